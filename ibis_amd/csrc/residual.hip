@@ -1,0 +1,1296 @@
+// residual.hip -- MI355X (gfx950) kernels and C ABI for ibis's cell residual.
+//
+// One residual evaluation (FiniteVolume<T>::compute_dudt,
+// src/finite_volume/finite_volume/finite_volume.cpp:57-96) is:
+//
+//   k_bc          boundary faces -> ghost rows of prim (pre-reconstruction
+//                 actions; boundaries/boundary.cpp:15-399)
+//   k_cell_grad   per cell: LSQ gradients of the reconstruction variables
+//                 and their limiters in one pass over the neighbour stencil
+//                 (gradient.h:66-116, limiter.cpp:24-62)
+//   k_face_flux   per face: MUSCL reconstruction of both sides, thermo
+//                 closure, rotation into the face frame, flux calculator,
+//                 rotation back (convective_flux.cpp:85-420)
+//   [k_post_flux] constant-flux boundary override (boundary.cpp:405-443)
+//   [k_bc, k_visc_grad, k_visc_flux]  viscous path (viscous_flux.cpp:16-184)
+//   k_integral    per cell: deterministic gather of its faces' fluxes in the
+//                 cell's own face order (finite_volume.cpp:221-255)
+//
+// Layout in HBM: all state, geometry and scratch are structure-of-arrays
+// FP64 with int32 indices.  The cell->face and cell->neighbour lists are
+// stored column-major ("ELL": entry j of cell i at j*N + i) so that a
+// wavefront reading entry j of 64 consecutive cells issues one coalesced
+// load.  Cells and faces are numbered in a locality order by the grid
+// builder, so neighbour and face gathers hit the same cache lines.
+//
+// There are no atomics in the residual: the face->cell accumulation is a
+// gather, so results are deterministic and independent of launch shape.
+// All code is compiled without floating-point contraction so that it
+// evaluates the reference's expression trees operation by operation.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "flux.hpp"
+#include "ibis_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            std::string m_ = std::string(#expr) + ": " + hipGetErrorString(e_);         \
+            return ibis::fail(IBIS_ERR_HIP, m_.c_str());                                \
+        }                                                                               \
+    } while (0)
+
+// primitive rows and gradient/limiter slots
+enum { P_RHO = 0, P_P = 1, P_T = 2, P_E = 3, P_VX = 4, P_VY = 5, P_VZ = 6, NPRIM = 7 };
+enum { G_P = 0, G_RHO = 1, G_T = 2, G_U = 3, G_VX = 4, G_VY = 5, G_VZ = 6, NGRAD = 7 };
+
+constexpr int BLOCK = 256;
+constexpr int MAXF = 6;
+
+struct DevMesh {
+    int dim, nl, maxf;
+    int N, G, Nt, F, nb;
+    const int* L;
+    const int* R;
+    const double* geom;  // 13 x F
+    const double* cen;   // 3 x Nt
+    const double* vol;   // Nt
+    const double* lsq;   // nl x N
+    const int* ell_face; // maxf x N
+    const int* ell_nbr;  // maxf x N
+    const signed char* ell_sign;  // maxf x N
+    const unsigned char* nfaces;  // N
+    const int* b_face;   // nb
+    const int* b_ghost;
+    const int* b_valid;
+    const int* b_tag;
+};
+
+__device__ __forceinline__ double gget(const double* p, int stride, int row, int i) {
+    return p[(size_t)row * stride + i];
+}
+
+// ---------------------------------------------------------------------------
+// Boundary ghost-cell actions (src/finite_volume/finite_volume/boundaries/boundary.cpp)
+// ---------------------------------------------------------------------------
+struct GasParams {
+    double R, Cv, Cp, gamma;
+};
+
+__global__ void __launch_bounds__(BLOCK) k_bc(DevMesh m, const int* __restrict__ kinds,
+                                              const double* __restrict__ pars, GasParams gp,
+                                              double* __restrict__ prim) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.nb) return;
+    const int f = m.b_face[b], gh = m.b_ghost[b], va = m.b_valid[b], t = m.b_tag[b];
+    const int Nt = m.Nt, F = m.F;
+    double* rho = prim + (size_t)P_RHO * Nt;
+    double* p = prim + (size_t)P_P * Nt;
+    double* temp = prim + (size_t)P_T * Nt;
+    double* e = prim + (size_t)P_E * Nt;
+    double* vx = prim + (size_t)P_VX * Nt;
+    double* vy = prim + (size_t)P_VY * Nt;
+    double* vz = prim + (size_t)P_VZ * Nt;
+    for (int a = 0; a < IBIS_MAX_ACT; a++) {
+        const int kind = kinds[t * IBIS_MAX_ACT + a];
+        if (kind == IBIS_BC_NONE) continue;
+        const double* par = pars + (t * IBIS_MAX_ACT + a) * IBIS_NPAR;
+        switch (kind) {
+            case IBIS_BC_FLOW_STATE_COPY:  // boundary.cpp:15-45
+                temp[gh] = par[0];
+                p[gh] = par[1];
+                rho[gh] = par[2];
+                e[gh] = par[3];
+                vx[gh] = par[4];
+                vy[gh] = par[5];
+                vz[gh] = par[6];
+                break;
+            case IBIS_BC_INTERNAL_COPY:  // boundary.cpp:100-136
+                temp[gh] = temp[va];
+                p[gh] = p[va];
+                rho[gh] = rho[va];
+                e[gh] = e[va];
+                vx[gh] = vx[va];
+                vy[gh] = vy[va];
+                vz[gh] = vz[va];
+                break;
+            case IBIS_BC_REFLECT_NORMAL: {  // boundary.cpp:142-206
+                temp[gh] = temp[va];
+                p[gh] = p[va];
+                rho[gh] = rho[va];
+                e[gh] = e[va];
+                double x = vx[va], y = vy[va], z = vz[va];
+                double nx = m.geom[0 * F + f], ny = m.geom[1 * F + f], nz = m.geom[2 * F + f];
+                double t1x = m.geom[3 * F + f], t1y = m.geom[4 * F + f], t1z = m.geom[5 * F + f];
+                double t2x = m.geom[6 * F + f], t2y = m.geom[7 * F + f], t2z = m.geom[8 * F + f];
+                double xs = -(x * nx + y * ny + z * nz);
+                double ys = x * t1x + y * t1y + z * t1z;
+                double zs = x * t2x + y * t2y + z * t2z;
+                vx[gh] = xs * nx + ys * t1x + zs * t2x;
+                vy[gh] = xs * ny + ys * t1y + zs * t2y;
+                vz[gh] = xs * nz + ys * t1z + zs * t2z;
+                break;
+            }
+            case IBIS_BC_VEL_REFLECT:  // boundary.cpp:209-241
+                vx[gh] = -vx[va];
+                vy[gh] = -vy[va];
+                vz[gh] = -vz[va];
+                break;
+            case IBIS_BC_FIX_TEMPERATURE:  // boundary.cpp:244-277
+                temp[gh] = 2 * par[0] - temp[va];
+                break;
+            case IBIS_BC_SUBSONIC_INFLOW: {  // boundary.cpp:286-342
+                double nx = m.geom[0 * F + f], ny = m.geom[1 * F + f], nz = m.geom[2 * F + f];
+                double in_vx = par[4], in_vy = par[5], in_vz = par[6];
+                double vel_correct =
+                    nx * (in_vx - vx[va]) + ny * (in_vy - vy[va]) + nz * (in_vz - vz[va]);
+                double a = sqrt(gp.gamma * gp.R * temp[va]);
+                double rho_ref = rho[va];
+                double p_in = par[1];
+                double p_face = 0.5 * (p_in + p[va] - rho_ref * a * vel_correct);
+                double rho_face = par[2] + (p_face - p_in) / a / a;
+                double vx_face = in_vx - nx * (p_in - p_face) / (rho_ref * a);
+                double vy_face = in_vy - ny * (p_in - p_face) / (rho_ref * a);
+                double vz_face = in_vz - nz * (p_in - p_face) / (rho_ref * a);
+                p[gh] = 2 * p_face - p[va];
+                rho[gh] = 2 * rho_face - rho[va];
+                temp[gh] = p[gh] / (rho[gh] * gp.R);
+                e[gh] = gp.Cv * temp[gh];
+                vx[gh] = 2 * vx_face - vx[va];
+                vy[gh] = 2 * vy_face - vy[va];
+                vz[gh] = 2 * vz_face - vz[va];
+                break;
+            }
+            case IBIS_BC_SUBSONIC_OUTFLOW: {  // boundary.cpp:348-399
+                double nx = m.geom[0 * F + f], ny = m.geom[1 * F + f], nz = m.geom[2 * F + f];
+                double p_face = par[0];
+                double rho_ref = rho[va];
+                double p_ref = p[va];
+                double a = sqrt(gp.gamma * gp.R * temp[va]);
+                double rho_face = rho[va] + (p_face - p_ref) / a / a;
+                double vx_face = vx[va] + nx * (p_ref - p_face) / (rho_ref * a);
+                double vy_face = vy[va] + ny * (p_ref - p_face) / (rho_ref * a);
+                double vz_face = vz[va] + nz * (p_ref - p_face) / (rho_ref * a);
+                p[gh] = 2 * p_face - p[va];
+                rho[gh] = 2 * rho_face - rho[va];
+                temp[gh] = p[gh] / (rho[gh] * gp.R);
+                e[gh] = gp.Cv * temp[gh];
+                vx[gh] = 2 * vx_face - vx[va];
+                vy[gh] = 2 * vy_face - vy[va];
+                vz[gh] = 2 * vz_face - vz[va];
+                break;
+            }
+            default:
+                break;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cell kernel: LSQ gradients (GR/gradient.h:66-116) of NV variables in one
+// pass over the stencil, then the limiter of each (FV/limiter.cpp:24-62, or
+// the Venkatakrishnan limiter).
+//   vrow[v]  : prim row of variable v
+//   vslot[v] : gradient/limiter slot of variable v
+//   ngrad    : variables [0, ngrad) get their gradient computed here; the
+//              rest use the gradient already stored in their slot (2D vz)
+//   LIM      : 0 none, 1 Barth-Jespersen, 2 Venkatakrishnan
+// ---------------------------------------------------------------------------
+struct VarSet {
+    int n, ngrad;
+    int row[5];
+    int slot[5];
+};
+
+template <int DIM, int LIM, int NV>
+__global__ void __launch_bounds__(BLOCK)
+    k_cell_grad(DevMesh m, VarSet vs, const double* __restrict__ prim, double* __restrict__ grad,
+                double* __restrict__ lim, double eps, double venkat_K) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int N = m.N, Nt = m.Nt, F = m.F;
+    if (i >= N) return;
+    const double* cx = m.cen;
+    const double* cy = m.cen + Nt;
+    const double* cz = m.cen + 2 * (size_t)Nt;
+    const double xi = cx[i], yi = cy[i], zi = cz[i];
+    const int nf = m.nfaces[i];
+    const double r11 = m.lsq[i], r12 = m.lsq[N + i], r22 = m.lsq[2 * (size_t)N + i];
+    double r13 = 0.0, r23 = 0.0, r33 = 0.0;
+    if (DIM == 3) {
+        r13 = m.lsq[3 * (size_t)N + i];
+        r23 = m.lsq[4 * (size_t)N + i];
+        r33 = m.lsq[5 * (size_t)N + i];
+    }
+    // per-cell constants of the weight formulae, hoisted exactly as written
+    const double beta = (r12 * r23 - r13 * r23) / (r11 * r22);  // GR/gradient.h:89 (sic)
+    const double r11s = r11 * r11;
+    const double i22 = 1.0 / (r22 * r22);
+    const double q12 = r12 / r11;
+    const double i33 = 1.0 / (r33 * r33);
+    const double q23 = r23 / r22;
+
+    double ui[NV], gx[NV], gy[NV], gz[NV], umin[NV], umax[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        ui[v] = prim[(size_t)vs.row[v] * Nt + i];
+        gx[v] = 0.0;
+        gy[v] = 0.0;
+        gz[v] = 0.0;
+        umin[v] = ui[v];
+        umax[v] = ui[v];
+    }
+    for (int j = 0; j < nf; j++) {
+        const int n = m.ell_nbr[(size_t)j * N + i];
+        const double dx = cx[n] - xi;
+        const double dy = cy[n] - yi;
+        const double dz = cz[n] - zi;
+        const double alpha_1 = dx / r11s;
+        const double alpha_2 = i22 * (dy - q12 * dx);
+        double alpha_3 = 0.0;
+        if (DIM == 3) alpha_3 = i33 * (dz - q23 * dy + beta * dx);
+        const double w_1 = alpha_1 - q12 * alpha_2 + beta * alpha_3;
+        const double w_2 = alpha_2 - q23 * alpha_3;
+        const double w_3 = alpha_3;
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            const double un = prim[(size_t)vs.row[v] * Nt + n];
+            const double diff = un - ui[v];
+            gx[v] += w_1 * diff;
+            gy[v] += w_2 * diff;
+            gz[v] += w_3 * diff;
+            umin[v] = ibis::kmin(umin[v], un);
+            umax[v] = ibis::kmax(umax[v], un);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        double* g = grad + (size_t)vs.slot[v] * 3 * N;
+        if (v < vs.ngrad) {
+            g[i] = gx[v];
+            g[N + i] = gy[v];
+            g[2 * (size_t)N + i] = gz[v];
+        } else {
+            gx[v] = g[i];
+            gy[v] = g[N + i];
+            gz[v] = g[2 * (size_t)N + i];
+        }
+    }
+    if (LIM == 0) return;
+    double phi[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) phi[v] = 1.0;
+    double eps2 = 0.0;
+    if (LIM == 2) {
+        const double V = m.vol[i];
+        const double K3 = venkat_K * venkat_K * venkat_K;
+        eps2 = (DIM == 3) ? K3 * V : K3 * (V * sqrt(V));
+    }
+    for (int j = 0; j < nf; j++) {
+        const int f = m.ell_face[(size_t)j * N + i];
+        const double dx = m.geom[9 * (size_t)F + f] - xi;
+        const double dy = m.geom[10 * (size_t)F + f] - yi;
+        const double dz = m.geom[11 * (size_t)F + f] - zi;
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            double d2 = gx[v] * dx + gy[v] * dy + gz[v] * dz;
+            if (LIM == 1) {  // Barth-Jespersen, FV/limiter.cpp:289-303
+                const int s = (d2 > 0) - (d2 < 0);
+                d2 = s * (fabs(d2) + eps);
+                if (s > 0)
+                    phi[v] = ibis::kmin(phi[v], (umax[v] - ui[v]) / d2);
+                else if (s < 0)
+                    phi[v] = ibis::kmin(phi[v], (umin[v] - ui[v]) / d2);
+            } else {  // Venkatakrishnan
+                double d1;
+                if (d2 > 0.0)
+                    d1 = umax[v] - ui[v];
+                else if (d2 < 0.0)
+                    d1 = umin[v] - ui[v];
+                else
+                    continue;
+                const double d1s = d1 * d1;
+                const double d2s = d2 * d2;
+                const double num = d1s + eps2 + 2.0 * (d1 * d2);
+                const double den = d1s + 2.0 * d2s + d1 * d2 + eps2;
+                phi[v] = ibis::kmin(phi[v], num / den);
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) lim[(size_t)vs.slot[v] * N + i] = phi[v];
+}
+
+// ---------------------------------------------------------------------------
+// Face kernel: reconstruction + flux (FV/convective_flux.cpp:85-420)
+// ---------------------------------------------------------------------------
+struct FaceParams {
+    int thermo;   // 0 rho_p, 1 rho_T, 2 rho_u, 3 p_T
+    int limited;  // limiter enabled
+    int slotA, rowA, slotB, rowB;  // the two thermo reconstruction variables
+    double R, Cv, gamma, delta;
+};
+
+// value + limiter * (grad . d), FV/convective_flux.cpp:243-255
+__device__ __forceinline__ double lin_interp(double value, const double* grad, int N, int slot,
+                                             int c, bool valid, double L, double dx, double dy,
+                                             double dz) {
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    if (valid) {
+        const double* g = grad + (size_t)slot * 3 * N;
+        gx = g[c];
+        gy = g[N + c];
+        gz = g[2 * (size_t)N + c];
+    }
+    return value + L * (gx * dx + gy * dy + gz * dz);
+}
+
+template <bool RECON>
+__device__ __forceinline__ void reconstruct(const DevMesh& m, const FaceParams& fp,
+                                            const double* __restrict__ prim,
+                                            const double* __restrict__ grad,
+                                            const double* __restrict__ lim, int f, int c,
+                                            double* st) {
+    const int N = m.N, Nt = m.Nt, F = m.F;
+    if (!RECON) {  // copy_reconstruct, FV/convective_flux.cpp:212-240
+#pragma unroll
+        for (int v = 0; v < NPRIM; v++) st[v] = prim[(size_t)v * Nt + c];
+        return;
+    }
+    const bool valid = c < N;
+    const bool limit = fp.limited && valid;
+    const double dx = m.geom[9 * (size_t)F + f] - m.cen[c];
+    const double dy = m.geom[10 * (size_t)F + f] - m.cen[Nt + c];
+    const double dz = m.geom[11 * (size_t)F + f] - m.cen[2 * (size_t)Nt + c];
+    const double vA = prim[(size_t)fp.rowA * Nt + c];
+    const double vB = prim[(size_t)fp.rowB * Nt + c];
+    const double LA = limit ? lim[(size_t)fp.slotA * N + c] : 1.0;
+    const double LB = limit ? lim[(size_t)fp.slotB * N + c] : 1.0;
+    const double a = lin_interp(vA, grad, N, fp.slotA, c, valid, LA, dx, dy, dz);
+    const double b = lin_interp(vB, grad, N, fp.slotB, c, valid, LB, dx, dy, dz);
+    double rho = 0.0, p = 0.0, T = 0.0, e = 0.0;
+    switch (fp.thermo) {
+        case IBIS_THERMO_RHO_P: p = a; rho = b; break;
+        case IBIS_THERMO_RHO_T: rho = a; T = b; break;
+        case IBIS_THERMO_RHO_U: rho = a; e = b; break;
+        default: p = a; T = b; break;
+    }
+    ibis::thermo_update<double>(fp.thermo, fp.R, fp.Cv, rho, p, T, e);
+    st[ibis::S_RHO] = rho;
+    st[ibis::S_P] = p;
+    st[ibis::S_T] = T;
+    st[ibis::S_E] = e;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int slot = G_VX + k;
+        const double L = limit ? lim[(size_t)slot * N + c] : 1.0;
+        st[ibis::S_U + k] =
+            lin_interp(prim[(size_t)(P_VX + k) * Nt + c], grad, N, slot, c, valid, L, dx, dy, dz);
+    }
+}
+
+template <int DIM, int FLUX, bool RECON, bool PREMUL>
+__global__ void __launch_bounds__(BLOCK)
+    k_face_flux(DevMesh m, FaceParams fp, const double* __restrict__ prim,
+                const double* __restrict__ grad, const double* __restrict__ lim,
+                double* __restrict__ flux) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int F = m.F;
+    if (f >= F) return;
+    const int l = m.L[f], r = m.R[f];
+    double Ls[NPRIM], Rs[NPRIM];
+    reconstruct<RECON>(m, fp, prim, grad, lim, f, l, Ls);
+    reconstruct<RECON>(m, fp, prim, grad, lim, f, r, Rs);
+    const double nx = m.geom[f], ny = m.geom[(size_t)F + f], nz = m.geom[2 * (size_t)F + f];
+    const double t1x = m.geom[3 * (size_t)F + f], t1y = m.geom[4 * (size_t)F + f],
+                 t1z = m.geom[5 * (size_t)F + f];
+    const double t2x = m.geom[6 * (size_t)F + f], t2y = m.geom[7 * (size_t)F + f],
+                 t2z = m.geom[8 * (size_t)F + f];
+    // transform_to_local_frame, UT/vector3.cpp:142-153
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+        double* s = side ? Rs : Ls;
+        const double a = s[4], b = s[5], c = s[6];
+        s[4] = a * nx + b * ny + c * nz;
+        s[5] = a * t1x + b * t1y + c * t1z;
+        s[6] = a * t2x + b * t2y + c * t2z;
+    }
+    double out[5];
+    constexpr bool T3 = DIM == 3;
+    if (FLUX == IBIS_FLUX_AUSMDV)
+        ibis::flux_ausmdv<double, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, out);
+    else if (FLUX == IBIS_FLUX_HANEL)
+        ibis::flux_hanel<double, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, out);
+    else if (FLUX == IBIS_FLUX_LDFSS2)
+        ibis::flux_ldfss2<double, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, fp.delta, out);
+    else
+        ibis::flux_rusanov<double, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, out);
+    // flux::transform_to_global, FV/convective_flux.cpp:148-177
+    const double px = out[1], py = out[2];
+    double pz = 0.0;
+    if (T3) pz = out[3];
+    const double x = px * nx + py * t1x + pz * t2x;
+    const double y = px * ny + py * t1y + pz * t2y;
+    const double z = px * nz + py * t1z + pz * t2z;
+    double res[5] = {out[0], x, y, z, out[4]};
+    const double A = PREMUL ? m.geom[12 * (size_t)F + f] : 1.0;
+    flux[f] = PREMUL ? res[0] * A : res[0];
+    flux[(size_t)F + f] = PREMUL ? res[1] * A : res[1];
+    flux[2 * (size_t)F + f] = PREMUL ? res[2] * A : res[2];
+    if (T3) {
+        flux[3 * (size_t)F + f] = PREMUL ? res[3] * A : res[3];
+        flux[4 * (size_t)F + f] = PREMUL ? res[4] * A : res[4];
+    } else {
+        flux[3 * (size_t)F + f] = PREMUL ? res[4] * A : res[4];
+    }
+}
+
+// ConstantFlux::apply (boundary.cpp:405-443), static grid
+__global__ void __launch_bounds__(BLOCK)
+    k_post_flux(DevMesh m, const int* __restrict__ kinds, const double* __restrict__ pars,
+                GasParams gp, int premul, double* __restrict__ flux) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.nb) return;
+    const int f = m.b_face[b], t = m.b_tag[b], F = m.F;
+    const bool three_d = m.dim == 3;
+    for (int a = 0; a < IBIS_MAX_ACT; a++) {
+        if (kinds[t * IBIS_MAX_ACT + a] != IBIS_BC_CONSTANT_FLUX) continue;
+        const double* par = pars + (t * IBIS_MAX_ACT + a) * IBIS_NPAR;
+        const double nx = m.geom[f], ny = m.geom[(size_t)F + f], nz = m.geom[2 * (size_t)F + f];
+        const double u = gp.Cv * par[0];
+        const double fvx = par[4], fvy = par[5], fvz = par[6];
+        const double rvx = fvx - 0.0, rvy = fvy - 0.0, rvz = fvz - 0.0;
+        const double mass_flux = par[2] * (rvx * nx + rvy * ny + rvz * nz);
+        const double p = par[1];
+        double v[5];
+        v[0] = mass_flux;
+        v[1] = p * nx + fvx * mass_flux;
+        v[2] = p * ny + fvy * mass_flux;
+        v[3] = p * nz + fvz * mass_flux;
+        v[4] = mass_flux * (u + 0.5 * (fvx * fvx + fvy * fvy + fvz * fvz)) +
+               p * (fvx * nx + fvy * nx + fvz * nz);  // v_y * n_x as in the reference
+        const double A = premul ? m.geom[12 * (size_t)F + f] : 1.0;
+        const int ne = three_d ? 5 : 4;
+        for (int q = 0; q < ne; q++) {
+            const double val = (!three_d && q == 3) ? v[4] : v[q];
+            flux[(size_t)q * F + f] = premul ? val * A : val;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Viscous flux (FV/viscous_flux.cpp:36-184)
+// ---------------------------------------------------------------------------
+struct ViscParams {
+    double R, Cv, Cp, mu0, T0, Ts, Pr;
+};
+
+__device__ __forceinline__ double sutherland(double temp, const ViscParams& vp) {
+    // GAS/transport_properties.h:27-42
+    return vp.mu0 * pow(temp / vp.T0, 3.0 / 2.0) * (vp.T0 + vp.Ts) / (temp + vp.Ts);
+}
+
+__device__ __forceinline__ void hasselbacher(const double* __restrict__ grad, int N, int slot,
+                                             double vl, double vr, int l, int r,
+                                             const double* ehat, const double* n, double len_e,
+                                             double edn, double* out) {
+    const double* g = grad + (size_t)slot * 3 * N;
+    const double ax = 0.5 * (g[l] + g[r]);
+    const double ay = 0.5 * (g[N + l] + g[N + r]);
+    const double az = 0.5 * (g[2 * (size_t)N + l] + g[2 * (size_t)N + r]);
+    const double avg_dot_ehat = ax * ehat[0] + ay * ehat[1] + az * ehat[2];
+    const double correction = avg_dot_ehat - (vr - vl) / len_e;
+    out[0] = ax - correction * n[0] / edn;
+    out[1] = ay - correction * n[1] / edn;
+    out[2] = az - correction * n[2] / edn;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(BLOCK)
+    k_visc_flux(DevMesh m, ViscParams vp, const double* __restrict__ prim,
+                const double* __restrict__ grad, double* __restrict__ flux,
+                double* __restrict__ face_rho, double* __restrict__ face_T) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int F = m.F, N = m.N, Nt = m.Nt;
+    if (f >= F) return;
+    const int l = m.L[f], r = m.R[f];
+    const bool lv = l < N, rv = r < N;
+    double gT[3], gvx[3], gvy[3], gvz[3];
+    const double n[3] = {m.geom[f], m.geom[(size_t)F + f], m.geom[2 * (size_t)F + f]};
+    if (!lv || !rv) {  // copy_gradients_to_face, viscous_flux.cpp:16-34
+        const int c = lv ? l : r;
+        for (int k = 0; k < 3; k++) {
+            gT[k] = grad[((size_t)G_T * 3 + k) * N + c];
+            gvx[k] = grad[((size_t)G_VX * 3 + k) * N + c];
+            gvy[k] = grad[((size_t)G_VY * 3 + k) * N + c];
+            gvz[k] = grad[((size_t)G_VZ * 3 + k) * N + c];
+        }
+    } else {  // hasselbacher_average, viscous_flux.cpp:52-80
+        const double ex = m.cen[r] - m.cen[l];
+        const double ey = m.cen[Nt + r] - m.cen[Nt + l];
+        const double ez = m.cen[2 * (size_t)Nt + r] - m.cen[2 * (size_t)Nt + l];
+        const double len_e = sqrt(ex * ex + ey * ey + ez * ez);
+        const double ehat[3] = {ex / len_e, ey / len_e, ez / len_e};
+        const double edn = ehat[0] * n[0] + ehat[1] * n[1] + ehat[2] * n[2];
+        hasselbacher(grad, N, G_VX, prim[(size_t)P_VX * Nt + l], prim[(size_t)P_VX * Nt + r], l,
+                     r, ehat, n, len_e, edn, gvx);
+        hasselbacher(grad, N, G_VY, prim[(size_t)P_VY * Nt + l], prim[(size_t)P_VY * Nt + r], l,
+                     r, ehat, n, len_e, edn, gvy);
+        hasselbacher(grad, N, G_VZ, prim[(size_t)P_VZ * Nt + l], prim[(size_t)P_VZ * Nt + r], l,
+                     r, ehat, n, len_e, edn, gvz);
+        hasselbacher(grad, N, G_T, prim[(size_t)P_T * Nt + l], prim[(size_t)P_T * Nt + r], l, r,
+                     ehat, n, len_e, edn, gT);
+    }
+    // average_flow_states_pT + update_thermo_from_pT (GAS/flow_state.h:89-91)
+    const double p = 0.5 * (prim[(size_t)P_P * Nt + l] + prim[(size_t)P_P * Nt + r]);
+    const double temp = 0.5 * (prim[(size_t)P_T * Nt + l] + prim[(size_t)P_T * Nt + r]);
+    const double vx = 0.5 * (prim[(size_t)P_VX * Nt + l] + prim[(size_t)P_VX * Nt + r]);
+    const double vy = 0.5 * (prim[(size_t)P_VY * Nt + l] + prim[(size_t)P_VY * Nt + r]);
+    const double vz = 0.5 * (prim[(size_t)P_VZ * Nt + l] + prim[(size_t)P_VZ * Nt + r]);
+    face_rho[f] = p / (vp.R * temp);
+    face_T[f] = temp;
+    const double mu = sutherland(temp, vp);
+    const double k = vp.Cp * sutherland(temp, vp) / vp.Pr;
+    const double lambda = -2.0 / 3.0 * mu;
+    const double bulk = lambda * (gvx[0] + gvy[1] + gvz[2]);
+    const double tau_xx = 2.0 * mu * gvx[0] + bulk;
+    const double tau_yy = 2.0 * mu * gvy[1] + bulk;
+    const double tau_zz = 2.0 * mu * gvz[2] + bulk;
+    const double tau_xy = mu * (gvx[1] + gvy[0]);
+    const double tau_xz = mu * (gvx[2] + gvz[0]);
+    const double tau_yz = mu * (gvy[2] + gvz[1]);
+    const double theta_x = vx * tau_xx + vy * tau_xy + vz * tau_xz + k * gT[0];
+    const double theta_y = vx * tau_xy + vy * tau_yy + vz * tau_yz + k * gT[1];
+    const double theta_z = vx * tau_xz + vy * tau_yz + vz * tau_zz + k * gT[2];
+    flux[(size_t)F + f] -= n[0] * tau_xx + n[1] * tau_xy + n[2] * tau_xz;
+    flux[2 * (size_t)F + f] -= n[0] * tau_xy + n[1] * tau_yy + n[2] * tau_yz;
+    if (DIM == 3) {
+        flux[3 * (size_t)F + f] -= n[0] * tau_xz + n[1] * tau_yz + n[2] * tau_zz;
+        flux[4 * (size_t)F + f] -= n[0] * theta_x + n[1] * theta_y + n[2] * theta_z;
+    } else {
+        flux[3 * (size_t)F + f] -= n[0] * theta_x + n[1] * theta_y + n[2] * theta_z;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Surface integral (FV/finite_volume.cpp:221-255): per cell, in the cell's
+// face order.  With PREMUL the flux already holds flux*area and
+// flux*(-area*outsign) == (outsign > 0 ? -(flux*area) : flux*area) exactly.
+// ---------------------------------------------------------------------------
+template <int DIM, bool PREMUL>
+__global__ void __launch_bounds__(BLOCK)
+    k_integral(DevMesh m, const double* __restrict__ flux, double* __restrict__ dudt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int N = m.N, Nt = m.Nt, F = m.F;
+    if (i >= N) return;
+    constexpr int NC = DIM + 2;
+    double d[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) d[q] = 0.0;
+    const int nf = m.nfaces[i];
+    for (int j = 0; j < nf; j++) {
+        const int f = m.ell_face[(size_t)j * N + i];
+        const int s = m.ell_sign[(size_t)j * N + i];
+        if (PREMUL) {
+#pragma unroll
+            for (int q = 0; q < NC; q++) {
+                const double fa = flux[(size_t)q * F + f];
+                d[q] += (s > 0) ? -fa : fa;
+            }
+        } else {
+            const double area = -m.geom[12 * (size_t)F + f] * s;
+#pragma unroll
+            for (int q = 0; q < NC; q++) d[q] += flux[(size_t)q * F + f] * area;
+        }
+    }
+    const double V = m.vol[i];
+#pragma unroll
+    for (int q = 0; q < NC; q++) dudt[(size_t)q * Nt + i] = d[q] / V;
+}
+
+// ---------------------------------------------------------------------------
+// estimate_dt (FV/finite_volume.cpp:131-184): block minimum, then a single
+// block reduces the partial minima.  Minimum is order independent.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double block_min(double v) {
+    __shared__ double sm[BLOCK / 64];
+    for (int o = 32; o > 0; o >>= 1) v = ibis::kmin(__shfl_down(v, o, 64), v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sm[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < BLOCK / 64; k++) v = ibis::kmin(sm[k], v);
+    }
+    return v;
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_dt_partial(DevMesh m, const double* __restrict__ prim, const double* __restrict__ face_rho,
+                 const double* __restrict__ face_T, int viscous, double sf, ViscParams vp,
+                 double gamma, double* __restrict__ partial) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int N = m.N, Nt = m.Nt, F = m.F;
+    double dt = 1.79769313486231570815e+308;
+    if (i < N) {
+        const double vx = prim[(size_t)P_VX * Nt + i], vy = prim[(size_t)P_VY * Nt + i],
+                     vz = prim[(size_t)P_VZ * Nt + i];
+        const double a = sqrt(gamma * vp.R * prim[(size_t)P_T * Nt + i]);
+        double src = 0.0, srv = 0.0;
+        const double volume = m.vol[i];
+        const int nf = m.nfaces[i];
+        for (int j = 0; j < nf; j++) {
+            const int f = m.ell_face[(size_t)j * N + i];
+            const double area = m.geom[12 * (size_t)F + f];
+            const double dot = vx * m.geom[f] + vy * m.geom[(size_t)F + f] +
+                               vz * m.geom[2 * (size_t)F + f];
+            const double sig_vel = fabs(dot) + a;
+            src += sig_vel * area;
+            if (viscous) {
+                const double tf = face_T[f];
+                const double mu = sutherland(tf, vp);
+                const double k = vp.Cp * sutherland(tf, vp) / vp.Pr;
+                const double rho = face_rho[f];
+                const double Pr = mu * vp.Cp / k;
+                const double tmp = (gamma / rho) * (mu / Pr) * area * area;
+                srv += tmp / volume;
+            }
+        }
+        dt = volume / (src + sf * srv);
+    }
+    dt = block_min(dt);
+    if (threadIdx.x == 0) partial[blockIdx.x] = dt;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_min_final(const double* __restrict__ partial, int n,
+                                                     double* __restrict__ out) {
+    double v = 1.79769313486231570815e+308;
+    for (int k = threadIdx.x; k < n; k += BLOCK) v = ibis::kmin(partial[k], v);
+    v = block_min(v);
+    if (threadIdx.x == 0) out[0] = v;
+}
+
+// count_bad_cells (FV/finite_volume.cpp:258-270): integer sum, order independent
+__global__ void __launch_bounds__(BLOCK) k_zero_u64(unsigned long long* p) { p[0] = 0ull; }
+
+__global__ void __launch_bounds__(BLOCK)
+    k_count_bad(int N, int Nt, const double* __restrict__ prim, unsigned long long* count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int bad = 0;
+    if (i < N) {
+        const double temp = prim[(size_t)P_T * Nt + i], rho = prim[(size_t)P_RHO * Nt + i];
+        bad = (temp < 0.0 || rho < 0.0 || isnan(rho) || isinf(rho)) ? 1 : 0;
+    }
+    unsigned long long ballot = __ballot(bad);
+    if ((threadIdx.x & 63) == 0 && ballot) atomicAdd(count, (unsigned long long)__popcll(ballot));
+}
+
+// conversions, FV/primative_conserved_conversion.cpp:5-51
+template <int DIM>
+__global__ void __launch_bounds__(BLOCK)
+    k_cons_to_prim(int Nt, double R, double Cv, const double* __restrict__ cq,
+                   double* __restrict__ fs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nt) return;
+    const double rho = cq[i];
+    const double vx = cq[(size_t)Nt + i] / rho;
+    const double vy = cq[2 * (size_t)Nt + i] / rho;
+    const double vz = DIM == 3 ? cq[3 * (size_t)Nt + i] / rho : 0.0;
+    const double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
+    const double u = cq[(size_t)(DIM + 1) * Nt + i] / rho - ke;
+    const double temp = u / Cv;
+    fs[(size_t)P_RHO * Nt + i] = rho;
+    fs[(size_t)P_VX * Nt + i] = vx;
+    fs[(size_t)P_VY * Nt + i] = vy;
+    fs[(size_t)P_VZ * Nt + i] = vz;
+    fs[(size_t)P_E * Nt + i] = u;
+    fs[(size_t)P_T * Nt + i] = temp;
+    fs[(size_t)P_P * Nt + i] = rho * R * temp;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(BLOCK)
+    k_prim_to_cons(int Nt, const double* __restrict__ fs, double* __restrict__ cq) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nt) return;
+    const double vx = fs[(size_t)P_VX * Nt + i], vy = fs[(size_t)P_VY * Nt + i],
+                 vz = fs[(size_t)P_VZ * Nt + i];
+    const double rho = fs[(size_t)P_RHO * Nt + i];
+    cq[i] = rho;
+    cq[(size_t)Nt + i] = rho * vx;
+    cq[2 * (size_t)Nt + i] = rho * vy;
+    if (DIM == 3) cq[3 * (size_t)Nt + i] = rho * vz;
+    const double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
+    cq[(size_t)(DIM + 1) * Nt + i] = rho * (ke + fs[(size_t)P_E * Nt + i]);
+}
+
+// u1 = u0 + k*dt  (apply_time_derivative, FV/conserved_quantities.cpp:80-94)
+__global__ void __launch_bounds__(BLOCK)
+    k_axpy(long n, const double* __restrict__ u0, double* u1, const double* __restrict__ k,
+           double dt) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u1[i] = u0[i] + k[i] * dt;
+}
+
+inline unsigned nblocks(long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+
+}  // namespace
+
+// ===========================================================================
+// host side
+// ===========================================================================
+namespace ibis {
+ibis_status fail(ibis_status s, const char* msg) {
+    g_last_error = msg ? msg : "";
+    return s;
+}
+}  // namespace ibis
+
+extern "C" const char* ibis_last_error(void) { return g_last_error.c_str(); }
+
+struct ibis_mesh {
+    DevMesh d;
+    std::vector<void*> allocs;
+};
+
+template <class T>
+static ibis_status upload(ibis_mesh* m, const std::vector<T>& h, const T** out) {
+    void* p = nullptr;
+    size_t bytes = std::max<size_t>(h.size() * sizeof(T), 8);
+    HIP_TRY(hipMalloc(&p, bytes));
+    m->allocs.push_back(p);
+    if (!h.empty()) HIP_TRY(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    *out = static_cast<const T*>(p);
+    return IBIS_OK;
+}
+
+void ibis_mesh_destroy(ibis_mesh* m) {
+    if (!m) return;
+    for (void* p : m->allocs) (void)hipFree(p);
+    delete m;
+}
+
+ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
+    if (!d || !out) return ibis::fail(IBIS_ERR_INVALID, "ibis_mesh_create: null argument");
+    if (d->dim != 2 && d->dim != 3) return ibis::fail(IBIS_ERR_INVALID, "dim must be 2 or 3");
+    const int64_t N = d->n_cells, G = d->n_ghost, F = d->n_faces, Nt = N + G;
+    if (N <= 0 || F <= 0 || Nt >= (int64_t)INT32_MAX || F >= (int64_t)INT32_MAX)
+        return ibis::fail(IBIS_ERR_INVALID, "ibis_mesh_create: mesh size out of range");
+    if (d->ntags < 0 || d->ntags > IBIS_MAX_TAGS)
+        return ibis::fail(IBIS_ERR_INVALID, "ibis_mesh_create: too many boundary tags");
+    // validate connectivity on the host: every index the kernels follow
+    int maxf = 0;
+    for (int64_t c = 0; c < N; c++) {
+        int64_t n = d->cell_face_off[c + 1] - d->cell_face_off[c];
+        if (n <= 0 || n > MAXF)
+            return ibis::fail(IBIS_ERR_INVALID, "ibis_mesh_create: cells must have 1..6 faces");
+        maxf = std::max<int>(maxf, (int)n);
+    }
+    const int64_t nnz = d->cell_face_off[N];
+    for (int64_t j = 0; j < nnz; j++) {
+        if (d->cell_faces[j] < 0 || d->cell_faces[j] >= F || d->cell_neighbours[j] < 0 ||
+            d->cell_neighbours[j] >= Nt)
+            return ibis::fail(IBIS_ERR_INVALID, "ibis_mesh_create: cell list index out of range");
+    }
+    for (int64_t f = 0; f < F; f++)
+        if (d->face_left[f] < 0 || d->face_left[f] >= Nt || d->face_right[f] < 0 ||
+            d->face_right[f] >= Nt)
+            return ibis::fail(IBIS_ERR_INVALID, "ibis_mesh_create: face cell index out of range");
+    const int64_t nb = d->tag_off[d->ntags];
+    std::vector<int> b_face(nb), b_ghost(nb), b_valid(nb), b_tag(nb);
+    for (int t = 0; t < d->ntags; t++)
+        for (int64_t k = d->tag_off[t]; k < d->tag_off[t + 1]; k++) {
+            int64_t f = d->bfaces[k];
+            if (f < 0 || f >= F) return ibis::fail(IBIS_ERR_INVALID, "boundary face out of range");
+            int64_t l = d->face_left[f], r = d->face_right[f];
+            b_face[k] = (int)f;
+            b_valid[k] = (int)(l < N ? l : r);
+            b_ghost[k] = (int)(l < N ? r : l);
+            b_tag[k] = t;
+            if (b_ghost[k] < N)
+                return ibis::fail(IBIS_ERR_INVALID, "boundary face without a ghost cell");
+        }
+    ibis_mesh* m = new ibis_mesh();
+    DevMesh& D = m->d;
+    D.dim = d->dim;
+    D.nl = d->dim == 2 ? 3 : 6;
+    D.maxf = maxf;
+    D.N = (int)N;
+    D.G = (int)G;
+    D.Nt = (int)Nt;
+    D.F = (int)F;
+    D.nb = (int)nb;
+    std::vector<int> L(F), R(F);
+    for (int64_t f = 0; f < F; f++) {
+        L[f] = (int)d->face_left[f];
+        R[f] = (int)d->face_right[f];
+    }
+    std::vector<int> ef((size_t)maxf * N, 0), en((size_t)maxf * N, 0);
+    std::vector<signed char> es((size_t)maxf * N, 0);
+    std::vector<unsigned char> nf(N);
+    for (int64_t c = 0; c < N; c++) {
+        int64_t o = d->cell_face_off[c];
+        nf[c] = (unsigned char)(d->cell_face_off[c + 1] - o);
+        for (int j = 0; j < nf[c]; j++) {
+            ef[(size_t)j * N + c] = (int)d->cell_faces[o + j];
+            en[(size_t)j * N + c] = (int)d->cell_neighbours[o + j];
+            es[(size_t)j * N + c] = (signed char)d->cell_outsigns[o + j];
+        }
+    }
+    std::vector<double> geom(d->face_geom, d->face_geom + 13 * F);
+    std::vector<double> cen(d->centroid, d->centroid + 3 * Nt);
+    std::vector<double> vol(d->volume, d->volume + Nt);
+    std::vector<double> lsq(d->lsq, d->lsq + (size_t)D.nl * N);
+    ibis_status s;
+#define UP(vec, field)                                      \
+    if ((s = upload(m, vec, &D.field)) != IBIS_OK) {        \
+        ibis_mesh_destroy(m);                               \
+        return s;                                           \
+    }
+    UP(L, L);
+    UP(R, R);
+    UP(geom, geom);
+    UP(cen, cen);
+    UP(vol, vol);
+    UP(lsq, lsq);
+    UP(ef, ell_face);
+    UP(en, ell_nbr);
+    UP(es, ell_sign);
+    UP(nf, nfaces);
+    UP(b_face, b_face);
+    UP(b_ghost, b_ghost);
+    UP(b_valid, b_valid);
+    UP(b_tag, b_tag);
+#undef UP
+    *out = m;
+    return IBIS_OK;
+}
+
+ibis_status ibis_mesh_from_grid(const ibis_grid* g, ibis_mesh** out) {
+    if (!g) return ibis::fail(IBIS_ERR_INVALID, "ibis_mesh_from_grid: null grid");
+    ibis_mesh_desc d;
+    ibis_grid_fill_desc(g, &d);
+    return ibis_mesh_create(&d, out);
+}
+
+struct ibis_residual {
+    const ibis_mesh* m = nullptr;
+    ibis_fv_desc desc;
+    bool premul = false;
+    double* grad = nullptr;
+    double* lim = nullptr;
+    double* flux = nullptr;
+    double* face_rho = nullptr;
+    double* face_T = nullptr;
+    int* kinds = nullptr;    // pre, visc, post
+    double* pars = nullptr;  // pre, visc, post
+    double* partial = nullptr;
+    double* dt_dev = nullptr;
+    unsigned long long* count_dev = nullptr;
+    double* host_buf = nullptr;  // pinned: [0] dt, [1] count
+    // RK stage storage
+    std::vector<double*> k;
+    double* k_tmp = nullptr;
+    double* flow_tmp = nullptr;
+};
+
+void ibis_residual_destroy(ibis_residual* r) {
+    if (!r) return;
+    for (void* p : {(void*)r->grad, (void*)r->lim, (void*)r->flux, (void*)r->face_rho,
+                    (void*)r->face_T, (void*)r->kinds, (void*)r->pars, (void*)r->partial,
+                    (void*)r->dt_dev, (void*)r->count_dev, (void*)r->k_tmp, (void*)r->flow_tmp})
+        if (p) (void)hipFree(p);
+    for (double* p : r->k) (void)hipFree(p);
+    if (r->host_buf) (void)hipHostFree(r->host_buf);
+    delete r;
+}
+
+ibis_status ibis_residual_create(const ibis_mesh* m, const ibis_fv_desc* desc,
+                                 ibis_residual** out) {
+    if (!m || !desc || !out) return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_create: null argument");
+    const ibis_fv_desc& d = *desc;
+    if (d.flux < 0 || d.flux > 3) return ibis::fail(IBIS_ERR_INVALID, "Unknown flux calculator");
+    if (d.order != 1 && d.order != 2) return ibis::fail(IBIS_ERR_INVALID, "Invalid reconstruction order");
+    if (d.order > 1 && (d.thermo < 0 || d.thermo > 3))
+        return ibis::fail(IBIS_ERR_INVALID, "Unknown thermo interpolator");
+    if (d.order > 1 && (d.limiter < 0 || d.limiter > 2))
+        return ibis::fail(IBIS_ERR_INVALID, "Unknown limiter");
+    for (int q = 0; q < IBIS_MAX_TAGS * IBIS_MAX_ACT; q++) {
+        if (d.pre_kind[q] < 0 || d.pre_kind[q] > 7 || d.visc_kind[q] < 0 || d.visc_kind[q] > 7 ||
+            (d.post_kind[q] != 0 && d.post_kind[q] != IBIS_BC_CONSTANT_FLUX))
+            return ibis::fail(IBIS_ERR_INVALID, "Unknown boundary action");
+    }
+    const DevMesh& D = m->d;
+    ibis_residual* r = new ibis_residual();
+    r->m = m;
+    r->desc = d;
+    r->premul = !d.viscous;
+    const size_t N = D.N, F = D.F;
+    const int nc = D.dim + 2;
+    auto alloc0 = [&](void** p, size_t bytes) -> hipError_t {
+        hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 8));
+        if (e != hipSuccess) return e;
+        return hipMemset(*p, 0, std::max<size_t>(bytes, 8));
+    };
+#define AL(ptr, bytes)                                                         \
+    do {                                                                       \
+        hipError_t e_ = alloc0((void**)&(ptr), (bytes));                       \
+        if (e_ != hipSuccess) {                                                \
+            ibis_residual_destroy(r);                                          \
+            return ibis::fail(IBIS_ERR_HIP, hipGetErrorString(e_));            \
+        }                                                                      \
+    } while (0)
+    AL(r->grad, sizeof(double) * NGRAD * 3 * N);
+    AL(r->lim, sizeof(double) * NGRAD * N);
+    AL(r->flux, sizeof(double) * nc * F);
+    if (d.viscous) {
+        AL(r->face_rho, sizeof(double) * F);
+        AL(r->face_T, sizeof(double) * F);
+    }
+    const size_t ntab = IBIS_MAX_TAGS * IBIS_MAX_ACT;
+    AL(r->kinds, sizeof(int) * 3 * ntab);
+    AL(r->pars, sizeof(double) * 3 * ntab * IBIS_NPAR);
+    AL(r->partial, sizeof(double) * (nblocks(D.N) + 1));
+    AL(r->dt_dev, sizeof(double));
+    AL(r->count_dev, sizeof(unsigned long long));
+#undef AL
+    std::vector<int> kinds(3 * ntab);
+    std::vector<double> pars(3 * ntab * IBIS_NPAR);
+    std::memcpy(&kinds[0], d.pre_kind, ntab * sizeof(int));
+    std::memcpy(&kinds[ntab], d.visc_kind, ntab * sizeof(int));
+    std::memcpy(&kinds[2 * ntab], d.post_kind, ntab * sizeof(int));
+    std::memcpy(&pars[0], d.pre_par, ntab * IBIS_NPAR * sizeof(double));
+    std::memcpy(&pars[ntab * IBIS_NPAR], d.visc_par, ntab * IBIS_NPAR * sizeof(double));
+    std::memcpy(&pars[2 * ntab * IBIS_NPAR], d.post_par, ntab * IBIS_NPAR * sizeof(double));
+    if (hipMemcpy(r->kinds, kinds.data(), kinds.size() * sizeof(int), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(r->pars, pars.data(), pars.size() * sizeof(double), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipHostMalloc((void**)&r->host_buf, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        ibis_residual_destroy(r);
+        return ibis::fail(IBIS_ERR_HIP, "ibis_residual_create: upload failed");
+    }
+    *out = r;
+    return IBIS_OK;
+}
+
+namespace {
+
+template <int DIM>
+void launch_cell_grad(const DevMesh& D, const VarSet& vs, int lim, const double* prim,
+                      double* grad, double* limv, double eps, double K, hipStream_t st) {
+    const unsigned nb = nblocks(D.N);
+    // NV is 5 in both dimensions (rho/e-like pair, vx, vy, vz); in 2D the vz
+    // gradient is taken from its slot rather than computed.
+    if (lim == 0)
+        k_cell_grad<DIM, 0, 5><<<nb, BLOCK, 0, st>>>(D, vs, prim, grad, limv, eps, K);
+    else if (lim == 1)
+        k_cell_grad<DIM, 1, 5><<<nb, BLOCK, 0, st>>>(D, vs, prim, grad, limv, eps, K);
+    else
+        k_cell_grad<DIM, 2, 5><<<nb, BLOCK, 0, st>>>(D, vs, prim, grad, limv, eps, K);
+}
+
+template <int DIM, int FLUX>
+void launch_face_flux_f(const DevMesh& D, const FaceParams& fp, bool recon, bool premul,
+                        const double* prim, const double* grad, const double* lim, double* flux,
+                        hipStream_t st) {
+    const unsigned nb = nblocks(D.F);
+    if (recon) {
+        if (premul)
+            k_face_flux<DIM, FLUX, true, true><<<nb, BLOCK, 0, st>>>(D, fp, prim, grad, lim, flux);
+        else
+            k_face_flux<DIM, FLUX, true, false><<<nb, BLOCK, 0, st>>>(D, fp, prim, grad, lim, flux);
+    } else {
+        if (premul)
+            k_face_flux<DIM, FLUX, false, true><<<nb, BLOCK, 0, st>>>(D, fp, prim, grad, lim, flux);
+        else
+            k_face_flux<DIM, FLUX, false, false><<<nb, BLOCK, 0, st>>>(D, fp, prim, grad, lim, flux);
+    }
+}
+
+template <int DIM>
+void launch_face_flux(const DevMesh& D, int flux_kind, const FaceParams& fp, bool recon,
+                      bool premul, const double* prim, const double* grad, const double* lim,
+                      double* flux, hipStream_t st) {
+    switch (flux_kind) {
+        case IBIS_FLUX_HANEL:
+            launch_face_flux_f<DIM, IBIS_FLUX_HANEL>(D, fp, recon, premul, prim, grad, lim, flux, st);
+            break;
+        case IBIS_FLUX_AUSMDV:
+            launch_face_flux_f<DIM, IBIS_FLUX_AUSMDV>(D, fp, recon, premul, prim, grad, lim, flux, st);
+            break;
+        case IBIS_FLUX_LDFSS2:
+            launch_face_flux_f<DIM, IBIS_FLUX_LDFSS2>(D, fp, recon, premul, prim, grad, lim, flux, st);
+            break;
+        default:
+            launch_face_flux_f<DIM, IBIS_FLUX_RUSANOV>(D, fp, recon, premul, prim, grad, lim, flux, st);
+            break;
+    }
+}
+
+// thermo reconstruction pair: (slot, prim row) of the two variables,
+// FV/convective_flux.cpp:181-209
+void thermo_pair(int thermo, int* sA, int* rA, int* sB, int* rB) {
+    switch (thermo) {
+        case IBIS_THERMO_RHO_P: *sA = G_P; *rA = P_P; *sB = G_RHO; *rB = P_RHO; break;
+        case IBIS_THERMO_RHO_T: *sA = G_RHO; *rA = P_RHO; *sB = G_T; *rB = P_T; break;
+        case IBIS_THERMO_RHO_U: *sA = G_RHO; *rA = P_RHO; *sB = G_U; *rB = P_E; break;
+        default: *sA = G_P; *rA = P_P; *sB = G_T; *rB = P_T; break;
+    }
+}
+
+bool has_kind(const int32_t* kinds, int ntags, int kind) {
+    for (int q = 0; q < ntags * IBIS_MAX_ACT; q++)
+        if (kinds[q] == kind) return true;
+    return false;
+}
+
+bool has_any(const int32_t* kinds, int ntags) {
+    for (int q = 0; q < ntags * IBIS_MAX_ACT; q++)
+        if (kinds[q] != IBIS_BC_NONE) return true;
+    return false;
+}
+
+}  // namespace
+
+ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
+                              int allow_reconstruction, ibis_stream_t stream) {
+    if (!r || !prim || !dudt) return ibis::fail(IBIS_ERR_INVALID, "ibis_compute_dudt: null argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const DevMesh& D = r->m->d;
+    const ibis_fv_desc& d = r->desc;
+    const size_t ntab = IBIS_MAX_TAGS * IBIS_MAX_ACT;
+    const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
+    const bool recon = allow_reconstruction && d.order == 2;
+    // 1. pre-reconstruction ghost fill
+    if (D.nb > 0 && has_any(d.pre_kind, d.ntags))
+        k_bc<<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, prim);
+    // 2. gradients and limiters
+    FaceParams fp;
+    fp.thermo = d.thermo;
+    fp.limited = d.limiter != IBIS_LIM_UNLIMITED;
+    thermo_pair(d.thermo, &fp.slotA, &fp.rowA, &fp.slotB, &fp.rowB);
+    fp.R = d.R;
+    fp.Cv = d.Cv;
+    fp.gamma = d.gamma;
+    fp.delta = d.ldfss_delta;
+    if (recon) {
+        VarSet vs;
+        vs.n = 5;
+        vs.ngrad = D.dim == 3 ? 5 : 4;
+        vs.row[0] = fp.rowA;
+        vs.slot[0] = fp.slotA;
+        vs.row[1] = fp.rowB;
+        vs.slot[1] = fp.slotB;
+        vs.row[2] = P_VX;
+        vs.slot[2] = G_VX;
+        vs.row[3] = P_VY;
+        vs.slot[3] = G_VY;
+        vs.row[4] = P_VZ;
+        vs.slot[4] = G_VZ;
+        if (D.dim == 3)
+            launch_cell_grad<3>(D, vs, d.limiter, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, st);
+        else
+            launch_cell_grad<2>(D, vs, d.limiter, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, st);
+    }
+    // 3. convective face flux
+    if (D.dim == 3)
+        launch_face_flux<3>(D, d.flux, fp, recon, r->premul, prim, r->grad, r->lim, r->flux, st);
+    else
+        launch_face_flux<2>(D, d.flux, fp, recon, r->premul, prim, r->grad, r->lim, r->flux, st);
+    // 4. post-convective flux actions
+    if (D.nb > 0 && has_kind(d.post_kind, d.ntags, IBIS_BC_CONSTANT_FLUX))
+        k_post_flux<<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds + 2 * ntab,
+                                                     r->pars + 2 * ntab * IBIS_NPAR, gp,
+                                                     r->premul ? 1 : 0, r->flux);
+    // 5. viscous flux
+    if (d.viscous) {
+        if (D.nb > 0 && has_any(d.visc_kind, d.ntags))
+            k_bc<<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds + ntab, r->pars + ntab * IBIS_NPAR,
+                                                  gp, prim);
+        VarSet vs;
+        vs.n = 4;
+        vs.ngrad = 4;
+        vs.row[0] = P_T;
+        vs.slot[0] = G_T;
+        vs.row[1] = P_VX;
+        vs.slot[1] = G_VX;
+        vs.row[2] = P_VY;
+        vs.slot[2] = G_VY;
+        vs.row[3] = P_VZ;
+        vs.slot[3] = G_VZ;
+        vs.row[4] = P_VZ;
+        vs.slot[4] = G_VZ;
+        const unsigned nbc = nblocks(D.N);
+        ViscParams vp{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr};
+        if (D.dim == 3) {
+            k_cell_grad<3, 0, 4><<<nbc, BLOCK, 0, st>>>(D, vs, prim, r->grad, r->lim, 0.0, 0.0);
+            k_visc_flux<3><<<nblocks(D.F), BLOCK, 0, st>>>(D, vp, prim, r->grad, r->flux,
+                                                           r->face_rho, r->face_T);
+        } else {
+            k_cell_grad<2, 0, 4><<<nbc, BLOCK, 0, st>>>(D, vs, prim, r->grad, r->lim, 0.0, 0.0);
+            k_visc_flux<2><<<nblocks(D.F), BLOCK, 0, st>>>(D, vp, prim, r->grad, r->flux,
+                                                           r->face_rho, r->face_T);
+        }
+    }
+    // 6. surface integral
+    const unsigned nbc = nblocks(D.N);
+    if (D.dim == 3) {
+        if (r->premul)
+            k_integral<3, true><<<nbc, BLOCK, 0, st>>>(D, r->flux, dudt);
+        else
+            k_integral<3, false><<<nbc, BLOCK, 0, st>>>(D, r->flux, dudt);
+    } else {
+        if (r->premul)
+            k_integral<2, true><<<nbc, BLOCK, 0, st>>>(D, r->flux, dudt);
+        else
+            k_integral<2, false><<<nbc, BLOCK, 0, st>>>(D, r->flux, dudt);
+    }
+    HIP_TRY(hipGetLastError());
+    return IBIS_OK;
+}
+
+ibis_status ibis_estimate_dt(ibis_residual* r, const double* prim, double* dt_host,
+                             ibis_stream_t stream) {
+    if (!r || !prim || !dt_host) return ibis::fail(IBIS_ERR_INVALID, "ibis_estimate_dt: null argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const DevMesh& D = r->m->d;
+    const ibis_fv_desc& d = r->desc;
+    ViscParams vp{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr};
+    const unsigned nb = nblocks(D.N);
+    k_dt_partial<<<nb, BLOCK, 0, st>>>(D, prim, r->face_rho, r->face_T, d.viscous ? 1 : 0,
+                                       d.signal_factor, vp, d.gamma, r->partial);
+    k_min_final<<<1, BLOCK, 0, st>>>(r->partial, (int)nb, r->dt_dev);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(r->host_buf, r->dt_dev, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    *dt_host = r->host_buf[0];
+    return IBIS_OK;
+}
+
+ibis_status ibis_count_bad_cells(ibis_residual* r, const double* prim, uint64_t* n_host,
+                                 ibis_stream_t stream) {
+    if (!r || !prim || !n_host) return ibis::fail(IBIS_ERR_INVALID, "ibis_count_bad_cells: null argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const DevMesh& D = r->m->d;
+    k_zero_u64<<<1, 1, 0, st>>>(r->count_dev);
+    k_count_bad<<<nblocks(D.N), BLOCK, 0, st>>>(D.N, D.Nt, prim, r->count_dev);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(r->host_buf + 1, r->count_dev, sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    uint64_t v;
+    std::memcpy(&v, r->host_buf + 1, 8);
+    *n_host = v;
+    return IBIS_OK;
+}
+
+ibis_status ibis_cons_to_prim(ibis_residual* r, const double* cons, double* prim,
+                              ibis_stream_t stream) {
+    if (!r || !cons || !prim) return ibis::fail(IBIS_ERR_INVALID, "ibis_cons_to_prim: null argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const DevMesh& D = r->m->d;
+    if (D.dim == 3)
+        k_cons_to_prim<3><<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, r->desc.R, r->desc.Cv, cons, prim);
+    else
+        k_cons_to_prim<2><<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, r->desc.R, r->desc.Cv, cons, prim);
+    HIP_TRY(hipGetLastError());
+    return IBIS_OK;
+}
+
+ibis_status ibis_prim_to_cons(ibis_residual* r, const double* prim, double* cons,
+                              ibis_stream_t stream) {
+    if (!r || !cons || !prim) return ibis::fail(IBIS_ERR_INVALID, "ibis_prim_to_cons: null argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const DevMesh& D = r->m->d;
+    if (D.dim == 3)
+        k_prim_to_cons<3><<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, prim, cons);
+    else
+        k_prim_to_cons<2><<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, prim, cons);
+    HIP_TRY(hipGetLastError());
+    return IBIS_OK;
+}
+
+ibis_status ibis_apply_time_derivative(ibis_residual* r, const double* u0, double* u1,
+                                       const double* k, double dt, ibis_stream_t stream) {
+    if (!r || !u0 || !u1 || !k) return ibis::fail(IBIS_ERR_INVALID, "ibis_apply_time_derivative: null argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const DevMesh& D = r->m->d;
+    const long n = (long)(D.dim + 2) * D.Nt;
+    k_axpy<<<nblocks(n), BLOCK, 0, st>>>(n, u0, u1, k, dt);
+    HIP_TRY(hipGetLastError());
+    return IBIS_OK;
+}
+
+ibis_status ibis_rk_step(ibis_residual* r, int s, const double* a, const double* b, double cfl,
+                         double* cons, double* prim, double* dt_prev, double t, double max_time,
+                         double plot_remaining, double* dt_taken, ibis_stream_t stream) {
+    if (!r || !a || !b || !cons || !prim || !dt_prev || s < 1 || s > 8)
+        return ibis::fail(IBIS_ERR_INVALID, "ibis_rk_step: bad argument");
+    const DevMesh& D = r->m->d;
+    const size_t ncv = (size_t)(D.dim + 2) * D.Nt;
+    while ((int)r->k.size() < s) {
+        double* p = nullptr;
+        HIP_TRY(hipMalloc(&p, ncv * sizeof(double)));
+        HIP_TRY(hipMemset(p, 0, ncv * sizeof(double)));
+        r->k.push_back(p);
+    }
+    if (!r->k_tmp) {
+        HIP_TRY(hipMalloc(&r->k_tmp, ncv * sizeof(double)));
+        HIP_TRY(hipMemset(r->k_tmp, 0, ncv * sizeof(double)));
+        HIP_TRY(hipMalloc(&r->flow_tmp, (size_t)NPRIM * D.Nt * sizeof(double)));
+        HIP_TRY(hipMemset(r->flow_tmp, 0, (size_t)NPRIM * D.Nt * sizeof(double)));
+    }
+    ibis_status rc;
+    // SO/runge_kutta.cpp:143-147
+    if ((rc = ibis_compute_dudt(r, prim, r->k[0], 1, stream)) != IBIS_OK) return rc;
+    double stable_dt;
+    if ((rc = ibis_estimate_dt(r, prim, &stable_dt, stream)) != IBIS_OK) return rc;
+    // RungeKutta::estimate_dt, SO/runge_kutta.cpp:108-119 (Kokkos::min ordering)
+    auto kmin = [](double x, double y) { return (y < x) ? y : x; };
+    double dt = kmin(cfl * stable_dt, 1.5 * (*dt_prev));
+    dt = kmin(dt, max_time - t);
+    if (plot_remaining > 0.0) dt = kmin(dt, plot_remaining);
+    for (int i = 1; i < s; i++) {  // SO/runge_kutta.cpp:151-182
+        if ((rc = ibis_apply_time_derivative(r, cons, r->k_tmp, r->k[0], a[i * s + 0] * dt,
+                                             stream)) != IBIS_OK)
+            return rc;
+        for (int j = 1; j < i; j++) {
+            if (a[i * s + j] < 1e-14) continue;
+            if ((rc = ibis_apply_time_derivative(r, r->k_tmp, r->k_tmp, r->k[j], a[i * s + j] * dt,
+                                                 stream)) != IBIS_OK)
+                return rc;
+        }
+        if ((rc = ibis_cons_to_prim(r, r->k_tmp, r->flow_tmp, stream)) != IBIS_OK) return rc;
+        if ((rc = ibis_compute_dudt(r, r->flow_tmp, r->k[i], 1, stream)) != IBIS_OK) return rc;
+    }
+    for (int i = 0; i < s; i++)  // SO/runge_kutta.cpp:185-191
+        if ((rc = ibis_apply_time_derivative(r, cons, cons, r->k[i], b[i] * dt, stream)) != IBIS_OK)
+            return rc;
+    if ((rc = ibis_cons_to_prim(r, cons, prim, stream)) != IBIS_OK) return rc;
+    *dt_prev = dt;
+    if (dt_taken) *dt_taken = dt;
+    return IBIS_OK;
+}
+
+ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** limiters,
+                                  double** flux, int* flux_has_area) {
+    if (!r) return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_scratch: null residual");
+    if (grads) *grads = r->grad;
+    if (limiters) *limiters = r->lim;
+    if (flux) *flux = r->flux;
+    if (flux_has_area) *flux_has_area = r->premul ? 1 : 0;
+    return IBIS_OK;
+}

@@ -1,0 +1,206 @@
+/*
+ * ibis_gpu.h -- C ABI of the MI355X cell-residual library (libibis_gpu.so).
+ *
+ * The reference's boundary is the C++ class template FiniteVolume<T>
+ * (src/finite_volume/finite_volume/finite_volume.h:28-142) constructed from a
+ * GridBlock<T> (src/grid/grid/grid.h:26) and a json config.  This header is
+ * the same operator surface as plain C: handles, pointers and sizes, no C++
+ * types and no exceptions across it.  Every entry point returns an
+ * ibis_status; ibis_last_error() gives the message of the last failure on the
+ * calling thread.
+ *
+ * Conventions
+ *   - FP64 arrays are structure-of-arrays with the row stride given below.
+ *   - prim  (device, caller-owned): 7 x n_total, rows rho, p, T, e, vx, vy, vz
+ *     (GasStates {rho,p,T,e} + velocity, src/gas/gas/gas_state.h:40-46).
+ *     Ghost rows [n_cells, n_total) are written by the boundary actions,
+ *     exactly as the reference mutates its FlowStates.
+ *   - cons / dudt (device, caller-owned): (dim+2) x n_total, rows mass,
+ *     mom_x, mom_y, [mom_z], energy (conserved_quantities.cpp:16-23); the
+ *     residual writes rows [0, n_cells).
+ *   - Cells and faces are in the numbering of the ibis_grid the mesh was
+ *     made from.  ibis_grid_build with reorder=1 renumbers cells and faces
+ *     for memory locality and reports the permutation.
+ *   - Calls taking a hipStream_t are asynchronous on that stream unless
+ *     documented as synchronising.  Handles are single-stream and not
+ *     thread-safe, like the reference's FiniteVolume.
+ */
+#ifndef IBIS_GPU_H
+#define IBIS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* ibis_stream_t; /* == hipStream_t */
+
+typedef enum {
+    IBIS_OK = 0,
+    IBIS_ERR_INVALID = 1, /* bad argument or configuration */
+    IBIS_ERR_HIP = 2,     /* a HIP runtime call failed */
+    IBIS_ERR_GRID = 3     /* grid construction failed */
+} ibis_status;
+
+#define IBIS_MAX_TAGS 16
+#define IBIS_MAX_ACT 4
+#define IBIS_NPAR 8
+
+/* --------------------------------------------------------------------------
+ * Host grid (replaces GridBlock construction, src/grid/grid/grid.h:89-167,
+ * and the SU2 ingest of src/grid/grid/grid_io.cpp:218-348).
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    int32_t dim;
+    int32_t ntags;            /* boundary tags, in the reference's sorted order */
+    int64_t nv;
+    const double* xyz;        /* nv x 3 (AoS) */
+    int64_t nc;
+    const int32_t* ctype;     /* SU2 element code per cell */
+    const int64_t* cvtx;      /* nc x 8, padded with -1 */
+    const int32_t* tag_ghost; /* per tag: 1 -> boundary has ghost cells */
+    int64_t nb;
+    const int32_t* btag;      /* boundary element -> tag */
+    const int32_t* btype;     /* SU2 element code of the boundary element */
+    const int64_t* bvtx;      /* nb x 4, padded with -1 */
+} ibis_grid_io;
+
+typedef struct ibis_grid ibis_grid;
+
+/* Build connectivity, geometry and LSQ factors exactly as the reference
+ * does (faces in first-appearance order, ghosts by sorted tag).  With
+ * reorder != 0 the result is renumbered (valid cells in Morton order of
+ * their centroids, ghosts after their valid neighbour, faces by owner). */
+ibis_status ibis_grid_build(const ibis_grid_io* io, int reorder, ibis_grid** out);
+void ibis_grid_destroy(ibis_grid* g);
+/* out[8] = dim, n_cells, n_ghost, n_faces, n_cell_faces, n_lsq, ntags, n_bfaces */
+ibis_status ibis_grid_sizes(const ibis_grid* g, int64_t* out);
+/* Permutations: new id of reference cell c (n_total) and of reference face f. */
+ibis_status ibis_grid_permutation(const ibis_grid* g, int64_t* cell_new_of_old,
+                                  int64_t* face_new_of_old);
+/* Geometry/connectivity in the grid's numbering (any pointer may be NULL):
+ * face_geom 13 x F (n, t1, t2, centre, area), face_lr 2 x F, cell_off N+1,
+ * cell_faces, outsigns, neighbours (n_cell_faces), centroid 3 x n_total,
+ * volume n_total, lsq n_lsq x N, tag_off ntags+1, bfaces, ghosts. */
+ibis_status ibis_grid_export(const ibis_grid* g, double* face_geom, int64_t* face_lr,
+                             int64_t* cell_off, int64_t* cell_faces, int32_t* outsigns,
+                             int64_t* neighbours, double* centroid, double* volume, double* lsq,
+                             int64_t* tag_off, int64_t* bfaces, int64_t* ghosts);
+
+/* --------------------------------------------------------------------------
+ * Device mesh: a GridBlock's arrays uploaded to HBM in the kernels' layout.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    int32_t dim, ntags;
+    int64_t n_cells, n_ghost, n_faces;
+    const int64_t* face_left;       /* F (GR/interface.h:335-336) */
+    const int64_t* face_right;      /* F */
+    const double* face_geom;        /* 13 x F: n(3) t1(3) t2(3) centre(3) area */
+    const int64_t* cell_face_off;   /* n_cells + 1 (GR/cell.h:112-114) */
+    const int64_t* cell_faces;      /* cell -> faces, in the cell's face order */
+    const int32_t* cell_outsigns;
+    const int64_t* cell_neighbours; /* GR/cell.h:327 */
+    const double* centroid;         /* 3 x n_total */
+    const double* volume;           /* n_total */
+    const double* lsq;              /* (dim==2 ? 3 : 6) x n_cells (GR/gradient.h:56-61) */
+    const int64_t* tag_off;         /* ntags + 1 */
+    const int64_t* bfaces;          /* boundary faces of each tag */
+} ibis_mesh_desc;
+
+typedef struct ibis_mesh ibis_mesh;
+
+/* Upload a mesh described by host arrays (any GridBlock). */
+ibis_status ibis_mesh_create(const ibis_mesh_desc* desc, ibis_mesh** out);
+/* Convenience: upload an ibis_grid. */
+ibis_status ibis_mesh_from_grid(const ibis_grid* g, ibis_mesh** out);
+void ibis_mesh_destroy(ibis_mesh* m);
+
+/* --------------------------------------------------------------------------
+ * Residual operator (replaces FiniteVolume<T>,
+ * src/finite_volume/finite_volume/finite_volume.cpp:13-54).
+ * ------------------------------------------------------------------------ */
+enum { IBIS_FLUX_HANEL = 0, IBIS_FLUX_AUSMDV = 1, IBIS_FLUX_LDFSS2 = 2, IBIS_FLUX_RUSANOV = 3 };
+enum { IBIS_THERMO_RHO_P = 0, IBIS_THERMO_RHO_T = 1, IBIS_THERMO_RHO_U = 2, IBIS_THERMO_P_T = 3 };
+enum { IBIS_LIM_UNLIMITED = 0, IBIS_LIM_BARTH_JESPERSEN = 1, IBIS_LIM_VENKATAKRISHNAN = 2 };
+/* boundary actions (src/finite_volume/finite_volume/boundaries/boundary.cpp:449-492) */
+enum {
+    IBIS_BC_NONE = 0,
+    IBIS_BC_FLOW_STATE_COPY = 1, /* par: T, p, rho, e, vx, vy, vz */
+    IBIS_BC_INTERNAL_COPY = 2,
+    IBIS_BC_REFLECT_NORMAL = 3,
+    IBIS_BC_VEL_REFLECT = 4,
+    IBIS_BC_FIX_TEMPERATURE = 5, /* par: Twall */
+    IBIS_BC_SUBSONIC_INFLOW = 6, /* par: T, p, rho, e, vx, vy, vz */
+    IBIS_BC_SUBSONIC_OUTFLOW = 7,/* par: p */
+    IBIS_BC_CONSTANT_FLUX = 8    /* post-convective flux action; par as flow_state_copy */
+};
+enum { IBIS_PREC_FP64 = 0, IBIS_PREC_FP32_FLUX = 1 };
+
+typedef struct {
+    int32_t flux, order, thermo, limiter, viscous, ntags;
+    double ldfss_delta, lim_eps, venkat_K, signal_factor;
+    double R, Cv, Cp, gamma, mu0, T0, Ts, Pr;
+    int32_t pre_kind[IBIS_MAX_TAGS * IBIS_MAX_ACT];
+    double pre_par[IBIS_MAX_TAGS * IBIS_MAX_ACT * IBIS_NPAR];
+    int32_t visc_kind[IBIS_MAX_TAGS * IBIS_MAX_ACT];
+    double visc_par[IBIS_MAX_TAGS * IBIS_MAX_ACT * IBIS_NPAR];
+    int32_t post_kind[IBIS_MAX_TAGS * IBIS_MAX_ACT];
+    double post_par[IBIS_MAX_TAGS * IBIS_MAX_ACT * IBIS_NPAR];
+} ibis_fv_desc;
+
+typedef struct ibis_residual ibis_residual;
+
+ibis_status ibis_residual_create(const ibis_mesh* m, const ibis_fv_desc* desc,
+                                 ibis_residual** out);
+void ibis_residual_destroy(ibis_residual* r);
+
+/* FiniteVolume<T>::compute_dudt (finite_volume.cpp:57-96, static grid).
+ * Writes ghost rows of prim, rows [0, n_cells) of dudt.  Asynchronous. */
+ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
+                              int allow_reconstruction, ibis_stream_t stream);
+
+/* FiniteVolume<T>::estimate_dt (finite_volume.cpp:131-184).  Synchronises
+ * the stream and writes the global minimum to *dt_host. */
+ibis_status ibis_estimate_dt(ibis_residual* r, const double* prim, double* dt_host,
+                             ibis_stream_t stream);
+
+/* FiniteVolume<T>::count_bad_cells (finite_volume.cpp:258-270).  Synchronises. */
+ibis_status ibis_count_bad_cells(ibis_residual* r, const double* prim, uint64_t* n_host,
+                                 ibis_stream_t stream);
+
+/* conserved_to_primatives / primatives_to_conserved
+ * (primative_conserved_conversion.cpp:5-51), all n_total rows. */
+ibis_status ibis_cons_to_prim(ibis_residual* r, const double* cons, double* prim,
+                              ibis_stream_t stream);
+ibis_status ibis_prim_to_cons(ibis_residual* r, const double* prim, double* cons,
+                              ibis_stream_t stream);
+
+/* u1 = u0 + k * dt over all (dim+2) x n_total entries
+ * (apply_time_derivative, conserved_quantities.cpp:80-94); u1 may alias u0. */
+ibis_status ibis_apply_time_derivative(ibis_residual* r, const double* u0, double* u1,
+                                       const double* k, double dt, ibis_stream_t stream);
+
+/* One explicit Runge-Kutta step (RungeKutta::take_step + estimate_dt,
+ * src/solvers/solvers/runge_kutta.cpp:108-203).  a is nstages x nstages
+ * (a[i*nstages+j] = stage-i weight of k_j), b has nstages entries.  cons and
+ * prim are the device solution; *dt_prev is read and updated.  Stage
+ * buffers are library-owned.  Synchronises once (the dt reduction). */
+ibis_status ibis_rk_step(ibis_residual* r, int nstages, const double* a, const double* b,
+                         double cfl, double* cons, double* prim, double* dt_prev, double t,
+                         double max_time, double plot_remaining, double* dt_taken,
+                         ibis_stream_t stream);
+
+/* Device pointers of the library's intermediates (for inspection and
+ * tests): gradients 7 x 3 x n_cells (slots p, rho, T, u, vx, vy, vz),
+ * limiters 7 x n_cells, face flux (dim+2) x n_faces.  The face flux is
+ * multiplied by the face area when the configuration is inviscid. */
+ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** limiters,
+                                  double** flux, int* flux_has_area);
+
+const char* ibis_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

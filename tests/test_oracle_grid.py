@@ -1,0 +1,112 @@
+"""Pin the oracle's grid builder to the reference's own known answers.
+
+Expected values are the ones the reference's doctest cases assert on its
+fixture grids (copied unchanged to tests/golden/grid.su2 and cube.su2):
+  src/grid/grid/grid.cpp:145-319, src/grid/grid/interface.cpp:143-200,
+  src/grid/grid/cell.cpp:68-99, src/grid/grid/gradient.cpp:29-82.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from ibis_amd.gridio import read_su2
+from oracle import OracleGrid
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+TAGS_2D = sorted(["slip_wall_bottom", "slip_wall_top", "inflow", "outflow"])
+TAGS_3D = sorted(["bottom", "top", "west", "east", "north", "south"])
+
+
+@pytest.fixture(scope="module")
+def g2():
+    return OracleGrid(read_su2(os.path.join(GOLD, "grid.su2")), TAGS_2D, [1] * 4)
+
+
+@pytest.fixture(scope="module")
+def g3():
+    return OracleGrid(read_su2(os.path.join(GOLD, "cube.su2")), TAGS_3D, [1] * 6)
+
+
+def test_interfaces_vertices_and_cell_faces(g2):
+    # grid.cpp:197-215 "grid cell faces 2"
+    face_ids = [[0, 1, 2, 3], [4, 5, 6, 1], [7, 8, 9, 5], [2, 10, 11, 12], [6, 13, 14, 10],
+                [9, 15, 16, 13], [11, 17, 18, 19], [14, 20, 21, 17], [16, 22, 23, 20]]
+    assert g2.N == 9 and g2.F == 24 and g2.G == 12
+    for c in range(9):
+        got = g2.cell_faces[g2.cell_off[c]:g2.cell_off[c + 1]]
+        assert list(got) == face_ids[c]
+
+
+def test_outsigns(g2):
+    # grid.cpp:217-232
+    outsigns = [[1, 1, 1, 1], [1, 1, 1, -1], [1, 1, 1, -1], [-1, 1, 1, 1], [-1, 1, 1, -1],
+                [-1, 1, 1, -1], [-1, 1, 1, 1], [-1, 1, 1, -1], [-1, 1, 1, -1]]
+    for c in range(9):
+        assert list(g2.outsigns[g2.cell_off[c]:g2.cell_off[c + 1]]) == outsigns[c]
+
+
+def test_neighbours(g2):
+    # grid.cpp:234-244
+    nb = lambda c, j: g2.neighbours[g2.cell_off[c] + j]  # noqa: E731
+    assert nb(0, 1) == 1 and nb(0, 2) == 3 and nb(4, 0) == 1 and nb(8, 3) == 7
+
+
+def test_ghost_centres(g2):
+    # grid.cpp:246-319
+    expect = {"inflow": [(-0.5, 0.5), (-0.5, 1.5), (-0.5, 2.5)],
+              "slip_wall_bottom": [(0.5, -0.5), (1.5, -0.5), (2.5, -0.5)],
+              "outflow": [(3.5, 0.5), (3.5, 1.5), (3.5, 2.5)],
+              "slip_wall_top": [(0.5, 3.5), (1.5, 3.5), (2.5, 3.5)]}
+    for tag, pts in expect.items():
+        gh = g2.ghost_cells(tag)
+        for k, (x, y) in enumerate(pts):
+            assert g2.centroid[0, gh[k]] == x
+            assert g2.centroid[1, gh[k]] == y
+            assert g2.centroid[2, gh[k]] == 0.0
+    # ghost ids follow the sorted tag order (grid.h:414-433)
+    assert list(g2.ghost_cells("inflow")) == [9, 10, 11]
+    assert list(g2.ghost_cells("slip_wall_top")) == [18, 19, 20]
+
+
+def test_face_geometry(g2):
+    # interface.cpp:143-200 (same 4x4 vertex grid and face list)
+    assert np.all(np.abs(g2.area() - 1.0) < 1e-14)
+    n = g2.norm()
+    expect_n = [(0, -1), (1, 0), (0, 1), (-1, 0), (0, -1), (1, 0)]
+    for f, (x, y) in enumerate(expect_n):
+        assert abs(n[0, f] - x) < 1e-14 and abs(n[1, f] - y) < 1e-14 and abs(n[2, f]) < 1e-14
+    xs = [0.5, 1.0, 0.5, 0.0, 1.5, 2.0, 1.5, 2.5, 3.0, 2.5, 1.0, 0.5, 0.0, 2.0, 1.5, 3.0,
+          2.5, 1.0, 0.5, 0.0, 2.0, 1.5, 3.0, 2.5]
+    ys = [0.0, 0.5, 1.0, 0.5, 0.0, 0.5, 1.0, 0.0, 0.5, 1.0, 1.5, 2.0, 1.5, 1.5, 2.0, 1.5,
+          2.0, 2.5, 3.0, 2.5, 2.5, 3.0, 2.5, 3.0]
+    c = g2.face_centre()
+    assert np.all(np.abs(c[0] - xs) < 1e-14) and np.all(np.abs(c[1] - ys) < 1e-14)
+
+
+def test_cell_volumes_and_centroids(g2):
+    # cell.cpp:68-99
+    assert np.all(np.abs(g2.volume[:9] - 1.0) < 1e-14)
+    assert np.allclose(g2.centroid[0, :9], [0.5, 1.5, 2.5] * 3, atol=1e-14, rtol=0)
+    assert np.allclose(g2.centroid[1, :9], [0.5] * 3 + [1.5] * 3 + [2.5] * 3, atol=1e-14, rtol=0)
+
+
+def test_3d_volumes_and_unit_vectors(g3):
+    # grid.cpp:145-182
+    assert g3.N == 27
+    assert np.allclose(g3.volume[:27], 1.0 / 27, rtol=1e-12, atol=0)
+    for vec in (g3.norm(), g3.tan1(), g3.tan2()):
+        assert np.allclose(np.sqrt((vec ** 2).sum(axis=0)), 1.0, atol=1e-14, rtol=0)
+
+
+def test_lsq_gradient_of_linear_field(g2):
+    # gradient.cpp:29-82: values linear in (x, y), ghosts set linearly
+    vals = np.zeros(21)
+    vals[:9] = [1.0, 2.0, 3.0, 1.5, 2.5, 3.5, 2.0, 3.0, 4.0]
+    vals[g2.ghost_cells("slip_wall_bottom")] = [0.5, 1.5, 2.5]
+    vals[g2.ghost_cells("slip_wall_top")] = [2.5, 3.5, 4.5]
+    vals[g2.ghost_cells("inflow")] = [0.0, 0.5, 1.0]
+    vals[g2.ghost_cells("outflow")] = [4.0, 4.5, 5.0]
+    grad = g2.gradient(vals)
+    assert np.allclose(grad[0], 1.0, rtol=1e-12, atol=0)
+    assert np.allclose(grad[1], 0.5, rtol=1e-12, atol=0)
