@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Residual throughput of the MI355X cell residual (dU/dt).
+
+Workload (default): 3D hex box 160^3 (4,096,000 cells), interior nodes
+jittered by +-10% of the spacing (seed 3), cells in a seeded random order
+(seed 4), smooth seeded Mach 0.3-3 state (seed 5), Hanel flux, 2nd-order
+MUSCL with the Venkatakrishnan limiter (K=5), rho_u thermo interpolation,
+FP64, all six faces supersonic outflow.
+
+One "step" is one full residual evaluation (ghost fill, gradients and
+limiters, face fluxes, face-to-cell integral) with the state already in HBM.
+Prints one JSON line:
+  value       -- Mcells/s of the residual (N / t_residual)
+  roofline    -- HBM roofline of the residual: algorithmic bytes per residual
+                 (compulsory arrays counted once) / measured residual time
+  cpu_baseline-- the CPU oracle (OpenMP) on the same grid and state, timed
+                 on this host for a bounded number of residuals
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(N, F, Nt, dim, ncons, faces_per_cell):
+    """Compulsory bytes per residual: state in (ghosts included), dU/dt out,
+    centroid + volume + LSQ factors, face normal/area/centre/L-R pair,
+    cell->face lists + offsets (FP64 values, int32 indices)."""
+    n_lsq = 3 if dim == 2 else 6
+    return (Nt * 8 * ncons + N * 8 * ncons + N * 8 * (dim + 1 + n_lsq)
+            + F * (8 * dim + 8 + 8 * dim + 8) + N * (4 * faces_per_cell + 4))
+
+
+def read_traffic(workload):
+    """HBM bytes per residual measured with rocprofv3 --pmc, if committed."""
+    path = os.path.join(ROOT, "profiles", f"residual_{workload}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_residual:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def build_workload(name):
+    from ibis_amd import cases
+    if name == "hex160":
+        g_io, cfg = cases.hex_case(160)
+        prim = cases.hex_prim(g_io, 160 ** 3 + 6 * 160 ** 2)
+        return g_io, cfg, prim, 6
+    if name == "plate":
+        g_io, cfg, fs = cases.plate_case(1024, 512)
+        prim = cases.perturbed_prim(g_io, fs, 1024 * 512 + 2 * (1024 + 512), seed=2)
+        return g_io, cfg, prim, 4
+    if name == "wedge":
+        g_io, cfg, fs = cases.wedge_case(192, 96)
+        prim = cases.perturbed_prim(g_io, fs, 192 * 96 + 2 * (192 + 96), seed=14)
+        return g_io, cfg, prim, 4
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(g_io, cfg, prim, n_res, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import OracleFV, OracleGrid, set_threads
+    set_threads(threads)
+    enc = cfg.encode()
+    og = OracleGrid(g_io, enc["tags"], enc["ghost"])
+    fv = OracleFV(og, cfg)
+    p = prim.copy()
+    fv.compute_dudt(p)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(n_res):
+        fv.compute_dudt(p)
+    dt = (time.perf_counter() - t0) / n_res
+    return og.N / dt / 1e6, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="hex160", choices=["hex160", "plate", "wedge"])
+    ap.add_argument("--cpu-residuals", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the residual of one block runs on one MI355X "
+              f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from ibis_amd.finite_volume import FiniteVolume, GridBlock
+
+    t_setup = time.perf_counter()
+    g_io, cfg, prim_ref, fpc = build_workload(args.workload)
+    print(f"[bench] workload {args.workload}: grid generated", file=sys.stderr, flush=True)
+    gb = GridBlock(g_io, cfg, reorder=True)
+    fv = FiniteVolume(gb, cfg)
+    print(f"[bench] grid built in {time.perf_counter() - t_setup:.1f}s: N={gb.N} F={gb.F} "
+          f"G={gb.G}", file=sys.stderr, flush=True)
+    prim = torch.from_numpy(gb.to_grid_order(prim_ref)).to("cuda")
+    dudt = torch.zeros((gb.dim + 2, gb.Nt), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        fv.compute_dudt(prim, gb, dudt)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        fv.compute_dudt(prim, gb, dudt)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = ev0.elapsed_time(ev1)
+    ms_per_step = ms_total / args.steps
+    print(f"[bench] timed {args.steps} residuals: {ms_per_step * 1e3:.1f} us/residual "
+          f"(wall {wall * 1e3 / args.steps:.1f} us)", file=sys.stderr, flush=True)
+
+    # per-kernel durations, HIP events on the launch stream
+    fv.profile(True)
+    for _ in range(args.steps):
+        fv.compute_dudt(prim, gb, dudt)
+    torch.cuda.synchronize()
+    kt = fv.kernel_times(reset=True)
+    fv.profile(False)
+    kernels_us = {k: 1e3 * v[0] / v[1] for k, v in kt.items()}
+    residual_kernel_us = sum(kernels_us.values())
+    dominant = max(kernels_us, key=kernels_us.get)
+
+    N, F, Nt = gb.N, gb.F, gb.Nt
+    ncons = gb.dim + 2
+    B = algorithmic_bytes(N, F, Nt, gb.dim, ncons, fpc)
+    t_res = ms_per_step * 1e-3
+    achieved = B / t_res / 1e9
+    traffic = read_traffic(args.workload)
+    mcells = N / t_res / 1e6
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        print(f"[bench] CPU oracle on {threads} threads ...", file=sys.stderr, flush=True)
+        v, dt = cpu_baseline(g_io, cfg, prim_ref, args.cpu_residuals, threads)
+        cpu = {"value": round(v, 3), "unit": "Mcells/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_residuals} residuals of the same {args.workload} grid and "
+                         f"state (after 1 warm-up), {dt * 1e3:.0f} ms each, oracle/ibis_oracle.cpp "
+                         f"built -O3 with OpenMP, reference cell order"}
+
+    out = {
+        "metric": "residual Mcells/s (FP64, 2nd-order residual, % HBM peak in roofline)",
+        "value": round(mcells, 2),
+        "unit": "Mcells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": {"hex160": "hex box 160^3 (4.1M cells) jittered +-10%, cell order "
+                                          "permuted, Hanel, 2nd order, Venkatakrishnan, rho_u",
+                                "plate": "flat plate 1024x512 viscous, AUSMDV, Barth-Jespersen",
+                                "wedge": "wedge 192x96 inviscid Mach 3, AUSMDV, Barth-Jespersen"
+                                }[args.workload],
+                   "cells": N, "faces": F, "ghosts": Nt - N},
+        "roofline": {"bound": "hbm", "kernel": "residual (all kernels of one compute_dudt)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "algorithmic_bytes": B,
+                     "dominant_kernel": dominant,
+                     "kernels_us": {k: round(v, 2) for k, v in kernels_us.items()},
+                     "sum_kernels_us": round(residual_kernel_us, 2)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -268,8 +268,8 @@ class FVConfig:
                 if len(acts) > MAX_ACT:
                     raise ValueError(f"at most {MAX_ACT} actions per boundary list")
                 for a, act in enumerate(acts):
-                    kind[t * MAX_ACT + a] = BC_KIND[act["type"]]
                     pp = _action_params(act)
+                    kind[t * MAX_ACT + a] = BC_KIND[act["type"]]
                     par[(t * MAX_ACT + a) * NPAR:(t * MAX_ACT + a) * NPAR + len(pp)] = pp
             tabs[name] = (kind, par)
         ghost = np.array([1 if self.boundaries[t].ghost_cells else 0 for t in tags], np.int32)

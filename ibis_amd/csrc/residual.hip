@@ -900,6 +900,16 @@ struct ibis_residual {
     double* dt_dev = nullptr;
     unsigned long long* count_dev = nullptr;
     double* host_buf = nullptr;  // pinned: [0] dt, [1] count
+    // per-kernel event timing
+    bool profiling = false;
+    struct Ev {
+        int slot;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> pending;
+    std::vector<hipEvent_t> pool;
+    double acc_ms[IBIS_NKERNELS] = {0};
+    int64_t acc_n[IBIS_NKERNELS] = {0};
     // RK stage storage
     std::vector<double*> k;
     double* k_tmp = nullptr;
@@ -913,6 +923,11 @@ void ibis_residual_destroy(ibis_residual* r) {
                     (void*)r->dt_dev, (void*)r->count_dev, (void*)r->k_tmp, (void*)r->flow_tmp})
         if (p) (void)hipFree(p);
     for (double* p : r->k) (void)hipFree(p);
+    for (auto& e : r->pending) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    for (hipEvent_t e : r->pool) (void)hipEventDestroy(e);
     if (r->host_buf) (void)hipHostFree(r->host_buf);
     delete r;
 }
@@ -1065,6 +1080,17 @@ bool has_any(const int32_t* kinds, int ntags) {
 
 }  // namespace
 
+static hipEvent_t take_event(ibis_residual* r) {
+    if (!r->pool.empty()) {
+        hipEvent_t e = r->pool.back();
+        r->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
 ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
                               int allow_reconstruction, ibis_stream_t stream) {
     if (!r || !prim || !dudt) return ibis::fail(IBIS_ERR_INVALID, "ibis_compute_dudt: null argument");
@@ -1074,9 +1100,24 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
     const size_t ntab = IBIS_MAX_TAGS * IBIS_MAX_ACT;
     const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
     const bool recon = allow_reconstruction && d.order == 2;
+    hipEvent_t ev_a = nullptr;
+    auto beg = [&]() {
+        if (!r->profiling) return;
+        ev_a = take_event(r);
+        (void)hipEventRecord(ev_a, st);
+    };
+    auto end = [&](int slot) {
+        if (!r->profiling) return;
+        hipEvent_t b = take_event(r);
+        (void)hipEventRecord(b, st);
+        r->pending.push_back({slot, ev_a, b});
+    };
     // 1. pre-reconstruction ghost fill
-    if (D.nb > 0 && has_any(d.pre_kind, d.ntags))
+    if (D.nb > 0 && has_any(d.pre_kind, d.ntags)) {
+        beg();
         k_bc<<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, prim);
+        end(0);
+    }
     // 2. gradients and limiters
     FaceParams fp;
     fp.thermo = d.thermo;
@@ -1100,26 +1141,36 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
         vs.slot[3] = G_VY;
         vs.row[4] = P_VZ;
         vs.slot[4] = G_VZ;
+        beg();
         if (D.dim == 3)
             launch_cell_grad<3>(D, vs, d.limiter, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, st);
         else
             launch_cell_grad<2>(D, vs, d.limiter, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, st);
+        end(1);
     }
     // 3. convective face flux
+    beg();
     if (D.dim == 3)
         launch_face_flux<3>(D, d.flux, fp, recon, r->premul, prim, r->grad, r->lim, r->flux, st);
     else
         launch_face_flux<2>(D, d.flux, fp, recon, r->premul, prim, r->grad, r->lim, r->flux, st);
+    end(2);
     // 4. post-convective flux actions
-    if (D.nb > 0 && has_kind(d.post_kind, d.ntags, IBIS_BC_CONSTANT_FLUX))
+    if (D.nb > 0 && has_kind(d.post_kind, d.ntags, IBIS_BC_CONSTANT_FLUX)) {
+        beg();
         k_post_flux<<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds + 2 * ntab,
                                                      r->pars + 2 * ntab * IBIS_NPAR, gp,
                                                      r->premul ? 1 : 0, r->flux);
+        end(3);
+    }
     // 5. viscous flux
     if (d.viscous) {
-        if (D.nb > 0 && has_any(d.visc_kind, d.ntags))
+        if (D.nb > 0 && has_any(d.visc_kind, d.ntags)) {
+            beg();
             k_bc<<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds + ntab, r->pars + ntab * IBIS_NPAR,
                                                   gp, prim);
+            end(4);
+        }
         VarSet vs;
         vs.n = 4;
         vs.ngrad = 4;
@@ -1135,18 +1186,24 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
         vs.slot[4] = G_VZ;
         const unsigned nbc = nblocks(D.N);
         ViscParams vp{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr};
-        if (D.dim == 3) {
+        beg();
+        if (D.dim == 3)
             k_cell_grad<3, 0, 4><<<nbc, BLOCK, 0, st>>>(D, vs, prim, r->grad, r->lim, 0.0, 0.0);
+        else
+            k_cell_grad<2, 0, 4><<<nbc, BLOCK, 0, st>>>(D, vs, prim, r->grad, r->lim, 0.0, 0.0);
+        end(5);
+        beg();
+        if (D.dim == 3)
             k_visc_flux<3><<<nblocks(D.F), BLOCK, 0, st>>>(D, vp, prim, r->grad, r->flux,
                                                            r->face_rho, r->face_T);
-        } else {
-            k_cell_grad<2, 0, 4><<<nbc, BLOCK, 0, st>>>(D, vs, prim, r->grad, r->lim, 0.0, 0.0);
+        else
             k_visc_flux<2><<<nblocks(D.F), BLOCK, 0, st>>>(D, vp, prim, r->grad, r->flux,
                                                            r->face_rho, r->face_T);
-        }
+        end(6);
     }
     // 6. surface integral
     const unsigned nbc = nblocks(D.N);
+    beg();
     if (D.dim == 3) {
         if (r->premul)
             k_integral<3, true><<<nbc, BLOCK, 0, st>>>(D, r->flux, dudt);
@@ -1158,7 +1215,38 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
         else
             k_integral<2, false><<<nbc, BLOCK, 0, st>>>(D, r->flux, dudt);
     }
+    end(7);
     HIP_TRY(hipGetLastError());
+    return IBIS_OK;
+}
+
+ibis_status ibis_residual_profile(ibis_residual* r, int enable) {
+    if (!r) return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_profile: null residual");
+    r->profiling = enable != 0;
+    return IBIS_OK;
+}
+
+ibis_status ibis_residual_kernel_times(ibis_residual* r, double* total_ms, int64_t* counts,
+                                       int reset) {
+    if (!r) return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_kernel_times: null residual");
+    for (auto& e : r->pending) {
+        HIP_TRY(hipEventSynchronize(e.b));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, e.a, e.b));
+        r->acc_ms[e.slot] += ms;
+        r->acc_n[e.slot] += 1;
+        r->pool.push_back(e.a);
+        r->pool.push_back(e.b);
+    }
+    r->pending.clear();
+    for (int q = 0; q < IBIS_NKERNELS; q++) {
+        if (total_ms) total_ms[q] = r->acc_ms[q];
+        if (counts) counts[q] = r->acc_n[q];
+        if (reset) {
+            r->acc_ms[q] = 0.0;
+            r->acc_n[q] = 0;
+        }
+    }
     return IBIS_OK;
 }
 

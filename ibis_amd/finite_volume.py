@@ -250,6 +250,19 @@ class FiniteVolume:
                                           C.byref(pa)), "ibis_residual_scratch")
         return g.value, lm.value, fx.value, bool(pa.value)
 
+    def profile(self, enable: bool = True):
+        """Record HIP events around every kernel launch of compute_dudt."""
+        check(lib().ibis_residual_profile(self._h, 1 if enable else 0), "ibis_residual_profile")
+
+    def kernel_times(self, reset: bool = True) -> dict:
+        """{kernel name: (total ms, launches)} of the profiled launches."""
+        from .native import KERNEL_NAMES, NKERNELS
+        ms = np.zeros(NKERNELS)
+        n = np.zeros(NKERNELS, np.int64)
+        check(lib().ibis_residual_kernel_times(self._h, ptr(ms), ptr(n), 1 if reset else 0),
+              "ibis_residual_kernel_times")
+        return {KERNEL_NAMES[q]: (float(ms[q]), int(n[q])) for q in range(NKERNELS) if n[q] > 0}
+
     def close(self):
         if getattr(self, "_h", None):
             lib().ibis_residual_destroy(self._h)

@@ -13,6 +13,9 @@ import numpy as np
 from . import build as _build
 
 MAX_TAGS, MAX_ACT, NPAR = 16, 4, 8
+NKERNELS = 8
+KERNEL_NAMES = ("k_bc", "k_cell_grad", "k_face_flux", "k_post_flux", "k_bc_visc",
+                "k_cell_grad_visc", "k_visc_flux", "k_integral")
 
 HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
                       "ibis_gpu.h")
@@ -77,6 +80,8 @@ _SIGS = {
                                C.POINTER(C.c_double), _P]),
     "ibis_residual_scratch": (C.c_int, [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                         C.POINTER(C.c_void_p), C.POINTER(C.c_int)]),
+    "ibis_residual_profile": (C.c_int, [_P, C.c_int]),
+    "ibis_residual_kernel_times": (C.c_int, [_P, _P, _P, C.c_int]),
     "ibis_last_error": (C.c_char_p, []),
 }
 

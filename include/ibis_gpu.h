@@ -198,6 +198,19 @@ ibis_status ibis_rk_step(ibis_residual* r, int nstages, const double* a, const d
 ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** limiters,
                                   double** flux, int* flux_has_area);
 
+/* Per-kernel timing with HIP events recorded around every launch of
+ * ibis_compute_dudt on its stream.  Kernel slots:
+ *   0 k_bc (pre-reconstruction)  1 k_cell_grad  2 k_face_flux  3 k_post_flux
+ *   4 k_bc (pre-viscous)         5 k_cell_grad (viscous)  6 k_visc_flux
+ *   7 k_integral
+ * ibis_residual_kernel_times synchronises, adds the elapsed times of all
+ * completed launches into total_ms[slot] / counts[slot] (IBIS_NKERNELS
+ * entries each) and, with reset != 0, clears the accumulators. */
+#define IBIS_NKERNELS 8
+ibis_status ibis_residual_profile(ibis_residual* r, int enable);
+ibis_status ibis_residual_kernel_times(ibis_residual* r, double* total_ms, int64_t* counts,
+                                       int reset);
+
 const char* ibis_last_error(void);
 
 #ifdef __cplusplus
