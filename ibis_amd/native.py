@@ -100,6 +100,11 @@ def lib():
         if not os.path.exists(path):
             raise IbisError(f"{path} is missing: run ibis_amd.build.build() (or "
                             "__graft_entry__.build()) first")
+        # torch ships its own HIP runtime.  Import it first so that this
+        # library's libamdhip64.so.7 dependency binds to the runtime already
+        # in the process; loading a second HIP runtime leaves torch without
+        # a device.
+        import torch  # noqa: F401
         L = C.CDLL(path)
         for name, (res, args) in _SIGS.items():
             fn = getattr(L, name)
