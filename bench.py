@@ -128,7 +128,7 @@ def main():
     ms_total = ev0.elapsed_time(ev1)
     ms_per_step = ms_total / args.steps
     print(f"[bench] timed {args.steps} residuals: {ms_per_step * 1e3:.1f} us/residual "
-          f"(wall {wall * 1e3 / args.steps:.1f} us)", file=sys.stderr, flush=True)
+          f"(wall {wall * 1e6 / args.steps:.1f} us)", file=sys.stderr, flush=True)
 
     # per-kernel durations, HIP events on the launch stream
     fv.profile(True)
