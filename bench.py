@@ -93,7 +93,21 @@ def main():
     ap.add_argument("--workload", default="hex160", choices=["hex160", "plate", "wedge"])
     ap.add_argument("--cpu-residuals", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (5 times), to calibrate the "
+                         "FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_mib": args.copy_calibration, "copies": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the residual of one block runs on one MI355X "
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)

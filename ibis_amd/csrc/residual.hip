@@ -78,7 +78,25 @@ struct DevMesh {
     const int* b_ghost;
     const int* b_valid;
     const int* b_tag;
+    // tiles of TILE consecutive cells and the faces touching them
+    int ntiles, maxtf;
+    const int* tile_face_off;         // ntiles + 1
+    const int* tile_faces;            // face ids, ascending within a tile
+    const unsigned short* cell_lds;   // maxf x N: position of a cell's face in its tile list
 };
+
+constexpr int TILE = 256;
+constexpr int NXCD = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin across them
+
+// Logical block for XCD locality: hardware block b runs on XCD b % 8, so
+// handing XCD x the logical blocks [x*per, (x+1)*per) keeps neighbouring
+// cells and faces in the same XCD's L2.  Launch grids are padded to 8*per
+// blocks; a returned value >= nlogical means "no work".
+__device__ __forceinline__ int xcd_block(int nlogical) {
+    const int per = (nlogical + NXCD - 1) / NXCD;
+    const int b = blockIdx.x;
+    return (b % NXCD) * per + b / NXCD;
+}
 
 __device__ __forceinline__ double gget(const double* p, int stride, int row, int i) {
     return p[(size_t)row * stride + i];
@@ -201,14 +219,14 @@ __global__ void __launch_bounds__(BLOCK) k_bc(DevMesh m, const int* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Cell kernel: LSQ gradients (GR/gradient.h:66-116) of NV variables in one
-// pass over the stencil, then the limiter of each (FV/limiter.cpp:24-62, or
-// the Venkatakrishnan limiter).
-//   vrow[v]  : prim row of variable v
-//   vslot[v] : gradient/limiter slot of variable v
-//   ngrad    : variables [0, ngrad) get their gradient computed here; the
-//              rest use the gradient already stored in their slot (2D vz)
-//   LIM      : 0 none, 1 Barth-Jespersen, 2 Venkatakrishnan
+// Cell kernel: LSQ gradients (GR/gradient.h:66-116) of the reconstruction
+// variables, then the limiter of each (FV/limiter.cpp:24-62, or the
+// Venkatakrishnan limiter).
+//   row[v]  : prim row of variable v
+//   slot[v] : gradient/limiter slot of variable v
+//   ngrad   : variables [0, ngrad) get their gradient computed here; the
+//             rest use the gradient already stored in their slot (2D vz)
+//   LIM     : 0 none, 1 Barth-Jespersen, 2 Venkatakrishnan
 // ---------------------------------------------------------------------------
 struct VarSet {
     int n, ngrad;
@@ -216,16 +234,26 @@ struct VarSet {
     int slot[5];
 };
 
-template <int DIM, int LIM, int NV>
-__global__ void __launch_bounds__(BLOCK)
-    k_cell_grad(DevMesh m, VarSet vs, const double* __restrict__ prim, double* __restrict__ grad,
-                double* __restrict__ lim, double eps, double venkat_K) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Cell kernel, one wavefront per (variable, 64 cells): wave v of a block
+// handles variable v of cells [64*b, 64*b + 64).  The weight formulae are
+// evaluated by each wave (they depend on geometry only); per-thread state is
+// one variable, which keeps register use low enough for 8 waves per SIMD.
+template <int DIM, int LIM>
+__global__ void __launch_bounds__(320)
+    k_cell_grad_var(DevMesh m, VarSet vs, const double* __restrict__ prim,
+                    double* __restrict__ grad, double* __restrict__ lim, double eps,
+                    double venkat_K, int nlogical) {
+    const int lb = xcd_block(nlogical);
+    if (lb >= nlogical) return;
+    const int v = threadIdx.x >> 6;
+    const int i = lb * 64 + (threadIdx.x & 63);
     const int N = m.N, Nt = m.Nt, F = m.F;
     if (i >= N) return;
+    const int row = vs.row[v], slot = vs.slot[v];
     const double* cx = m.cen;
     const double* cy = m.cen + Nt;
     const double* cz = m.cen + 2 * (size_t)Nt;
+    const double* u = prim + (size_t)row * Nt;
     const double xi = cx[i], yi = cy[i], zi = cz[i];
     const int nf = m.nfaces[i];
     const double r11 = m.lsq[i], r12 = m.lsq[N + i], r22 = m.lsq[2 * (size_t)N + i];
@@ -235,24 +263,14 @@ __global__ void __launch_bounds__(BLOCK)
         r23 = m.lsq[4 * (size_t)N + i];
         r33 = m.lsq[5 * (size_t)N + i];
     }
-    // per-cell constants of the weight formulae, hoisted exactly as written
     const double beta = (r12 * r23 - r13 * r23) / (r11 * r22);  // GR/gradient.h:89 (sic)
     const double r11s = r11 * r11;
     const double i22 = 1.0 / (r22 * r22);
     const double q12 = r12 / r11;
     const double i33 = 1.0 / (r33 * r33);
     const double q23 = r23 / r22;
-
-    double ui[NV], gx[NV], gy[NV], gz[NV], umin[NV], umax[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-        ui[v] = prim[(size_t)vs.row[v] * Nt + i];
-        gx[v] = 0.0;
-        gy[v] = 0.0;
-        gz[v] = 0.0;
-        umin[v] = ui[v];
-        umax[v] = ui[v];
-    }
+    const double ui = u[i];
+    double gx = 0.0, gy = 0.0, gz = 0.0, umin = ui, umax = ui;
     for (int j = 0; j < nf; j++) {
         const int n = m.ell_nbr[(size_t)j * N + i];
         const double dx = cx[n] - xi;
@@ -265,34 +283,26 @@ __global__ void __launch_bounds__(BLOCK)
         const double w_1 = alpha_1 - q12 * alpha_2 + beta * alpha_3;
         const double w_2 = alpha_2 - q23 * alpha_3;
         const double w_3 = alpha_3;
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-            const double un = prim[(size_t)vs.row[v] * Nt + n];
-            const double diff = un - ui[v];
-            gx[v] += w_1 * diff;
-            gy[v] += w_2 * diff;
-            gz[v] += w_3 * diff;
-            umin[v] = ibis::kmin(umin[v], un);
-            umax[v] = ibis::kmax(umax[v], un);
-        }
+        const double un = u[n];
+        const double diff = un - ui;
+        gx += w_1 * diff;
+        gy += w_2 * diff;
+        gz += w_3 * diff;
+        umin = ibis::kmin(umin, un);
+        umax = ibis::kmax(umax, un);
     }
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-        double* g = grad + (size_t)vs.slot[v] * 3 * N;
-        if (v < vs.ngrad) {
-            g[i] = gx[v];
-            g[N + i] = gy[v];
-            g[2 * (size_t)N + i] = gz[v];
-        } else {
-            gx[v] = g[i];
-            gy[v] = g[N + i];
-            gz[v] = g[2 * (size_t)N + i];
-        }
+    double* g = grad + (size_t)slot * 3 * N;
+    if (v < vs.ngrad) {
+        g[i] = gx;
+        g[N + i] = gy;
+        g[2 * (size_t)N + i] = gz;
+    } else {
+        gx = g[i];
+        gy = g[N + i];
+        gz = g[2 * (size_t)N + i];
     }
     if (LIM == 0) return;
-    double phi[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++) phi[v] = 1.0;
+    double phi = 1.0;
     double eps2 = 0.0;
     if (LIM == 2) {
         const double V = m.vol[i];
@@ -304,34 +314,30 @@ __global__ void __launch_bounds__(BLOCK)
         const double dx = m.geom[9 * (size_t)F + f] - xi;
         const double dy = m.geom[10 * (size_t)F + f] - yi;
         const double dz = m.geom[11 * (size_t)F + f] - zi;
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-            double d2 = gx[v] * dx + gy[v] * dy + gz[v] * dz;
-            if (LIM == 1) {  // Barth-Jespersen, FV/limiter.cpp:289-303
-                const int s = (d2 > 0) - (d2 < 0);
-                d2 = s * (fabs(d2) + eps);
-                if (s > 0)
-                    phi[v] = ibis::kmin(phi[v], (umax[v] - ui[v]) / d2);
-                else if (s < 0)
-                    phi[v] = ibis::kmin(phi[v], (umin[v] - ui[v]) / d2);
-            } else {  // Venkatakrishnan
-                double d1;
-                if (d2 > 0.0)
-                    d1 = umax[v] - ui[v];
-                else if (d2 < 0.0)
-                    d1 = umin[v] - ui[v];
-                else
-                    continue;
-                const double d1s = d1 * d1;
-                const double d2s = d2 * d2;
-                const double num = d1s + eps2 + 2.0 * (d1 * d2);
-                const double den = d1s + 2.0 * d2s + d1 * d2 + eps2;
-                phi[v] = ibis::kmin(phi[v], num / den);
-            }
+        double d2 = gx * dx + gy * dy + gz * dz;
+        if (LIM == 1) {  // Barth-Jespersen, FV/limiter.cpp:289-303
+            const int sg = (d2 > 0) - (d2 < 0);
+            d2 = sg * (fabs(d2) + eps);
+            if (sg > 0)
+                phi = ibis::kmin(phi, (umax - ui) / d2);
+            else if (sg < 0)
+                phi = ibis::kmin(phi, (umin - ui) / d2);
+        } else {  // Venkatakrishnan
+            double d1;
+            if (d2 > 0.0)
+                d1 = umax - ui;
+            else if (d2 < 0.0)
+                d1 = umin - ui;
+            else
+                continue;
+            const double d1s = d1 * d1;
+            const double d2s = d2 * d2;
+            const double num = d1s + eps2 + 2.0 * (d1 * d2);
+            const double den = d1s + 2.0 * d2s + d1 * d2 + eps2;
+            phi = ibis::kmin(phi, num / den);
         }
     }
-#pragma unroll
-    for (int v = 0; v < NV; v++) lim[(size_t)vs.slot[v] * N + i] = phi[v];
+    lim[(size_t)slot * N + i] = phi;
 }
 
 // ---------------------------------------------------------------------------
@@ -402,14 +408,16 @@ __device__ __forceinline__ void reconstruct(const DevMesh& m, const FaceParams& 
     }
 }
 
-template <int DIM, int FLUX, bool RECON, bool PREMUL>
-__global__ void __launch_bounds__(BLOCK)
-    k_face_flux(DevMesh m, FaceParams fp, const double* __restrict__ prim,
-                const double* __restrict__ grad, const double* __restrict__ lim,
-                double* __restrict__ flux) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+// Flux of face f in the global frame, rows in conserved order (mass,
+// momentum..., energy): reconstruction of both sides, rotation to the face
+// frame, flux calculator, rotation back (FV/convective_flux.cpp:85-178).
+template <int DIM, int FLUX, bool RECON>
+__device__ __forceinline__ void face_flux_eval(const DevMesh& m, const FaceParams& fp,
+                                               const double* __restrict__ prim,
+                                               const double* __restrict__ grad,
+                                               const double* __restrict__ lim, int f,
+                                               double* res) {
     const int F = m.F;
-    if (f >= F) return;
     const int l = m.L[f], r = m.R[f];
     double Ls[NPRIM], Rs[NPRIM];
     reconstruct<RECON>(m, fp, prim, grad, lim, f, l, Ls);
@@ -442,20 +450,78 @@ __global__ void __launch_bounds__(BLOCK)
     const double px = out[1], py = out[2];
     double pz = 0.0;
     if (T3) pz = out[3];
-    const double x = px * nx + py * t1x + pz * t2x;
-    const double y = px * ny + py * t1y + pz * t2y;
-    const double z = px * nz + py * t1z + pz * t2z;
-    double res[5] = {out[0], x, y, z, out[4]};
-    const double A = PREMUL ? m.geom[12 * (size_t)F + f] : 1.0;
-    flux[f] = PREMUL ? res[0] * A : res[0];
-    flux[(size_t)F + f] = PREMUL ? res[1] * A : res[1];
-    flux[2 * (size_t)F + f] = PREMUL ? res[2] * A : res[2];
+    res[0] = out[0];
+    res[1] = px * nx + py * t1x + pz * t2x;
+    res[2] = px * ny + py * t1y + pz * t2y;
     if (T3) {
-        flux[3 * (size_t)F + f] = PREMUL ? res[3] * A : res[3];
-        flux[4 * (size_t)F + f] = PREMUL ? res[4] * A : res[4];
+        res[3] = px * nz + py * t1z + pz * t2z;
+        res[4] = out[4];
     } else {
-        flux[3 * (size_t)F + f] = PREMUL ? res[4] * A : res[4];
+        res[3] = out[4];
     }
+}
+
+template <int DIM, int FLUX, bool RECON, bool PREMUL>
+__global__ void __launch_bounds__(BLOCK)
+    k_face_flux(DevMesh m, FaceParams fp, const double* __restrict__ prim,
+                const double* __restrict__ grad, const double* __restrict__ lim,
+                double* __restrict__ flux) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int F = m.F;
+    if (f >= F) return;
+    constexpr int NC = DIM + 2;
+    double res[NC];
+    face_flux_eval<DIM, FLUX, RECON>(m, fp, prim, grad, lim, f, res);
+    const double A = PREMUL ? m.geom[12 * (size_t)F + f] : 1.0;
+#pragma unroll
+    for (int q = 0; q < NC; q++) flux[(size_t)q * F + f] = PREMUL ? res[q] * A : res[q];
+}
+
+// Fused face flux + surface integral for one tile of TILE cells.
+// Phase 1: every face touching the tile is evaluated once per tile and its
+// flux*area kept in LDS (faces on the tile border are evaluated by both
+// neighbouring tiles).  Phase 2: each cell sums its faces from LDS in its
+// own face order, so the result is the gather of
+// FV/finite_volume.cpp:221-255 with no flux array in HBM and no atomics.
+template <int DIM, int FLUX, bool RECON>
+__global__ void __launch_bounds__(TILE)
+    k_face_tile(DevMesh m, FaceParams fp, const double* __restrict__ prim,
+                const double* __restrict__ grad, const double* __restrict__ lim,
+                double* __restrict__ dudt) {
+    extern __shared__ double sflux[];  // NC x maxtf
+    const int t = xcd_block(m.ntiles);
+    if (t >= m.ntiles) return;
+    constexpr int NC = DIM + 2;
+    const int F = m.F, N = m.N, Nt = m.Nt, S = m.maxtf;
+    const int f0 = m.tile_face_off[t];
+    const int nfl = m.tile_face_off[t + 1] - f0;
+    for (int k = threadIdx.x; k < nfl; k += TILE) {
+        const int f = m.tile_faces[f0 + k];
+        double res[NC];
+        face_flux_eval<DIM, FLUX, RECON>(m, fp, prim, grad, lim, f, res);
+        const double A = m.geom[12 * (size_t)F + f];
+#pragma unroll
+        for (int q = 0; q < NC; q++) sflux[q * S + k] = res[q] * A;
+    }
+    __syncthreads();
+    const int c = t * TILE + threadIdx.x;
+    if (c >= N) return;
+    double d[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) d[q] = 0.0;
+    const int nf = m.nfaces[c];
+    for (int j = 0; j < nf; j++) {
+        const int k = m.cell_lds[(size_t)j * N + c];
+        const int s = m.ell_sign[(size_t)j * N + c];
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+            const double fa = sflux[q * S + k];
+            d[q] += (s > 0) ? -fa : fa;  // == flux * (-area * outsign), exactly
+        }
+    }
+    const double V = m.vol[c];
+#pragma unroll
+    for (int q = 0; q < NC; q++) dudt[(size_t)q * Nt + c] = d[q] / V;
 }
 
 // ConstantFlux::apply (boundary.cpp:405-443), static grid
@@ -745,6 +811,10 @@ __global__ void __launch_bounds__(BLOCK)
 }
 
 inline unsigned nblocks(long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+inline unsigned xcd_grid(long nlogical) {
+    const long per = (nlogical + NXCD - 1) / NXCD;
+    return (unsigned)(per * NXCD);
+}
 
 }  // namespace
 
@@ -849,6 +919,34 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
             es[(size_t)j * N + c] = (signed char)d->cell_outsigns[o + j];
         }
     }
+    // tiles of TILE consecutive cells: faces touching each tile, and for
+    // every (cell, face slot) the face's position in its tile's list
+    const int ntiles = (int)((N + TILE - 1) / TILE);
+    std::vector<int> toff(ntiles + 1, 0), tfaces;
+    std::vector<unsigned short> clds((size_t)maxf * N, 0);
+    int maxtf = 0;
+    {
+        std::vector<int> fl;
+        for (int t = 0; t < ntiles; t++) {
+            fl.clear();
+            const int64_t c0 = (int64_t)t * TILE, c1 = std::min<int64_t>(N, c0 + TILE);
+            for (int64_t c = c0; c < c1; c++)
+                for (int64_t j = d->cell_face_off[c]; j < d->cell_face_off[c + 1]; j++)
+                    fl.push_back((int)d->cell_faces[j]);
+            std::sort(fl.begin(), fl.end());
+            fl.erase(std::unique(fl.begin(), fl.end()), fl.end());
+            maxtf = std::max<int>(maxtf, (int)fl.size());
+            for (int64_t c = c0; c < c1; c++) {
+                const int64_t o = d->cell_face_off[c];
+                for (int64_t j = o; j < d->cell_face_off[c + 1]; j++) {
+                    auto it = std::lower_bound(fl.begin(), fl.end(), (int)d->cell_faces[j]);
+                    clds[(size_t)(j - o) * N + c] = (unsigned short)(it - fl.begin());
+                }
+            }
+            tfaces.insert(tfaces.end(), fl.begin(), fl.end());
+            toff[t + 1] = (int)tfaces.size();
+        }
+    }
     std::vector<double> geom(d->face_geom, d->face_geom + 13 * F);
     std::vector<double> cen(d->centroid, d->centroid + 3 * Nt);
     std::vector<double> vol(d->volume, d->volume + Nt);
@@ -873,7 +971,12 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
     UP(b_ghost, b_ghost);
     UP(b_valid, b_valid);
     UP(b_tag, b_tag);
+    UP(toff, tile_face_off);
+    UP(tfaces, tile_faces);
+    UP(clds, cell_lds);
 #undef UP
+    D.ntiles = ntiles;
+    D.maxtf = maxtf;
     *out = m;
     return IBIS_OK;
 }
@@ -951,7 +1054,8 @@ ibis_status ibis_residual_create(const ibis_mesh* m, const ibis_fv_desc* desc,
     ibis_residual* r = new ibis_residual();
     r->m = m;
     r->desc = d;
-    r->premul = !d.viscous;
+    // the unfused path keeps the flux unmultiplied (the viscous kernel adds to it)
+    r->premul = false;
     const size_t N = D.N, F = D.F;
     const int nc = D.dim + 2;
     auto alloc0 = [&](void** p, size_t bytes) -> hipError_t {
@@ -1006,15 +1110,46 @@ namespace {
 template <int DIM>
 void launch_cell_grad(const DevMesh& D, const VarSet& vs, int lim, const double* prim,
                       double* grad, double* limv, double eps, double K, hipStream_t st) {
-    const unsigned nb = nblocks(D.N);
-    // NV is 5 in both dimensions (rho/e-like pair, vx, vy, vz); in 2D the vz
-    // gradient is taken from its slot rather than computed.
+    const int nlog = (D.N + 63) / 64;
+    const unsigned g = xcd_grid(nlog);
+    const unsigned b = 64 * vs.n;
     if (lim == 0)
-        k_cell_grad<DIM, 0, 5><<<nb, BLOCK, 0, st>>>(D, vs, prim, grad, limv, eps, K);
+        k_cell_grad_var<DIM, 0><<<g, b, 0, st>>>(D, vs, prim, grad, limv, eps, K, nlog);
     else if (lim == 1)
-        k_cell_grad<DIM, 1, 5><<<nb, BLOCK, 0, st>>>(D, vs, prim, grad, limv, eps, K);
+        k_cell_grad_var<DIM, 1><<<g, b, 0, st>>>(D, vs, prim, grad, limv, eps, K, nlog);
     else
-        k_cell_grad<DIM, 2, 5><<<nb, BLOCK, 0, st>>>(D, vs, prim, grad, limv, eps, K);
+        k_cell_grad_var<DIM, 2><<<g, b, 0, st>>>(D, vs, prim, grad, limv, eps, K, nlog);
+}
+
+template <int DIM, int FLUX>
+void launch_face_tile_f(const DevMesh& D, const FaceParams& fp, bool recon, const double* prim,
+                        const double* grad, const double* lim, double* dudt, hipStream_t st) {
+    const unsigned g = xcd_grid(D.ntiles);
+    const size_t shm = (size_t)(DIM + 2) * D.maxtf * sizeof(double);
+    if (recon)
+        k_face_tile<DIM, FLUX, true><<<g, TILE, shm, st>>>(D, fp, prim, grad, lim, dudt);
+    else
+        k_face_tile<DIM, FLUX, false><<<g, TILE, shm, st>>>(D, fp, prim, grad, lim, dudt);
+}
+
+template <int DIM>
+void launch_face_tile(const DevMesh& D, int flux_kind, const FaceParams& fp, bool recon,
+                      const double* prim, const double* grad, const double* lim, double* dudt,
+                      hipStream_t st) {
+    switch (flux_kind) {
+        case IBIS_FLUX_HANEL:
+            launch_face_tile_f<DIM, IBIS_FLUX_HANEL>(D, fp, recon, prim, grad, lim, dudt, st);
+            break;
+        case IBIS_FLUX_AUSMDV:
+            launch_face_tile_f<DIM, IBIS_FLUX_AUSMDV>(D, fp, recon, prim, grad, lim, dudt, st);
+            break;
+        case IBIS_FLUX_LDFSS2:
+            launch_face_tile_f<DIM, IBIS_FLUX_LDFSS2>(D, fp, recon, prim, grad, lim, dudt, st);
+            break;
+        default:
+            launch_face_tile_f<DIM, IBIS_FLUX_RUSANOV>(D, fp, recon, prim, grad, lim, dudt, st);
+            break;
+    }
 }
 
 template <int DIM, int FLUX>
@@ -1148,7 +1283,20 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
             launch_cell_grad<2>(D, vs, d.limiter, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, st);
         end(1);
     }
-    // 3. convective face flux
+    // 3. convective face flux.  Without viscous or flux-override actions the
+    //    face flux and the surface integral run as one tiled kernel.
+    const bool has_post = D.nb > 0 && has_kind(d.post_kind, d.ntags, IBIS_BC_CONSTANT_FLUX);
+    const bool fused = !d.viscous && !has_post;
+    if (fused) {
+        beg();
+        if (D.dim == 3)
+            launch_face_tile<3>(D, d.flux, fp, recon, prim, r->grad, r->lim, dudt, st);
+        else
+            launch_face_tile<2>(D, d.flux, fp, recon, prim, r->grad, r->lim, dudt, st);
+        end(2);
+        HIP_TRY(hipGetLastError());
+        return IBIS_OK;
+    }
     beg();
     if (D.dim == 3)
         launch_face_flux<3>(D, d.flux, fp, recon, r->premul, prim, r->grad, r->lim, r->flux, st);
@@ -1156,7 +1304,7 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
         launch_face_flux<2>(D, d.flux, fp, recon, r->premul, prim, r->grad, r->lim, r->flux, st);
     end(2);
     // 4. post-convective flux actions
-    if (D.nb > 0 && has_kind(d.post_kind, d.ntags, IBIS_BC_CONSTANT_FLUX)) {
+    if (has_post) {
         beg();
         k_post_flux<<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds + 2 * ntab,
                                                      r->pars + 2 * ntab * IBIS_NPAR, gp,
@@ -1184,13 +1332,12 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
         vs.slot[3] = G_VZ;
         vs.row[4] = P_VZ;
         vs.slot[4] = G_VZ;
-        const unsigned nbc = nblocks(D.N);
         ViscParams vp{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr};
         beg();
         if (D.dim == 3)
-            k_cell_grad<3, 0, 4><<<nbc, BLOCK, 0, st>>>(D, vs, prim, r->grad, r->lim, 0.0, 0.0);
+            launch_cell_grad<3>(D, vs, 0, prim, r->grad, r->lim, 0.0, 0.0, st);
         else
-            k_cell_grad<2, 0, 4><<<nbc, BLOCK, 0, st>>>(D, vs, prim, r->grad, r->lim, 0.0, 0.0);
+            launch_cell_grad<2>(D, vs, 0, prim, r->grad, r->lim, 0.0, 0.0, st);
         end(5);
         beg();
         if (D.dim == 3)
