@@ -83,6 +83,10 @@ struct DevMesh {
     const int* tile_face_off;         // ntiles + 1
     const int* tile_faces;            // face ids, ascending within a tile
     const unsigned short* cell_lds;   // maxf x N: position of a cell's face in its tile list
+    // face-state records: slot of face f in the face list of its left/right cell
+    const unsigned char* slotL;
+    const unsigned char* slotR;
+    int n_from_t;  // face normals equal cross(t1, t2) bit for bit (grid-built meshes)
 };
 
 constexpr int TILE = 256;
@@ -524,6 +528,293 @@ __global__ void __launch_bounds__(TILE)
     for (int q = 0; q < NC; q++) dudt[(size_t)q * Nt + c] = d[q] / V;
 }
 
+// ---------------------------------------------------------------------------
+// Record path for the 2nd-order residual without viscous or flux-override
+// actions.  The cell kernel reconstructs every face value it owns (value +
+// limiter * (grad . (x_face - x_cell)), FV/convective_flux.cpp:243-255) and
+// stores the 5 reconstruction variables per (cell, face slot) as one 40-byte
+// record; ghost cells get one record each.  The face kernel then reads two
+// records per face and never touches gradients, limiters or centroids, so
+// every byte it reads is used once.
+//   record of valid cell c, slot j : frec[(c*S + j)*5 + v]
+//   record of ghost cell g         : frec[(N*S + g - N)*5 + v]
+//   v = thermo variable A, B, vx, vy, vz;  S = m.maxf
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ size_t rec_index(const DevMesh& m, int c, int slot) {
+    return c < m.N ? (size_t)c * m.maxf + slot : (size_t)m.N * m.maxf + (c - m.N);
+}
+
+// Ghost-side records: grad = 0 and limiter = 1 for a ghost cell, so the
+// reconstructed value is value + 1.0 * (0.0*dx + 0.0*dy + 0.0*dz).
+__global__ void __launch_bounds__(BLOCK)
+    k_ghost_rec(DevMesh m, VarSet vs, const double* __restrict__ prim, double* __restrict__ frec) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.nb) return;
+    const int f = m.b_face[b], g = m.b_ghost[b];
+    const int F = m.F, Nt = m.Nt;
+    const double dx = m.geom[9 * (size_t)F + f] - m.cen[g];
+    const double dy = m.geom[10 * (size_t)F + f] - m.cen[Nt + g];
+    const double dz = m.geom[11 * (size_t)F + f] - m.cen[2 * (size_t)Nt + g];
+    double* rec = frec + rec_index(m, g, 0) * 5;
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        const double gx = 0.0, gy = 0.0, gz = 0.0;
+        rec[v] = prim[(size_t)vs.row[v] * Nt + g] + 1.0 * (gx * dx + gy * dy + gz * dz);
+    }
+}
+
+// Cell kernel of the record path: wave v of a 320-thread block handles
+// variable v of 64 cells (LSQ gradient, limiter, face values); the face
+// values are staged in LDS and written out as contiguous records.
+template <int DIM, int LIM>
+__global__ void __launch_bounds__(320)
+    k_cell_rec(DevMesh m, VarSet vs, const double* __restrict__ prim, double* __restrict__ grad,
+               double* __restrict__ lim, double eps, double venkat_K, int nlogical,
+               int write_slots, double* __restrict__ frec) {
+    __shared__ double stage[64 * MAXF * 5];
+    const int lb = xcd_block(nlogical);
+    if (lb >= nlogical) return;
+    const int v = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int i = lb * 64 + lane;
+    const int N = m.N, Nt = m.Nt, F = m.F;
+    if (i < N) {
+        const int row = vs.row[v], slot = vs.slot[v];
+        const double* cx = m.cen;
+        const double* cy = m.cen + Nt;
+        const double* cz = m.cen + 2 * (size_t)Nt;
+        const double* u = prim + (size_t)row * Nt;
+        const double xi = cx[i], yi = cy[i], zi = cz[i];
+        const int nf = m.nfaces[i];
+        const double ui = u[i];
+        double gx = 0.0, gy = 0.0, gz = 0.0, umin = ui, umax = ui;
+        if (v < vs.ngrad) {
+            const double r11 = m.lsq[i], r12 = m.lsq[N + i], r22 = m.lsq[2 * (size_t)N + i];
+            double r13 = 0.0, r23 = 0.0, r33 = 0.0;
+            if (DIM == 3) {
+                r13 = m.lsq[3 * (size_t)N + i];
+                r23 = m.lsq[4 * (size_t)N + i];
+                r33 = m.lsq[5 * (size_t)N + i];
+            }
+            const double beta = (r12 * r23 - r13 * r23) / (r11 * r22);  // GR/gradient.h:89 (sic)
+            const double r11s = r11 * r11;
+            const double i22 = 1.0 / (r22 * r22);
+            const double q12 = r12 / r11;
+            const double i33 = 1.0 / (r33 * r33);
+            const double q23 = r23 / r22;
+            for (int j = 0; j < nf; j++) {
+                const int n = m.ell_nbr[(size_t)j * N + i];
+                const double dx = cx[n] - xi;
+                const double dy = cy[n] - yi;
+                const double dz = cz[n] - zi;
+                const double alpha_1 = dx / r11s;
+                const double alpha_2 = i22 * (dy - q12 * dx);
+                double alpha_3 = 0.0;
+                if (DIM == 3) alpha_3 = i33 * (dz - q23 * dy + beta * dx);
+                const double w_1 = alpha_1 - q12 * alpha_2 + beta * alpha_3;
+                const double w_2 = alpha_2 - q23 * alpha_3;
+                const double w_3 = alpha_3;
+                const double un = u[n];
+                const double diff = un - ui;
+                gx += w_1 * diff;
+                gy += w_2 * diff;
+                gz += w_3 * diff;
+                umin = ibis::kmin(umin, un);
+                umax = ibis::kmax(umax, un);
+            }
+        } else {
+            for (int j = 0; j < nf; j++) {
+                const double un = u[m.ell_nbr[(size_t)j * N + i]];
+                umin = ibis::kmin(umin, un);
+                umax = ibis::kmax(umax, un);
+            }
+        }
+        double* g = grad + (size_t)slot * 3 * N;
+        if (v < vs.ngrad) {
+            if (write_slots) {
+                g[i] = gx;
+                g[N + i] = gy;
+                g[2 * (size_t)N + i] = gz;
+            }
+        } else {
+            gx = g[i];
+            gy = g[N + i];
+            gz = g[2 * (size_t)N + i];
+        }
+        double ddx[MAXF], ddy[MAXF], ddz[MAXF];
+#pragma unroll
+        for (int j = 0; j < MAXF; j++) {
+            if (j < nf) {
+                const int f = m.ell_face[(size_t)j * N + i];
+                ddx[j] = m.geom[9 * (size_t)F + f] - xi;
+                ddy[j] = m.geom[10 * (size_t)F + f] - yi;
+                ddz[j] = m.geom[11 * (size_t)F + f] - zi;
+            } else {
+                ddx[j] = ddy[j] = ddz[j] = 0.0;
+            }
+        }
+        double phi = 1.0;
+        if (LIM != 0) {
+            double eps2 = 0.0;
+            if (LIM == 2) {
+                const double V = m.vol[i];
+                const double K3 = venkat_K * venkat_K * venkat_K;
+                eps2 = (DIM == 3) ? K3 * V : K3 * (V * sqrt(V));
+            }
+#pragma unroll
+            for (int j = 0; j < MAXF; j++) {
+                if (j >= nf) break;
+                double d2 = gx * ddx[j] + gy * ddy[j] + gz * ddz[j];
+                if (LIM == 1) {  // Barth-Jespersen, FV/limiter.cpp:289-303
+                    const int sg = (d2 > 0) - (d2 < 0);
+                    d2 = sg * (fabs(d2) + eps);
+                    if (sg > 0)
+                        phi = ibis::kmin(phi, (umax - ui) / d2);
+                    else if (sg < 0)
+                        phi = ibis::kmin(phi, (umin - ui) / d2);
+                } else {  // Venkatakrishnan
+                    double d1;
+                    if (d2 > 0.0)
+                        d1 = umax - ui;
+                    else if (d2 < 0.0)
+                        d1 = umin - ui;
+                    else
+                        continue;
+                    const double d1s = d1 * d1;
+                    const double d2s = d2 * d2;
+                    const double num = d1s + eps2 + 2.0 * (d1 * d2);
+                    const double den = d1s + 2.0 * d2s + d1 * d2 + eps2;
+                    phi = ibis::kmin(phi, num / den);
+                }
+            }
+            if (write_slots) lim[(size_t)slot * N + i] = phi;
+        }
+#pragma unroll
+        for (int j = 0; j < MAXF; j++) {
+            if (j >= nf) break;
+            stage[(lane * MAXF + j) * 5 + v] = ui + phi * (gx * ddx[j] + gy * ddy[j] + gz * ddz[j]);
+        }
+    }
+    __syncthreads();
+    const int S = m.maxf;
+    const int nc = min(64, N - lb * 64);
+    const int total = nc * S * 5;
+    double* out = frec + (size_t)lb * 64 * S * 5;
+    for (int k = threadIdx.x; k < total; k += blockDim.x) {
+        const int cl = k / (S * 5), rem = k - cl * (S * 5);
+        out[k] = stage[cl * MAXF * 5 + rem];
+    }
+}
+
+// Fused face flux + surface integral from face-state records.
+template <int DIM, int FLUX>
+__global__ void __launch_bounds__(TILE)
+    k_face_tile_rec(DevMesh m, FaceParams fp, const double* __restrict__ frec,
+                    double* __restrict__ dudt) {
+    extern __shared__ double sflux[];  // NC x maxtf
+    const int t = xcd_block(m.ntiles);
+    if (t >= m.ntiles) return;
+    constexpr int NC = DIM + 2;
+    constexpr bool T3 = DIM == 3;
+    const int F = m.F, N = m.N, Nt = m.Nt, S = m.maxtf;
+    const int f0 = m.tile_face_off[t];
+    const int nfl = m.tile_face_off[t + 1] - f0;
+    for (int k = threadIdx.x; k < nfl; k += TILE) {
+        const int f = m.tile_faces[f0 + k];
+        const int l = m.L[f], r = m.R[f];
+        const double* rl = frec + rec_index(m, l, m.slotL[f]) * 5;
+        const double* rr = frec + rec_index(m, r, m.slotR[f]) * 5;
+        double Ls[NPRIM], Rs[NPRIM];
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const double* rec = side ? rr : rl;
+            double* st = side ? Rs : Ls;
+            const double a = rec[0], b = rec[1];
+            double rho = 0.0, p = 0.0, T = 0.0, e = 0.0;
+            switch (fp.thermo) {
+                case IBIS_THERMO_RHO_P: p = a; rho = b; break;
+                case IBIS_THERMO_RHO_T: rho = a; T = b; break;
+                case IBIS_THERMO_RHO_U: rho = a; e = b; break;
+                default: p = a; T = b; break;
+            }
+            ibis::thermo_update<double>(fp.thermo, fp.R, fp.Cv, rho, p, T, e);
+            st[ibis::S_RHO] = rho;
+            st[ibis::S_P] = p;
+            st[ibis::S_T] = T;
+            st[ibis::S_E] = e;
+            st[4] = rec[2];
+            st[5] = rec[3];
+            st[6] = rec[4];
+        }
+        const double t1x = m.geom[3 * (size_t)F + f], t1y = m.geom[4 * (size_t)F + f],
+                     t1z = m.geom[5 * (size_t)F + f];
+        const double t2x = m.geom[6 * (size_t)F + f], t2y = m.geom[7 * (size_t)F + f],
+                     t2z = m.geom[8 * (size_t)F + f];
+        double nx, ny, nz;
+        if (m.n_from_t) {  // cross(tan1, tan2), GR/interface.h:249
+            nx = t1y * t2z - t1z * t2y;
+            ny = t1z * t2x - t1x * t2z;
+            nz = t1x * t2y - t1y * t2x;
+        } else {
+            nx = m.geom[f];
+            ny = m.geom[(size_t)F + f];
+            nz = m.geom[2 * (size_t)F + f];
+        }
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            double* s = side ? Rs : Ls;
+            const double a = s[4], b = s[5], c = s[6];
+            s[4] = a * nx + b * ny + c * nz;
+            s[5] = a * t1x + b * t1y + c * t1z;
+            s[6] = a * t2x + b * t2y + c * t2z;
+        }
+        double out[5];
+        if (FLUX == IBIS_FLUX_AUSMDV)
+            ibis::flux_ausmdv<double, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, out);
+        else if (FLUX == IBIS_FLUX_HANEL)
+            ibis::flux_hanel<double, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, out);
+        else if (FLUX == IBIS_FLUX_LDFSS2)
+            ibis::flux_ldfss2<double, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, fp.delta, out);
+        else
+            ibis::flux_rusanov<double, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, out);
+        const double px = out[1], py = out[2];
+        double pz = 0.0;
+        if (T3) pz = out[3];
+        double res[NC];
+        res[0] = out[0];
+        res[1] = px * nx + py * t1x + pz * t2x;
+        res[2] = px * ny + py * t1y + pz * t2y;
+        if (T3) {
+            res[3] = px * nz + py * t1z + pz * t2z;
+            res[4] = out[4];
+        } else {
+            res[3] = out[4];
+        }
+        const double A = m.geom[12 * (size_t)F + f];
+#pragma unroll
+        for (int q = 0; q < NC; q++) sflux[q * S + k] = res[q] * A;
+    }
+    __syncthreads();
+    const int c = t * TILE + threadIdx.x;
+    if (c >= N) return;
+    double d[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) d[q] = 0.0;
+    const int nf = m.nfaces[c];
+    for (int j = 0; j < nf; j++) {
+        const int k = m.cell_lds[(size_t)j * N + c];
+        const int s = m.ell_sign[(size_t)j * N + c];
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+            const double fa = sflux[q * S + k];
+            d[q] += (s > 0) ? -fa : fa;
+        }
+    }
+    const double V = m.vol[c];
+#pragma unroll
+    for (int q = 0; q < NC; q++) dudt[(size_t)q * Nt + c] = d[q] / V;
+}
+
 // ConstantFlux::apply (boundary.cpp:405-443), static grid
 __global__ void __launch_bounds__(BLOCK)
     k_post_flux(DevMesh m, const int* __restrict__ kinds, const double* __restrict__ pars,
@@ -947,6 +1238,27 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
             toff[t + 1] = (int)tfaces.size();
         }
     }
+    std::vector<unsigned char> slotL(F, 0), slotR(F, 0);
+    for (int64_t c = 0; c < N; c++)
+        for (int64_t j = d->cell_face_off[c]; j < d->cell_face_off[c + 1]; j++) {
+            const int64_t f = d->cell_faces[j];
+            if (d->face_left[f] == c)
+                slotL[f] = (unsigned char)(j - d->cell_face_off[c]);
+            else
+                slotR[f] = (unsigned char)(j - d->cell_face_off[c]);
+        }
+    int n_from_t = 1;
+    for (int64_t f = 0; f < F && n_from_t; f++) {
+        const double* gg = d->face_geom;
+        const double t1x = gg[3 * F + f], t1y = gg[4 * F + f], t1z = gg[5 * F + f];
+        const double t2x = gg[6 * F + f], t2y = gg[7 * F + f], t2z = gg[8 * F + f];
+        const double nx = t1y * t2z - t1z * t2y;
+        const double ny = t1z * t2x - t1x * t2z;
+        const double nz = t1x * t2y - t1y * t2x;
+        if (std::memcmp(&nx, gg + f, 8) || std::memcmp(&ny, gg + F + f, 8) ||
+            std::memcmp(&nz, gg + 2 * F + f, 8))
+            n_from_t = 0;
+    }
     std::vector<double> geom(d->face_geom, d->face_geom + 13 * F);
     std::vector<double> cen(d->centroid, d->centroid + 3 * Nt);
     std::vector<double> vol(d->volume, d->volume + Nt);
@@ -974,7 +1286,10 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
     UP(toff, tile_face_off);
     UP(tfaces, tile_faces);
     UP(clds, cell_lds);
+    UP(slotL, slotL);
+    UP(slotR, slotR);
 #undef UP
+    D.n_from_t = n_from_t;
     D.ntiles = ntiles;
     D.maxtf = maxtf;
     *out = m;
@@ -995,6 +1310,7 @@ struct ibis_residual {
     double* grad = nullptr;
     double* lim = nullptr;
     double* flux = nullptr;
+    double* frec = nullptr;  // face-state records of the fused 2nd-order path
     double* face_rho = nullptr;
     double* face_T = nullptr;
     int* kinds = nullptr;    // pre, visc, post
@@ -1021,7 +1337,7 @@ struct ibis_residual {
 
 void ibis_residual_destroy(ibis_residual* r) {
     if (!r) return;
-    for (void* p : {(void*)r->grad, (void*)r->lim, (void*)r->flux, (void*)r->face_rho,
+    for (void* p : {(void*)r->grad, (void*)r->lim, (void*)r->flux, (void*)r->frec, (void*)r->face_rho,
                     (void*)r->face_T, (void*)r->kinds, (void*)r->pars, (void*)r->partial,
                     (void*)r->dt_dev, (void*)r->count_dev, (void*)r->k_tmp, (void*)r->flow_tmp})
         if (p) (void)hipFree(p);
@@ -1074,6 +1390,7 @@ ibis_status ibis_residual_create(const ibis_mesh* m, const ibis_fv_desc* desc,
     AL(r->grad, sizeof(double) * NGRAD * 3 * N);
     AL(r->lim, sizeof(double) * NGRAD * N);
     AL(r->flux, sizeof(double) * nc * F);
+    AL(r->frec, sizeof(double) * 5 * ((size_t)D.N * D.maxf + D.G));
     if (d.viscous) {
         AL(r->face_rho, sizeof(double) * F);
         AL(r->face_T, sizeof(double) * F);
@@ -1262,20 +1579,69 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
     fp.Cv = d.Cv;
     fp.gamma = d.gamma;
     fp.delta = d.ldfss_delta;
+    VarSet vs;
+    vs.n = 5;
+    vs.ngrad = D.dim == 3 ? 5 : 4;
+    vs.row[0] = fp.rowA;
+    vs.slot[0] = fp.slotA;
+    vs.row[1] = fp.rowB;
+    vs.slot[1] = fp.slotB;
+    vs.row[2] = P_VX;
+    vs.slot[2] = G_VX;
+    vs.row[3] = P_VY;
+    vs.slot[3] = G_VY;
+    vs.row[4] = P_VZ;
+    vs.slot[4] = G_VZ;
+    const bool has_post = D.nb > 0 && has_kind(d.post_kind, d.ntags, IBIS_BC_CONSTANT_FLUX);
+    const bool fused = !d.viscous && !has_post;
+    if (fused && recon) {
+        // record path: ghost records, cell reconstruction, fused face flux + integral
+        if (D.nb > 0) {
+            beg();
+            k_ghost_rec<<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, r->frec);
+            end(0);
+        }
+        const int nlog = (D.N + 63) / 64;
+        beg();
+        if (D.dim == 3) {
+            if (d.limiter == 0)
+                k_cell_rec<3, 0><<<xcd_grid(nlog), 320, 0, st>>>(D, vs, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->frec);
+            else if (d.limiter == 1)
+                k_cell_rec<3, 1><<<xcd_grid(nlog), 320, 0, st>>>(D, vs, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->frec);
+            else
+                k_cell_rec<3, 2><<<xcd_grid(nlog), 320, 0, st>>>(D, vs, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->frec);
+        } else {
+            if (d.limiter == 0)
+                k_cell_rec<2, 0><<<xcd_grid(nlog), 320, 0, st>>>(D, vs, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->frec);
+            else if (d.limiter == 1)
+                k_cell_rec<2, 1><<<xcd_grid(nlog), 320, 0, st>>>(D, vs, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->frec);
+            else
+                k_cell_rec<2, 2><<<xcd_grid(nlog), 320, 0, st>>>(D, vs, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->frec);
+        }
+        end(1);
+        beg();
+        const unsigned g = xcd_grid(D.ntiles);
+        const size_t shm = (size_t)(D.dim + 2) * D.maxtf * sizeof(double);
+        if (D.dim == 3) {
+            switch (d.flux) {
+                case IBIS_FLUX_HANEL: k_face_tile_rec<3, IBIS_FLUX_HANEL><<<g, TILE, shm, st>>>(D, fp, r->frec, dudt); break;
+                case IBIS_FLUX_AUSMDV: k_face_tile_rec<3, IBIS_FLUX_AUSMDV><<<g, TILE, shm, st>>>(D, fp, r->frec, dudt); break;
+                case IBIS_FLUX_LDFSS2: k_face_tile_rec<3, IBIS_FLUX_LDFSS2><<<g, TILE, shm, st>>>(D, fp, r->frec, dudt); break;
+                default: k_face_tile_rec<3, IBIS_FLUX_RUSANOV><<<g, TILE, shm, st>>>(D, fp, r->frec, dudt); break;
+            }
+        } else {
+            switch (d.flux) {
+                case IBIS_FLUX_HANEL: k_face_tile_rec<2, IBIS_FLUX_HANEL><<<g, TILE, shm, st>>>(D, fp, r->frec, dudt); break;
+                case IBIS_FLUX_AUSMDV: k_face_tile_rec<2, IBIS_FLUX_AUSMDV><<<g, TILE, shm, st>>>(D, fp, r->frec, dudt); break;
+                case IBIS_FLUX_LDFSS2: k_face_tile_rec<2, IBIS_FLUX_LDFSS2><<<g, TILE, shm, st>>>(D, fp, r->frec, dudt); break;
+                default: k_face_tile_rec<2, IBIS_FLUX_RUSANOV><<<g, TILE, shm, st>>>(D, fp, r->frec, dudt); break;
+            }
+        }
+        end(2);
+        HIP_TRY(hipGetLastError());
+        return IBIS_OK;
+    }
     if (recon) {
-        VarSet vs;
-        vs.n = 5;
-        vs.ngrad = D.dim == 3 ? 5 : 4;
-        vs.row[0] = fp.rowA;
-        vs.slot[0] = fp.slotA;
-        vs.row[1] = fp.rowB;
-        vs.slot[1] = fp.slotB;
-        vs.row[2] = P_VX;
-        vs.slot[2] = G_VX;
-        vs.row[3] = P_VY;
-        vs.slot[3] = G_VY;
-        vs.row[4] = P_VZ;
-        vs.slot[4] = G_VZ;
         beg();
         if (D.dim == 3)
             launch_cell_grad<3>(D, vs, d.limiter, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, st);
@@ -1285,8 +1651,6 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
     }
     // 3. convective face flux.  Without viscous or flux-override actions the
     //    face flux and the surface integral run as one tiled kernel.
-    const bool has_post = D.nb > 0 && has_kind(d.post_kind, d.ntags, IBIS_BC_CONSTANT_FLUX);
-    const bool fused = !d.viscous && !has_post;
     if (fused) {
         beg();
         if (D.dim == 3)
