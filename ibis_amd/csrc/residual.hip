@@ -529,25 +529,42 @@ __global__ void __launch_bounds__(TILE)
 }
 
 // ---------------------------------------------------------------------------
-// Record path for the 2nd-order residual without viscous or flux-override
-// actions.  The cell kernel reconstructs every face value it owns (value +
-// limiter * (grad . (x_face - x_cell)), FV/convective_flux.cpp:243-255) and
-// stores the 5 reconstruction variables per (cell, face slot) as one 40-byte
-// record; ghost cells get one record each.  The face kernel then reads two
-// records per face and never touches gradients, limiters or centroids, so
-// every byte it reads is used once.
-//   record of valid cell c, slot j : frec[(c*S + j)*5 + v]
-//   record of ghost cell g         : frec[(N*S + g - N)*5 + v]
-//   v = thermo variable A, B, vx, vy, vz;  S = m.maxf
+// Face-state path for the 2nd-order residual without viscous or
+// flux-override actions.
+//
+//   pack[c]  (8 doubles, every cell incl. ghosts): the 5 reconstruction
+//            variables (thermo pair A, B, then vx, vy, vz) and the centroid,
+//            so that a neighbour is read with four 16-byte loads.
+//   FL[f], FR[f] (5 doubles each): the reconstructed values on the left and
+//            right side of face f (value + limiter * grad . (x_f - x_c),
+//            FV/convective_flux.cpp:243-255).
+//
+// The cell kernel computes gradients and limiters and writes the face
+// values it owns; the face kernel reads FL/FR in face order (coalesced),
+// applies the thermo closure, and never touches gradients or centroids.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ size_t rec_index(const DevMesh& m, int c, int slot) {
-    return c < m.N ? (size_t)c * m.maxf + slot : (size_t)m.N * m.maxf + (c - m.N);
+__global__ void __launch_bounds__(BLOCK)
+    k_pack_geom(int Nt, const double* __restrict__ cen, double* __restrict__ pack) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nt) return;
+    pack[(size_t)i * 8 + 5] = cen[i];
+    pack[(size_t)i * 8 + 6] = cen[Nt + i];
+    pack[(size_t)i * 8 + 7] = cen[2 * (size_t)Nt + i];
 }
 
-// Ghost-side records: grad = 0 and limiter = 1 for a ghost cell, so the
-// reconstructed value is value + 1.0 * (0.0*dx + 0.0*dy + 0.0*dz).
 __global__ void __launch_bounds__(BLOCK)
-    k_ghost_rec(DevMesh m, VarSet vs, const double* __restrict__ prim, double* __restrict__ frec) {
+    k_pack(int Nt, VarSet vs, const double* __restrict__ prim, double* __restrict__ pack) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nt) return;
+#pragma unroll
+    for (int v = 0; v < 5; v++) pack[(size_t)i * 8 + v] = prim[(size_t)vs.row[v] * Nt + i];
+}
+
+// Ghost side of boundary faces: grad = 0 and limiter = 1 for a ghost cell,
+// so the reconstructed value is value + 1.0 * (0.0*dx + 0.0*dy + 0.0*dz).
+__global__ void __launch_bounds__(BLOCK)
+    k_ghost_fs(DevMesh m, VarSet vs, const double* __restrict__ prim, double* __restrict__ FL,
+               double* __restrict__ FR) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= m.nb) return;
     const int f = m.b_face[b], g = m.b_ghost[b];
@@ -555,7 +572,7 @@ __global__ void __launch_bounds__(BLOCK)
     const double dx = m.geom[9 * (size_t)F + f] - m.cen[g];
     const double dy = m.geom[10 * (size_t)F + f] - m.cen[Nt + g];
     const double dz = m.geom[11 * (size_t)F + f] - m.cen[2 * (size_t)Nt + g];
-    double* rec = frec + rec_index(m, g, 0) * 5;
+    double* rec = (m.L[f] == g ? FL : FR) + (size_t)f * 5;
 #pragma unroll
     for (int v = 0; v < 5; v++) {
         const double gx = 0.0, gy = 0.0, gz = 0.0;
@@ -563,154 +580,154 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
-// Cell kernel of the record path: wave v of a 320-thread block handles
-// variable v of 64 cells (LSQ gradient, limiter, face values); the face
-// values are staged in LDS and written out as contiguous records.
 template <int DIM, int LIM>
-__global__ void __launch_bounds__(320)
-    k_cell_rec(DevMesh m, VarSet vs, const double* __restrict__ prim, double* __restrict__ grad,
-               double* __restrict__ lim, double eps, double venkat_K, int nlogical,
-               int write_slots, double* __restrict__ frec) {
-    __shared__ double stage[64 * MAXF * 5];
+__global__ void __launch_bounds__(BLOCK)
+    k_cell_fs(DevMesh m, VarSet vs, const double* __restrict__ pack, double* __restrict__ grad,
+              double* __restrict__ lim, double eps, double venkat_K, int nlogical,
+              int write_slots, double* __restrict__ FL, double* __restrict__ FR) {
     const int lb = xcd_block(nlogical);
     if (lb >= nlogical) return;
-    const int v = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
-    const int i = lb * 64 + lane;
-    const int N = m.N, Nt = m.Nt, F = m.F;
-    if (i < N) {
-        const int row = vs.row[v], slot = vs.slot[v];
-        const double* cx = m.cen;
-        const double* cy = m.cen + Nt;
-        const double* cz = m.cen + 2 * (size_t)Nt;
-        const double* u = prim + (size_t)row * Nt;
-        const double xi = cx[i], yi = cy[i], zi = cz[i];
-        const int nf = m.nfaces[i];
-        const double ui = u[i];
-        double gx = 0.0, gy = 0.0, gz = 0.0, umin = ui, umax = ui;
-        if (v < vs.ngrad) {
-            const double r11 = m.lsq[i], r12 = m.lsq[N + i], r22 = m.lsq[2 * (size_t)N + i];
-            double r13 = 0.0, r23 = 0.0, r33 = 0.0;
-            if (DIM == 3) {
-                r13 = m.lsq[3 * (size_t)N + i];
-                r23 = m.lsq[4 * (size_t)N + i];
-                r33 = m.lsq[5 * (size_t)N + i];
+    const int i = lb * BLOCK + threadIdx.x;
+    const int N = m.N, F = m.F;
+    if (i >= N) return;
+    constexpr int NG = DIM == 3 ? 5 : 4;  // variables whose gradient is computed here
+    const double2* P = reinterpret_cast<const double2*>(pack);
+    double ui[5];
+    double xi, yi, zi;
+    {
+        const double2 a0 = P[(size_t)i * 4 + 0], a1 = P[(size_t)i * 4 + 1];
+        const double2 a2 = P[(size_t)i * 4 + 2], a3 = P[(size_t)i * 4 + 3];
+        ui[0] = a0.x;
+        ui[1] = a0.y;
+        ui[2] = a1.x;
+        ui[3] = a1.y;
+        ui[4] = a2.x;
+        xi = a2.y;
+        yi = a3.x;
+        zi = a3.y;
+    }
+    const int nf = m.nfaces[i];
+    const double r11 = m.lsq[i], r12 = m.lsq[N + i], r22 = m.lsq[2 * (size_t)N + i];
+    double r13 = 0.0, r23 = 0.0, r33 = 0.0;
+    if (DIM == 3) {
+        r13 = m.lsq[3 * (size_t)N + i];
+        r23 = m.lsq[4 * (size_t)N + i];
+        r33 = m.lsq[5 * (size_t)N + i];
+    }
+    const double beta = (r12 * r23 - r13 * r23) / (r11 * r22);  // GR/gradient.h:89 (sic)
+    const double r11s = r11 * r11;
+    const double i22 = 1.0 / (r22 * r22);
+    const double q12 = r12 / r11;
+    const double i33 = 1.0 / (r33 * r33);
+    const double q23 = r23 / r22;
+    double gx[5], gy[5], gz[5], umin[5], umax[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        gx[v] = gy[v] = gz[v] = 0.0;
+        umin[v] = umax[v] = ui[v];
+    }
+    for (int j = 0; j < nf; j++) {
+        const int n = m.ell_nbr[(size_t)j * N + i];
+        const double2 b0 = P[(size_t)n * 4 + 0], b1 = P[(size_t)n * 4 + 1];
+        const double2 b2 = P[(size_t)n * 4 + 2], b3 = P[(size_t)n * 4 + 3];
+        const double un[5] = {b0.x, b0.y, b1.x, b1.y, b2.x};
+        const double dx = b2.y - xi;
+        const double dy = b3.x - yi;
+        const double dz = b3.y - zi;
+        const double alpha_1 = dx / r11s;
+        const double alpha_2 = i22 * (dy - q12 * dx);
+        double alpha_3 = 0.0;
+        if (DIM == 3) alpha_3 = i33 * (dz - q23 * dy + beta * dx);
+        const double w_1 = alpha_1 - q12 * alpha_2 + beta * alpha_3;
+        const double w_2 = alpha_2 - q23 * alpha_3;
+        const double w_3 = alpha_3;
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            if (v < NG) {
+                const double diff = un[v] - ui[v];
+                gx[v] += w_1 * diff;
+                gy[v] += w_2 * diff;
+                gz[v] += w_3 * diff;
             }
-            const double beta = (r12 * r23 - r13 * r23) / (r11 * r22);  // GR/gradient.h:89 (sic)
-            const double r11s = r11 * r11;
-            const double i22 = 1.0 / (r22 * r22);
-            const double q12 = r12 / r11;
-            const double i33 = 1.0 / (r33 * r33);
-            const double q23 = r23 / r22;
-            for (int j = 0; j < nf; j++) {
-                const int n = m.ell_nbr[(size_t)j * N + i];
-                const double dx = cx[n] - xi;
-                const double dy = cy[n] - yi;
-                const double dz = cz[n] - zi;
-                const double alpha_1 = dx / r11s;
-                const double alpha_2 = i22 * (dy - q12 * dx);
-                double alpha_3 = 0.0;
-                if (DIM == 3) alpha_3 = i33 * (dz - q23 * dy + beta * dx);
-                const double w_1 = alpha_1 - q12 * alpha_2 + beta * alpha_3;
-                const double w_2 = alpha_2 - q23 * alpha_3;
-                const double w_3 = alpha_3;
-                const double un = u[n];
-                const double diff = un - ui;
-                gx += w_1 * diff;
-                gy += w_2 * diff;
-                gz += w_3 * diff;
-                umin = ibis::kmin(umin, un);
-                umax = ibis::kmax(umax, un);
-            }
-        } else {
-            for (int j = 0; j < nf; j++) {
-                const double un = u[m.ell_nbr[(size_t)j * N + i]];
-                umin = ibis::kmin(umin, un);
-                umax = ibis::kmax(umax, un);
-            }
+            umin[v] = ibis::kmin(umin[v], un[v]);
+            umax[v] = ibis::kmax(umax[v], un[v]);
         }
-        double* g = grad + (size_t)slot * 3 * N;
-        if (v < vs.ngrad) {
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        double* g = grad + (size_t)vs.slot[v] * 3 * N;
+        if (v < NG) {
             if (write_slots) {
-                g[i] = gx;
-                g[N + i] = gy;
-                g[2 * (size_t)N + i] = gz;
+                g[i] = gx[v];
+                g[N + i] = gy[v];
+                g[2 * (size_t)N + i] = gz[v];
             }
-        } else {
-            gx = g[i];
-            gy = g[N + i];
-            gz = g[2 * (size_t)N + i];
+        } else {  // 2D vz: the gradient stored in its slot
+            gx[v] = g[i];
+            gy[v] = g[N + i];
+            gz[v] = g[2 * (size_t)N + i];
         }
-        double ddx[MAXF], ddy[MAXF], ddz[MAXF];
-#pragma unroll
-        for (int j = 0; j < MAXF; j++) {
-            if (j < nf) {
-                const int f = m.ell_face[(size_t)j * N + i];
-                ddx[j] = m.geom[9 * (size_t)F + f] - xi;
-                ddy[j] = m.geom[10 * (size_t)F + f] - yi;
-                ddz[j] = m.geom[11 * (size_t)F + f] - zi;
-            } else {
-                ddx[j] = ddy[j] = ddz[j] = 0.0;
-            }
+    }
+    double phi[5] = {1.0, 1.0, 1.0, 1.0, 1.0};
+    if (LIM != 0) {
+        double eps2 = 0.0;
+        if (LIM == 2) {
+            const double V = m.vol[i];
+            const double K3 = venkat_K * venkat_K * venkat_K;
+            eps2 = (DIM == 3) ? K3 * V : K3 * (V * sqrt(V));
         }
-        double phi = 1.0;
-        if (LIM != 0) {
-            double eps2 = 0.0;
-            if (LIM == 2) {
-                const double V = m.vol[i];
-                const double K3 = venkat_K * venkat_K * venkat_K;
-                eps2 = (DIM == 3) ? K3 * V : K3 * (V * sqrt(V));
-            }
+        for (int j = 0; j < nf; j++) {
+            const int f = m.ell_face[(size_t)j * N + i];
+            const double dx = m.geom[9 * (size_t)F + f] - xi;
+            const double dy = m.geom[10 * (size_t)F + f] - yi;
+            const double dz = m.geom[11 * (size_t)F + f] - zi;
 #pragma unroll
-            for (int j = 0; j < MAXF; j++) {
-                if (j >= nf) break;
-                double d2 = gx * ddx[j] + gy * ddy[j] + gz * ddz[j];
+            for (int v = 0; v < 5; v++) {
+                double d2 = gx[v] * dx + gy[v] * dy + gz[v] * dz;
                 if (LIM == 1) {  // Barth-Jespersen, FV/limiter.cpp:289-303
                     const int sg = (d2 > 0) - (d2 < 0);
                     d2 = sg * (fabs(d2) + eps);
                     if (sg > 0)
-                        phi = ibis::kmin(phi, (umax - ui) / d2);
+                        phi[v] = ibis::kmin(phi[v], (umax[v] - ui[v]) / d2);
                     else if (sg < 0)
-                        phi = ibis::kmin(phi, (umin - ui) / d2);
+                        phi[v] = ibis::kmin(phi[v], (umin[v] - ui[v]) / d2);
                 } else {  // Venkatakrishnan
                     double d1;
                     if (d2 > 0.0)
-                        d1 = umax - ui;
+                        d1 = umax[v] - ui[v];
                     else if (d2 < 0.0)
-                        d1 = umin - ui;
+                        d1 = umin[v] - ui[v];
                     else
                         continue;
                     const double d1s = d1 * d1;
                     const double d2s = d2 * d2;
                     const double num = d1s + eps2 + 2.0 * (d1 * d2);
                     const double den = d1s + 2.0 * d2s + d1 * d2 + eps2;
-                    phi = ibis::kmin(phi, num / den);
+                    phi[v] = ibis::kmin(phi[v], num / den);
                 }
             }
-            if (write_slots) lim[(size_t)slot * N + i] = phi;
         }
+        if (write_slots) {
 #pragma unroll
-        for (int j = 0; j < MAXF; j++) {
-            if (j >= nf) break;
-            stage[(lane * MAXF + j) * 5 + v] = ui + phi * (gx * ddx[j] + gy * ddy[j] + gz * ddz[j]);
+            for (int v = 0; v < 5; v++) lim[(size_t)vs.slot[v] * N + i] = phi[v];
         }
     }
-    __syncthreads();
-    const int S = m.maxf;
-    const int nc = min(64, N - lb * 64);
-    const int total = nc * S * 5;
-    double* out = frec + (size_t)lb * 64 * S * 5;
-    for (int k = threadIdx.x; k < total; k += blockDim.x) {
-        const int cl = k / (S * 5), rem = k - cl * (S * 5);
-        out[k] = stage[cl * MAXF * 5 + rem];
+    for (int j = 0; j < nf; j++) {
+        const int f = m.ell_face[(size_t)j * N + i];
+        const double dx = m.geom[9 * (size_t)F + f] - xi;
+        const double dy = m.geom[10 * (size_t)F + f] - yi;
+        const double dz = m.geom[11 * (size_t)F + f] - zi;
+        double* rec = (m.ell_sign[(size_t)j * N + i] > 0 ? FL : FR) + (size_t)f * 5;
+#pragma unroll
+        for (int v = 0; v < 5; v++) rec[v] = ui[v] + phi[v] * (gx[v] * dx + gy[v] * dy + gz[v] * dz);
     }
 }
 
-// Fused face flux + surface integral from face-state records.
+// Fused face flux + surface integral from the face states.
 template <int DIM, int FLUX>
 __global__ void __launch_bounds__(TILE)
-    k_face_tile_rec(DevMesh m, FaceParams fp, const double* __restrict__ frec,
-                    double* __restrict__ dudt) {
+    k_face_tile_fs(DevMesh m, FaceParams fp, const double* __restrict__ FL,
+                   const double* __restrict__ FR, double* __restrict__ dudt) {
     extern __shared__ double sflux[];  // NC x maxtf
     const int t = xcd_block(m.ntiles);
     if (t >= m.ntiles) return;
@@ -721,13 +738,10 @@ __global__ void __launch_bounds__(TILE)
     const int nfl = m.tile_face_off[t + 1] - f0;
     for (int k = threadIdx.x; k < nfl; k += TILE) {
         const int f = m.tile_faces[f0 + k];
-        const int l = m.L[f], r = m.R[f];
-        const double* rl = frec + rec_index(m, l, m.slotL[f]) * 5;
-        const double* rr = frec + rec_index(m, r, m.slotR[f]) * 5;
         double Ls[NPRIM], Rs[NPRIM];
 #pragma unroll
         for (int side = 0; side < 2; side++) {
-            const double* rec = side ? rr : rl;
+            const double* rec = (side ? FR : FL) + (size_t)f * 5;
             double* st = side ? Rs : Ls;
             const double a = rec[0], b = rec[1];
             double rho = 0.0, p = 0.0, T = 0.0, e = 0.0;
@@ -807,7 +821,7 @@ __global__ void __launch_bounds__(TILE)
 #pragma unroll
         for (int q = 0; q < NC; q++) {
             const double fa = sflux[q * S + k];
-            d[q] += (s > 0) ? -fa : fa;
+            d[q] += (s > 0) ? -fa : fa;  // == flux * (-area * outsign), exactly
         }
     }
     const double V = m.vol[c];
@@ -1169,6 +1183,15 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
         if (d->face_left[f] < 0 || d->face_left[f] >= Nt || d->face_right[f] < 0 ||
             d->face_right[f] >= Nt)
             return ibis::fail(IBIS_ERR_INVALID, "ibis_mesh_create: face cell index out of range");
+    // outsign +1 <=> the cell is the face's left cell (GR/grid.h:204-212)
+    for (int64_t c = 0; c < N; c++)
+        for (int64_t j = d->cell_face_off[c]; j < d->cell_face_off[c + 1]; j++) {
+            const int64_t f = d->cell_faces[j];
+            const bool left = d->face_left[f] == c;
+            if (!(left ? d->cell_outsigns[j] == 1 : (d->cell_outsigns[j] == -1 && d->face_right[f] == c)))
+                return ibis::fail(IBIS_ERR_INVALID,
+                                  "ibis_mesh_create: outsigns disagree with face left/right cells");
+        }
     const int64_t nb = d->tag_off[d->ntags];
     std::vector<int> b_face(nb), b_ghost(nb), b_valid(nb), b_tag(nb);
     for (int t = 0; t < d->ntags; t++)
@@ -1310,7 +1333,9 @@ struct ibis_residual {
     double* grad = nullptr;
     double* lim = nullptr;
     double* flux = nullptr;
-    double* frec = nullptr;  // face-state records of the fused 2nd-order path
+    double* pack = nullptr;  // per-cell AoS values + centroid (face-state path)
+    double* FL = nullptr;    // left / right face states (face-state path)
+    double* FR = nullptr;
     double* face_rho = nullptr;
     double* face_T = nullptr;
     int* kinds = nullptr;    // pre, visc, post
@@ -1337,7 +1362,7 @@ struct ibis_residual {
 
 void ibis_residual_destroy(ibis_residual* r) {
     if (!r) return;
-    for (void* p : {(void*)r->grad, (void*)r->lim, (void*)r->flux, (void*)r->frec, (void*)r->face_rho,
+    for (void* p : {(void*)r->grad, (void*)r->lim, (void*)r->flux, (void*)r->pack, (void*)r->FL, (void*)r->FR, (void*)r->face_rho,
                     (void*)r->face_T, (void*)r->kinds, (void*)r->pars, (void*)r->partial,
                     (void*)r->dt_dev, (void*)r->count_dev, (void*)r->k_tmp, (void*)r->flow_tmp})
         if (p) (void)hipFree(p);
@@ -1390,7 +1415,9 @@ ibis_status ibis_residual_create(const ibis_mesh* m, const ibis_fv_desc* desc,
     AL(r->grad, sizeof(double) * NGRAD * 3 * N);
     AL(r->lim, sizeof(double) * NGRAD * N);
     AL(r->flux, sizeof(double) * nc * F);
-    AL(r->frec, sizeof(double) * 5 * ((size_t)D.N * D.maxf + D.G));
+    AL(r->pack, sizeof(double) * 8 * (size_t)D.Nt);
+    AL(r->FL, sizeof(double) * 5 * F);
+    AL(r->FR, sizeof(double) * 5 * F);
     if (d.viscous) {
         AL(r->face_rho, sizeof(double) * F);
         AL(r->face_T, sizeof(double) * F);
@@ -1417,6 +1444,11 @@ ibis_status ibis_residual_create(const ibis_mesh* m, const ibis_fv_desc* desc,
         hipHostMalloc((void**)&r->host_buf, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         ibis_residual_destroy(r);
         return ibis::fail(IBIS_ERR_HIP, "ibis_residual_create: upload failed");
+    }
+    k_pack_geom<<<nblocks(D.Nt), BLOCK>>>(D.Nt, D.cen, r->pack);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        ibis_residual_destroy(r);
+        return ibis::fail(IBIS_ERR_HIP, "ibis_residual_create: pack initialisation failed");
     }
     *out = r;
     return IBIS_OK;
@@ -1595,28 +1627,29 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
     const bool has_post = D.nb > 0 && has_kind(d.post_kind, d.ntags, IBIS_BC_CONSTANT_FLUX);
     const bool fused = !d.viscous && !has_post;
     if (fused && recon) {
-        // record path: ghost records, cell reconstruction, fused face flux + integral
-        if (D.nb > 0) {
-            beg();
-            k_ghost_rec<<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, r->frec);
-            end(0);
-        }
-        const int nlog = (D.N + 63) / 64;
+        // face-state path: pack, ghost face states, cell reconstruction,
+        // fused face flux + integral
+        beg();
+        k_pack<<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, vs, prim, r->pack);
+        if (D.nb > 0) k_ghost_fs<<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
+        end(0);
+        const int nlog = (D.N + BLOCK - 1) / BLOCK;
+        const unsigned gc = xcd_grid(nlog);
         beg();
         if (D.dim == 3) {
             if (d.limiter == 0)
-                k_cell_rec<3, 0><<<xcd_grid(nlog), 320, 0, st>>>(D, vs, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->frec);
+                k_cell_fs<3, 0><<<gc, BLOCK, 0, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR);
             else if (d.limiter == 1)
-                k_cell_rec<3, 1><<<xcd_grid(nlog), 320, 0, st>>>(D, vs, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->frec);
+                k_cell_fs<3, 1><<<gc, BLOCK, 0, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR);
             else
-                k_cell_rec<3, 2><<<xcd_grid(nlog), 320, 0, st>>>(D, vs, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->frec);
+                k_cell_fs<3, 2><<<gc, BLOCK, 0, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR);
         } else {
             if (d.limiter == 0)
-                k_cell_rec<2, 0><<<xcd_grid(nlog), 320, 0, st>>>(D, vs, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->frec);
+                k_cell_fs<2, 0><<<gc, BLOCK, 0, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR);
             else if (d.limiter == 1)
-                k_cell_rec<2, 1><<<xcd_grid(nlog), 320, 0, st>>>(D, vs, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->frec);
+                k_cell_fs<2, 1><<<gc, BLOCK, 0, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR);
             else
-                k_cell_rec<2, 2><<<xcd_grid(nlog), 320, 0, st>>>(D, vs, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->frec);
+                k_cell_fs<2, 2><<<gc, BLOCK, 0, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR);
         }
         end(1);
         beg();
@@ -1624,17 +1657,17 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
         const size_t shm = (size_t)(D.dim + 2) * D.maxtf * sizeof(double);
         if (D.dim == 3) {
             switch (d.flux) {
-                case IBIS_FLUX_HANEL: k_face_tile_rec<3, IBIS_FLUX_HANEL><<<g, TILE, shm, st>>>(D, fp, r->frec, dudt); break;
-                case IBIS_FLUX_AUSMDV: k_face_tile_rec<3, IBIS_FLUX_AUSMDV><<<g, TILE, shm, st>>>(D, fp, r->frec, dudt); break;
-                case IBIS_FLUX_LDFSS2: k_face_tile_rec<3, IBIS_FLUX_LDFSS2><<<g, TILE, shm, st>>>(D, fp, r->frec, dudt); break;
-                default: k_face_tile_rec<3, IBIS_FLUX_RUSANOV><<<g, TILE, shm, st>>>(D, fp, r->frec, dudt); break;
+                case IBIS_FLUX_HANEL: k_face_tile_fs<3, IBIS_FLUX_HANEL><<<g, TILE, shm, st>>>(D, fp, r->FL, r->FR, dudt); break;
+                case IBIS_FLUX_AUSMDV: k_face_tile_fs<3, IBIS_FLUX_AUSMDV><<<g, TILE, shm, st>>>(D, fp, r->FL, r->FR, dudt); break;
+                case IBIS_FLUX_LDFSS2: k_face_tile_fs<3, IBIS_FLUX_LDFSS2><<<g, TILE, shm, st>>>(D, fp, r->FL, r->FR, dudt); break;
+                default: k_face_tile_fs<3, IBIS_FLUX_RUSANOV><<<g, TILE, shm, st>>>(D, fp, r->FL, r->FR, dudt); break;
             }
         } else {
             switch (d.flux) {
-                case IBIS_FLUX_HANEL: k_face_tile_rec<2, IBIS_FLUX_HANEL><<<g, TILE, shm, st>>>(D, fp, r->frec, dudt); break;
-                case IBIS_FLUX_AUSMDV: k_face_tile_rec<2, IBIS_FLUX_AUSMDV><<<g, TILE, shm, st>>>(D, fp, r->frec, dudt); break;
-                case IBIS_FLUX_LDFSS2: k_face_tile_rec<2, IBIS_FLUX_LDFSS2><<<g, TILE, shm, st>>>(D, fp, r->frec, dudt); break;
-                default: k_face_tile_rec<2, IBIS_FLUX_RUSANOV><<<g, TILE, shm, st>>>(D, fp, r->frec, dudt); break;
+                case IBIS_FLUX_HANEL: k_face_tile_fs<2, IBIS_FLUX_HANEL><<<g, TILE, shm, st>>>(D, fp, r->FL, r->FR, dudt); break;
+                case IBIS_FLUX_AUSMDV: k_face_tile_fs<2, IBIS_FLUX_AUSMDV><<<g, TILE, shm, st>>>(D, fp, r->FL, r->FR, dudt); break;
+                case IBIS_FLUX_LDFSS2: k_face_tile_fs<2, IBIS_FLUX_LDFSS2><<<g, TILE, shm, st>>>(D, fp, r->FL, r->FR, dudt); break;
+                default: k_face_tile_fs<2, IBIS_FLUX_RUSANOV><<<g, TILE, shm, st>>>(D, fp, r->FL, r->FR, dudt); break;
             }
         }
         end(2);
