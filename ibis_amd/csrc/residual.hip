@@ -562,6 +562,14 @@ __global__ void __launch_bounds__(BLOCK)
 
 // Ghost side of boundary faces: grad = 0 and limiter = 1 for a ghost cell,
 // so the reconstructed value is value + 1.0 * (0.0*dx + 0.0*dy + 0.0*dz).
+// LAYOUT 0: face-indexed FL[f] / FR[f].  LAYOUT 1: cell-major records
+// (record of valid cell c, slot j at (c*S + j); ghost g at (N*S + g - N),
+// S = maxf) in FL, read through the per-face slot of each side.
+__device__ __forceinline__ size_t rec_index(const DevMesh& m, int c, int slot) {
+    return c < m.N ? (size_t)c * m.maxf + slot : (size_t)m.N * m.maxf + (c - m.N);
+}
+
+template <int LAYOUT>
 __global__ void __launch_bounds__(BLOCK)
     k_ghost_fs(DevMesh m, VarSet vs, const double* __restrict__ prim, double* __restrict__ FL,
                double* __restrict__ FR) {
@@ -572,7 +580,8 @@ __global__ void __launch_bounds__(BLOCK)
     const double dx = m.geom[9 * (size_t)F + f] - m.cen[g];
     const double dy = m.geom[10 * (size_t)F + f] - m.cen[Nt + g];
     const double dz = m.geom[11 * (size_t)F + f] - m.cen[2 * (size_t)Nt + g];
-    double* rec = (m.L[f] == g ? FL : FR) + (size_t)f * 5;
+    double* rec = LAYOUT == 0 ? (m.L[f] == g ? FL : FR) + (size_t)f * 5
+                              : FL + rec_index(m, g, 0) * 5;
 #pragma unroll
     for (int v = 0; v < 5; v++) {
         const double gx = 0.0, gy = 0.0, gz = 0.0;
@@ -580,11 +589,27 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
-template <int DIM, int LIM>
+// Offsets x_face - x_cell per (cell, face slot), column-major like the other
+// cell lists: offs[(j*3 + k)*N + i].  Same expression as the kernels use.
+__global__ void __launch_bounds__(BLOCK) k_offsets(DevMesh m, double* __restrict__ offs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int N = m.N, F = m.F, Nt = m.Nt;
+    if (i >= N) return;
+    const double xi = m.cen[i], yi = m.cen[Nt + i], zi = m.cen[2 * (size_t)Nt + i];
+    for (int j = 0; j < m.nfaces[i]; j++) {
+        const int f = m.ell_face[(size_t)j * N + i];
+        offs[(size_t)(j * 3 + 0) * N + i] = m.geom[9 * (size_t)F + f] - xi;
+        offs[(size_t)(j * 3 + 1) * N + i] = m.geom[10 * (size_t)F + f] - yi;
+        offs[(size_t)(j * 3 + 2) * N + i] = m.geom[11 * (size_t)F + f] - zi;
+    }
+}
+
+template <int DIM, int LIM, int LAYOUT, bool OFFS>
 __global__ void __launch_bounds__(BLOCK)
     k_cell_fs(DevMesh m, VarSet vs, const double* __restrict__ pack, double* __restrict__ grad,
               double* __restrict__ lim, double eps, double venkat_K, int nlogical,
-              int write_slots, double* __restrict__ FL, double* __restrict__ FR) {
+              int write_slots, double* __restrict__ FL, double* __restrict__ FR,
+              const double* __restrict__ offs) {
     const int lb = xcd_block(nlogical);
     if (lb >= nlogical) return;
     const int i = lb * BLOCK + threadIdx.x;
@@ -677,10 +702,17 @@ __global__ void __launch_bounds__(BLOCK)
             eps2 = (DIM == 3) ? K3 * V : K3 * (V * sqrt(V));
         }
         for (int j = 0; j < nf; j++) {
-            const int f = m.ell_face[(size_t)j * N + i];
-            const double dx = m.geom[9 * (size_t)F + f] - xi;
-            const double dy = m.geom[10 * (size_t)F + f] - yi;
-            const double dz = m.geom[11 * (size_t)F + f] - zi;
+            double dx, dy, dz;
+            if (OFFS) {
+                dx = offs[(size_t)(j * 3 + 0) * N + i];
+                dy = offs[(size_t)(j * 3 + 1) * N + i];
+                dz = offs[(size_t)(j * 3 + 2) * N + i];
+            } else {
+                const int f = m.ell_face[(size_t)j * N + i];
+                dx = m.geom[9 * (size_t)F + f] - xi;
+                dy = m.geom[10 * (size_t)F + f] - yi;
+                dz = m.geom[11 * (size_t)F + f] - zi;
+            }
 #pragma unroll
             for (int v = 0; v < 5; v++) {
                 double d2 = gx[v] * dx + gy[v] * dy + gz[v] * dz;
@@ -714,17 +746,25 @@ __global__ void __launch_bounds__(BLOCK)
     }
     for (int j = 0; j < nf; j++) {
         const int f = m.ell_face[(size_t)j * N + i];
-        const double dx = m.geom[9 * (size_t)F + f] - xi;
-        const double dy = m.geom[10 * (size_t)F + f] - yi;
-        const double dz = m.geom[11 * (size_t)F + f] - zi;
-        double* rec = (m.ell_sign[(size_t)j * N + i] > 0 ? FL : FR) + (size_t)f * 5;
+        double dx, dy, dz;
+        if (OFFS) {
+            dx = offs[(size_t)(j * 3 + 0) * N + i];
+            dy = offs[(size_t)(j * 3 + 1) * N + i];
+            dz = offs[(size_t)(j * 3 + 2) * N + i];
+        } else {
+            dx = m.geom[9 * (size_t)F + f] - xi;
+            dy = m.geom[10 * (size_t)F + f] - yi;
+            dz = m.geom[11 * (size_t)F + f] - zi;
+        }
+        double* rec = LAYOUT == 0 ? (m.ell_sign[(size_t)j * N + i] > 0 ? FL : FR) + (size_t)f * 5
+                                  : FL + ((size_t)i * m.maxf + j) * 5;
 #pragma unroll
         for (int v = 0; v < 5; v++) rec[v] = ui[v] + phi[v] * (gx[v] * dx + gy[v] * dy + gz[v] * dz);
     }
 }
 
 // Fused face flux + surface integral from the face states.
-template <int DIM, int FLUX>
+template <int DIM, int FLUX, int LAYOUT>
 __global__ void __launch_bounds__(TILE)
     k_face_tile_fs(DevMesh m, FaceParams fp, const double* __restrict__ FL,
                    const double* __restrict__ FR, double* __restrict__ dudt) {
@@ -741,7 +781,11 @@ __global__ void __launch_bounds__(TILE)
         double Ls[NPRIM], Rs[NPRIM];
 #pragma unroll
         for (int side = 0; side < 2; side++) {
-            const double* rec = (side ? FR : FL) + (size_t)f * 5;
+            const double* rec;
+            if (LAYOUT == 0)
+                rec = (side ? FR : FL) + (size_t)f * 5;
+            else
+                rec = FL + (side ? rec_index(m, m.R[f], m.slotR[f]) : rec_index(m, m.L[f], m.slotL[f])) * 5;
             double* st = side ? Rs : Ls;
             const double a = rec[0], b = rec[1];
             double rho = 0.0, p = 0.0, T = 0.0, e = 0.0;
@@ -1336,6 +1380,9 @@ struct ibis_residual {
     double* pack = nullptr;  // per-cell AoS values + centroid (face-state path)
     double* FL = nullptr;    // left / right face states (face-state path)
     double* FR = nullptr;
+    double* offs = nullptr;  // precomputed face offsets (option precomputed_offsets)
+    int opt_layout = 0;      // option face_state_layout
+    int opt_offsets = 0;     // option precomputed_offsets
     double* face_rho = nullptr;
     double* face_T = nullptr;
     int* kinds = nullptr;    // pre, visc, post
@@ -1362,7 +1409,7 @@ struct ibis_residual {
 
 void ibis_residual_destroy(ibis_residual* r) {
     if (!r) return;
-    for (void* p : {(void*)r->grad, (void*)r->lim, (void*)r->flux, (void*)r->pack, (void*)r->FL, (void*)r->FR, (void*)r->face_rho,
+    for (void* p : {(void*)r->grad, (void*)r->lim, (void*)r->flux, (void*)r->pack, (void*)r->FL, (void*)r->FR, (void*)r->offs, (void*)r->face_rho,
                     (void*)r->face_T, (void*)r->kinds, (void*)r->pars, (void*)r->partial,
                     (void*)r->dt_dev, (void*)r->count_dev, (void*)r->k_tmp, (void*)r->flow_tmp})
         if (p) (void)hipFree(p);
@@ -1416,7 +1463,7 @@ ibis_status ibis_residual_create(const ibis_mesh* m, const ibis_fv_desc* desc,
     AL(r->lim, sizeof(double) * NGRAD * N);
     AL(r->flux, sizeof(double) * nc * F);
     AL(r->pack, sizeof(double) * 8 * (size_t)D.Nt);
-    AL(r->FL, sizeof(double) * 5 * F);
+    AL(r->FL, sizeof(double) * 5 * std::max<size_t>(F, (size_t)D.N * D.maxf + D.G));
     AL(r->FR, sizeof(double) * 5 * F);
     if (d.viscous) {
         AL(r->face_rho, sizeof(double) * F);
@@ -1539,6 +1586,58 @@ void launch_face_flux(const DevMesh& D, int flux_kind, const FaceParams& fp, boo
     }
 }
 
+template <int DIM, int LIM, int LAYOUT, bool OFFS>
+void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d, ibis_residual* r,
+                      int nlog, hipStream_t st) {
+    k_cell_fs<DIM, LIM, LAYOUT, OFFS><<<xcd_grid(nlog), BLOCK, 0, st>>>(
+        D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR, r->offs);
+}
+
+template <int DIM, int LIM>
+void launch_cell_fs_l(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d, ibis_residual* r,
+                      int nlog, hipStream_t st) {
+    const int sel = r->opt_layout * 2 + (r->opt_offsets ? 1 : 0);
+    switch (sel) {
+        case 0: launch_cell_fs_t<DIM, LIM, 0, false>(D, vs, d, r, nlog, st); break;
+        case 1: launch_cell_fs_t<DIM, LIM, 0, true>(D, vs, d, r, nlog, st); break;
+        case 2: launch_cell_fs_t<DIM, LIM, 1, false>(D, vs, d, r, nlog, st); break;
+        default: launch_cell_fs_t<DIM, LIM, 1, true>(D, vs, d, r, nlog, st); break;
+    }
+}
+
+template <int DIM>
+void launch_cell_fs(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d, ibis_residual* r,
+                    int nlog, const double* /*prim*/, hipStream_t st) {
+    if (d.limiter == 0)
+        launch_cell_fs_l<DIM, 0>(D, vs, d, r, nlog, st);
+    else if (d.limiter == 1)
+        launch_cell_fs_l<DIM, 1>(D, vs, d, r, nlog, st);
+    else
+        launch_cell_fs_l<DIM, 2>(D, vs, d, r, nlog, st);
+}
+
+template <int DIM, int FLUX>
+void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, const double* FL,
+                      const double* FR, double* dudt, hipStream_t st) {
+    const unsigned g = xcd_grid(D.ntiles);
+    const size_t shm = (size_t)(DIM + 2) * D.maxtf * sizeof(double);
+    if (layout == 0)
+        k_face_tile_fs<DIM, FLUX, 0><<<g, TILE, shm, st>>>(D, fp, FL, FR, dudt);
+    else
+        k_face_tile_fs<DIM, FLUX, 1><<<g, TILE, shm, st>>>(D, fp, FL, FR, dudt);
+}
+
+template <int DIM>
+void launch_face_fs(const DevMesh& D, const FaceParams& fp, int flux, int layout,
+                    const double* FL, const double* FR, double* dudt, hipStream_t st) {
+    switch (flux) {
+        case IBIS_FLUX_HANEL: launch_face_fs_f<DIM, IBIS_FLUX_HANEL>(D, fp, layout, FL, FR, dudt, st); break;
+        case IBIS_FLUX_AUSMDV: launch_face_fs_f<DIM, IBIS_FLUX_AUSMDV>(D, fp, layout, FL, FR, dudt, st); break;
+        case IBIS_FLUX_LDFSS2: launch_face_fs_f<DIM, IBIS_FLUX_LDFSS2>(D, fp, layout, FL, FR, dudt, st); break;
+        default: launch_face_fs_f<DIM, IBIS_FLUX_RUSANOV>(D, fp, layout, FL, FR, dudt, st); break;
+    }
+}
+
 // thermo reconstruction pair: (slot, prim row) of the two variables,
 // FV/convective_flux.cpp:181-209
 void thermo_pair(int thermo, int* sA, int* rA, int* sB, int* rB) {
@@ -1631,45 +1730,25 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
         // fused face flux + integral
         beg();
         k_pack<<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, vs, prim, r->pack);
-        if (D.nb > 0) k_ghost_fs<<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
+        if (D.nb > 0) {
+            if (r->opt_layout == 0)
+                k_ghost_fs<0><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
+            else
+                k_ghost_fs<1><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
+        }
         end(0);
         const int nlog = (D.N + BLOCK - 1) / BLOCK;
-        const unsigned gc = xcd_grid(nlog);
         beg();
-        if (D.dim == 3) {
-            if (d.limiter == 0)
-                k_cell_fs<3, 0><<<gc, BLOCK, 0, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR);
-            else if (d.limiter == 1)
-                k_cell_fs<3, 1><<<gc, BLOCK, 0, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR);
-            else
-                k_cell_fs<3, 2><<<gc, BLOCK, 0, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR);
-        } else {
-            if (d.limiter == 0)
-                k_cell_fs<2, 0><<<gc, BLOCK, 0, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR);
-            else if (d.limiter == 1)
-                k_cell_fs<2, 1><<<gc, BLOCK, 0, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR);
-            else
-                k_cell_fs<2, 2><<<gc, BLOCK, 0, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR);
-        }
+        if (D.dim == 3)
+            launch_cell_fs<3>(D, vs, d, r, nlog, prim, st);
+        else
+            launch_cell_fs<2>(D, vs, d, r, nlog, prim, st);
         end(1);
         beg();
-        const unsigned g = xcd_grid(D.ntiles);
-        const size_t shm = (size_t)(D.dim + 2) * D.maxtf * sizeof(double);
-        if (D.dim == 3) {
-            switch (d.flux) {
-                case IBIS_FLUX_HANEL: k_face_tile_fs<3, IBIS_FLUX_HANEL><<<g, TILE, shm, st>>>(D, fp, r->FL, r->FR, dudt); break;
-                case IBIS_FLUX_AUSMDV: k_face_tile_fs<3, IBIS_FLUX_AUSMDV><<<g, TILE, shm, st>>>(D, fp, r->FL, r->FR, dudt); break;
-                case IBIS_FLUX_LDFSS2: k_face_tile_fs<3, IBIS_FLUX_LDFSS2><<<g, TILE, shm, st>>>(D, fp, r->FL, r->FR, dudt); break;
-                default: k_face_tile_fs<3, IBIS_FLUX_RUSANOV><<<g, TILE, shm, st>>>(D, fp, r->FL, r->FR, dudt); break;
-            }
-        } else {
-            switch (d.flux) {
-                case IBIS_FLUX_HANEL: k_face_tile_fs<2, IBIS_FLUX_HANEL><<<g, TILE, shm, st>>>(D, fp, r->FL, r->FR, dudt); break;
-                case IBIS_FLUX_AUSMDV: k_face_tile_fs<2, IBIS_FLUX_AUSMDV><<<g, TILE, shm, st>>>(D, fp, r->FL, r->FR, dudt); break;
-                case IBIS_FLUX_LDFSS2: k_face_tile_fs<2, IBIS_FLUX_LDFSS2><<<g, TILE, shm, st>>>(D, fp, r->FL, r->FR, dudt); break;
-                default: k_face_tile_fs<2, IBIS_FLUX_RUSANOV><<<g, TILE, shm, st>>>(D, fp, r->FL, r->FR, dudt); break;
-            }
-        }
+        if (D.dim == 3)
+            launch_face_fs<3>(D, fp, d.flux, r->opt_layout, r->FL, r->FR, dudt, st);
+        else
+            launch_face_fs<2>(D, fp, d.flux, r->opt_layout, r->FL, r->FR, dudt, st);
         end(2);
         HIP_TRY(hipGetLastError());
         return IBIS_OK;
@@ -1762,6 +1841,29 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
     end(7);
     HIP_TRY(hipGetLastError());
     return IBIS_OK;
+}
+
+ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t value) {
+    if (!r || !key) return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_set_option: null argument");
+    const std::string k(key);
+    if (k == "face_state_layout") {
+        if (value != 0 && value != 1)
+            return ibis::fail(IBIS_ERR_INVALID, "face_state_layout must be 0 or 1");
+        r->opt_layout = (int)value;
+        return IBIS_OK;
+    }
+    if (k == "precomputed_offsets") {
+        if (value && !r->offs) {
+            const DevMesh& D = r->m->d;
+            HIP_TRY(hipMalloc(&r->offs, sizeof(double) * 3 * (size_t)D.maxf * D.N));
+            k_offsets<<<nblocks(D.N), BLOCK>>>(D, r->offs);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipDeviceSynchronize());
+        }
+        r->opt_offsets = value ? 1 : 0;
+        return IBIS_OK;
+    }
+    return ibis::fail(IBIS_ERR_INVALID, ("unknown option " + k).c_str());
 }
 
 ibis_status ibis_residual_profile(ibis_residual* r, int enable) {

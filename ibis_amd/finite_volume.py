@@ -250,6 +250,11 @@ class FiniteVolume:
                                           C.byref(pa)), "ibis_residual_scratch")
         return g.value, lm.value, fx.value, bool(pa.value)
 
+    def set_option(self, key: str, value: int):
+        """Implementation choice of the kernels (same results for every value)."""
+        check(lib().ibis_residual_set_option(self._h, key.encode(), int(value)),
+              "ibis_residual_set_option")
+
     def profile(self, enable: bool = True):
         """Record HIP events around every kernel launch of compute_dudt."""
         check(lib().ibis_residual_profile(self._h, 1 if enable else 0), "ibis_residual_profile")

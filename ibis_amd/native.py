@@ -81,6 +81,7 @@ _SIGS = {
     "ibis_residual_scratch": (C.c_int, [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                         C.POINTER(C.c_void_p), C.POINTER(C.c_int)]),
     "ibis_residual_profile": (C.c_int, [_P, C.c_int]),
+    "ibis_residual_set_option": (C.c_int, [_P, C.c_char_p, C.c_int64]),
     "ibis_residual_kernel_times": (C.c_int, [_P, _P, _P, C.c_int]),
     "ibis_last_error": (C.c_char_p, []),
 }

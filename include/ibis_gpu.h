@@ -198,6 +198,15 @@ ibis_status ibis_rk_step(ibis_residual* r, int nstages, const double* a, const d
 ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** limiters,
                                   double** flux, int* flux_has_area);
 
+/* Implementation choices of the 2nd-order inviscid path; every choice gives
+ * the same bits.  Keys:
+ *   "face_state_layout"   0: face states indexed by face (left/right arrays)
+ *                         1: face states stored per (cell, face slot)
+ *   "precomputed_offsets" 0: x_face - x_cell computed from face centres
+ *                         1: precomputed once per (cell, face slot)
+ * Setting precomputed_offsets allocates and synchronises once. */
+ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t value);
+
 /* Per-kernel timing with HIP events recorded around every launch of
  * ibis_compute_dudt on its stream.  Kernel slots:
  *   0 k_bc (pre-reconstruction)  1 k_cell_grad  2 k_face_flux  3 k_post_flux

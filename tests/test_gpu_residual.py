@@ -17,11 +17,13 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def _check(grid_io, cfg, prim_ref, exact, allow_reconstruction=True):
+def _check(grid_io, cfg, prim_ref, exact, allow_reconstruction=True, options=None):
     og, ofv = oracle_pair(grid_io, cfg)
     p_cpu = prim_ref.copy()
     d_cpu = ofv.compute_dudt(p_cpu, allow_reconstruction=allow_reconstruction)
     gb, fv = gpu_objects(grid_io, cfg)
+    for k, v in (options or {}).items():
+        fv.set_option(k, v)
     d_gpu, p_gpu = gpu_dudt(gb, fv, prim_ref, allow_reconstruction)
     n = og.N
     print(f"case N={n} F={og.F} flux={cfg.flux} lim={cfg.limiter} visc={cfg.viscous}", flush=True)
@@ -91,6 +93,17 @@ def test_hex_jittered_permuted(flux, limiter):
     g_io, cfg = cases.hex_case(16, flux=flux, limiter=limiter)
     prim = cases.hex_prim(g_io, 16 ** 3 + 6 * 16 ** 2)
     _check(g_io, cfg, prim, exact=True)
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("offsets", [0, 1])
+def test_implementation_options_give_identical_bits(layout, offsets):
+    opts = {"face_state_layout": layout, "precomputed_offsets": offsets}
+    g_io, cfg = cases.hex_case(12, flux="hanel", limiter="venkatakrishnan")
+    _check(g_io, cfg, cases.hex_prim(g_io, 12 ** 3 + 6 * 12 ** 2), exact=True, options=opts)
+    g_io, cfg, fs = cases.wedge_case(40, 20)
+    prim = cases.perturbed_prim(g_io, fs, 40 * 20 + 2 * (40 + 20), seed=21)
+    _check(g_io, cfg, prim, exact=True, options=opts)
 
 
 def test_plate_viscous():
