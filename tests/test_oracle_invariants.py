@@ -1,0 +1,383 @@
+"""Pin the oracle's residual where the reference's own tests say nothing.
+
+The reference has no test case for anything in src/finite_volume (its
+finite_volume_unittest has zero TEST_CASEs), so each part of the oracle's
+residual is pinned here by a fact that does not come from the oracle's own
+formulas: closed-form Euler fluxes, free-stream preservation, discrete
+conservation, exactness on linear fields, boundary-action identities,
+viscous stress of a linear shear, the time-step formula on a uniform grid,
+conversion round trips, and finite differences for the dual-number JVP.
+Each check is chosen so that a plausible mistake (a dropped term, a wrong
+sign or index, a swapped operand) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import oracle_pair
+from ibis_amd import cases, config as C, gridgen
+from ibis_amd.gridio import GridIO
+
+GAS = C.IdealGas()
+
+
+def euler_flux(rho, p, T, vel, n):
+    """Exact inviscid flux F.n of one state (rho u_n, rho u_n v + p n, rho u_n H)."""
+    un = vel @ n
+    e = GAS.Cv * T
+    H = e + p / rho + 0.5 * (vel @ vel)
+    return np.concatenate([[rho * un], rho * un * vel + p * n, [rho * un * H]])
+
+
+def unit_box_2d(nx, ny, jitter=0.0, seed=1):
+    x = np.linspace(0, 1, nx + 1)
+    y = np.linspace(0, 1, ny + 1)
+    X = np.repeat(x[:, None], ny + 1, 1)
+    Y = np.repeat(y[None, :], nx + 1, 0)
+    if jitter:
+        u = gridgen.uniform01(seed, 2 * X.size).reshape(2, *X.shape)
+        inner = np.zeros_like(X, bool)
+        inner[1:-1, 1:-1] = True
+        X = X + np.where(inner, (2 * u[0] - 1) * jitter / nx, 0)
+        Y = Y + np.where(inner, (2 * u[1] - 1) * jitter / ny, 0)
+    xyz, ctype, cvtx, vid = gridgen._quad_block(X, Y)
+    i, j = np.arange(nx), np.arange(ny)
+    markers = {"left": gridgen._lines(vid(0, j + 1), vid(0, j)),
+               "right": gridgen._lines(vid(nx, j), vid(nx, j + 1)),
+               "bottom": gridgen._lines(vid(i, 0), vid(i + 1, 0)),
+               "top": gridgen._lines(vid(i + 1, ny), vid(i, ny))}
+    return GridIO(2, xyz, ctype, cvtx, markers)
+
+
+def uniform_stream_cfg(fs, flux, limiter="barth_jespersen", order=2, thermo="rho_u"):
+    # tests/minimal/uniform_stream/job.py: inflow on left/bottom, outflow on right/top
+    return C.FVConfig(flux=flux, limiter=limiter, reconstruction_order=order,
+                      thermo_interpolator=thermo,
+                      boundaries={"left": C.supersonic_inflow(fs),
+                                  "bottom": C.supersonic_inflow(fs),
+                                  "right": C.supersonic_outflow(),
+                                  "top": C.supersonic_outflow()})
+
+
+FS_UNIFORM = C.FlowState.from_pT(101325.0, 300.0, vx=1000.0, vy=500.0)
+
+
+@pytest.mark.parametrize("flux", ["hanel", "ausmdv", "ldfss2", "rusanov"])
+@pytest.mark.parametrize("limiter,order,thermo", [("barth_jespersen", 2, "rho_u"),
+                                                  ("venkatakrishnan", 2, "rho_p"),
+                                                  ("unlimited", 2, "rho_T"),
+                                                  ("barth_jespersen", 1, "rho_u")])
+def test_free_stream_is_preserved(flux, limiter, order, thermo):
+    """Uniform supersonic state on a jittered grid: dU/dt = 0 to round-off,
+    limiters stay 1, and every face flux is the exact Euler flux."""
+    g = unit_box_2d(12, 9, jitter=0.3)
+    cfg = uniform_stream_cfg(FS_UNIFORM, flux, limiter, order, thermo)
+    og, fv = oracle_pair(g, cfg)
+    prim = cases.uniform_prim(FS_UNIFORM, og.N, og.Nt)
+    d = fv.compute_dudt(prim)
+    fs = FS_UNIFORM
+    scale = (fs.rho * 1200.0 ** 2 + fs.p) / (1.0 / 12)  # |F| / h
+    assert np.abs(d[:, :og.N]).max() <= 1e-11 * scale * fs.rho * 1000 / fs.rho
+    inter = fv.intermediates()
+    if order == 2 and limiter != "unlimited":
+        slots = {"rho_u": [1, 3], "rho_p": [0, 1], "rho_T": [1, 2]}[thermo] + [4, 5, 6]
+        assert np.all(inter["lim"][slots] == 1.0)
+    vel = np.array([fs.vx, fs.vy])
+    n = og.norm()
+    for f in range(0, og.F, 7):
+        nf = n[:2, f]
+        exact = euler_flux(fs.rho, fs.p, fs.T, vel, nf)
+        got = inter["flux"][:, f]
+        assert np.allclose(got, exact, rtol=1e-12, atol=1e-9 * np.abs(exact).max())
+
+
+def _two_state_fluxes(flux, left, right, normal=(1.0, 0.0)):
+    """Flux of one interior face between two cells with given states (2x1 grid)."""
+    x = np.array([0.0, 1.0, 2.0])
+    y = np.array([0.0, 1.0])
+    X = np.repeat(x[:, None], 2, 1)
+    Y = np.repeat(y[None, :], 3, 0)
+    xyz, ctype, cvtx, vid = gridgen._quad_block(X, Y)
+    markers = {"all": gridgen._cat(gridgen._lines(vid(0, np.array([1])), vid(0, np.array([0]))),
+                                   gridgen._lines(vid(2, np.array([0])), vid(2, np.array([1]))),
+                                   gridgen._lines(vid(np.array([0, 1]), 0), vid(np.array([1, 2]), 0)),
+                                   gridgen._lines(vid(np.array([1, 2]), 1), vid(np.array([0, 1]), 1)))}
+    g = GridIO(2, xyz, ctype, cvtx, markers)
+    cfg = C.FVConfig(flux=flux, reconstruction_order=1,
+                     boundaries={"all": C.supersonic_outflow()})
+    og, fv = oracle_pair(g, cfg)
+    prim = np.zeros((7, og.Nt))
+    for c, st in enumerate((left, right)):
+        rho, p, vx, vy = st
+        T = p / (rho * GAS.R)
+        prim[:, c] = [rho, p, T, GAS.Cv * T, vx, vy, 0.0]
+    fv.compute_dudt(prim)
+    inter = fv.intermediates()
+    # the interior face is the one between cells 0 and 1
+    lr = og.face_lr
+    f = int(np.where(((lr[0] == 0) & (lr[1] == 1)) | ((lr[0] == 1) & (lr[1] == 0)))[0][0])
+    return og, inter, f
+
+
+@pytest.mark.parametrize("flux", ["hanel", "ausmdv"])
+def test_supersonic_upwinding_gives_left_flux(flux):
+    """M_L, M_R > 1 along the normal: AUSMDV and Hanel return F(U_upwind)
+    exactly in exact arithmetic (ausmdv.cpp:51-70 else-branches, entropy fix
+    inactive; hanel.cpp:38-53)."""
+    a = GAS.speed_of_sound(300.0)
+    left = (1.2, 1.0e5, 2.5 * a, 0.3 * a)
+    right = (0.9, 0.7e5, 2.2 * a, -0.1 * a)
+    og, inter, f = _two_state_fluxes(flux, left, right)
+    n = og.norm()[:2, f]
+    up = left if og.face_lr[0, f] == 0 else right
+    if n[0] < 0:  # flow goes the other way through this face's normal
+        up = right if og.face_lr[0, f] == 0 else left
+    rho, p, vx, vy = up
+    exact = euler_flux(rho, p, p / (rho * GAS.R), np.array([vx, vy]), n)
+    assert np.allclose(inter["flux"][:, f], exact, rtol=1e-13, atol=0)
+
+
+def test_rusanov_closed_form():
+    """Rusanov: 1/2 (F_L + F_R) - 1/2 S (W_R - W_L), S = max|u +- a|, on a
+    subsonic pair.  The reference dissipates (rho, rho u, rho v, rho H), i.e.
+    rho*H rather than the total energy in the last row (rusanov.cpp:142-143);
+    a textbook Rusanov (rho*E) differs from it by 1/2 S (p_R - p_L)."""
+    left = (1.3, 1.2e5, 80.0, 20.0)
+    right = (1.0, 0.9e5, -40.0, 10.0)
+    og, inter, f = _two_state_fluxes("rusanov", left, right)
+    n = og.norm()[:2, f]
+    sides = {og.face_lr[0, f]: "L", og.face_lr[1, f]: "R"}
+    st = {sides[0]: left, sides[1]: right}
+
+    def cons(s):
+        rho, p, vx, vy = s
+        T = p / (rho * GAS.R)
+        rhoH = rho * (GAS.Cv * T + p / rho + 0.5 * (vx * vx + vy * vy))
+        return np.array([rho, rho * vx, rho * vy, rhoH])
+
+    def F(s):
+        rho, p, vx, vy = s
+        return euler_flux(rho, p, p / (rho * GAS.R), np.array([vx, vy]), n)
+
+    def a(s):
+        rho, p, _, _ = s
+        return math.sqrt(GAS.gamma * p / rho)
+
+    unL = np.array(st["L"][2:]) @ n
+    unR = np.array(st["R"][2:]) @ n
+    S = max(abs(unL - a(st["L"])), abs(unR - a(st["R"])), abs(unL + a(st["L"])),
+            abs(unR + a(st["R"])))
+    exact = 0.5 * (F(st["L"]) + F(st["R"])) - 0.5 * S * (cons(st["R"]) - cons(st["L"]))
+    assert np.allclose(inter["flux"][:, f], exact, rtol=1e-12, atol=1e-9 * np.abs(exact).max())
+
+
+@pytest.mark.parametrize("case", ["wedge", "hex", "plate"])
+def test_discrete_conservation(case):
+    """Interior face fluxes cancel: sum_i V_i dU_i/dt equals minus the sum of
+    boundary-face fluxes F.n A (outward) to round-off."""
+    if case == "wedge":
+        g, cfg, fs = cases.wedge_case(24, 12)
+        prim = cases.perturbed_prim(g, fs, 24 * 12 + 2 * 36, seed=5)
+    elif case == "plate":
+        g, cfg, fs = cases.plate_case(24, 12)
+        prim = cases.perturbed_prim(g, fs, 24 * 12 + 2 * 36, seed=6)
+    else:
+        g, cfg = cases.hex_case(6)
+        prim = cases.hex_prim(g, 6 ** 3 + 6 * 36)
+    og, fv = oracle_pair(g, cfg)
+    d = fv.compute_dudt(prim)
+    flux = fv.intermediates()["flux"]
+    lhs = (d[:, :og.N] * og.volume[:og.N]).sum(axis=1)
+    rhs = np.zeros(d.shape[0])
+    terms = np.zeros(d.shape[0])
+    for f in og.bfaces:
+        out = 1.0 if og.face_lr[0, f] < og.N else -1.0  # valid cell on the left: n points out
+        rhs -= flux[:, f] * og.area()[f] * out
+    for c in range(og.N):
+        for j in range(og.cell_off[c], og.cell_off[c + 1]):
+            terms += np.abs(flux[:, og.cell_faces[j]] * og.area()[og.cell_faces[j]])
+    assert np.all(np.abs(lhs - rhs) <= 1e-12 * terms)
+
+
+def test_lsq_gradient_linear_exact_2d_jittered():
+    g = unit_box_2d(10, 7, jitter=0.4, seed=3)
+    cfg = uniform_stream_cfg(FS_UNIFORM, "hanel")
+    og, _ = oracle_pair(g, cfg)
+    vals = 2.0 - 3.0 * og.centroid[0] + 0.5 * og.centroid[1]
+    grad = og.gradient(vals)
+    assert np.allclose(grad[0], -3.0, rtol=1e-11) and np.allclose(grad[1], 0.5, rtol=1e-11)
+
+
+def test_lsq_gradient_3d_beta_quirk():
+    """The reference's 3D weights use beta = (r12 r23 - r13 r23)/(r11 r22)
+    (gradient.h:89).  They are exact on an unjittered box (r13 = 0, so beta
+    is 0 either way) and not exact on a jittered one, where the textbook
+    beta = (r12 r23 - r13 r22)/(r11 r22) would be."""
+    for jitter, exact in ((0.0, True), (0.1, False)):
+        g = gridgen.hex_box(6, jitter=jitter, permute=False)
+        cfg = cases.hex_config()
+        og, _ = oracle_pair(g, cfg)
+        vals = 1.0 + 2.0 * og.centroid[0] - 1.0 * og.centroid[1] + 4.0 * og.centroid[2]
+        grad = og.gradient(vals)
+        ok = (np.allclose(grad[0], 2.0, rtol=1e-10) and np.allclose(grad[1], -1.0, rtol=1e-10)
+              and np.allclose(grad[2], 4.0, rtol=1e-10))
+        assert ok == exact
+
+
+def test_linear_field_reconstruction_and_bj_on_uniform_grid():
+    """Linear rho and e on a uniform grid with interior-cell stencils: the
+    BJ limiter is 1 in cells whose neighbours all lie on the linear field
+    and the face values equal the linear field at the face centre."""
+    g = unit_box_2d(8, 8)
+    cfg = uniform_stream_cfg(FS_UNIFORM, "hanel")
+    og, fv = oracle_pair(g, cfg)
+    prim = cases.uniform_prim(FS_UNIFORM, og.N, og.Nt)
+    x, y = og.centroid[0], og.centroid[1]
+    prim[0] = 1.0 + 0.1 * x + 0.05 * y
+    prim[3] = GAS.Cv * 300.0 * (1.0 + 0.02 * x - 0.03 * y)
+    fv.compute_dudt(prim)
+    inter = fv.intermediates()
+    interior = [c for c in range(og.N)
+                if np.all(og.neighbours[og.cell_off[c]:og.cell_off[c + 1]] < og.N)]
+    assert np.allclose(inter["grads"][1, 0, interior], 0.1, rtol=1e-10)
+    assert np.all(inter["lim"][1, interior] == 1.0)
+    fc = og.face_centre()
+    for f in range(og.F):
+        l, r = og.face_lr[:, f]
+        if l in interior:
+            want = 1.0 + 0.1 * fc[0, f] + 0.05 * fc[1, f]
+            assert abs(inter["left"][0, f] - want) < 1e-12
+
+
+def test_boundary_actions():
+    """Ghost values: slip wall reflects the normal velocity (v - 2 (v.n) n),
+    keeps T, p, rho, e; fixed-temperature wall gives T_g = 2 T_w - T_i and
+    reverses the velocity; supersonic inflow copies the prescribed state."""
+    g, cfg, fs = cases.plate_case(8, 4)
+    og, fv = oracle_pair(g, cfg)
+    prim = cases.perturbed_prim(g, fs, og.Nt, seed=3)
+    fv.compute_dudt(prim)
+    lr = og.face_lr
+    for f, gh in zip(og.boundary_faces("wall"), og.ghost_cells("wall")):
+        va = lr[0, f] if lr[0, f] < og.N else lr[1, f]
+        assert prim[2, gh] == 2 * 300.0 - prim[2, va]
+        assert np.array_equal(prim[4:7, gh], -prim[4:7, va])
+        assert prim[1, gh] == prim[1, va] and prim[0, gh] == prim[0, va]
+    for gh in og.ghost_cells("inflow"):
+        assert np.array_equal(prim[:, gh], np.array([fs.rho, fs.p, fs.T, fs.energy, fs.vx,
+                                                     fs.vy, fs.vz]))
+    g, cfg, fs = cases.wedge_case(12, 6)
+    og, fv = oracle_pair(g, cfg)
+    prim = cases.perturbed_prim(g, fs, og.Nt, seed=4)
+    fv.compute_dudt(prim)
+    n = og.norm()
+    for f, gh in zip(og.boundary_faces("wall"), og.ghost_cells("wall")):
+        va = og.face_lr[0, f] if og.face_lr[0, f] < og.N else og.face_lr[1, f]
+        v = prim[4:7, va]
+        want = v - 2 * (v @ n[:, f]) * n[:, f]
+        assert np.allclose(prim[4:7, gh], want, rtol=1e-13, atol=1e-12)
+        assert np.array_equal(prim[:4, gh], prim[:4, va])
+
+
+def test_viscous_couette_shear_stress():
+    """u = a y, v = 0, T uniform on the plate grid: the LSQ gradient is exact
+    in 2D and Hasselbacher averaging keeps it, so on every interior face with
+    normal (0, +-1) the viscous x-momentum flux is -n_y mu(T) a."""
+    g, cfg, fs = cases.plate_case(10, 8)
+    cfg.boundaries = {"inflow": C.supersonic_outflow(), "outflow": C.supersonic_outflow(),
+                      "wall": C.supersonic_outflow()}
+    inv = C.FVConfig(**{**cfg.__dict__, "viscous": False})
+    og, fv = oracle_pair(g, cfg)
+    _, fv_inv = oracle_pair(g, inv)
+    a = 50.0
+    prim = cases.uniform_prim(fs, og.N, og.Nt)
+    prim[4, :og.N] = a * og.centroid[1, :og.N]
+    prim[5, :og.N] = 0.0
+    d_v = fv.compute_dudt(prim.copy())
+    f_v = fv.intermediates()["flux"]
+    fv_inv.compute_dudt(prim.copy())
+    f_i = fv_inv.intermediates()["flux"]
+    visc = f_v - f_i
+    mu = C.TransportProperties().viscosity(fs.T)
+    n = og.norm()
+    checked = 0
+    for f in range(og.F):
+        l, r = og.face_lr[:, f]
+        if l < og.N and r < og.N and abs(abs(n[1, f]) - 1.0) < 1e-12:
+            # both cells' stencils must be interior for the gradient to be exact
+            nb = [og.neighbours[og.cell_off[c]:og.cell_off[c + 1]] for c in (l, r)]
+            if all(np.all(x < og.N) for x in nb):
+                assert visc[1, f] == pytest.approx(-n[1, f] * mu * a, rel=1e-9)
+                checked += 1
+    assert checked > 10
+    assert np.isfinite(d_v).all()
+
+
+def test_estimate_dt_uniform_grid_closed_form():
+    """Uniform state on a uniform nx x ny unit box: dt = V / sum_f (|v.n|+a) A
+    = (h_x h_y) / (2 (|u| + a) h_y + 2 (|v| + a) h_x)."""
+    nx, ny = 10, 5
+    g = unit_box_2d(nx, ny)
+    cfg = uniform_stream_cfg(FS_UNIFORM, "hanel")
+    og, fv = oracle_pair(g, cfg)
+    prim = cases.uniform_prim(FS_UNIFORM, og.N, og.Nt)
+    a = GAS.speed_of_sound(FS_UNIFORM.T)
+    hx, hy = 1.0 / nx, 1.0 / ny
+    want = hx * hy / (2 * (abs(FS_UNIFORM.vx) + a) * hy + 2 * (abs(FS_UNIFORM.vy) + a) * hx)
+    assert fv.estimate_dt(prim) == pytest.approx(want, rel=1e-13)
+
+
+def test_conversion_round_trip():
+    g, cfg, fs = cases.wedge_case(16, 8)
+    og, fv = oracle_pair(g, cfg)
+    prim = cases.perturbed_prim(g, fs, og.Nt, seed=9)
+    cons = fv.prim_to_cons(prim)
+    back = fv.cons_to_prim(cons)
+    n = og.N
+    assert np.allclose(back[:, :n], prim[:, :n], rtol=1e-13, atol=1e-12)
+    E = prim[0, :n] * (prim[3, :n] + 0.5 * (prim[4, :n] ** 2 + prim[5, :n] ** 2))
+    assert np.allclose(cons[3, :n], E, rtol=1e-15)
+
+
+def test_rk_step_keeps_free_stream_and_follows_dt_rule():
+    g = unit_box_2d(8, 6, jitter=0.2)
+    cfg = uniform_stream_cfg(FS_UNIFORM, "ausmdv")
+    og, fv = oracle_pair(g, cfg)
+    prim = cases.uniform_prim(FS_UNIFORM, og.N, og.Nt)
+    cons = fv.prim_to_cons(prim)
+    c0 = cons.copy()
+    a, b = C.ssp_rk3()
+    stable = fv.estimate_dt(prim)
+    dt = fv.rk_step(a, b, 0.5, cons, prim, np.finfo(float).max, 0.0, 1.0)
+    assert dt == 0.5 * stable
+    assert np.allclose(cons[:, :og.N], c0[:, :og.N], rtol=1e-12)
+    # growth limit: dt <= 1.5 * previous (SO/runge_kutta.cpp:114)
+    dt2 = fv.rk_step(a, b, 0.5, cons, prim, 0.1 * stable, dt, 1.0)
+    assert dt2 == pytest.approx(0.15 * stable, rel=1e-15)
+    # end time clip
+    dt3 = fv.rk_step(a, b, 0.5, cons, prim, dt, 0.0, 0.25 * stable)
+    assert dt3 == 0.25 * stable
+
+
+@pytest.mark.parametrize("flux", ["ausmdv", "hanel"])
+def test_dual_jvp_matches_central_differences(flux):
+    """The Dual<double> residual's tangent equals the central finite
+    difference of the real residual on a smooth state (step 1e-6 relative)."""
+    g, cfg, fs = cases.wedge_case(12, 6, flux=flux)
+    og, fv = oracle_pair(g, cfg)
+    prim = cases.perturbed_prim(g, fs, og.Nt, seed=10, amp=0.02)
+    cons = fv.prim_to_cons(prim)
+    u = gridgen.uniform01(77, cons.size).reshape(cons.shape) - 0.5
+    u[:, og.N:] = 0.0
+    v = u * np.abs(cons).max(axis=1, keepdims=True) * 1e-3
+    p, pdot = fv.cons_to_prim_jvp(cons, v)
+    d0, ddot = fv.compute_dudt_jvp(p.copy(), pdot.copy())
+    h = 1e-6
+    dp = fv.compute_dudt(fv.cons_to_prim(cons + h * v))
+    dm = fv.compute_dudt(fv.cons_to_prim(cons - h * v))
+    fd = (dp - dm) / (2 * h)
+    n = og.N
+    scale = np.abs(ddot[:, :n]).max(axis=1, keepdims=True)
+    assert np.all(np.abs(fd[:, :n] - ddot[:, :n]) <= 1e-5 * scale)
+    assert np.array_equal(d0[:, :n], fv.compute_dudt(p.copy())[:, :n])
