@@ -93,6 +93,10 @@ def main():
     ap.add_argument("--workload", default="hex160", choices=["hex160", "plate", "wedge"])
     ap.add_argument("--cpu-residuals", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--layout", type=int, default=None, choices=[0, 1, 2],
+                    help="fix the face-state layout instead of autotuning")
+    ap.add_argument("--offsets", type=int, default=None, choices=[0, 1],
+                    help="fix the precomputed-offsets option instead of autotuning")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 times), to calibrate the "
                          "FETCH_SIZE/WRITE_SIZE counters")
@@ -130,8 +134,8 @@ def main():
     # autotune the implementation choices (all give identical results)
     variants = {}
     best = None
-    for layout in (0, 1):
-        for offs in (0, 1):
+    for layout in ((0, 1, 2) if args.layout is None else (args.layout,)):
+        for offs in ((0, 1) if args.offsets is None else (args.offsets,)):
             fv.set_option("face_state_layout", layout)
             fv.set_option("precomputed_offsets", offs)
             for _ in range(2):

@@ -13,15 +13,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include "dual.hpp"
+
 namespace ibis {
 
 enum { S_RHO = 0, S_P = 1, S_T = 2, S_E = 3, S_U = 4, S_V = 5, S_W = 6, NSTATE = 7 };
-
-// Kokkos::min / Kokkos::max on arithmetic types
-template <class S>
-__device__ __forceinline__ S kmin(S a, S b) { return (b < a) ? b : a; }
-template <class S>
-__device__ __forceinline__ S kmax(S a, S b) { return (a < b) ? b : a; }
 
 struct GasConst {
     double R, Cv, Cp, gamma;
@@ -29,42 +25,44 @@ struct GasConst {
 
 // Ausmdv::compute_flux, src/finite_volume/finite_volume/flux_calculators/ausmdv.cpp:9-131
 template <class S, bool THREE_D>
-__device__ __forceinline__ void flux_ausmdv(const S* L, const S* Rs, S R, S Cv, S gamma,
-                                            S* out) {
+__device__ __forceinline__ void flux_ausmdv(const S* L, const S* Rs, typename lit<S>::type R,
+                                            typename lit<S>::type Cv,
+                                            typename lit<S>::type gamma, S* out) {
+    using K = typename lit<S>::type;
     S rL = L[S_RHO], pL = L[S_P];
     S pLrL = pL / rL;
     S uL = L[S_U], vL = L[S_V], wL = L[S_W];
     S eL = Cv * L[S_T];
     S aL = sqrt(gamma * R * L[S_T]);
-    S keL = S(0.5) * (uL * uL + vL * vL + wL * wL);
+    S keL = K(0.5) * (uL * uL + vL * vL + wL * wL);
     S HL = eL + pLrL + keL;
     S rR = Rs[S_RHO], pR = Rs[S_P];
     S pRrR = pR / rR;
     S uR = Rs[S_U], vR = Rs[S_V], wR = Rs[S_W];
     S eR = Cv * Rs[S_T];
     S aR = sqrt(gamma * R * Rs[S_T]);
-    S keR = S(0.5) * (uR * uR + vR * vR + wR * wR);
+    S keR = K(0.5) * (uR * uR + vR * vR + wR * wR);
     S HR = eR + pRrR + keR;
 
-    S alphaL = S(2.0) * pLrL / (pLrL + pRrR);
-    S alphaR = S(2.0) * pRrR / (pLrL + pRrR);
+    S alphaL = K(2.0) * pLrL / (pLrL + pRrR);
+    S alphaR = K(2.0) * pRrR / (pLrL + pRrR);
     S am = kmax(aL, aR);
     S ML = uL / am;
     S MR = uR / am;
     S pLplus, uLplus;
-    S duL = S(0.5) * (uL + fabs(uL));
-    if (fabs(ML) <= S(1.0)) {
-        pLplus = pL * (ML + S(1.0)) * (ML + S(1.0)) * (S(2.0) - ML) * S(0.25);
-        uLplus = alphaL * ((uL + am) * (uL + am) / (S(4.0) * am) - duL) + duL;
+    S duL = K(0.5) * (uL + fabs(uL));
+    if (fabs(ML) <= K(1.0)) {
+        pLplus = pL * (ML + K(1.0)) * (ML + K(1.0)) * (K(2.0) - ML) * K(0.25);
+        uLplus = alphaL * ((uL + am) * (uL + am) / (K(4.0) * am) - duL) + duL;
     } else {
         pLplus = pL * duL / uL;
         uLplus = duL;
     }
     S pRminus, uRminus;
-    S duR = S(0.5) * (uR - fabs(uR));
-    if (fabs(MR) <= S(1.0)) {
-        pRminus = pR * (MR - S(1.0)) * (MR - S(1.0)) * (S(2.0) + MR) * S(0.25);
-        uRminus = alphaR * (-(uR - am) * (uR - am) / (S(4.0) * am) - duR) + duR;
+    S duR = K(0.5) * (uR - fabs(uR));
+    if (fabs(MR) <= K(1.0)) {
+        pRminus = pR * (MR - K(1.0)) * (MR - K(1.0)) * (K(2.0) + MR) * K(0.25);
+        uRminus = alphaR * (-(uR - am) * (uR - am) / (K(4.0) * am) - duR) + duR;
     } else {
         pRminus = pR * duR / uR;
         uRminus = duR;
@@ -72,13 +70,13 @@ __device__ __forceinline__ void flux_ausmdv(const S* L, const S* Rs, S R, S Cv, 
     S ru_half = uLplus * rL + uRminus * rR;
     S p_half = pLplus + pRminus;
     S dp = pL - pR;
-    dp = S(10.0) * fabs(dp) / kmin(pL, pR);
-    S s = S(0.5) * kmin(S(1.0), dp);
+    dp = K(10.0) * fabs(dp) / kmin(pL, pR);
+    S s = K(0.5) * kmin(K(1.0), dp);
     S ru2_AUSMV = uLplus * rL * uL + uRminus * rR * uR;
-    S ru2_AUSMD = S(0.5) * (ru_half * (uL + uR) - fabs(ru_half) * (uR - uL));
-    S ru2_half = (S(0.5) + s) * ru2_AUSMV + (S(0.5) - s) * ru2_AUSMD;
-    S m = ru_half, mx = ru2_half + p_half, my, mz = S(0.0), en;
-    if (ru_half >= S(0.0)) {
+    S ru2_AUSMD = K(0.5) * (ru_half * (uL + uR) - fabs(ru_half) * (uR - uL));
+    S ru2_half = (K(0.5) + s) * ru2_AUSMV + (K(0.5) - s) * ru2_AUSMD;
+    S m = ru_half, mx = ru2_half + p_half, my, mz = K(0.0), en;
+    if (ru_half >= K(0.0)) {
         my = ru_half * vL;
         if (THREE_D) mz = ru_half * wL;
         en = ru_half * HL;
@@ -87,13 +85,13 @@ __device__ __forceinline__ void flux_ausmdv(const S* L, const S* Rs, S R, S Cv, 
         if (THREE_D) mz = ru_half * wR;
         en = ru_half * HR;
     }
-    const S C_EFIX = S(0.125);
-    bool caseA = ((uL - aL) < S(0.0)) && ((uR - aR) > S(0.0));
-    bool caseB = ((uL + aL) < S(0.0)) && ((uR + aR) > S(0.0));
-    S d_ua = S(0.0);
+    const K C_EFIX = K(0.125);
+    bool caseA = ((uL - aL) < K(0.0)) && ((uR - aR) > K(0.0));
+    bool caseB = ((uL + aL) < K(0.0)) && ((uR + aR) > K(0.0));
+    S d_ua = K(0.0);
     if (caseA && !caseB) d_ua = C_EFIX * ((uR - aR) - (uL - aL));
     if (caseB && !caseA) d_ua = C_EFIX * ((uR + aR) - (uL + aL));
-    if (d_ua != S(0.0)) {
+    if (d_ua != K(0.0)) {
         m -= d_ua * (rR - rL);
         mx -= d_ua * (rR * uR - rL * uL);
         my -= d_ua * (rR * vR - rL * vL);
@@ -109,102 +107,109 @@ __device__ __forceinline__ void flux_ausmdv(const S* L, const S* Rs, S R, S Cv, 
 
 // Hanel::compute_flux, flux_calculators/hanel.cpp:8-64
 template <class S, bool THREE_D>
-__device__ __forceinline__ void flux_hanel(const S* L, const S* Rs, S R, S Cv, S gamma,
-                                           S* out) {
+__device__ __forceinline__ void flux_hanel(const S* L, const S* Rs, typename lit<S>::type R,
+                                           typename lit<S>::type Cv,
+                                           typename lit<S>::type gamma, S* out) {
+    using K = typename lit<S>::type;
     S rL = L[S_RHO], pL = L[S_P];
     S eL = Cv * L[S_T];
     S aL = sqrt(gamma * R * L[S_T]);
     S uL = L[S_U], vL = L[S_V], wL = L[S_W];
-    S keL = S(0.5) * (uL * uL + vL * vL + wL * wL);
+    S keL = K(0.5) * (uL * uL + vL * vL + wL * wL);
     S pLrL = pL / rL;
     S HL = eL + pLrL + keL;
     S rR = Rs[S_RHO], pR = Rs[S_P];
     S eR = Cv * Rs[S_T];
     S aR = sqrt(gamma * R * Rs[S_T]);
     S uR = Rs[S_U], vR = Rs[S_V], wR = Rs[S_W];
-    S keR = S(0.5) * (uR * uR + vR * vR + wR * wR);
+    S keR = K(0.5) * (uR * uR + vR * vR + wR * wR);
     S pRrR = pR / rR;
     S HR = eR + pRrR + keR;
     S pLplus, uLplus;
     if (fabs(uL) <= aL) {
-        uLplus = S(1.0) / (S(4.0) * aL) * (uL + aL) * (uL + aL);
-        pLplus = pL * uLplus * (S(1.0) / aL * (S(2.0) - uL / aL));
+        uLplus = K(1.0) / (K(4.0) * aL) * (uL + aL) * (uL + aL);
+        pLplus = pL * uLplus * (K(1.0) / aL * (K(2.0) - uL / aL));
     } else {
-        uLplus = S(0.5) * (uL + fabs(uL));
-        pLplus = pL * uLplus * (S(1.0) / uL);
+        uLplus = K(0.5) * (uL + fabs(uL));
+        pLplus = pL * uLplus * (K(1.0) / uL);
     }
     S pRminus, uRminus;
     if (fabs(uR) <= aR) {
-        uRminus = S(-1.0) / (S(4.0) * aR) * (uR - aR) * (uR - aR);
-        pRminus = pR * uRminus * (S(1.0) / aR * (S(-2.0) - uR / aR));
+        uRminus = K(-1.0) / (K(4.0) * aR) * (uR - aR) * (uR - aR);
+        pRminus = pR * uRminus * (K(1.0) / aR * (K(-2.0) - uR / aR));
     } else {
-        uRminus = S(0.5) * (uR - fabs(uR));
-        pRminus = pR * uRminus * (S(1.0) / uR);
+        uRminus = K(0.5) * (uR - fabs(uR));
+        pRminus = pR * uRminus * (K(1.0) / uR);
     }
     S p_half = pLplus + pRminus;
     out[0] = uLplus * rL + uRminus * rR;
     out[1] = uLplus * rL * uL + uRminus * rR * uR + p_half;
     out[2] = uLplus * rL * vL + uRminus * rR * vR;
-    out[3] = THREE_D ? (uLplus * rL * wL + uRminus * rR * wR) : S(0.0);
+    out[3] = THREE_D ? (uLplus * rL * wL + uRminus * rR * wR) : K(0.0);
     out[4] = uLplus * rL * HL + uRminus * rR * HR;
 }
 
 // Ldfss2::compute_flux, flux_calculators/ldfss2.cpp:10-89
 template <class S, bool THREE_D>
-__device__ __forceinline__ void flux_ldfss2(const S* L, const S* Rs, S R, S Cv, S gamma,
-                                            S delta, S* out) {
+__device__ __forceinline__ void flux_ldfss2(const S* L, const S* Rs, typename lit<S>::type R,
+                                            typename lit<S>::type Cv,
+                                            typename lit<S>::type gamma,
+                                            typename lit<S>::type delta, S* out) {
+    using K = typename lit<S>::type;
     S rL = L[S_RHO], pL = L[S_P];
     S pLrL = pL / rL;
     S uL = L[S_U], vL = L[S_V], wL = L[S_W];
     S eL = Cv * L[S_T];
     S aL = sqrt(gamma * R * L[S_T]);
-    S keL = S(0.5) * (uL * uL + vL * vL + wL * wL);
+    S keL = K(0.5) * (uL * uL + vL * vL + wL * wL);
     S HL = eL + pLrL + keL;
     S rR = Rs[S_RHO], pR = Rs[S_P];
     S pRrR = pR / rR;
     S uR = Rs[S_U], vR = Rs[S_V], wR = Rs[S_W];
     S eR = Cv * Rs[S_T];
     S aR = sqrt(gamma * R * Rs[S_T]);
-    S keR = S(0.5) * (uR * uR + vR * vR + wR * wR);
+    S keR = K(0.5) * (uR * uR + vR * vR + wR * wR);
     S HR = eR + pRrR + keR;
-    S am = S(0.5) * (aL + aR);
+    S am = K(0.5) * (aL + aR);
     S ML = uL / am;
     S MR = uR / am;
-    S MpL = S(0.25) * (ML + S(1.0)) * (ML + S(1.0));
-    S MmR = S(-0.25) * (MR - S(1.0)) * (MR - S(1.0));
-    S alphaL = S(0.5) * (S(1.0) + copysign(S(1.0), ML));
-    S alphaR = S(0.5) * (S(1.0) - copysign(S(1.0), MR));
-    S betaL = -kmax(S(0.0), S(1.0) - floor(fabs(ML)));
-    S betaR = -kmax(S(0.0), S(1.0) - floor(fabs(MR)));
-    S PL = S(0.25) * (ML + S(1.0)) * (ML + S(1.0)) * (S(2.0) - ML);
-    S PR = S(0.25) * (MR - S(1.0)) * (MR - S(1.0)) * (S(2.0) + MR);
-    S DL = alphaL * (S(1.0) + betaL) - betaL * PL;
-    S DR = alphaR * (S(1.0) + betaR) - betaR * PR;
-    S Mhalf = S(0.25) * betaL * betaR * pow(sqrt(S(0.5) * (ML * ML + MR * MR)) - S(1.0), S(2.0));
-    S MhalfL = Mhalf * (S(1.0) - ((pL - pR) / (pL + pR) + delta * (fabs(pL - pR) / pL)));
-    S MhalfR = Mhalf * (S(1.0) + ((pL - pR) / (pL + pR) - delta * (fabs(pL - pR) / pR)));
-    S CL = alphaL * (S(1.0) + betaL) * ML - betaL * MpL - MhalfL;
-    S CR = alphaR * (S(1.0) + betaR) * MR - betaR * MmR + MhalfR;
+    S MpL = K(0.25) * (ML + K(1.0)) * (ML + K(1.0));
+    S MmR = K(-0.25) * (MR - K(1.0)) * (MR - K(1.0));
+    S alphaL = K(0.5) * (K(1.0) + copysign(S(1.0), ML));
+    S alphaR = K(0.5) * (K(1.0) - copysign(S(1.0), MR));
+    S betaL = -kmax(K(0.0), K(1.0) - floor(fabs(ML)));
+    S betaR = -kmax(K(0.0), K(1.0) - floor(fabs(MR)));
+    S PL = K(0.25) * (ML + K(1.0)) * (ML + K(1.0)) * (K(2.0) - ML);
+    S PR = K(0.25) * (MR - K(1.0)) * (MR - K(1.0)) * (K(2.0) + MR);
+    S DL = alphaL * (K(1.0) + betaL) - betaL * PL;
+    S DR = alphaR * (K(1.0) + betaR) - betaR * PR;
+    S Mhalf = K(0.25) * betaL * betaR * pow(sqrt(K(0.5) * (ML * ML + MR * MR)) - K(1.0), S(2.0));
+    S MhalfL = Mhalf * (K(1.0) - ((pL - pR) / (pL + pR) + delta * (fabs(pL - pR) / pL)));
+    S MhalfR = Mhalf * (K(1.0) + ((pL - pR) / (pL + pR) - delta * (fabs(pL - pR) / pR)));
+    S CL = alphaL * (K(1.0) + betaL) * ML - betaL * MpL - MhalfL;
+    S CR = alphaR * (K(1.0) + betaR) * MR - betaR * MmR + MhalfR;
     S ru_half = am * rL * CL + am * rR * CR;
     S ru2_half = am * rL * CL * uL + am * rR * CR * uR;
     S p_half = DL * pL + DR * pR;
     out[0] = ru_half;
     out[1] = ru2_half + p_half;
     out[2] = am * rL * CL * vL + am * rR * CR * vR;
-    out[3] = THREE_D ? (am * rL * CL * wL + am * rR * CR * wR) : S(0.0);
+    out[3] = THREE_D ? (am * rL * CL * wL + am * rR * CR * wR) : K(0.0);
     out[4] = am * rL * CL * HL + am * rR * CR * HR;
 }
 
 // Rusanov::compute_flux, flux_calculators/rusanov.cpp:10-52
 template <class S, bool THREE_D>
-__device__ __forceinline__ void flux_rusanov(const S* L, const S* Rs, S R, S Cv, S gamma,
-                                             S* out) {
+__device__ __forceinline__ void flux_rusanov(const S* L, const S* Rs, typename lit<S>::type R,
+                                             typename lit<S>::type Cv,
+                                             typename lit<S>::type gamma, S* out) {
+    using K = typename lit<S>::type;
     S rL = L[S_RHO];
     S eL = Cv * L[S_T];
     S pL = L[S_P];
     S uL = L[S_U], vL = L[S_V], wL = L[S_W];
     S ruL = rL * uL;
-    S keL = S(0.5) * (uL * uL + vL * vL + wL * wL);
+    S keL = K(0.5) * (uL * uL + vL * vL + wL * wL);
     S HL = eL + pL / rL + keL;
     S aL = sqrt(gamma * R * L[S_T]);
     S rR = Rs[S_RHO];
@@ -212,23 +217,25 @@ __device__ __forceinline__ void flux_rusanov(const S* L, const S* Rs, S R, S Cv,
     S pR = Rs[S_P];
     S uR = Rs[S_U], vR = Rs[S_V], wR = Rs[S_W];
     S ruR = rR * uR;
-    S keR = S(0.5) * (uR * uR + vR * vR + wR * wR);
+    S keR = K(0.5) * (uR * uR + vR * vR + wR * wR);
     S HR = eR + pR / rR + keR;
     S aR = sqrt(gamma * R * Rs[S_T]);
     S S_plus = kmax(fabs(uL - aL), fabs(uR - aR));
     S_plus = kmax(S_plus, fabs(uL + aL));
     S_plus = kmax(S_plus, fabs(uR + aR));
-    out[0] = S(0.5) * (ruL + ruR) - S(0.5) * S_plus * (rR - rL);
-    out[1] = S(0.5) * (ruL * uL + pL + ruR * uR + pR) - S(0.5) * S_plus * (rR * uR - rL * uL);
-    out[2] = S(0.5) * (ruL * vL + ruR * vR) - S(0.5) * S_plus * (rR * vR - rL * vL);
-    out[3] = THREE_D ? (S(0.5) * (ruL * wL + ruR * wR) - S(0.5) * S_plus * (rR * wR - rL * wL))
-                     : S(0.0);
-    out[4] = S(0.5) * (ruL * HL + ruR * HR) - S(0.5) * S_plus * (rR * HR - rL * HL);
+    out[0] = K(0.5) * (ruL + ruR) - K(0.5) * S_plus * (rR - rL);
+    out[1] = K(0.5) * (ruL * uL + pL + ruR * uR + pR) - K(0.5) * S_plus * (rR * uR - rL * uL);
+    out[2] = K(0.5) * (ruL * vL + ruR * vR) - K(0.5) * S_plus * (rR * vR - rL * vL);
+    out[3] = THREE_D ? (K(0.5) * (ruL * wL + ruR * wR) - K(0.5) * S_plus * (rR * wR - rL * wL))
+                     : K(0.0);
+    out[4] = K(0.5) * (ruL * HL + ruR * HR) - K(0.5) * S_plus * (rR * HR - rL * HL);
 }
 
 // Thermodynamic closure after reconstruction (src/gas/gas/gas_model.h:91-116)
 template <class S>
-__device__ __forceinline__ void thermo_update(int mode, S R, S Cv, S& rho, S& p, S& T, S& e) {
+__device__ __forceinline__ void thermo_update(int mode, typename lit<S>::type R,
+                                              typename lit<S>::type Cv, S& rho, S& p, S& T,
+                                              S& e) {
     switch (mode) {
         case 0:  // rho_p: update_thermo_from_rhop
             T = p / (rho * R);

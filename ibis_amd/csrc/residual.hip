@@ -113,33 +113,36 @@ struct GasParams {
     double R, Cv, Cp, gamma;
 };
 
+// S is double for the residual and ibis::Dual for its tangent (JVP); the
+// geometry and the action parameters are plain doubles in both.
+template <class S>
 __global__ void __launch_bounds__(BLOCK) k_bc(DevMesh m, const int* __restrict__ kinds,
                                               const double* __restrict__ pars, GasParams gp,
-                                              double* __restrict__ prim) {
+                                              S* __restrict__ prim) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= m.nb) return;
     const int f = m.b_face[b], gh = m.b_ghost[b], va = m.b_valid[b], t = m.b_tag[b];
     const int Nt = m.Nt, F = m.F;
-    double* rho = prim + (size_t)P_RHO * Nt;
-    double* p = prim + (size_t)P_P * Nt;
-    double* temp = prim + (size_t)P_T * Nt;
-    double* e = prim + (size_t)P_E * Nt;
-    double* vx = prim + (size_t)P_VX * Nt;
-    double* vy = prim + (size_t)P_VY * Nt;
-    double* vz = prim + (size_t)P_VZ * Nt;
+    S* rho = prim + (size_t)P_RHO * Nt;
+    S* p = prim + (size_t)P_P * Nt;
+    S* temp = prim + (size_t)P_T * Nt;
+    S* e = prim + (size_t)P_E * Nt;
+    S* vx = prim + (size_t)P_VX * Nt;
+    S* vy = prim + (size_t)P_VY * Nt;
+    S* vz = prim + (size_t)P_VZ * Nt;
     for (int a = 0; a < IBIS_MAX_ACT; a++) {
         const int kind = kinds[t * IBIS_MAX_ACT + a];
         if (kind == IBIS_BC_NONE) continue;
         const double* par = pars + (t * IBIS_MAX_ACT + a) * IBIS_NPAR;
         switch (kind) {
             case IBIS_BC_FLOW_STATE_COPY:  // boundary.cpp:15-45
-                temp[gh] = par[0];
-                p[gh] = par[1];
-                rho[gh] = par[2];
-                e[gh] = par[3];
-                vx[gh] = par[4];
-                vy[gh] = par[5];
-                vz[gh] = par[6];
+                temp[gh] = S(par[0]);
+                p[gh] = S(par[1]);
+                rho[gh] = S(par[2]);
+                e[gh] = S(par[3]);
+                vx[gh] = S(par[4]);
+                vy[gh] = S(par[5]);
+                vz[gh] = S(par[6]);
                 break;
             case IBIS_BC_INTERNAL_COPY:  // boundary.cpp:100-136
                 temp[gh] = temp[va];
@@ -155,13 +158,13 @@ __global__ void __launch_bounds__(BLOCK) k_bc(DevMesh m, const int* __restrict__
                 p[gh] = p[va];
                 rho[gh] = rho[va];
                 e[gh] = e[va];
-                double x = vx[va], y = vy[va], z = vz[va];
+                const S x = vx[va], y = vy[va], z = vz[va];
                 double nx = m.geom[0 * F + f], ny = m.geom[1 * F + f], nz = m.geom[2 * F + f];
                 double t1x = m.geom[3 * F + f], t1y = m.geom[4 * F + f], t1z = m.geom[5 * F + f];
                 double t2x = m.geom[6 * F + f], t2y = m.geom[7 * F + f], t2z = m.geom[8 * F + f];
-                double xs = -(x * nx + y * ny + z * nz);
-                double ys = x * t1x + y * t1y + z * t1z;
-                double zs = x * t2x + y * t2y + z * t2z;
+                const S xs = -(x * nx + y * ny + z * nz);
+                const S ys = x * t1x + y * t1y + z * t1z;
+                const S zs = x * t2x + y * t2y + z * t2z;
                 vx[gh] = xs * nx + ys * t1x + zs * t2x;
                 vy[gh] = xs * ny + ys * t1y + zs * t2y;
                 vz[gh] = xs * nz + ys * t1z + zs * t2z;
@@ -173,47 +176,47 @@ __global__ void __launch_bounds__(BLOCK) k_bc(DevMesh m, const int* __restrict__
                 vz[gh] = -vz[va];
                 break;
             case IBIS_BC_FIX_TEMPERATURE:  // boundary.cpp:244-277
-                temp[gh] = 2 * par[0] - temp[va];
+                temp[gh] = 2.0 * S(par[0]) - temp[va];
                 break;
             case IBIS_BC_SUBSONIC_INFLOW: {  // boundary.cpp:286-342
                 double nx = m.geom[0 * F + f], ny = m.geom[1 * F + f], nz = m.geom[2 * F + f];
-                double in_vx = par[4], in_vy = par[5], in_vz = par[6];
-                double vel_correct =
+                const S in_vx = S(par[4]), in_vy = S(par[5]), in_vz = S(par[6]);
+                const S vel_correct =
                     nx * (in_vx - vx[va]) + ny * (in_vy - vy[va]) + nz * (in_vz - vz[va]);
-                double a = sqrt(gp.gamma * gp.R * temp[va]);
-                double rho_ref = rho[va];
-                double p_in = par[1];
-                double p_face = 0.5 * (p_in + p[va] - rho_ref * a * vel_correct);
-                double rho_face = par[2] + (p_face - p_in) / a / a;
-                double vx_face = in_vx - nx * (p_in - p_face) / (rho_ref * a);
-                double vy_face = in_vy - ny * (p_in - p_face) / (rho_ref * a);
-                double vz_face = in_vz - nz * (p_in - p_face) / (rho_ref * a);
-                p[gh] = 2 * p_face - p[va];
-                rho[gh] = 2 * rho_face - rho[va];
+                const S a = sqrt(S(gp.gamma * gp.R) * temp[va]);
+                const S rho_ref = rho[va];
+                const S p_in = S(par[1]);
+                const S p_face = 0.5 * (p_in + p[va] - rho_ref * a * vel_correct);
+                const S rho_face = S(par[2]) + (p_face - p_in) / a / a;
+                const S vx_face = in_vx - nx * (p_in - p_face) / (rho_ref * a);
+                const S vy_face = in_vy - ny * (p_in - p_face) / (rho_ref * a);
+                const S vz_face = in_vz - nz * (p_in - p_face) / (rho_ref * a);
+                p[gh] = 2.0 * p_face - p[va];
+                rho[gh] = 2.0 * rho_face - rho[va];
                 temp[gh] = p[gh] / (rho[gh] * gp.R);
                 e[gh] = gp.Cv * temp[gh];
-                vx[gh] = 2 * vx_face - vx[va];
-                vy[gh] = 2 * vy_face - vy[va];
-                vz[gh] = 2 * vz_face - vz[va];
+                vx[gh] = 2.0 * vx_face - vx[va];
+                vy[gh] = 2.0 * vy_face - vy[va];
+                vz[gh] = 2.0 * vz_face - vz[va];
                 break;
             }
             case IBIS_BC_SUBSONIC_OUTFLOW: {  // boundary.cpp:348-399
                 double nx = m.geom[0 * F + f], ny = m.geom[1 * F + f], nz = m.geom[2 * F + f];
-                double p_face = par[0];
-                double rho_ref = rho[va];
-                double p_ref = p[va];
-                double a = sqrt(gp.gamma * gp.R * temp[va]);
-                double rho_face = rho[va] + (p_face - p_ref) / a / a;
-                double vx_face = vx[va] + nx * (p_ref - p_face) / (rho_ref * a);
-                double vy_face = vy[va] + ny * (p_ref - p_face) / (rho_ref * a);
-                double vz_face = vz[va] + nz * (p_ref - p_face) / (rho_ref * a);
-                p[gh] = 2 * p_face - p[va];
-                rho[gh] = 2 * rho_face - rho[va];
+                const S p_face = S(par[0]);
+                const S rho_ref = rho[va];
+                const S p_ref = p[va];
+                const S a = sqrt(S(gp.gamma * gp.R) * temp[va]);
+                const S rho_face = rho[va] + (p_face - p_ref) / a / a;
+                const S vx_face = vx[va] + nx * (p_ref - p_face) / (rho_ref * a);
+                const S vy_face = vy[va] + ny * (p_ref - p_face) / (rho_ref * a);
+                const S vz_face = vz[va] + nz * (p_ref - p_face) / (rho_ref * a);
+                p[gh] = 2.0 * p_face - p[va];
+                rho[gh] = 2.0 * rho_face - rho[va];
                 temp[gh] = p[gh] / (rho[gh] * gp.R);
                 e[gh] = gp.Cv * temp[gh];
-                vx[gh] = 2 * vx_face - vx[va];
-                vy[gh] = 2 * vy_face - vy[va];
-                vz[gh] = 2 * vz_face - vz[va];
+                vx[gh] = 2.0 * vx_face - vx[va];
+                vy[gh] = 2.0 * vy_face - vy[va];
+                vz[gh] = 2.0 * vz_face - vz[va];
                 break;
             }
             default:
@@ -242,11 +245,10 @@ struct VarSet {
 // handles variable v of cells [64*b, 64*b + 64).  The weight formulae are
 // evaluated by each wave (they depend on geometry only); per-thread state is
 // one variable, which keeps register use low enough for 8 waves per SIMD.
-template <int DIM, int LIM>
+template <class S, int DIM, int LIM>
 __global__ void __launch_bounds__(320)
-    k_cell_grad_var(DevMesh m, VarSet vs, const double* __restrict__ prim,
-                    double* __restrict__ grad, double* __restrict__ lim, double eps,
-                    double venkat_K, int nlogical) {
+    k_cell_grad_var(DevMesh m, VarSet vs, const S* __restrict__ prim, S* __restrict__ grad,
+                    S* __restrict__ lim, double eps, double venkat_K, int nlogical) {
     const int lb = xcd_block(nlogical);
     if (lb >= nlogical) return;
     const int v = threadIdx.x >> 6;
@@ -257,7 +259,7 @@ __global__ void __launch_bounds__(320)
     const double* cx = m.cen;
     const double* cy = m.cen + Nt;
     const double* cz = m.cen + 2 * (size_t)Nt;
-    const double* u = prim + (size_t)row * Nt;
+    const S* u = prim + (size_t)row * Nt;
     const double xi = cx[i], yi = cy[i], zi = cz[i];
     const int nf = m.nfaces[i];
     const double r11 = m.lsq[i], r12 = m.lsq[N + i], r22 = m.lsq[2 * (size_t)N + i];
@@ -273,8 +275,8 @@ __global__ void __launch_bounds__(320)
     const double q12 = r12 / r11;
     const double i33 = 1.0 / (r33 * r33);
     const double q23 = r23 / r22;
-    const double ui = u[i];
-    double gx = 0.0, gy = 0.0, gz = 0.0, umin = ui, umax = ui;
+    const S ui = u[i];
+    S gx = S(0.0), gy = S(0.0), gz = S(0.0), umin = ui, umax = ui;
     for (int j = 0; j < nf; j++) {
         const int n = m.ell_nbr[(size_t)j * N + i];
         const double dx = cx[n] - xi;
@@ -287,15 +289,15 @@ __global__ void __launch_bounds__(320)
         const double w_1 = alpha_1 - q12 * alpha_2 + beta * alpha_3;
         const double w_2 = alpha_2 - q23 * alpha_3;
         const double w_3 = alpha_3;
-        const double un = u[n];
-        const double diff = un - ui;
+        const S un = u[n];
+        const S diff = un - ui;
         gx += w_1 * diff;
         gy += w_2 * diff;
         gz += w_3 * diff;
         umin = ibis::kmin(umin, un);
         umax = ibis::kmax(umax, un);
     }
-    double* g = grad + (size_t)slot * 3 * N;
+    S* g = grad + (size_t)slot * 3 * N;
     if (v < vs.ngrad) {
         g[i] = gx;
         g[N + i] = gy;
@@ -306,7 +308,7 @@ __global__ void __launch_bounds__(320)
         gz = g[2 * (size_t)N + i];
     }
     if (LIM == 0) return;
-    double phi = 1.0;
+    S phi = S(1.0);
     double eps2 = 0.0;
     if (LIM == 2) {
         const double V = m.vol[i];
@@ -318,26 +320,26 @@ __global__ void __launch_bounds__(320)
         const double dx = m.geom[9 * (size_t)F + f] - xi;
         const double dy = m.geom[10 * (size_t)F + f] - yi;
         const double dz = m.geom[11 * (size_t)F + f] - zi;
-        double d2 = gx * dx + gy * dy + gz * dz;
+        S d2 = gx * dx + gy * dy + gz * dz;
         if (LIM == 1) {  // Barth-Jespersen, FV/limiter.cpp:289-303
-            const int sg = (d2 > 0) - (d2 < 0);
-            d2 = sg * (fabs(d2) + eps);
+            const int sg = (d2 > S(0.0)) - (d2 < S(0.0));
+            d2 = (double)sg * (fabs(d2) + eps);
             if (sg > 0)
                 phi = ibis::kmin(phi, (umax - ui) / d2);
             else if (sg < 0)
                 phi = ibis::kmin(phi, (umin - ui) / d2);
         } else {  // Venkatakrishnan
-            double d1;
-            if (d2 > 0.0)
+            S d1;
+            if (d2 > S(0.0))
                 d1 = umax - ui;
-            else if (d2 < 0.0)
+            else if (d2 < S(0.0))
                 d1 = umin - ui;
             else
                 continue;
-            const double d1s = d1 * d1;
-            const double d2s = d2 * d2;
-            const double num = d1s + eps2 + 2.0 * (d1 * d2);
-            const double den = d1s + 2.0 * d2s + d1 * d2 + eps2;
+            const S d1s = d1 * d1;
+            const S d2s = d2 * d2;
+            const S num = d1s + eps2 + 2.0 * (d1 * d2);
+            const S den = d1s + 2.0 * d2s + d1 * d2 + eps2;
             phi = ibis::kmin(phi, num / den);
         }
     }
@@ -355,12 +357,12 @@ struct FaceParams {
 };
 
 // value + limiter * (grad . d), FV/convective_flux.cpp:243-255
-__device__ __forceinline__ double lin_interp(double value, const double* grad, int N, int slot,
-                                             int c, bool valid, double L, double dx, double dy,
-                                             double dz) {
-    double gx = 0.0, gy = 0.0, gz = 0.0;
+template <class S>
+__device__ __forceinline__ S lin_interp(S value, const S* grad, int N, int slot, int c,
+                                        bool valid, S L, double dx, double dy, double dz) {
+    S gx = S(0.0), gy = S(0.0), gz = S(0.0);
     if (valid) {
-        const double* g = grad + (size_t)slot * 3 * N;
+        const S* g = grad + (size_t)slot * 3 * N;
         gx = g[c];
         gy = g[N + c];
         gz = g[2 * (size_t)N + c];
@@ -368,12 +370,10 @@ __device__ __forceinline__ double lin_interp(double value, const double* grad, i
     return value + L * (gx * dx + gy * dy + gz * dz);
 }
 
-template <bool RECON>
+template <class S, bool RECON>
 __device__ __forceinline__ void reconstruct(const DevMesh& m, const FaceParams& fp,
-                                            const double* __restrict__ prim,
-                                            const double* __restrict__ grad,
-                                            const double* __restrict__ lim, int f, int c,
-                                            double* st) {
+                                            const S* __restrict__ prim, const S* __restrict__ grad,
+                                            const S* __restrict__ lim, int f, int c, S* st) {
     const int N = m.N, Nt = m.Nt, F = m.F;
     if (!RECON) {  // copy_reconstruct, FV/convective_flux.cpp:212-240
 #pragma unroll
@@ -385,20 +385,20 @@ __device__ __forceinline__ void reconstruct(const DevMesh& m, const FaceParams& 
     const double dx = m.geom[9 * (size_t)F + f] - m.cen[c];
     const double dy = m.geom[10 * (size_t)F + f] - m.cen[Nt + c];
     const double dz = m.geom[11 * (size_t)F + f] - m.cen[2 * (size_t)Nt + c];
-    const double vA = prim[(size_t)fp.rowA * Nt + c];
-    const double vB = prim[(size_t)fp.rowB * Nt + c];
-    const double LA = limit ? lim[(size_t)fp.slotA * N + c] : 1.0;
-    const double LB = limit ? lim[(size_t)fp.slotB * N + c] : 1.0;
-    const double a = lin_interp(vA, grad, N, fp.slotA, c, valid, LA, dx, dy, dz);
-    const double b = lin_interp(vB, grad, N, fp.slotB, c, valid, LB, dx, dy, dz);
-    double rho = 0.0, p = 0.0, T = 0.0, e = 0.0;
+    const S vA = prim[(size_t)fp.rowA * Nt + c];
+    const S vB = prim[(size_t)fp.rowB * Nt + c];
+    const S LA = limit ? lim[(size_t)fp.slotA * N + c] : S(1.0);
+    const S LB = limit ? lim[(size_t)fp.slotB * N + c] : S(1.0);
+    const S a = lin_interp<S>(vA, grad, N, fp.slotA, c, valid, LA, dx, dy, dz);
+    const S b = lin_interp<S>(vB, grad, N, fp.slotB, c, valid, LB, dx, dy, dz);
+    S rho = S(0.0), p = S(0.0), T = S(0.0), e = S(0.0);
     switch (fp.thermo) {
         case IBIS_THERMO_RHO_P: p = a; rho = b; break;
         case IBIS_THERMO_RHO_T: rho = a; T = b; break;
         case IBIS_THERMO_RHO_U: rho = a; e = b; break;
         default: p = a; T = b; break;
     }
-    ibis::thermo_update<double>(fp.thermo, fp.R, fp.Cv, rho, p, T, e);
+    ibis::thermo_update<S>(fp.thermo, fp.R, fp.Cv, rho, p, T, e);
     st[ibis::S_RHO] = rho;
     st[ibis::S_P] = p;
     st[ibis::S_T] = T;
@@ -406,26 +406,25 @@ __device__ __forceinline__ void reconstruct(const DevMesh& m, const FaceParams& 
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         const int slot = G_VX + k;
-        const double L = limit ? lim[(size_t)slot * N + c] : 1.0;
-        st[ibis::S_U + k] =
-            lin_interp(prim[(size_t)(P_VX + k) * Nt + c], grad, N, slot, c, valid, L, dx, dy, dz);
+        const S L = limit ? lim[(size_t)slot * N + c] : S(1.0);
+        st[ibis::S_U + k] = lin_interp<S>(prim[(size_t)(P_VX + k) * Nt + c], grad, N, slot, c,
+                                          valid, L, dx, dy, dz);
     }
 }
 
 // Flux of face f in the global frame, rows in conserved order (mass,
 // momentum..., energy): reconstruction of both sides, rotation to the face
 // frame, flux calculator, rotation back (FV/convective_flux.cpp:85-178).
-template <int DIM, int FLUX, bool RECON>
+template <class S, int DIM, int FLUX, bool RECON>
 __device__ __forceinline__ void face_flux_eval(const DevMesh& m, const FaceParams& fp,
-                                               const double* __restrict__ prim,
-                                               const double* __restrict__ grad,
-                                               const double* __restrict__ lim, int f,
-                                               double* res) {
+                                               const S* __restrict__ prim,
+                                               const S* __restrict__ grad,
+                                               const S* __restrict__ lim, int f, S* res) {
     const int F = m.F;
     const int l = m.L[f], r = m.R[f];
-    double Ls[NPRIM], Rs[NPRIM];
-    reconstruct<RECON>(m, fp, prim, grad, lim, f, l, Ls);
-    reconstruct<RECON>(m, fp, prim, grad, lim, f, r, Rs);
+    S Ls[NPRIM], Rs[NPRIM];
+    reconstruct<S, RECON>(m, fp, prim, grad, lim, f, l, Ls);
+    reconstruct<S, RECON>(m, fp, prim, grad, lim, f, r, Rs);
     const double nx = m.geom[f], ny = m.geom[(size_t)F + f], nz = m.geom[2 * (size_t)F + f];
     const double t1x = m.geom[3 * (size_t)F + f], t1y = m.geom[4 * (size_t)F + f],
                  t1z = m.geom[5 * (size_t)F + f];
@@ -434,25 +433,25 @@ __device__ __forceinline__ void face_flux_eval(const DevMesh& m, const FaceParam
     // transform_to_local_frame, UT/vector3.cpp:142-153
 #pragma unroll
     for (int side = 0; side < 2; side++) {
-        double* s = side ? Rs : Ls;
-        const double a = s[4], b = s[5], c = s[6];
+        S* s = side ? Rs : Ls;
+        const S a = s[4], b = s[5], c = s[6];
         s[4] = a * nx + b * ny + c * nz;
         s[5] = a * t1x + b * t1y + c * t1z;
         s[6] = a * t2x + b * t2y + c * t2z;
     }
-    double out[5];
+    S out[5];
     constexpr bool T3 = DIM == 3;
     if (FLUX == IBIS_FLUX_AUSMDV)
-        ibis::flux_ausmdv<double, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, out);
+        ibis::flux_ausmdv<S, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, out);
     else if (FLUX == IBIS_FLUX_HANEL)
-        ibis::flux_hanel<double, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, out);
+        ibis::flux_hanel<S, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, out);
     else if (FLUX == IBIS_FLUX_LDFSS2)
-        ibis::flux_ldfss2<double, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, fp.delta, out);
+        ibis::flux_ldfss2<S, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, fp.delta, out);
     else
-        ibis::flux_rusanov<double, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, out);
+        ibis::flux_rusanov<S, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, out);
     // flux::transform_to_global, FV/convective_flux.cpp:148-177
-    const double px = out[1], py = out[2];
-    double pz = 0.0;
+    const S px = out[1], py = out[2];
+    S pz = S(0.0);
     if (T3) pz = out[3];
     res[0] = out[0];
     res[1] = px * nx + py * t1x + pz * t2x;
@@ -465,17 +464,16 @@ __device__ __forceinline__ void face_flux_eval(const DevMesh& m, const FaceParam
     }
 }
 
-template <int DIM, int FLUX, bool RECON, bool PREMUL>
+template <class S, int DIM, int FLUX, bool RECON, bool PREMUL>
 __global__ void __launch_bounds__(BLOCK)
-    k_face_flux(DevMesh m, FaceParams fp, const double* __restrict__ prim,
-                const double* __restrict__ grad, const double* __restrict__ lim,
-                double* __restrict__ flux) {
+    k_face_flux(DevMesh m, FaceParams fp, const S* __restrict__ prim, const S* __restrict__ grad,
+                const S* __restrict__ lim, S* __restrict__ flux) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     const int F = m.F;
     if (f >= F) return;
     constexpr int NC = DIM + 2;
-    double res[NC];
-    face_flux_eval<DIM, FLUX, RECON>(m, fp, prim, grad, lim, f, res);
+    S res[NC];
+    face_flux_eval<S, DIM, FLUX, RECON>(m, fp, prim, grad, lim, f, res);
     const double A = PREMUL ? m.geom[12 * (size_t)F + f] : 1.0;
 #pragma unroll
     for (int q = 0; q < NC; q++) flux[(size_t)q * F + f] = PREMUL ? res[q] * A : res[q];
@@ -502,7 +500,7 @@ __global__ void __launch_bounds__(TILE)
     for (int k = threadIdx.x; k < nfl; k += TILE) {
         const int f = m.tile_faces[f0 + k];
         double res[NC];
-        face_flux_eval<DIM, FLUX, RECON>(m, fp, prim, grad, lim, f, res);
+        face_flux_eval<double, DIM, FLUX, RECON>(m, fp, prim, grad, lim, f, res);
         const double A = m.geom[12 * (size_t)F + f];
 #pragma unroll
         for (int q = 0; q < NC; q++) sflux[q * S + k] = res[q] * A;
@@ -568,6 +566,13 @@ __global__ void __launch_bounds__(BLOCK)
 __device__ __forceinline__ size_t rec_index(const DevMesh& m, int c, int slot) {
     return c < m.N ? (size_t)c * m.maxf + slot : (size_t)m.N * m.maxf + (c - m.N);
 }
+// LAYOUT 2: slot-major structure of arrays, value v of slot j of valid cell c
+// at (j*5 + v)*N + c (one store instruction of a wavefront writes whole cache
+// lines); ghost g at maxf*5*N + v*G + (g - N).
+__device__ __forceinline__ size_t soa_index(const DevMesh& m, int c, int slot, int v) {
+    return c < m.N ? ((size_t)slot * 5 + v) * m.N + c
+                   : (size_t)m.maxf * 5 * m.N + (size_t)v * m.G + (c - m.N);
+}
 
 template <int LAYOUT>
 __global__ void __launch_bounds__(BLOCK)
@@ -585,7 +590,11 @@ __global__ void __launch_bounds__(BLOCK)
 #pragma unroll
     for (int v = 0; v < 5; v++) {
         const double gx = 0.0, gy = 0.0, gz = 0.0;
-        rec[v] = prim[(size_t)vs.row[v] * Nt + g] + 1.0 * (gx * dx + gy * dy + gz * dz);
+        const double val = prim[(size_t)vs.row[v] * Nt + g] + 1.0 * (gx * dx + gy * dy + gz * dz);
+        if (LAYOUT == 2)
+            FL[soa_index(m, g, 0, v)] = val;
+        else
+            rec[v] = val;
     }
 }
 
@@ -759,7 +768,13 @@ __global__ void __launch_bounds__(BLOCK)
         double* rec = LAYOUT == 0 ? (m.ell_sign[(size_t)j * N + i] > 0 ? FL : FR) + (size_t)f * 5
                                   : FL + ((size_t)i * m.maxf + j) * 5;
 #pragma unroll
-        for (int v = 0; v < 5; v++) rec[v] = ui[v] + phi[v] * (gx[v] * dx + gy[v] * dy + gz[v] * dz);
+        for (int v = 0; v < 5; v++) {
+            const double val = ui[v] + phi[v] * (gx[v] * dx + gy[v] * dy + gz[v] * dz);
+            if (LAYOUT == 2)
+                __builtin_nontemporal_store(val, FL + ((size_t)j * 5 + v) * N + i);
+            else
+                rec[v] = val;
+        }
     }
 }
 
@@ -781,11 +796,23 @@ __global__ void __launch_bounds__(TILE)
         double Ls[NPRIM], Rs[NPRIM];
 #pragma unroll
         for (int side = 0; side < 2; side++) {
-            const double* rec;
-            if (LAYOUT == 0)
-                rec = (side ? FR : FL) + (size_t)f * 5;
-            else
-                rec = FL + (side ? rec_index(m, m.R[f], m.slotR[f]) : rec_index(m, m.L[f], m.slotL[f])) * 5;
+            double rv[5];
+            if (LAYOUT == 0) {
+                const double* rec = (side ? FR : FL) + (size_t)f * 5;
+#pragma unroll
+                for (int v = 0; v < 5; v++) rv[v] = rec[v];
+            } else if (LAYOUT == 1) {
+                const double* rec =
+                    FL + (side ? rec_index(m, m.R[f], m.slotR[f]) : rec_index(m, m.L[f], m.slotL[f])) * 5;
+#pragma unroll
+                for (int v = 0; v < 5; v++) rv[v] = rec[v];
+            } else {
+                const int c = side ? m.R[f] : m.L[f];
+                const int sl = side ? m.slotR[f] : m.slotL[f];
+#pragma unroll
+                for (int v = 0; v < 5; v++) rv[v] = FL[soa_index(m, c, sl, v)];
+            }
+            const double* rec = rv;
             double* st = side ? Rs : Ls;
             const double a = rec[0], b = rec[1];
             double rho = 0.0, p = 0.0, T = 0.0, e = 0.0;
@@ -873,10 +900,12 @@ __global__ void __launch_bounds__(TILE)
     for (int q = 0; q < NC; q++) dudt[(size_t)q * Nt + c] = d[q] / V;
 }
 
-// ConstantFlux::apply (boundary.cpp:405-443), static grid
+// ConstantFlux::apply (boundary.cpp:405-443), static grid.  The override
+// depends on the action parameters only, so its tangent is zero.
+template <class S>
 __global__ void __launch_bounds__(BLOCK)
     k_post_flux(DevMesh m, const int* __restrict__ kinds, const double* __restrict__ pars,
-                GasParams gp, int premul, double* __restrict__ flux) {
+                GasParams gp, int premul, S* __restrict__ flux) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= m.nb) return;
     const int f = m.b_face[b], t = m.b_tag[b], F = m.F;
@@ -901,7 +930,7 @@ __global__ void __launch_bounds__(BLOCK)
         const int ne = three_d ? 5 : 4;
         for (int q = 0; q < ne; q++) {
             const double val = (!three_d && q == 3) ? v[4] : v[q];
-            flux[(size_t)q * F + f] = premul ? val * A : val;
+            flux[(size_t)q * F + f] = S(premul ? val * A : val);
         }
     }
 }
@@ -913,37 +942,45 @@ struct ViscParams {
     double R, Cv, Cp, mu0, T0, Ts, Pr;
 };
 
-__device__ __forceinline__ double sutherland(double temp, const ViscParams& vp) {
-    // GAS/transport_properties.h:27-42
-    return vp.mu0 * pow(temp / vp.T0, 3.0 / 2.0) * (vp.T0 + vp.Ts) / (temp + vp.Ts);
+// GAS/transport_properties.h:27-42; the constants have the scalar type
+template <class S>
+__device__ __forceinline__ S sutherland(S temp, const ViscParams& vp) {
+    const S mu0 = S(vp.mu0), T0 = S(vp.T0), Ts = S(vp.Ts);
+    return mu0 * pow(temp / T0, S(3.0 / 2.0)) * (T0 + Ts) / (temp + Ts);
 }
 
-__device__ __forceinline__ void hasselbacher(const double* __restrict__ grad, int N, int slot,
-                                             double vl, double vr, int l, int r,
-                                             const double* ehat, const double* n, double len_e,
-                                             double edn, double* out) {
-    const double* g = grad + (size_t)slot * 3 * N;
-    const double ax = 0.5 * (g[l] + g[r]);
-    const double ay = 0.5 * (g[N + l] + g[N + r]);
-    const double az = 0.5 * (g[2 * (size_t)N + l] + g[2 * (size_t)N + r]);
-    const double avg_dot_ehat = ax * ehat[0] + ay * ehat[1] + az * ehat[2];
-    const double correction = avg_dot_ehat - (vr - vl) / len_e;
+// GAS/transport_properties.h:64-75
+template <class S>
+__device__ __forceinline__ S conductivity(S temp, const ViscParams& vp) {
+    const S mu = sutherland<S>(temp, vp);
+    return vp.Cp * mu / S(vp.Pr);
+}
+
+template <class S>
+__device__ __forceinline__ void hasselbacher(const S* __restrict__ grad, int N, int slot, S vl,
+                                             S vr, int l, int r, const double* ehat,
+                                             const double* n, double len_e, double edn, S* out) {
+    const S* g = grad + (size_t)slot * 3 * N;
+    const S ax = 0.5 * (g[l] + g[r]);
+    const S ay = 0.5 * (g[N + l] + g[N + r]);
+    const S az = 0.5 * (g[2 * (size_t)N + l] + g[2 * (size_t)N + r]);
+    const S avg_dot_ehat = ax * ehat[0] + ay * ehat[1] + az * ehat[2];
+    const S correction = avg_dot_ehat - (vr - vl) / len_e;
     out[0] = ax - correction * n[0] / edn;
     out[1] = ay - correction * n[1] / edn;
     out[2] = az - correction * n[2] / edn;
 }
 
-template <int DIM>
+template <class S, int DIM>
 __global__ void __launch_bounds__(BLOCK)
-    k_visc_flux(DevMesh m, ViscParams vp, const double* __restrict__ prim,
-                const double* __restrict__ grad, double* __restrict__ flux,
-                double* __restrict__ face_rho, double* __restrict__ face_T) {
+    k_visc_flux(DevMesh m, ViscParams vp, const S* __restrict__ prim, const S* __restrict__ grad,
+                S* __restrict__ flux, double* __restrict__ face_rho, double* __restrict__ face_T) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     const int F = m.F, N = m.N, Nt = m.Nt;
     if (f >= F) return;
     const int l = m.L[f], r = m.R[f];
     const bool lv = l < N, rv = r < N;
-    double gT[3], gvx[3], gvy[3], gvz[3];
+    S gT[3], gvx[3], gvy[3], gvz[3];
     const double n[3] = {m.geom[f], m.geom[(size_t)F + f], m.geom[2 * (size_t)F + f]};
     if (!lv || !rv) {  // copy_gradients_to_face, viscous_flux.cpp:16-34
         const int c = lv ? l : r;
@@ -960,36 +997,38 @@ __global__ void __launch_bounds__(BLOCK)
         const double len_e = sqrt(ex * ex + ey * ey + ez * ez);
         const double ehat[3] = {ex / len_e, ey / len_e, ez / len_e};
         const double edn = ehat[0] * n[0] + ehat[1] * n[1] + ehat[2] * n[2];
-        hasselbacher(grad, N, G_VX, prim[(size_t)P_VX * Nt + l], prim[(size_t)P_VX * Nt + r], l,
-                     r, ehat, n, len_e, edn, gvx);
-        hasselbacher(grad, N, G_VY, prim[(size_t)P_VY * Nt + l], prim[(size_t)P_VY * Nt + r], l,
-                     r, ehat, n, len_e, edn, gvy);
-        hasselbacher(grad, N, G_VZ, prim[(size_t)P_VZ * Nt + l], prim[(size_t)P_VZ * Nt + r], l,
-                     r, ehat, n, len_e, edn, gvz);
-        hasselbacher(grad, N, G_T, prim[(size_t)P_T * Nt + l], prim[(size_t)P_T * Nt + r], l, r,
-                     ehat, n, len_e, edn, gT);
+        hasselbacher<S>(grad, N, G_VX, prim[(size_t)P_VX * Nt + l], prim[(size_t)P_VX * Nt + r], l,
+                        r, ehat, n, len_e, edn, gvx);
+        hasselbacher<S>(grad, N, G_VY, prim[(size_t)P_VY * Nt + l], prim[(size_t)P_VY * Nt + r], l,
+                        r, ehat, n, len_e, edn, gvy);
+        hasselbacher<S>(grad, N, G_VZ, prim[(size_t)P_VZ * Nt + l], prim[(size_t)P_VZ * Nt + r], l,
+                        r, ehat, n, len_e, edn, gvz);
+        hasselbacher<S>(grad, N, G_T, prim[(size_t)P_T * Nt + l], prim[(size_t)P_T * Nt + r], l, r,
+                        ehat, n, len_e, edn, gT);
     }
     // average_flow_states_pT + update_thermo_from_pT (GAS/flow_state.h:89-91)
-    const double p = 0.5 * (prim[(size_t)P_P * Nt + l] + prim[(size_t)P_P * Nt + r]);
-    const double temp = 0.5 * (prim[(size_t)P_T * Nt + l] + prim[(size_t)P_T * Nt + r]);
-    const double vx = 0.5 * (prim[(size_t)P_VX * Nt + l] + prim[(size_t)P_VX * Nt + r]);
-    const double vy = 0.5 * (prim[(size_t)P_VY * Nt + l] + prim[(size_t)P_VY * Nt + r]);
-    const double vz = 0.5 * (prim[(size_t)P_VZ * Nt + l] + prim[(size_t)P_VZ * Nt + r]);
-    face_rho[f] = p / (vp.R * temp);
-    face_T[f] = temp;
-    const double mu = sutherland(temp, vp);
-    const double k = vp.Cp * sutherland(temp, vp) / vp.Pr;
-    const double lambda = -2.0 / 3.0 * mu;
-    const double bulk = lambda * (gvx[0] + gvy[1] + gvz[2]);
-    const double tau_xx = 2.0 * mu * gvx[0] + bulk;
-    const double tau_yy = 2.0 * mu * gvy[1] + bulk;
-    const double tau_zz = 2.0 * mu * gvz[2] + bulk;
-    const double tau_xy = mu * (gvx[1] + gvy[0]);
-    const double tau_xz = mu * (gvx[2] + gvz[0]);
-    const double tau_yz = mu * (gvy[2] + gvz[1]);
-    const double theta_x = vx * tau_xx + vy * tau_xy + vz * tau_xz + k * gT[0];
-    const double theta_y = vx * tau_xy + vy * tau_yy + vz * tau_yz + k * gT[1];
-    const double theta_z = vx * tau_xz + vy * tau_yz + vz * tau_zz + k * gT[2];
+    const S p = 0.5 * (prim[(size_t)P_P * Nt + l] + prim[(size_t)P_P * Nt + r]);
+    const S temp = 0.5 * (prim[(size_t)P_T * Nt + l] + prim[(size_t)P_T * Nt + r]);
+    const S vx = 0.5 * (prim[(size_t)P_VX * Nt + l] + prim[(size_t)P_VX * Nt + r]);
+    const S vy = 0.5 * (prim[(size_t)P_VY * Nt + l] + prim[(size_t)P_VY * Nt + r]);
+    const S vz = 0.5 * (prim[(size_t)P_VZ * Nt + l] + prim[(size_t)P_VZ * Nt + r]);
+    if (face_rho) {
+        face_rho[f] = ibis::real_part(p / (vp.R * temp));
+        face_T[f] = ibis::real_part(temp);
+    }
+    const S mu = sutherland<S>(temp, vp);
+    const S k = conductivity<S>(temp, vp);
+    const S lambda = -2.0 / 3.0 * mu;
+    const S bulk = lambda * (gvx[0] + gvy[1] + gvz[2]);
+    const S tau_xx = 2.0 * mu * gvx[0] + bulk;
+    const S tau_yy = 2.0 * mu * gvy[1] + bulk;
+    const S tau_zz = 2.0 * mu * gvz[2] + bulk;
+    const S tau_xy = mu * (gvx[1] + gvy[0]);
+    const S tau_xz = mu * (gvx[2] + gvz[0]);
+    const S tau_yz = mu * (gvy[2] + gvz[1]);
+    const S theta_x = vx * tau_xx + vy * tau_xy + vz * tau_xz + k * gT[0];
+    const S theta_y = vx * tau_xy + vy * tau_yy + vz * tau_yz + k * gT[1];
+    const S theta_z = vx * tau_xz + vy * tau_yz + vz * tau_zz + k * gT[2];
     flux[(size_t)F + f] -= n[0] * tau_xx + n[1] * tau_xy + n[2] * tau_xz;
     flux[2 * (size_t)F + f] -= n[0] * tau_xy + n[1] * tau_yy + n[2] * tau_yz;
     if (DIM == 3) {
@@ -1005,16 +1044,16 @@ __global__ void __launch_bounds__(BLOCK)
 // face order.  With PREMUL the flux already holds flux*area and
 // flux*(-area*outsign) == (outsign > 0 ? -(flux*area) : flux*area) exactly.
 // ---------------------------------------------------------------------------
-template <int DIM, bool PREMUL>
+template <class S, int DIM, bool PREMUL>
 __global__ void __launch_bounds__(BLOCK)
-    k_integral(DevMesh m, const double* __restrict__ flux, double* __restrict__ dudt) {
+    k_integral(DevMesh m, const S* __restrict__ flux, S* __restrict__ dudt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int N = m.N, Nt = m.Nt, F = m.F;
     if (i >= N) return;
     constexpr int NC = DIM + 2;
-    double d[NC];
+    S d[NC];
 #pragma unroll
-    for (int q = 0; q < NC; q++) d[q] = 0.0;
+    for (int q = 0; q < NC; q++) d[q] = S(0.0);
     const int nf = m.nfaces[i];
     for (int j = 0; j < nf; j++) {
         const int f = m.ell_face[(size_t)j * N + i];
@@ -1022,7 +1061,7 @@ __global__ void __launch_bounds__(BLOCK)
         if (PREMUL) {
 #pragma unroll
             for (int q = 0; q < NC; q++) {
-                const double fa = flux[(size_t)q * F + f];
+                const S fa = flux[(size_t)q * F + f];
                 d[q] += (s > 0) ? -fa : fa;
             }
         } else {
@@ -1113,19 +1152,18 @@ __global__ void __launch_bounds__(BLOCK)
 }
 
 // conversions, FV/primative_conserved_conversion.cpp:5-51
-template <int DIM>
+template <class S, int DIM>
 __global__ void __launch_bounds__(BLOCK)
-    k_cons_to_prim(int Nt, double R, double Cv, const double* __restrict__ cq,
-                   double* __restrict__ fs) {
+    k_cons_to_prim(int Nt, double R, double Cv, const S* __restrict__ cq, S* __restrict__ fs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Nt) return;
-    const double rho = cq[i];
-    const double vx = cq[(size_t)Nt + i] / rho;
-    const double vy = cq[2 * (size_t)Nt + i] / rho;
-    const double vz = DIM == 3 ? cq[3 * (size_t)Nt + i] / rho : 0.0;
-    const double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
-    const double u = cq[(size_t)(DIM + 1) * Nt + i] / rho - ke;
-    const double temp = u / Cv;
+    const S rho = cq[i];
+    const S vx = cq[(size_t)Nt + i] / rho;
+    const S vy = cq[2 * (size_t)Nt + i] / rho;
+    const S vz = DIM == 3 ? cq[3 * (size_t)Nt + i] / rho : S(0.0);
+    const S ke = 0.5 * (vx * vx + vy * vy + vz * vz);
+    const S u = cq[(size_t)(DIM + 1) * Nt + i] / rho - ke;
+    const S temp = u / Cv;
     fs[(size_t)P_RHO * Nt + i] = rho;
     fs[(size_t)P_VX * Nt + i] = vx;
     fs[(size_t)P_VY * Nt + i] = vy;
@@ -1149,6 +1187,62 @@ __global__ void __launch_bounds__(BLOCK)
     if (DIM == 3) cq[3 * (size_t)Nt + i] = rho * vz;
     const double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
     cq[(size_t)(DIM + 1) * Nt + i] = rho * (ke + fs[(size_t)P_E * Nt + i]);
+}
+
+// SteadyStateLinearisation::matrix_vector_product set_dual
+// (steady_state.cpp:58-67): value = cons, tangent = vec[cell*ncons + q] for
+// valid cells; ghost rows hold zero (they are filled by the boundary
+// actions after the conversion).
+__global__ void __launch_bounds__(BLOCK)
+    k_seed_cons(int N, int Nt, int nc, const double* __restrict__ cons,
+                const double* __restrict__ vec, ibis::Dual* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nt) return;
+    for (int q = 0; q < nc; q++) {
+        const size_t k = (size_t)q * Nt + i;
+        out[k] = i < N ? ibis::Dual(cons[k], vec[(size_t)i * nc + q]) : ibis::Dual(0.0, 0.0);
+    }
+}
+
+// set_vector (steady_state.cpp:102-111): 1/dt* * v - dual(R)
+__global__ void __launch_bounds__(BLOCK)
+    k_jv_result(int N, int Nt, int nc, double dt_star, const double* __restrict__ vec,
+                const ibis::Dual* __restrict__ dudt, double* __restrict__ result) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    for (int q = 0; q < nc; q++) {
+        const size_t v = (size_t)i * nc + q;
+        result[v] = 1 / dt_star * vec[v] - dudt[(size_t)q * Nt + i].d;
+    }
+}
+
+// eval_rhs (steady_state.cpp:141-148): rhs[cell*ncons + q] = R(cell, q)
+__global__ void __launch_bounds__(BLOCK)
+    k_rhs_vector(int N, int Nt, int nc, const double* __restrict__ dudt,
+                 double* __restrict__ rhs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    for (int q = 0; q < nc; q++) rhs[(size_t)i * nc + q] = dudt[(size_t)q * Nt + i];
+}
+
+// (value, tangent) pairs <-> separate value and tangent arrays
+__global__ void __launch_bounds__(BLOCK)
+    k_to_dual(long n, const double* __restrict__ x, const double* __restrict__ xd,
+              ibis::Dual* __restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = ibis::Dual(x[i], xd[i]);
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_from_dual(int rows, int n, int stride, const ibis::Dual* __restrict__ in,
+                double* __restrict__ x, double* __restrict__ xd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int q = 0; q < rows; q++) {
+        const ibis::Dual v = in[(size_t)q * stride + i];
+        x[(size_t)q * stride + i] = v.r;
+        xd[(size_t)q * stride + i] = v.d;
+    }
 }
 
 // u1 = u0 + k*dt  (apply_time_derivative, FV/conserved_quantities.cpp:80-94)
@@ -1405,13 +1499,21 @@ struct ibis_residual {
     std::vector<double*> k;
     double* k_tmp = nullptr;
     double* flow_tmp = nullptr;
+    // tangent (JVP) state and scratch, allocated on first use
+    ibis::Dual* dual_prim = nullptr;
+    ibis::Dual* dual_dudt = nullptr;
+    ibis::Dual* dual_grad = nullptr;
+    ibis::Dual* dual_lim = nullptr;
+    ibis::Dual* dual_flux = nullptr;
 };
 
 void ibis_residual_destroy(ibis_residual* r) {
     if (!r) return;
     for (void* p : {(void*)r->grad, (void*)r->lim, (void*)r->flux, (void*)r->pack, (void*)r->FL, (void*)r->FR, (void*)r->offs, (void*)r->face_rho,
                     (void*)r->face_T, (void*)r->kinds, (void*)r->pars, (void*)r->partial,
-                    (void*)r->dt_dev, (void*)r->count_dev, (void*)r->k_tmp, (void*)r->flow_tmp})
+                    (void*)r->dt_dev, (void*)r->count_dev, (void*)r->k_tmp, (void*)r->flow_tmp,
+                    (void*)r->dual_prim, (void*)r->dual_dudt, (void*)r->dual_grad,
+                    (void*)r->dual_lim, (void*)r->dual_flux})
         if (p) (void)hipFree(p);
     for (double* p : r->k) (void)hipFree(p);
     for (auto& e : r->pending) {
@@ -1503,18 +1605,18 @@ ibis_status ibis_residual_create(const ibis_mesh* m, const ibis_fv_desc* desc,
 
 namespace {
 
-template <int DIM>
-void launch_cell_grad(const DevMesh& D, const VarSet& vs, int lim, const double* prim,
-                      double* grad, double* limv, double eps, double K, hipStream_t st) {
+template <class S, int DIM>
+void launch_cell_grad(const DevMesh& D, const VarSet& vs, int lim, const S* prim, S* grad,
+                      S* limv, double eps, double K, hipStream_t st) {
     const int nlog = (D.N + 63) / 64;
     const unsigned g = xcd_grid(nlog);
     const unsigned b = 64 * vs.n;
     if (lim == 0)
-        k_cell_grad_var<DIM, 0><<<g, b, 0, st>>>(D, vs, prim, grad, limv, eps, K, nlog);
+        k_cell_grad_var<S, DIM, 0><<<g, b, 0, st>>>(D, vs, prim, grad, limv, eps, K, nlog);
     else if (lim == 1)
-        k_cell_grad_var<DIM, 1><<<g, b, 0, st>>>(D, vs, prim, grad, limv, eps, K, nlog);
+        k_cell_grad_var<S, DIM, 1><<<g, b, 0, st>>>(D, vs, prim, grad, limv, eps, K, nlog);
     else
-        k_cell_grad_var<DIM, 2><<<g, b, 0, st>>>(D, vs, prim, grad, limv, eps, K, nlog);
+        k_cell_grad_var<S, DIM, 2><<<g, b, 0, st>>>(D, vs, prim, grad, limv, eps, K, nlog);
 }
 
 template <int DIM, int FLUX>
@@ -1548,40 +1650,39 @@ void launch_face_tile(const DevMesh& D, int flux_kind, const FaceParams& fp, boo
     }
 }
 
-template <int DIM, int FLUX>
+template <class S, int DIM, int FLUX>
 void launch_face_flux_f(const DevMesh& D, const FaceParams& fp, bool recon, bool premul,
-                        const double* prim, const double* grad, const double* lim, double* flux,
-                        hipStream_t st) {
+                        const S* prim, const S* grad, const S* lim, S* flux, hipStream_t st) {
     const unsigned nb = nblocks(D.F);
     if (recon) {
         if (premul)
-            k_face_flux<DIM, FLUX, true, true><<<nb, BLOCK, 0, st>>>(D, fp, prim, grad, lim, flux);
+            k_face_flux<S, DIM, FLUX, true, true><<<nb, BLOCK, 0, st>>>(D, fp, prim, grad, lim, flux);
         else
-            k_face_flux<DIM, FLUX, true, false><<<nb, BLOCK, 0, st>>>(D, fp, prim, grad, lim, flux);
+            k_face_flux<S, DIM, FLUX, true, false><<<nb, BLOCK, 0, st>>>(D, fp, prim, grad, lim, flux);
     } else {
         if (premul)
-            k_face_flux<DIM, FLUX, false, true><<<nb, BLOCK, 0, st>>>(D, fp, prim, grad, lim, flux);
+            k_face_flux<S, DIM, FLUX, false, true><<<nb, BLOCK, 0, st>>>(D, fp, prim, grad, lim, flux);
         else
-            k_face_flux<DIM, FLUX, false, false><<<nb, BLOCK, 0, st>>>(D, fp, prim, grad, lim, flux);
+            k_face_flux<S, DIM, FLUX, false, false><<<nb, BLOCK, 0, st>>>(D, fp, prim, grad, lim, flux);
     }
 }
 
-template <int DIM>
+template <class S, int DIM>
 void launch_face_flux(const DevMesh& D, int flux_kind, const FaceParams& fp, bool recon,
-                      bool premul, const double* prim, const double* grad, const double* lim,
-                      double* flux, hipStream_t st) {
+                      bool premul, const S* prim, const S* grad, const S* lim, S* flux,
+                      hipStream_t st) {
     switch (flux_kind) {
         case IBIS_FLUX_HANEL:
-            launch_face_flux_f<DIM, IBIS_FLUX_HANEL>(D, fp, recon, premul, prim, grad, lim, flux, st);
+            launch_face_flux_f<S, DIM, IBIS_FLUX_HANEL>(D, fp, recon, premul, prim, grad, lim, flux, st);
             break;
         case IBIS_FLUX_AUSMDV:
-            launch_face_flux_f<DIM, IBIS_FLUX_AUSMDV>(D, fp, recon, premul, prim, grad, lim, flux, st);
+            launch_face_flux_f<S, DIM, IBIS_FLUX_AUSMDV>(D, fp, recon, premul, prim, grad, lim, flux, st);
             break;
         case IBIS_FLUX_LDFSS2:
-            launch_face_flux_f<DIM, IBIS_FLUX_LDFSS2>(D, fp, recon, premul, prim, grad, lim, flux, st);
+            launch_face_flux_f<S, DIM, IBIS_FLUX_LDFSS2>(D, fp, recon, premul, prim, grad, lim, flux, st);
             break;
         default:
-            launch_face_flux_f<DIM, IBIS_FLUX_RUSANOV>(D, fp, recon, premul, prim, grad, lim, flux, st);
+            launch_face_flux_f<S, DIM, IBIS_FLUX_RUSANOV>(D, fp, recon, premul, prim, grad, lim, flux, st);
             break;
     }
 }
@@ -1601,7 +1702,9 @@ void launch_cell_fs_l(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
         case 0: launch_cell_fs_t<DIM, LIM, 0, false>(D, vs, d, r, nlog, st); break;
         case 1: launch_cell_fs_t<DIM, LIM, 0, true>(D, vs, d, r, nlog, st); break;
         case 2: launch_cell_fs_t<DIM, LIM, 1, false>(D, vs, d, r, nlog, st); break;
-        default: launch_cell_fs_t<DIM, LIM, 1, true>(D, vs, d, r, nlog, st); break;
+        case 3: launch_cell_fs_t<DIM, LIM, 1, true>(D, vs, d, r, nlog, st); break;
+        case 4: launch_cell_fs_t<DIM, LIM, 2, false>(D, vs, d, r, nlog, st); break;
+        default: launch_cell_fs_t<DIM, LIM, 2, true>(D, vs, d, r, nlog, st); break;
     }
 }
 
@@ -1623,8 +1726,10 @@ void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, const 
     const size_t shm = (size_t)(DIM + 2) * D.maxtf * sizeof(double);
     if (layout == 0)
         k_face_tile_fs<DIM, FLUX, 0><<<g, TILE, shm, st>>>(D, fp, FL, FR, dudt);
-    else
+    else if (layout == 1)
         k_face_tile_fs<DIM, FLUX, 1><<<g, TILE, shm, st>>>(D, fp, FL, FR, dudt);
+    else
+        k_face_tile_fs<DIM, FLUX, 2><<<g, TILE, shm, st>>>(D, fp, FL, FR, dudt);
 }
 
 template <int DIM>
@@ -1674,172 +1779,335 @@ static hipEvent_t take_event(ibis_residual* r) {
     return e;
 }
 
+namespace {
+
+// per-kernel event timing of one residual call
+struct Prof {
+    ibis_residual* r;
+    hipStream_t st;
+    hipEvent_t a = nullptr;
+    void beg() {
+        if (!r->profiling) return;
+        a = take_event(r);
+        (void)hipEventRecord(a, st);
+    }
+    void end(int slot) {
+        if (!r->profiling) return;
+        hipEvent_t b = take_event(r);
+        (void)hipEventRecord(b, st);
+        r->pending.push_back({slot, a, b});
+    }
+};
+
+template <class S>
+struct GenBufs {
+    S* grad;
+    S* lim;
+    S* flux;
+    double* face_rho;  // viscous face states for estimate_dt (null: not kept)
+    double* face_T;
+};
+
+void residual_params(const ibis_residual* r, FaceParams* fp, VarSet* vs) {
+    const ibis_fv_desc& d = r->desc;
+    fp->thermo = d.thermo;
+    fp->limited = d.limiter != IBIS_LIM_UNLIMITED;
+    thermo_pair(d.thermo, &fp->slotA, &fp->rowA, &fp->slotB, &fp->rowB);
+    fp->R = d.R;
+    fp->Cv = d.Cv;
+    fp->gamma = d.gamma;
+    fp->delta = d.ldfss_delta;
+    vs->n = 5;
+    vs->ngrad = r->m->d.dim == 3 ? 5 : 4;
+    vs->row[0] = fp->rowA;
+    vs->slot[0] = fp->slotA;
+    vs->row[1] = fp->rowB;
+    vs->slot[1] = fp->slotB;
+    vs->row[2] = P_VX;
+    vs->slot[2] = G_VX;
+    vs->row[3] = P_VY;
+    vs->slot[3] = G_VY;
+    vs->row[4] = P_VZ;
+    vs->slot[4] = G_VZ;
+}
+
+// The residual as the reference's sequence of passes, on scalar S: ghost
+// fill, gradients + limiters, face flux, flux override, viscous ghost fill,
+// viscous gradients and flux, surface integral (FV/finite_volume.cpp:57-96).
+template <class S>
+void run_general(ibis_residual* r, const GenBufs<S>& B, S* prim, S* dudt, bool recon,
+                 hipStream_t st, Prof& pf) {
+    const DevMesh& D = r->m->d;
+    const ibis_fv_desc& d = r->desc;
+    const size_t ntab = IBIS_MAX_TAGS * IBIS_MAX_ACT;
+    const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
+    FaceParams fp;
+    VarSet vs;
+    residual_params(r, &fp, &vs);
+    const bool has_post = D.nb > 0 && has_kind(d.post_kind, d.ntags, IBIS_BC_CONSTANT_FLUX);
+    if (D.nb > 0 && has_any(d.pre_kind, d.ntags)) {
+        pf.beg();
+        k_bc<S><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, prim);
+        pf.end(0);
+    }
+    if (recon) {
+        pf.beg();
+        if (D.dim == 3)
+            launch_cell_grad<S, 3>(D, vs, d.limiter, prim, B.grad, B.lim, d.lim_eps, d.venkat_K, st);
+        else
+            launch_cell_grad<S, 2>(D, vs, d.limiter, prim, B.grad, B.lim, d.lim_eps, d.venkat_K, st);
+        pf.end(1);
+    }
+    pf.beg();
+    if (D.dim == 3)
+        launch_face_flux<S, 3>(D, d.flux, fp, recon, r->premul, prim, B.grad, B.lim, B.flux, st);
+    else
+        launch_face_flux<S, 2>(D, d.flux, fp, recon, r->premul, prim, B.grad, B.lim, B.flux, st);
+    pf.end(2);
+    if (has_post) {
+        pf.beg();
+        k_post_flux<S><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds + 2 * ntab,
+                                                        r->pars + 2 * ntab * IBIS_NPAR, gp,
+                                                        r->premul ? 1 : 0, B.flux);
+        pf.end(3);
+    }
+    if (d.viscous) {
+        if (D.nb > 0 && has_any(d.visc_kind, d.ntags)) {
+            pf.beg();
+            k_bc<S><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds + ntab,
+                                                     r->pars + ntab * IBIS_NPAR, gp, prim);
+            pf.end(4);
+        }
+        VarSet vv;
+        vv.n = 4;
+        vv.ngrad = 4;
+        const int rows[5] = {P_T, P_VX, P_VY, P_VZ, P_VZ};
+        const int slots[5] = {G_T, G_VX, G_VY, G_VZ, G_VZ};
+        for (int q = 0; q < 5; q++) {
+            vv.row[q] = rows[q];
+            vv.slot[q] = slots[q];
+        }
+        ViscParams vp{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr};
+        pf.beg();
+        if (D.dim == 3)
+            launch_cell_grad<S, 3>(D, vv, 0, prim, B.grad, B.lim, 0.0, 0.0, st);
+        else
+            launch_cell_grad<S, 2>(D, vv, 0, prim, B.grad, B.lim, 0.0, 0.0, st);
+        pf.end(5);
+        pf.beg();
+        if (D.dim == 3)
+            k_visc_flux<S, 3><<<nblocks(D.F), BLOCK, 0, st>>>(D, vp, prim, B.grad, B.flux,
+                                                              B.face_rho, B.face_T);
+        else
+            k_visc_flux<S, 2><<<nblocks(D.F), BLOCK, 0, st>>>(D, vp, prim, B.grad, B.flux,
+                                                              B.face_rho, B.face_T);
+        pf.end(6);
+    }
+    const unsigned nbc = nblocks(D.N);
+    pf.beg();
+    if (D.dim == 3) {
+        if (r->premul)
+            k_integral<S, 3, true><<<nbc, BLOCK, 0, st>>>(D, B.flux, dudt);
+        else
+            k_integral<S, 3, false><<<nbc, BLOCK, 0, st>>>(D, B.flux, dudt);
+    } else {
+        if (r->premul)
+            k_integral<S, 2, true><<<nbc, BLOCK, 0, st>>>(D, B.flux, dudt);
+        else
+            k_integral<S, 2, false><<<nbc, BLOCK, 0, st>>>(D, B.flux, dudt);
+    }
+    pf.end(7);
+}
+
+}  // namespace
+
 ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
                               int allow_reconstruction, ibis_stream_t stream) {
     if (!r || !prim || !dudt) return ibis::fail(IBIS_ERR_INVALID, "ibis_compute_dudt: null argument");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const DevMesh& D = r->m->d;
     const ibis_fv_desc& d = r->desc;
-    const size_t ntab = IBIS_MAX_TAGS * IBIS_MAX_ACT;
-    const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
     const bool recon = allow_reconstruction && d.order == 2;
-    hipEvent_t ev_a = nullptr;
-    auto beg = [&]() {
-        if (!r->profiling) return;
-        ev_a = take_event(r);
-        (void)hipEventRecord(ev_a, st);
-    };
-    auto end = [&](int slot) {
-        if (!r->profiling) return;
-        hipEvent_t b = take_event(r);
-        (void)hipEventRecord(b, st);
-        r->pending.push_back({slot, ev_a, b});
-    };
-    // 1. pre-reconstruction ghost fill
-    if (D.nb > 0 && has_any(d.pre_kind, d.ntags)) {
-        beg();
-        k_bc<<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, prim);
-        end(0);
-    }
-    // 2. gradients and limiters
+    Prof pf{r, st};
     FaceParams fp;
-    fp.thermo = d.thermo;
-    fp.limited = d.limiter != IBIS_LIM_UNLIMITED;
-    thermo_pair(d.thermo, &fp.slotA, &fp.rowA, &fp.slotB, &fp.rowB);
-    fp.R = d.R;
-    fp.Cv = d.Cv;
-    fp.gamma = d.gamma;
-    fp.delta = d.ldfss_delta;
     VarSet vs;
-    vs.n = 5;
-    vs.ngrad = D.dim == 3 ? 5 : 4;
-    vs.row[0] = fp.rowA;
-    vs.slot[0] = fp.slotA;
-    vs.row[1] = fp.rowB;
-    vs.slot[1] = fp.slotB;
-    vs.row[2] = P_VX;
-    vs.slot[2] = G_VX;
-    vs.row[3] = P_VY;
-    vs.slot[3] = G_VY;
-    vs.row[4] = P_VZ;
-    vs.slot[4] = G_VZ;
+    residual_params(r, &fp, &vs);
     const bool has_post = D.nb > 0 && has_kind(d.post_kind, d.ntags, IBIS_BC_CONSTANT_FLUX);
     const bool fused = !d.viscous && !has_post;
-    if (fused && recon) {
-        // face-state path: pack, ghost face states, cell reconstruction,
-        // fused face flux + integral
-        beg();
-        k_pack<<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, vs, prim, r->pack);
-        if (D.nb > 0) {
-            if (r->opt_layout == 0)
-                k_ghost_fs<0><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
-            else
-                k_ghost_fs<1><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
-        }
-        end(0);
-        const int nlog = (D.N + BLOCK - 1) / BLOCK;
-        beg();
-        if (D.dim == 3)
-            launch_cell_fs<3>(D, vs, d, r, nlog, prim, st);
-        else
-            launch_cell_fs<2>(D, vs, d, r, nlog, prim, st);
-        end(1);
-        beg();
-        if (D.dim == 3)
-            launch_face_fs<3>(D, fp, d.flux, r->opt_layout, r->FL, r->FR, dudt, st);
-        else
-            launch_face_fs<2>(D, fp, d.flux, r->opt_layout, r->FL, r->FR, dudt, st);
-        end(2);
-        HIP_TRY(hipGetLastError());
-        return IBIS_OK;
-    }
-    if (recon) {
-        beg();
-        if (D.dim == 3)
-            launch_cell_grad<3>(D, vs, d.limiter, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, st);
-        else
-            launch_cell_grad<2>(D, vs, d.limiter, prim, r->grad, r->lim, d.lim_eps, d.venkat_K, st);
-        end(1);
-    }
-    // 3. convective face flux.  Without viscous or flux-override actions the
-    //    face flux and the surface integral run as one tiled kernel.
     if (fused) {
-        beg();
-        if (D.dim == 3)
-            launch_face_tile<3>(D, d.flux, fp, recon, prim, r->grad, r->lim, dudt, st);
-        else
-            launch_face_tile<2>(D, d.flux, fp, recon, prim, r->grad, r->lim, dudt, st);
-        end(2);
+        const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
+        if (D.nb > 0 && has_any(d.pre_kind, d.ntags)) {
+            pf.beg();
+            k_bc<double><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, prim);
+            pf.end(0);
+        }
+        if (recon) {
+            // face-state path: pack, ghost face states, cell reconstruction,
+            // fused face flux + integral
+            pf.beg();
+            k_pack<<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, vs, prim, r->pack);
+            if (D.nb > 0) {
+                if (r->opt_layout == 0)
+                    k_ghost_fs<0><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
+                else if (r->opt_layout == 1)
+                    k_ghost_fs<1><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
+                else
+                    k_ghost_fs<2><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
+            }
+            pf.end(0);
+            const int nlog = (D.N + BLOCK - 1) / BLOCK;
+            pf.beg();
+            if (D.dim == 3)
+                launch_cell_fs<3>(D, vs, d, r, nlog, prim, st);
+            else
+                launch_cell_fs<2>(D, vs, d, r, nlog, prim, st);
+            pf.end(1);
+            pf.beg();
+            if (D.dim == 3)
+                launch_face_fs<3>(D, fp, d.flux, r->opt_layout, r->FL, r->FR, dudt, st);
+            else
+                launch_face_fs<2>(D, fp, d.flux, r->opt_layout, r->FL, r->FR, dudt, st);
+            pf.end(2);
+        } else {
+            // first order: face flux and surface integral as one tiled kernel
+            pf.beg();
+            if (D.dim == 3)
+                launch_face_tile<3>(D, d.flux, fp, false, prim, r->grad, r->lim, dudt, st);
+            else
+                launch_face_tile<2>(D, d.flux, fp, false, prim, r->grad, r->lim, dudt, st);
+            pf.end(2);
+        }
         HIP_TRY(hipGetLastError());
         return IBIS_OK;
     }
-    beg();
-    if (D.dim == 3)
-        launch_face_flux<3>(D, d.flux, fp, recon, r->premul, prim, r->grad, r->lim, r->flux, st);
-    else
-        launch_face_flux<2>(D, d.flux, fp, recon, r->premul, prim, r->grad, r->lim, r->flux, st);
-    end(2);
-    // 4. post-convective flux actions
-    if (has_post) {
-        beg();
-        k_post_flux<<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds + 2 * ntab,
-                                                     r->pars + 2 * ntab * IBIS_NPAR, gp,
-                                                     r->premul ? 1 : 0, r->flux);
-        end(3);
-    }
-    // 5. viscous flux
-    if (d.viscous) {
-        if (D.nb > 0 && has_any(d.visc_kind, d.ntags)) {
-            beg();
-            k_bc<<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds + ntab, r->pars + ntab * IBIS_NPAR,
-                                                  gp, prim);
-            end(4);
-        }
-        VarSet vs;
-        vs.n = 4;
-        vs.ngrad = 4;
-        vs.row[0] = P_T;
-        vs.slot[0] = G_T;
-        vs.row[1] = P_VX;
-        vs.slot[1] = G_VX;
-        vs.row[2] = P_VY;
-        vs.slot[2] = G_VY;
-        vs.row[3] = P_VZ;
-        vs.slot[3] = G_VZ;
-        vs.row[4] = P_VZ;
-        vs.slot[4] = G_VZ;
-        ViscParams vp{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr};
-        beg();
-        if (D.dim == 3)
-            launch_cell_grad<3>(D, vs, 0, prim, r->grad, r->lim, 0.0, 0.0, st);
-        else
-            launch_cell_grad<2>(D, vs, 0, prim, r->grad, r->lim, 0.0, 0.0, st);
-        end(5);
-        beg();
-        if (D.dim == 3)
-            k_visc_flux<3><<<nblocks(D.F), BLOCK, 0, st>>>(D, vp, prim, r->grad, r->flux,
-                                                           r->face_rho, r->face_T);
-        else
-            k_visc_flux<2><<<nblocks(D.F), BLOCK, 0, st>>>(D, vp, prim, r->grad, r->flux,
-                                                           r->face_rho, r->face_T);
-        end(6);
-    }
-    // 6. surface integral
-    const unsigned nbc = nblocks(D.N);
-    beg();
-    if (D.dim == 3) {
-        if (r->premul)
-            k_integral<3, true><<<nbc, BLOCK, 0, st>>>(D, r->flux, dudt);
-        else
-            k_integral<3, false><<<nbc, BLOCK, 0, st>>>(D, r->flux, dudt);
-    } else {
-        if (r->premul)
-            k_integral<2, true><<<nbc, BLOCK, 0, st>>>(D, r->flux, dudt);
-        else
-            k_integral<2, false><<<nbc, BLOCK, 0, st>>>(D, r->flux, dudt);
-    }
-    end(7);
+    GenBufs<double> B{r->grad, r->lim, r->flux, r->face_rho, r->face_T};
+    run_general<double>(r, B, prim, dudt, recon, st, pf);
     HIP_TRY(hipGetLastError());
+    return IBIS_OK;
+}
+
+// tangent state and scratch, allocated (and zeroed) on first use
+static hipError_t ensure_dual(ibis_residual* r) {
+    if (r->dual_prim) return hipSuccess;
+    const DevMesh& D = r->m->d;
+    const size_t N = D.N, F = D.F, Nt = D.Nt;
+    const int nc = D.dim + 2;
+    const size_t sz = sizeof(ibis::Dual);
+    ibis::Dual** ptrs[5] = {&r->dual_prim, &r->dual_dudt, &r->dual_grad, &r->dual_lim,
+                            &r->dual_flux};
+    const size_t bytes[5] = {sz * NPRIM * Nt, sz * nc * Nt, sz * NGRAD * 3 * N, sz * NGRAD * N,
+                             sz * nc * F};
+    for (int q = 0; q < 5; q++) {
+        const size_t b = std::max<size_t>(bytes[q], 16);
+        hipError_t e = hipMalloc((void**)ptrs[q], b);
+        if (e == hipSuccess) e = hipMemset(*ptrs[q], 0, b);
+        if (e != hipSuccess) {
+            for (int k = 0; k <= q; k++) {
+                if (*ptrs[k]) (void)hipFree(*ptrs[k]);
+                *ptrs[k] = nullptr;
+            }
+            return e;
+        }
+    }
+    return hipSuccess;
+}
+
+// Tangent of the residual: the same passes on (value, tangent) pairs with
+// the reference's Dual arithmetic (UT/dual.h), replacing FiniteVolume<dual>.
+ibis_status ibis_compute_dudt_jvp(ibis_residual* r, double* prim, double* prim_dot, double* dudt,
+                                  double* dudt_dot, int allow_reconstruction,
+                                  ibis_stream_t stream) {
+    if (!r || !prim || !prim_dot || !dudt || !dudt_dot)
+        return ibis::fail(IBIS_ERR_INVALID, "ibis_compute_dudt_jvp: null argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const DevMesh& D = r->m->d;
+    const ibis_fv_desc& d = r->desc;
+    HIP_TRY(ensure_dual(r));
+    const long np = (long)NPRIM * D.Nt;
+    k_to_dual<<<nblocks(np), BLOCK, 0, st>>>(np, prim, prim_dot, r->dual_prim);
+    const bool recon = allow_reconstruction && d.order == 2;
+    Prof pf{r, st};
+    GenBufs<ibis::Dual> B{r->dual_grad, r->dual_lim, r->dual_flux, nullptr, nullptr};
+    run_general<ibis::Dual>(r, B, r->dual_prim, r->dual_dudt, recon, st, pf);
+    // ghost rows of the state are written by the boundary actions, as in the
+    // reference; the residual rows [0, N) of dudt
+    k_from_dual<<<nblocks(D.Nt), BLOCK, 0, st>>>(NPRIM, D.Nt, D.Nt, r->dual_prim, prim, prim_dot);
+    k_from_dual<<<nblocks(D.N), BLOCK, 0, st>>>(D.dim + 2, D.N, D.Nt, r->dual_dudt, dudt,
+                                                dudt_dot);
+    HIP_TRY(hipGetLastError());
+    return IBIS_OK;
+}
+
+ibis_status ibis_steady_matvec(ibis_residual* r, const double* cons, const double* vec,
+                               double* result, double dt_star, int allow_reconstruction,
+                               ibis_stream_t stream) {
+    if (!r || !cons || !vec || !result)
+        return ibis::fail(IBIS_ERR_INVALID, "ibis_steady_matvec: null argument");
+    if (!(dt_star > 0.0)) return ibis::fail(IBIS_ERR_INVALID, "ibis_steady_matvec: dt_star <= 0");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const DevMesh& D = r->m->d;
+    const ibis_fv_desc& d = r->desc;
+    HIP_TRY(ensure_dual(r));
+    const int nc = D.dim + 2;
+    // the conserved tangent goes through dual_dudt, which the residual
+    // overwrites only after the conversion has consumed it
+    k_seed_cons<<<nblocks(D.Nt), BLOCK, 0, st>>>(D.N, D.Nt, nc, cons, vec, r->dual_dudt);
+    if (D.dim == 3)
+        k_cons_to_prim<ibis::Dual, 3><<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, d.R, d.Cv,
+                                                                        r->dual_dudt, r->dual_prim);
+    else
+        k_cons_to_prim<ibis::Dual, 2><<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, d.R, d.Cv,
+                                                                        r->dual_dudt, r->dual_prim);
+    const bool recon = allow_reconstruction && d.order == 2;
+    Prof pf{r, st};
+    GenBufs<ibis::Dual> B{r->dual_grad, r->dual_lim, r->dual_flux, nullptr, nullptr};
+    run_general<ibis::Dual>(r, B, r->dual_prim, r->dual_dudt, recon, st, pf);
+    k_jv_result<<<nblocks(D.N), BLOCK, 0, st>>>(D.N, D.Nt, nc, dt_star, vec, r->dual_dudt, result);
+    HIP_TRY(hipGetLastError());
+    return IBIS_OK;
+}
+
+ibis_status ibis_steady_rhs(ibis_residual* r, double* prim, double* dudt, double* rhs,
+                            int allow_reconstruction, ibis_stream_t stream) {
+    if (!r || !prim || !dudt || !rhs) return ibis::fail(IBIS_ERR_INVALID, "ibis_steady_rhs: null argument");
+    ibis_status s = ibis_compute_dudt(r, prim, dudt, allow_reconstruction, stream);
+    if (s != IBIS_OK) return s;
+    const DevMesh& D = r->m->d;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    k_rhs_vector<<<nblocks(D.N), BLOCK, 0, st>>>(D.N, D.Nt, D.dim + 2, dudt, rhs);
+    HIP_TRY(hipGetLastError());
+    return IBIS_OK;
+}
+
+ibis_status ibis_cons_to_prim_jvp(ibis_residual* r, const double* cons, const double* cons_dot,
+                                  double* prim, double* prim_dot, ibis_stream_t stream) {
+    if (!r || !cons || !cons_dot || !prim || !prim_dot)
+        return ibis::fail(IBIS_ERR_INVALID, "ibis_cons_to_prim_jvp: null argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const DevMesh& D = r->m->d;
+    HIP_TRY(ensure_dual(r));
+    const int nc = D.dim + 2;
+    const long ncv = (long)nc * D.Nt;
+    k_to_dual<<<nblocks(ncv), BLOCK, 0, st>>>(ncv, cons, cons_dot, r->dual_dudt);
+    if (D.dim == 3)
+        k_cons_to_prim<ibis::Dual, 3><<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, r->desc.R, r->desc.Cv,
+                                                                        r->dual_dudt, r->dual_prim);
+    else
+        k_cons_to_prim<ibis::Dual, 2><<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, r->desc.R, r->desc.Cv,
+                                                                        r->dual_dudt, r->dual_prim);
+    k_from_dual<<<nblocks(D.Nt), BLOCK, 0, st>>>(NPRIM, D.Nt, D.Nt, r->dual_prim, prim, prim_dot);
+    HIP_TRY(hipGetLastError());
+    return IBIS_OK;
+}
+
+ibis_status ibis_residual_sizes(const ibis_residual* r, int64_t* sizes) {
+    if (!r || !sizes) return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_sizes: null argument");
+    const DevMesh& D = r->m->d;
+    sizes[0] = D.dim;
+    sizes[1] = D.N;
+    sizes[2] = D.Nt;
+    sizes[3] = D.F;
+    sizes[4] = D.G;
     return IBIS_OK;
 }
 
@@ -1847,8 +2115,8 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
     if (!r || !key) return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_set_option: null argument");
     const std::string k(key);
     if (k == "face_state_layout") {
-        if (value != 0 && value != 1)
-            return ibis::fail(IBIS_ERR_INVALID, "face_state_layout must be 0 or 1");
+        if (value < 0 || value > 2)
+            return ibis::fail(IBIS_ERR_INVALID, "face_state_layout must be 0, 1 or 2");
         r->opt_layout = (int)value;
         return IBIS_OK;
     }
@@ -1937,9 +2205,9 @@ ibis_status ibis_cons_to_prim(ibis_residual* r, const double* cons, double* prim
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const DevMesh& D = r->m->d;
     if (D.dim == 3)
-        k_cons_to_prim<3><<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, r->desc.R, r->desc.Cv, cons, prim);
+        k_cons_to_prim<double, 3><<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, r->desc.R, r->desc.Cv, cons, prim);
     else
-        k_cons_to_prim<2><<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, r->desc.R, r->desc.Cv, cons, prim);
+        k_cons_to_prim<double, 2><<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, r->desc.R, r->desc.Cv, cons, prim);
     HIP_TRY(hipGetLastError());
     return IBIS_OK;
 }

@@ -217,6 +217,27 @@ class FiniteVolume:
               "ibis_compute_dudt")
         return dudt
 
+    def compute_dudt_jvp(self, prim, prim_dot, dudt, dudt_dot, allow_reconstruction: bool = True,
+                         stream=None):
+        """Value and directional derivative of dU/dt along ``prim_dot``
+        (FiniteVolume<Ibis::dual>::compute_dudt, steady_state.cpp:52-128).
+        Ghost rows of ``prim`` and ``prim_dot`` are written."""
+        for t, rows, what in ((prim, NPRIM, "prim"), (prim_dot, NPRIM, "prim_dot"),
+                              (dudt, self.ncons, "dudt"), (dudt_dot, self.ncons, "dudt_dot")):
+            self._check_state(t, rows, self.grid.Nt, what)
+        check(lib().ibis_compute_dudt_jvp(self._h, ptr(prim), ptr(prim_dot), ptr(dudt),
+                                          ptr(dudt_dot), 1 if allow_reconstruction else 0,
+                                          _stream_handle(stream)), "ibis_compute_dudt_jvp")
+
+    def cons_to_prim_jvp(self, cons, cons_dot, prim, prim_dot, stream=None):
+        """conserved_to_primatives on (value, tangent) pairs."""
+        for t, rows, what in ((cons, self.ncons, "cons"), (cons_dot, self.ncons, "cons_dot"),
+                              (prim, NPRIM, "prim"), (prim_dot, NPRIM, "prim_dot")):
+            self._check_state(t, rows, self.grid.Nt, what)
+        check(lib().ibis_cons_to_prim_jvp(self._h, ptr(cons), ptr(cons_dot), ptr(prim),
+                                          ptr(prim_dot), _stream_handle(stream)),
+              "ibis_cons_to_prim_jvp")
+
     def estimate_dt(self, flow_state, grid=None, gas_model=None, trans_prop=None, stream=None):
         prim = flow_state.data if isinstance(flow_state, FlowStates) else flow_state
         self._check_state(prim, NPRIM, self.grid.Nt, "flow_state")

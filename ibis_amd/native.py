@@ -57,6 +57,35 @@ class FVDescStruct(C.Structure):
                 ("post_par", C.c_double * (MAX_TAGS * MAX_ACT * NPAR))]
 
 
+MAX_CFL_POINTS = 16
+NNORMS = 6
+
+
+class LinearResultStruct(C.Structure):
+    """ibis_linear_result (LinearSolveResult)."""
+    _fields_ = [("success", C.c_int32), ("n_iters", C.c_int32), ("tol", C.c_double),
+                ("residual", C.c_double)]
+
+
+class SteadyOpStruct(C.Structure):
+    _fields_ = [("r", C.c_void_p), ("cons", C.c_void_p), ("dt_star", C.c_double),
+                ("allow_reconstruction", C.c_int32)]
+
+
+class DenseOpStruct(C.Structure):
+    _fields_ = [("n", C.c_int64), ("a", C.c_void_p)]
+
+
+class JfnkDescStruct(C.Structure):
+    _fields_ = [("linear_solver", C.c_int32), ("max_iters", C.c_int32), ("tol", C.c_double),
+                ("max_precond_iters", C.c_int32), ("precond_tol", C.c_double),
+                ("cfl_kind", C.c_int32), ("n_cfl_points", C.c_int32), ("cfl_value", C.c_double),
+                ("cfl_times", C.c_double * MAX_CFL_POINTS),
+                ("cfl_values", C.c_double * MAX_CFL_POINTS),
+                ("growth_threshold", C.c_double), ("power", C.c_double),
+                ("start_cfl", C.c_double), ("max_cfl", C.c_double)]
+
+
 _P = C.c_void_p
 _SIGS = {
     "ibis_grid_build": (C.c_int, [C.POINTER(GridIOStruct), C.c_int, C.POINTER(C.c_void_p)]),
@@ -70,6 +99,8 @@ _SIGS = {
     "ibis_residual_create": (C.c_int, [_P, C.POINTER(FVDescStruct), C.POINTER(C.c_void_p)]),
     "ibis_residual_destroy": (None, [_P]),
     "ibis_compute_dudt": (C.c_int, [_P, _P, _P, C.c_int, _P]),
+    "ibis_compute_dudt_jvp": (C.c_int, [_P, _P, _P, _P, _P, C.c_int, _P]),
+    "ibis_cons_to_prim_jvp": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "ibis_estimate_dt": (C.c_int, [_P, _P, C.POINTER(C.c_double), _P]),
     "ibis_count_bad_cells": (C.c_int, [_P, _P, C.POINTER(C.c_uint64), _P]),
     "ibis_cons_to_prim": (C.c_int, [_P, _P, _P, _P]),
@@ -83,6 +114,20 @@ _SIGS = {
     "ibis_residual_profile": (C.c_int, [_P, C.c_int]),
     "ibis_residual_set_option": (C.c_int, [_P, C.c_char_p, C.c_int64]),
     "ibis_residual_kernel_times": (C.c_int, [_P, _P, _P, C.c_int]),
+    "ibis_residual_sizes": (C.c_int, [_P, _P]),
+    "ibis_steady_matvec": (C.c_int, [_P, _P, _P, _P, C.c_double, C.c_int, _P]),
+    "ibis_steady_rhs": (C.c_int, [_P, _P, _P, _P, C.c_int, _P]),
+    "ibis_steady_op_matvec": (C.c_int, [_P, _P, _P, _P]),
+    "ibis_dense_matvec": (C.c_int, [_P, _P, _P, _P]),
+    "ibis_gmres_create": (C.c_int, [C.c_int64, C.c_int32, C.c_double, C.POINTER(C.c_void_p)]),
+    "ibis_gmres_destroy": (None, [_P]),
+    "ibis_gmres_solve": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "ibis_fgmres_solve": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "ibis_jfnk_create": (C.c_int, [_P, _P, C.POINTER(C.c_void_p)]),
+    "ibis_jfnk_destroy": (None, [_P]),
+    "ibis_jfnk_initialise": (C.c_int, [_P, _P, _P, _P, _P]),
+    "ibis_jfnk_step": (C.c_int, [_P, _P, _P, C.c_int64, _P, _P, _P, _P]),
+    "ibis_cfl_eval": (C.c_int, [_P, C.c_double, _P, _P, _P]),
     "ibis_last_error": (C.c_char_p, []),
 }
 
@@ -134,4 +179,4 @@ def header_symbols():
     """Function names declared in include/ibis_gpu.h."""
     import re
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(ibis_[a-z0-9_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(ibis_[a-z0-9_]+)\s*\((?!\*)", txt)))

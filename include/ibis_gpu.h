@@ -155,10 +155,31 @@ ibis_status ibis_residual_create(const ibis_mesh* m, const ibis_fv_desc* desc,
                                  ibis_residual** out);
 void ibis_residual_destroy(ibis_residual* r);
 
+/* dim, n_cells, n_total, n_faces, n_ghost of the residual's mesh. */
+ibis_status ibis_residual_sizes(const ibis_residual* r, int64_t* sizes);
+
 /* FiniteVolume<T>::compute_dudt (finite_volume.cpp:57-96, static grid).
  * Writes ghost rows of prim, rows [0, n_cells) of dudt.  Asynchronous. */
 ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
                               int allow_reconstruction, ibis_stream_t stream);
+
+/* Tangent of the residual: FiniteVolume<Ibis::dual>::compute_dudt as
+ * SteadyStateLinearisation::matrix_vector_product runs it
+ * (src/solvers/solvers/steady_state.cpp:52-128), with the reference's Dual
+ * arithmetic (src/util/util/dual.h:10-346).  prim / prim_dot are the value
+ * and tangent of the state (7 x n_total; ghost rows of both are written),
+ * dudt / dudt_dot receive rows [0, n_cells) of the residual and of its
+ * directional derivative.  The tangent scratch (twice the FP64 scratch) is
+ * allocated on the first call.  Asynchronous after that first call. */
+ibis_status ibis_compute_dudt_jvp(ibis_residual* r, double* prim, double* prim_dot, double* dudt,
+                                  double* dudt_dot, int allow_reconstruction,
+                                  ibis_stream_t stream);
+
+/* conserved_to_primatives on Dual values (primative_conserved_conversion.cpp:
+ * 5-30), all n_total rows: value and tangent of the state from those of the
+ * conserved quantities. */
+ibis_status ibis_cons_to_prim_jvp(ibis_residual* r, const double* cons, const double* cons_dot,
+                                  double* prim, double* prim_dot, ibis_stream_t stream);
 
 /* FiniteVolume<T>::estimate_dt (finite_volume.cpp:131-184).  Synchronises
  * the stream and writes the global minimum to *dt_host. */
@@ -202,6 +223,7 @@ ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** lim
  * the same bits.  Keys:
  *   "face_state_layout"   0: face states indexed by face (left/right arrays)
  *                         1: face states stored per (cell, face slot)
+ *                         2: per (face slot, variable) planes over cells
  *   "precomputed_offsets" 0: x_face - x_cell computed from face centres
  *                         1: precomputed once per (cell, face slot)
  * Setting precomputed_offsets allocates and synchronises once. */
@@ -219,6 +241,126 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
 ibis_status ibis_residual_profile(ibis_residual* r, int enable);
 ibis_status ibis_residual_kernel_times(ibis_residual* r, double* total_ms, int64_t* counts,
                                        int reset);
+
+/* ------------------------------------------------------------------------
+ * Steady-state linearisation, GMRES / FGMRES and JFNK
+ * (src/solvers/solvers/steady_state.cpp:52-165, src/solvers/solvers/jfnk.cpp,
+ * src/linear_algebra/linear_algebra/gmres.cpp).  Vectors have n_cells *
+ * (dim+2) FP64 entries in the reference's order: index cell*(dim+2) + q
+ * (steady_state.cpp:60-68).  All vectors are device memory.
+ * ------------------------------------------------------------------------ */
+
+/* SteadyStateLinearisation::matrix_vector_product: result = vec / dt_star -
+ * dual(R(U + eps*vec)) with the tangent residual, cons the (dim+2) x n_total
+ * conserved state.  allow_reconstruction = 0 is the preconditioner's
+ * system (steady_state.cpp:47-50).  Asynchronous. */
+ibis_status ibis_steady_matvec(ibis_residual* r, const double* cons, const double* vec,
+                               double* result, double dt_star, int allow_reconstruction,
+                               ibis_stream_t stream);
+
+/* SteadyStateLinearisation::eval_rhs: residual of prim into dudt and into
+ * rhs in vector order.  Asynchronous. */
+ibis_status ibis_steady_rhs(ibis_residual* r, double* prim, double* dudt, double* rhs,
+                            int allow_reconstruction, ibis_stream_t stream);
+
+/* A linear operator for the Krylov solvers: w = A v on the given stream. */
+typedef ibis_status (*ibis_matvec_fn)(void* ctx, const double* v, double* w,
+                                      ibis_stream_t stream);
+
+/* Two ready-made operators (pass the function as A or P and a pointer to
+ * the struct as its context):
+ *   ibis_steady_op_matvec: the steady-state linearisation above;
+ *   ibis_dense_matvec: w = a v for a dense row-major n x n device matrix
+ *   (the LinearSystem of the reference's GMRES tests, gmres.cpp:320-500). */
+typedef struct {
+    ibis_residual* r;
+    const double* cons;
+    double dt_star;
+    int32_t allow_reconstruction;
+} ibis_steady_op;
+ibis_status ibis_steady_op_matvec(void* ctx, const double* v, double* w, ibis_stream_t stream);
+
+typedef struct {
+    int64_t n;
+    const double* a;
+} ibis_dense_op;
+ibis_status ibis_dense_matvec(void* ctx, const double* v, double* w, ibis_stream_t stream);
+
+/* LinearSolveResult (src/linear_algebra/linear_algebra/linear_system.h) */
+typedef struct {
+    int32_t success;
+    int32_t n_iters;
+    double tol;
+    double residual; /* |g_{m}| / beta */
+} ibis_linear_result;
+
+typedef struct ibis_gmres ibis_gmres;
+
+/* Gmres(system, max_iters, tol) (gmres.cpp:87-116): Krylov basis of
+ * min(num_vars, max_iters) + 1 device vectors. */
+ibis_status ibis_gmres_create(int64_t num_vars, int32_t max_iters, double tol,
+                              ibis_gmres** out);
+void ibis_gmres_destroy(ibis_gmres* g);
+
+/* Gmres::solve (gmres.cpp:119-182): x += the GMRES correction for A x = rhs
+ * from the initial guess x.  Modified Gram-Schmidt on the device (one host
+ * synchronisation per Arnoldi step for the Givens rotations, kept on the
+ * host as in the reference).  Deterministic reductions. */
+ibis_status ibis_gmres_solve(ibis_gmres* g, ibis_matvec_fn A, void* ctx, const double* rhs,
+                             double* x, ibis_linear_result* result, ibis_stream_t stream);
+
+/* FGmres::solve (gmres.cpp:234-303): right-preconditioned flexible GMRES;
+ * each preconditioner application solves P z = v with the inner solver
+ * `inner` from z = 0. */
+ibis_status ibis_fgmres_solve(ibis_gmres* g, ibis_matvec_fn A, void* ctx_a, const double* rhs,
+                              double* x, ibis_gmres* inner, ibis_matvec_fn P, void* ctx_p,
+                              ibis_linear_result* result, ibis_stream_t stream);
+
+/* CFL schedules (src/solvers/solvers/cfl.cpp:9-83) */
+enum { IBIS_CFL_CONSTANT = 0, IBIS_CFL_LINEAR = 1, IBIS_CFL_RESIDUAL = 2 };
+#define IBIS_MAX_CFL_POINTS 16
+
+/* Jfnk configuration (jfnk.cpp:7-22 and the linear_solver block read by
+ * make_linear_solver, gmres.cpp:305-318) */
+typedef struct {
+    int32_t linear_solver; /* 0 gmres, 1 fgmres */
+    int32_t max_iters;
+    double tol;
+    int32_t max_precond_iters;
+    double precond_tol;
+    int32_t cfl_kind;
+    int32_t n_cfl_points;
+    double cfl_value;                          /* constant */
+    double cfl_times[IBIS_MAX_CFL_POINTS];     /* linear_interpolate */
+    double cfl_values[IBIS_MAX_CFL_POINTS];
+    double growth_threshold, power, start_cfl, max_cfl; /* residual_based */
+} ibis_jfnk_desc;
+
+typedef struct ibis_jfnk ibis_jfnk;
+
+/* ConservedQuantitiesNorm: mass, momentum x, y, z, energy, global */
+#define IBIS_NNORMS 6
+
+ibis_status ibis_jfnk_create(ibis_residual* r, const ibis_jfnk_desc* desc, ibis_jfnk** out);
+void ibis_jfnk_destroy(ibis_jfnk* j);
+
+/* Jfnk::initialise (jfnk.cpp:24-29): residual of the state, its L2 norms
+ * (ConservedQuantities::L2_norms, conserved_quantities.cpp:41-69, over the
+ * valid cells) into norms[IBIS_NNORMS]; they become the initial norms of the
+ * residual-based CFL.  Synchronises. */
+ibis_status ibis_jfnk_initialise(ibis_jfnk* j, double* cons, double* prim, double* norms,
+                                 ibis_stream_t stream);
+
+/* Jfnk::step (jfnk.cpp:39-93): dt* = cfl(step) * estimate_dt, linear solve
+ * of (I/dt* - dR/dU) dU = R, U += dU on the valid cells, conversion to
+ * primitives, new residual and its norms.  Synchronises. */
+ibis_status ibis_jfnk_step(ibis_jfnk* j, double* cons, double* prim, int64_t step,
+                           ibis_linear_result* result, double* norms, double* dt_star,
+                           ibis_stream_t stream);
+
+/* CflSchedule::eval for a Jfnk's schedule (host only; for tests). */
+ibis_status ibis_cfl_eval(const ibis_jfnk_desc* desc, double t, double* previous_cfl,
+                          double* previous_residual, double* cfl);
 
 const char* ibis_last_error(void);
 

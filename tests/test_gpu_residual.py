@@ -95,7 +95,7 @@ def test_hex_jittered_permuted(flux, limiter):
     _check(g_io, cfg, prim, exact=True)
 
 
-@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("layout", [0, 1, 2])
 @pytest.mark.parametrize("offsets", [0, 1])
 def test_implementation_options_give_identical_bits(layout, offsets):
     opts = {"face_state_layout": layout, "precomputed_offsets": offsets}
