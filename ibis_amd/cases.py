@@ -118,3 +118,24 @@ def hex_prim(grid_io, n_total: int, seed: int = 5) -> np.ndarray:
     cc = gridgen.cell_centres(grid_io)
     prim[:, :grid_io.num_cells] = gridgen.smooth_field(cc, seed, 3)
     return prim
+
+
+def batch_instances(n: int = 64, seed: int = 7, n_min: int = 10_000, n_max: int = 400_000):
+    """Seeded wedge/ramp instances of uneven size: cell count U[n_min, n_max]
+    with nx:ny = 2:1, half wedges and half compression ramps, angle
+    U[5, 20] degrees, Mach U[1.5, 4] at p = 101325 Pa, T = 300 K."""
+    u = gridgen.uniform01(seed, 3 * n)
+    out = []
+    for i in range(n):
+        cells = n_min + u[3 * i] * (n_max - n_min)
+        ny = max(4, int(round(math.sqrt(cells / 2.0))))
+        out.append({"kind": "wedge" if i % 2 == 0 else "ramp", "nx": 2 * ny, "ny": ny,
+                    "angle": 5.0 + 15.0 * u[3 * i + 1], "mach": 1.5 + 2.5 * u[3 * i + 2]})
+    return out
+
+
+def batch_case(spec: dict):
+    """(GridIO, FVConfig, FlowState) of one batch instance: AUSMDV, 2nd
+    order, Barth-Jespersen, rho_u, FP64."""
+    make = wedge_case if spec["kind"] == "wedge" else ramp_case
+    return make(spec["nx"], spec["ny"], mach=spec["mach"], angle=spec["angle"])

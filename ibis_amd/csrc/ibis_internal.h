@@ -22,6 +22,20 @@ struct ibis_grid {
 
 void ibis_grid_fill_desc(const ibis_grid* g, ibis_mesh_desc* d);
 
+// Library-internal entry points shared between the compilation units.
+//
+// A residual whose boundary tables hold `ntags` tags (kinds3: pre, viscous,
+// post tables of ntags * IBIS_MAX_ACT entries each; pars3 likewise with
+// IBIS_NPAR values per entry), for meshes with more tags than an
+// ibis_fv_desc holds (a batch of instances); the rest of the desc applies.
+ibis_status ibis_residual_create_tables(const ibis_mesh* m, const ibis_fv_desc* desc, int ntags,
+                                        const int32_t* kinds3, const double* pars3,
+                                        ibis_residual** out);
+int ibis_mesh_ntags(const ibis_mesh* m);
+// local time step of every valid cell (the terms estimate_dt minimises)
+ibis_status ibis_residual_cell_dt(ibis_residual* r, const double* prim, double* dt_cells,
+                                  ibis_stream_t stream);
+
 namespace ibis {
 // Record a failure message for ibis_last_error() and return the status.
 ibis_status fail(ibis_status s, const char* msg);

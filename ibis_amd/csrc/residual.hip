@@ -574,10 +574,10 @@ __device__ __forceinline__ size_t soa_index(const DevMesh& m, int c, int slot, i
                    : (size_t)m.maxf * 5 * m.N + (size_t)v * m.G + (c - m.N);
 }
 
-template <int LAYOUT>
+template <int LAYOUT, class W = double>
 __global__ void __launch_bounds__(BLOCK)
-    k_ghost_fs(DevMesh m, VarSet vs, const double* __restrict__ prim, double* __restrict__ FL,
-               double* __restrict__ FR) {
+    k_ghost_fs(DevMesh m, VarSet vs, const double* __restrict__ prim, W* __restrict__ FL,
+               W* __restrict__ FR) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= m.nb) return;
     const int f = m.b_face[b], g = m.b_ghost[b];
@@ -585,16 +585,15 @@ __global__ void __launch_bounds__(BLOCK)
     const double dx = m.geom[9 * (size_t)F + f] - m.cen[g];
     const double dy = m.geom[10 * (size_t)F + f] - m.cen[Nt + g];
     const double dz = m.geom[11 * (size_t)F + f] - m.cen[2 * (size_t)Nt + g];
-    double* rec = LAYOUT == 0 ? (m.L[f] == g ? FL : FR) + (size_t)f * 5
-                              : FL + rec_index(m, g, 0) * 5;
+    W* rec = LAYOUT == 0 ? (m.L[f] == g ? FL : FR) + (size_t)f * 5 : FL + rec_index(m, g, 0) * 5;
 #pragma unroll
     for (int v = 0; v < 5; v++) {
         const double gx = 0.0, gy = 0.0, gz = 0.0;
         const double val = prim[(size_t)vs.row[v] * Nt + g] + 1.0 * (gx * dx + gy * dy + gz * dz);
         if (LAYOUT == 2)
-            FL[soa_index(m, g, 0, v)] = val;
+            FL[soa_index(m, g, 0, v)] = W(val);
         else
-            rec[v] = val;
+            rec[v] = W(val);
     }
 }
 
@@ -613,11 +612,13 @@ __global__ void __launch_bounds__(BLOCK) k_offsets(DevMesh m, double* __restrict
     }
 }
 
-template <int DIM, int LIM, int LAYOUT, bool OFFS>
+// W is the storage type of the face values (float in the reduced-precision
+// mode, layout 2 only); gradients and limiters are FP64 in both.
+template <int DIM, int LIM, int LAYOUT, bool OFFS, class W = double>
 __global__ void __launch_bounds__(BLOCK)
     k_cell_fs(DevMesh m, VarSet vs, const double* __restrict__ pack, double* __restrict__ grad,
               double* __restrict__ lim, double eps, double venkat_K, int nlogical,
-              int write_slots, double* __restrict__ FL, double* __restrict__ FR,
+              int write_slots, W* __restrict__ FL, W* __restrict__ FR,
               const double* __restrict__ offs) {
     const int lb = xcd_block(nlogical);
     if (lb >= nlogical) return;
@@ -765,24 +766,27 @@ __global__ void __launch_bounds__(BLOCK)
             dy = m.geom[10 * (size_t)F + f] - yi;
             dz = m.geom[11 * (size_t)F + f] - zi;
         }
-        double* rec = LAYOUT == 0 ? (m.ell_sign[(size_t)j * N + i] > 0 ? FL : FR) + (size_t)f * 5
-                                  : FL + ((size_t)i * m.maxf + j) * 5;
+        W* rec = LAYOUT == 0 ? (m.ell_sign[(size_t)j * N + i] > 0 ? FL : FR) + (size_t)f * 5
+                             : FL + ((size_t)i * m.maxf + j) * 5;
 #pragma unroll
         for (int v = 0; v < 5; v++) {
             const double val = ui[v] + phi[v] * (gx[v] * dx + gy[v] * dy + gz[v] * dz);
             if (LAYOUT == 2)
-                __builtin_nontemporal_store(val, FL + ((size_t)j * 5 + v) * N + i);
+                __builtin_nontemporal_store(W(val), FL + ((size_t)j * 5 + v) * N + i);
             else
-                rec[v] = val;
+                rec[v] = W(val);
         }
     }
 }
 
 // Fused face flux + surface integral from the face states.
-template <int DIM, int FLUX, int LAYOUT>
+// R is the arithmetic of the thermo closure, frame rotation and flux:
+// double, or float for the reduced-precision mode (face states stored as
+// float); the face-to-cell sum and the division by the volume stay FP64.
+template <int DIM, int FLUX, int LAYOUT, class R = double>
 __global__ void __launch_bounds__(TILE)
-    k_face_tile_fs(DevMesh m, FaceParams fp, const double* __restrict__ FL,
-                   const double* __restrict__ FR, double* __restrict__ dudt) {
+    k_face_tile_fs(DevMesh m, FaceParams fp, const R* __restrict__ FL,
+                   const R* __restrict__ FR, double* __restrict__ dudt) {
     extern __shared__ double sflux[];  // NC x maxtf
     const int t = xcd_block(m.ntiles);
     if (t >= m.ntiles) return;
@@ -793,16 +797,16 @@ __global__ void __launch_bounds__(TILE)
     const int nfl = m.tile_face_off[t + 1] - f0;
     for (int k = threadIdx.x; k < nfl; k += TILE) {
         const int f = m.tile_faces[f0 + k];
-        double Ls[NPRIM], Rs[NPRIM];
+        R Ls[NPRIM], Rs[NPRIM];
 #pragma unroll
         for (int side = 0; side < 2; side++) {
-            double rv[5];
+            R rv[5];
             if (LAYOUT == 0) {
-                const double* rec = (side ? FR : FL) + (size_t)f * 5;
+                const R* rec = (side ? FR : FL) + (size_t)f * 5;
 #pragma unroll
                 for (int v = 0; v < 5; v++) rv[v] = rec[v];
             } else if (LAYOUT == 1) {
-                const double* rec =
+                const R* rec =
                     FL + (side ? rec_index(m, m.R[f], m.slotR[f]) : rec_index(m, m.L[f], m.slotL[f])) * 5;
 #pragma unroll
                 for (int v = 0; v < 5; v++) rv[v] = rec[v];
@@ -812,17 +816,17 @@ __global__ void __launch_bounds__(TILE)
 #pragma unroll
                 for (int v = 0; v < 5; v++) rv[v] = FL[soa_index(m, c, sl, v)];
             }
-            const double* rec = rv;
-            double* st = side ? Rs : Ls;
-            const double a = rec[0], b = rec[1];
-            double rho = 0.0, p = 0.0, T = 0.0, e = 0.0;
+            const R* rec = rv;
+            R* st = side ? Rs : Ls;
+            const R a = rec[0], b = rec[1];
+            R rho = R(0.0), p = R(0.0), T = R(0.0), e = R(0.0);
             switch (fp.thermo) {
                 case IBIS_THERMO_RHO_P: p = a; rho = b; break;
                 case IBIS_THERMO_RHO_T: rho = a; T = b; break;
                 case IBIS_THERMO_RHO_U: rho = a; e = b; break;
                 default: p = a; T = b; break;
             }
-            ibis::thermo_update<double>(fp.thermo, fp.R, fp.Cv, rho, p, T, e);
+            ibis::thermo_update<R>(fp.thermo, R(fp.R), R(fp.Cv), rho, p, T, e);
             st[ibis::S_RHO] = rho;
             st[ibis::S_P] = p;
             st[ibis::S_T] = T;
@@ -831,41 +835,41 @@ __global__ void __launch_bounds__(TILE)
             st[5] = rec[3];
             st[6] = rec[4];
         }
-        const double t1x = m.geom[3 * (size_t)F + f], t1y = m.geom[4 * (size_t)F + f],
-                     t1z = m.geom[5 * (size_t)F + f];
-        const double t2x = m.geom[6 * (size_t)F + f], t2y = m.geom[7 * (size_t)F + f],
-                     t2z = m.geom[8 * (size_t)F + f];
-        double nx, ny, nz;
+        const R t1x = R(m.geom[3 * (size_t)F + f]), t1y = R(m.geom[4 * (size_t)F + f]),
+                t1z = R(m.geom[5 * (size_t)F + f]);
+        const R t2x = R(m.geom[6 * (size_t)F + f]), t2y = R(m.geom[7 * (size_t)F + f]),
+                t2z = R(m.geom[8 * (size_t)F + f]);
+        R nx, ny, nz;
         if (m.n_from_t) {  // cross(tan1, tan2), GR/interface.h:249
             nx = t1y * t2z - t1z * t2y;
             ny = t1z * t2x - t1x * t2z;
             nz = t1x * t2y - t1y * t2x;
         } else {
-            nx = m.geom[f];
-            ny = m.geom[(size_t)F + f];
-            nz = m.geom[2 * (size_t)F + f];
+            nx = R(m.geom[f]);
+            ny = R(m.geom[(size_t)F + f]);
+            nz = R(m.geom[2 * (size_t)F + f]);
         }
 #pragma unroll
         for (int side = 0; side < 2; side++) {
-            double* s = side ? Rs : Ls;
-            const double a = s[4], b = s[5], c = s[6];
+            R* s = side ? Rs : Ls;
+            const R a = s[4], b = s[5], c = s[6];
             s[4] = a * nx + b * ny + c * nz;
             s[5] = a * t1x + b * t1y + c * t1z;
             s[6] = a * t2x + b * t2y + c * t2z;
         }
-        double out[5];
+        R out[5];
         if (FLUX == IBIS_FLUX_AUSMDV)
-            ibis::flux_ausmdv<double, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, out);
+            ibis::flux_ausmdv<R, T3>(Ls, Rs, R(fp.R), R(fp.Cv), R(fp.gamma), out);
         else if (FLUX == IBIS_FLUX_HANEL)
-            ibis::flux_hanel<double, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, out);
+            ibis::flux_hanel<R, T3>(Ls, Rs, R(fp.R), R(fp.Cv), R(fp.gamma), out);
         else if (FLUX == IBIS_FLUX_LDFSS2)
-            ibis::flux_ldfss2<double, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, fp.delta, out);
+            ibis::flux_ldfss2<R, T3>(Ls, Rs, R(fp.R), R(fp.Cv), R(fp.gamma), R(fp.delta), out);
         else
-            ibis::flux_rusanov<double, T3>(Ls, Rs, fp.R, fp.Cv, fp.gamma, out);
-        const double px = out[1], py = out[2];
-        double pz = 0.0;
+            ibis::flux_rusanov<R, T3>(Ls, Rs, R(fp.R), R(fp.Cv), R(fp.gamma), out);
+        const R px = out[1], py = out[2];
+        R pz = R(0.0);
         if (T3) pz = out[3];
-        double res[NC];
+        R res[NC];
         res[0] = out[0];
         res[1] = px * nx + py * t1x + pz * t2x;
         res[2] = px * ny + py * t1y + pz * t2y;
@@ -877,7 +881,7 @@ __global__ void __launch_bounds__(TILE)
         }
         const double A = m.geom[12 * (size_t)F + f];
 #pragma unroll
-        for (int q = 0; q < NC; q++) sflux[q * S + k] = res[q] * A;
+        for (int q = 0; q < NC; q++) sflux[q * S + k] = (double)res[q] * A;
     }
     __syncthreads();
     const int c = t * TILE + threadIdx.x;
@@ -1091,41 +1095,55 @@ __device__ __forceinline__ double block_min(double v) {
     return v;
 }
 
+// local time step of valid cell i (FV/finite_volume.cpp:143-176)
+__device__ __forceinline__ double cell_dt(const DevMesh& m, const double* __restrict__ prim,
+                                          const double* __restrict__ face_rho,
+                                          const double* __restrict__ face_T, int viscous,
+                                          double sf, const ViscParams& vp, double gamma, int i) {
+    const int N = m.N, Nt = m.Nt, F = m.F;
+    const double vx = prim[(size_t)P_VX * Nt + i], vy = prim[(size_t)P_VY * Nt + i],
+                 vz = prim[(size_t)P_VZ * Nt + i];
+    const double a = sqrt(gamma * vp.R * prim[(size_t)P_T * Nt + i]);
+    double src = 0.0, srv = 0.0;
+    const double volume = m.vol[i];
+    const int nf = m.nfaces[i];
+    for (int j = 0; j < nf; j++) {
+        const int f = m.ell_face[(size_t)j * N + i];
+        const double area = m.geom[12 * (size_t)F + f];
+        const double dot =
+            vx * m.geom[f] + vy * m.geom[(size_t)F + f] + vz * m.geom[2 * (size_t)F + f];
+        const double sig_vel = fabs(dot) + a;
+        src += sig_vel * area;
+        if (viscous) {
+            const double tf = face_T[f];
+            const double mu = sutherland(tf, vp);
+            const double k = vp.Cp * sutherland(tf, vp) / vp.Pr;
+            const double rho = face_rho[f];
+            const double Pr = mu * vp.Cp / k;
+            const double tmp = (gamma / rho) * (mu / Pr) * area * area;
+            srv += tmp / volume;
+        }
+    }
+    return volume / (src + sf * srv);
+}
+
 __global__ void __launch_bounds__(BLOCK)
     k_dt_partial(DevMesh m, const double* __restrict__ prim, const double* __restrict__ face_rho,
                  const double* __restrict__ face_T, int viscous, double sf, ViscParams vp,
                  double gamma, double* __restrict__ partial) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int N = m.N, Nt = m.Nt, F = m.F;
     double dt = 1.79769313486231570815e+308;
-    if (i < N) {
-        const double vx = prim[(size_t)P_VX * Nt + i], vy = prim[(size_t)P_VY * Nt + i],
-                     vz = prim[(size_t)P_VZ * Nt + i];
-        const double a = sqrt(gamma * vp.R * prim[(size_t)P_T * Nt + i]);
-        double src = 0.0, srv = 0.0;
-        const double volume = m.vol[i];
-        const int nf = m.nfaces[i];
-        for (int j = 0; j < nf; j++) {
-            const int f = m.ell_face[(size_t)j * N + i];
-            const double area = m.geom[12 * (size_t)F + f];
-            const double dot = vx * m.geom[f] + vy * m.geom[(size_t)F + f] +
-                               vz * m.geom[2 * (size_t)F + f];
-            const double sig_vel = fabs(dot) + a;
-            src += sig_vel * area;
-            if (viscous) {
-                const double tf = face_T[f];
-                const double mu = sutherland(tf, vp);
-                const double k = vp.Cp * sutherland(tf, vp) / vp.Pr;
-                const double rho = face_rho[f];
-                const double Pr = mu * vp.Cp / k;
-                const double tmp = (gamma / rho) * (mu / Pr) * area * area;
-                srv += tmp / volume;
-            }
-        }
-        dt = volume / (src + sf * srv);
-    }
+    if (i < m.N) dt = cell_dt(m, prim, face_rho, face_T, viscous, sf, vp, gamma, i);
     dt = block_min(dt);
     if (threadIdx.x == 0) partial[blockIdx.x] = dt;
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_dt_cells(DevMesh m, const double* __restrict__ prim, const double* __restrict__ face_rho,
+               const double* __restrict__ face_T, int viscous, double sf, ViscParams vp,
+               double gamma, double* __restrict__ dt_cells) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m.N) dt_cells[i] = cell_dt(m, prim, face_rho, face_T, viscous, sf, vp, gamma, i);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_min_final(const double* __restrict__ partial, int n,
@@ -1275,8 +1293,11 @@ extern "C" const char* ibis_last_error(void) { return g_last_error.c_str(); }
 
 struct ibis_mesh {
     DevMesh d;
+    int ntags = 0;
     std::vector<void*> allocs;
 };
+
+int ibis_mesh_ntags(const ibis_mesh* m) { return m ? m->ntags : 0; }
 
 template <class T>
 static ibis_status upload(ibis_mesh* m, const std::vector<T>& h, const T** out) {
@@ -1301,7 +1322,7 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
     const int64_t N = d->n_cells, G = d->n_ghost, F = d->n_faces, Nt = N + G;
     if (N <= 0 || F <= 0 || Nt >= (int64_t)INT32_MAX || F >= (int64_t)INT32_MAX)
         return ibis::fail(IBIS_ERR_INVALID, "ibis_mesh_create: mesh size out of range");
-    if (d->ntags < 0 || d->ntags > IBIS_MAX_TAGS)
+    if (d->ntags < 0 || d->ntags > IBIS_MAX_MESH_TAGS)
         return ibis::fail(IBIS_ERR_INVALID, "ibis_mesh_create: too many boundary tags");
     // validate connectivity on the host: every index the kernels follow
     int maxf = 0;
@@ -1345,6 +1366,7 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
                 return ibis::fail(IBIS_ERR_INVALID, "boundary face without a ghost cell");
         }
     ibis_mesh* m = new ibis_mesh();
+    m->ntags = d->ntags;
     DevMesh& D = m->d;
     D.dim = d->dim;
     D.nl = d->dim == 2 ? 3 : 6;
@@ -1476,6 +1498,9 @@ struct ibis_residual {
     double* FR = nullptr;
     double* offs = nullptr;  // precomputed face offsets (option precomputed_offsets)
     int opt_layout = 0;      // option face_state_layout
+    int prec = IBIS_PREC_FP64;  // ibis_residual_set_precision
+    int ntab = 0;               // tag x action entries of each boundary table
+    bool has_pre = false, has_visc = false, has_post = false;
     int opt_offsets = 0;     // option precomputed_offsets
     double* face_rho = nullptr;
     double* face_T = nullptr;
@@ -1525,21 +1550,56 @@ void ibis_residual_destroy(ibis_residual* r) {
     delete r;
 }
 
-ibis_status ibis_residual_create(const ibis_mesh* m, const ibis_fv_desc* desc,
-                                 ibis_residual** out) {
-    if (!m || !desc || !out) return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_create: null argument");
-    const ibis_fv_desc& d = *desc;
+static bool kinds_valid(const int32_t* pre, const int32_t* visc, const int32_t* post, int n) {
+    for (int q = 0; q < n; q++) {
+        if (pre[q] < 0 || pre[q] > 7 || visc[q] < 0 || visc[q] > 7 ||
+            (post[q] != 0 && post[q] != IBIS_BC_CONSTANT_FLUX))
+            return false;
+    }
+    return true;
+}
+
+static ibis_status check_desc(const ibis_fv_desc& d) {
     if (d.flux < 0 || d.flux > 3) return ibis::fail(IBIS_ERR_INVALID, "Unknown flux calculator");
     if (d.order != 1 && d.order != 2) return ibis::fail(IBIS_ERR_INVALID, "Invalid reconstruction order");
     if (d.order > 1 && (d.thermo < 0 || d.thermo > 3))
         return ibis::fail(IBIS_ERR_INVALID, "Unknown thermo interpolator");
     if (d.order > 1 && (d.limiter < 0 || d.limiter > 2))
         return ibis::fail(IBIS_ERR_INVALID, "Unknown limiter");
-    for (int q = 0; q < IBIS_MAX_TAGS * IBIS_MAX_ACT; q++) {
-        if (d.pre_kind[q] < 0 || d.pre_kind[q] > 7 || d.visc_kind[q] < 0 || d.visc_kind[q] > 7 ||
-            (d.post_kind[q] != 0 && d.post_kind[q] != IBIS_BC_CONSTANT_FLUX))
-            return ibis::fail(IBIS_ERR_INVALID, "Unknown boundary action");
-    }
+    if (!kinds_valid(d.pre_kind, d.visc_kind, d.post_kind, IBIS_MAX_TAGS * IBIS_MAX_ACT))
+        return ibis::fail(IBIS_ERR_INVALID, "Unknown boundary action");
+    return IBIS_OK;
+}
+
+ibis_status ibis_residual_create(const ibis_mesh* m, const ibis_fv_desc* desc,
+                                 ibis_residual** out) {
+    if (!m || !desc || !out) return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_create: null argument");
+    ibis_status st = check_desc(*desc);
+    if (st != IBIS_OK) return st;
+    if (m->ntags > IBIS_MAX_TAGS)
+        return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_create: the mesh has more boundary tags "
+                                            "than a configuration holds");
+    const int ntab = IBIS_MAX_TAGS * IBIS_MAX_ACT;
+    std::vector<int32_t> kinds(3 * ntab);
+    std::vector<double> pars(3 * (size_t)ntab * IBIS_NPAR);
+    std::memcpy(&kinds[0], desc->pre_kind, ntab * sizeof(int32_t));
+    std::memcpy(&kinds[ntab], desc->visc_kind, ntab * sizeof(int32_t));
+    std::memcpy(&kinds[2 * ntab], desc->post_kind, ntab * sizeof(int32_t));
+    std::memcpy(&pars[0], desc->pre_par, ntab * IBIS_NPAR * sizeof(double));
+    std::memcpy(&pars[(size_t)ntab * IBIS_NPAR], desc->visc_par, ntab * IBIS_NPAR * sizeof(double));
+    std::memcpy(&pars[2 * (size_t)ntab * IBIS_NPAR], desc->post_par, ntab * IBIS_NPAR * sizeof(double));
+    return ibis_residual_create_tables(m, desc, IBIS_MAX_TAGS, kinds.data(), pars.data(), out);
+}
+
+ibis_status ibis_residual_create_tables(const ibis_mesh* m, const ibis_fv_desc* desc, int ntags,
+                                        const int32_t* kinds3, const double* pars3,
+                                        ibis_residual** out) {
+    if (!m || !desc || !kinds3 || !pars3 || !out || ntags < m->ntags || ntags <= 0)
+        return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_create_tables: bad argument");
+    const ibis_fv_desc& d = *desc;
+    const int ntab = ntags * IBIS_MAX_ACT;
+    if (!kinds_valid(kinds3, kinds3 + ntab, kinds3 + 2 * ntab, ntab))
+        return ibis::fail(IBIS_ERR_INVALID, "Unknown boundary action");
     const DevMesh& D = m->d;
     ibis_residual* r = new ibis_residual();
     r->m = m;
@@ -1571,25 +1631,26 @@ ibis_status ibis_residual_create(const ibis_mesh* m, const ibis_fv_desc* desc,
         AL(r->face_rho, sizeof(double) * F);
         AL(r->face_T, sizeof(double) * F);
     }
-    const size_t ntab = IBIS_MAX_TAGS * IBIS_MAX_ACT;
+    r->ntab = ntab;
     AL(r->kinds, sizeof(int) * 3 * ntab);
     AL(r->pars, sizeof(double) * 3 * ntab * IBIS_NPAR);
     AL(r->partial, sizeof(double) * (nblocks(D.N) + 1));
     AL(r->dt_dev, sizeof(double));
     AL(r->count_dev, sizeof(unsigned long long));
 #undef AL
-    std::vector<int> kinds(3 * ntab);
-    std::vector<double> pars(3 * ntab * IBIS_NPAR);
-    std::memcpy(&kinds[0], d.pre_kind, ntab * sizeof(int));
-    std::memcpy(&kinds[ntab], d.visc_kind, ntab * sizeof(int));
-    std::memcpy(&kinds[2 * ntab], d.post_kind, ntab * sizeof(int));
-    std::memcpy(&pars[0], d.pre_par, ntab * IBIS_NPAR * sizeof(double));
-    std::memcpy(&pars[ntab * IBIS_NPAR], d.visc_par, ntab * IBIS_NPAR * sizeof(double));
-    std::memcpy(&pars[2 * ntab * IBIS_NPAR], d.post_par, ntab * IBIS_NPAR * sizeof(double));
-    if (hipMemcpy(r->kinds, kinds.data(), kinds.size() * sizeof(int), hipMemcpyHostToDevice) !=
+    // which action lists are non-empty (tags the mesh does not have hold none)
+    for (int q = 0; q < m->ntags * IBIS_MAX_ACT; q++) {
+        r->has_pre = r->has_pre || kinds3[q] != IBIS_BC_NONE;
+        r->has_visc = r->has_visc || kinds3[ntab + q] != IBIS_BC_NONE;
+        r->has_post = r->has_post || kinds3[2 * ntab + q] == IBIS_BC_CONSTANT_FLUX;
+    }
+    r->has_pre = r->has_pre && D.nb > 0;
+    r->has_visc = r->has_visc && D.nb > 0;
+    r->has_post = r->has_post && D.nb > 0;
+    if (hipMemcpy(r->kinds, kinds3, 3 * (size_t)ntab * sizeof(int), hipMemcpyHostToDevice) !=
             hipSuccess ||
-        hipMemcpy(r->pars, pars.data(), pars.size() * sizeof(double), hipMemcpyHostToDevice) !=
-            hipSuccess ||
+        hipMemcpy(r->pars, pars3, 3 * (size_t)ntab * IBIS_NPAR * sizeof(double),
+                  hipMemcpyHostToDevice) != hipSuccess ||
         hipHostMalloc((void**)&r->host_buf, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         ibis_residual_destroy(r);
         return ibis::fail(IBIS_ERR_HIP, "ibis_residual_create: upload failed");
@@ -1690,6 +1751,12 @@ void launch_face_flux(const DevMesh& D, int flux_kind, const FaceParams& fp, boo
 template <int DIM, int LIM, int LAYOUT, bool OFFS>
 void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d, ibis_residual* r,
                       int nlog, hipStream_t st) {
+    if (r->prec == IBIS_PREC_FP32_FLUX) {
+        float* fl = reinterpret_cast<float*>(r->FL);
+        k_cell_fs<DIM, LIM, 2, OFFS, float><<<xcd_grid(nlog), BLOCK, 0, st>>>(
+            D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, fl, fl, r->offs);
+        return;
+    }
     k_cell_fs<DIM, LIM, LAYOUT, OFFS><<<xcd_grid(nlog), BLOCK, 0, st>>>(
         D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR, r->offs);
 }
@@ -1724,7 +1791,10 @@ void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, const 
                       const double* FR, double* dudt, hipStream_t st) {
     const unsigned g = xcd_grid(D.ntiles);
     const size_t shm = (size_t)(DIM + 2) * D.maxtf * sizeof(double);
-    if (layout == 0)
+    if (layout == 3) {  // layout 2 holding float values
+        const float* fl = reinterpret_cast<const float*>(FL);
+        k_face_tile_fs<DIM, FLUX, 2, float><<<g, TILE, shm, st>>>(D, fp, fl, fl, dudt);
+    } else if (layout == 0)
         k_face_tile_fs<DIM, FLUX, 0><<<g, TILE, shm, st>>>(D, fp, FL, FR, dudt);
     else if (layout == 1)
         k_face_tile_fs<DIM, FLUX, 1><<<g, TILE, shm, st>>>(D, fp, FL, FR, dudt);
@@ -1752,18 +1822,6 @@ void thermo_pair(int thermo, int* sA, int* rA, int* sB, int* rB) {
         case IBIS_THERMO_RHO_U: *sA = G_RHO; *rA = P_RHO; *sB = G_U; *rB = P_E; break;
         default: *sA = G_P; *rA = P_P; *sB = G_T; *rB = P_T; break;
     }
-}
-
-bool has_kind(const int32_t* kinds, int ntags, int kind) {
-    for (int q = 0; q < ntags * IBIS_MAX_ACT; q++)
-        if (kinds[q] == kind) return true;
-    return false;
-}
-
-bool has_any(const int32_t* kinds, int ntags) {
-    for (int q = 0; q < ntags * IBIS_MAX_ACT; q++)
-        if (kinds[q] != IBIS_BC_NONE) return true;
-    return false;
 }
 
 }  // namespace
@@ -1839,13 +1897,13 @@ void run_general(ibis_residual* r, const GenBufs<S>& B, S* prim, S* dudt, bool r
                  hipStream_t st, Prof& pf) {
     const DevMesh& D = r->m->d;
     const ibis_fv_desc& d = r->desc;
-    const size_t ntab = IBIS_MAX_TAGS * IBIS_MAX_ACT;
+    const size_t ntab = r->ntab;
     const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
     FaceParams fp;
     VarSet vs;
     residual_params(r, &fp, &vs);
-    const bool has_post = D.nb > 0 && has_kind(d.post_kind, d.ntags, IBIS_BC_CONSTANT_FLUX);
-    if (D.nb > 0 && has_any(d.pre_kind, d.ntags)) {
+    const bool has_post = r->has_post;
+    if (r->has_pre) {
         pf.beg();
         k_bc<S><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, prim);
         pf.end(0);
@@ -1872,7 +1930,7 @@ void run_general(ibis_residual* r, const GenBufs<S>& B, S* prim, S* dudt, bool r
         pf.end(3);
     }
     if (d.viscous) {
-        if (D.nb > 0 && has_any(d.visc_kind, d.ntags)) {
+        if (r->has_visc) {
             pf.beg();
             k_bc<S><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds + ntab,
                                                      r->pars + ntab * IBIS_NPAR, gp, prim);
@@ -1932,11 +1990,14 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
     FaceParams fp;
     VarSet vs;
     residual_params(r, &fp, &vs);
-    const bool has_post = D.nb > 0 && has_kind(d.post_kind, d.ntags, IBIS_BC_CONSTANT_FLUX);
-    const bool fused = !d.viscous && !has_post;
+    const bool fused = !d.viscous && !r->has_post;
+    if (r->prec == IBIS_PREC_FP32_FLUX && !(fused && recon))
+        return ibis::fail(IBIS_ERR_INVALID,
+                          "FP32 flux mode needs the inviscid second-order residual without "
+                          "flux-override boundaries");
     if (fused) {
         const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
-        if (D.nb > 0 && has_any(d.pre_kind, d.ntags)) {
+        if (r->has_pre) {
             pf.beg();
             k_bc<double><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, prim);
             pf.end(0);
@@ -1947,7 +2008,10 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
             pf.beg();
             k_pack<<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, vs, prim, r->pack);
             if (D.nb > 0) {
-                if (r->opt_layout == 0)
+                if (r->prec == IBIS_PREC_FP32_FLUX) {
+                    float* fl = reinterpret_cast<float*>(r->FL);
+                    k_ghost_fs<2, float><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, fl, fl);
+                } else if (r->opt_layout == 0)
                     k_ghost_fs<0><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
                 else if (r->opt_layout == 1)
                     k_ghost_fs<1><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
@@ -1963,10 +2027,11 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
                 launch_cell_fs<2>(D, vs, d, r, nlog, prim, st);
             pf.end(1);
             pf.beg();
+            const int lay = r->prec == IBIS_PREC_FP32_FLUX ? 3 : r->opt_layout;
             if (D.dim == 3)
-                launch_face_fs<3>(D, fp, d.flux, r->opt_layout, r->FL, r->FR, dudt, st);
+                launch_face_fs<3>(D, fp, d.flux, lay, r->FL, r->FR, dudt, st);
             else
-                launch_face_fs<2>(D, fp, d.flux, r->opt_layout, r->FL, r->FR, dudt, st);
+                launch_face_fs<2>(D, fp, d.flux, lay, r->FL, r->FR, dudt, st);
             pf.end(2);
         } else {
             // first order: face flux and surface integral as one tiled kernel
@@ -2100,6 +2165,14 @@ ibis_status ibis_cons_to_prim_jvp(ibis_residual* r, const double* cons, const do
     return IBIS_OK;
 }
 
+ibis_status ibis_residual_set_precision(ibis_residual* r, int mode) {
+    if (!r) return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_set_precision: null residual");
+    if (mode != IBIS_PREC_FP64 && mode != IBIS_PREC_FP32_FLUX)
+        return ibis::fail(IBIS_ERR_INVALID, "Unknown precision mode");
+    r->prec = mode;
+    return IBIS_OK;
+}
+
 ibis_status ibis_residual_sizes(const ibis_residual* r, int64_t* sizes) {
     if (!r || !sizes) return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_sizes: null argument");
     const DevMesh& D = r->m->d;
@@ -2179,6 +2252,19 @@ ibis_status ibis_estimate_dt(ibis_residual* r, const double* prim, double* dt_ho
     HIP_TRY(hipMemcpyAsync(r->host_buf, r->dt_dev, sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     *dt_host = r->host_buf[0];
+    return IBIS_OK;
+}
+
+ibis_status ibis_residual_cell_dt(ibis_residual* r, const double* prim, double* dt_cells,
+                                  ibis_stream_t stream) {
+    if (!r || !prim || !dt_cells) return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_cell_dt: null argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const DevMesh& D = r->m->d;
+    const ibis_fv_desc& d = r->desc;
+    ViscParams vp{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr};
+    k_dt_cells<<<nblocks(D.N), BLOCK, 0, st>>>(D, prim, r->face_rho, r->face_T, d.viscous ? 1 : 0,
+                                               d.signal_factor, vp, d.gamma, dt_cells);
+    HIP_TRY(hipGetLastError());
     return IBIS_OK;
 }
 

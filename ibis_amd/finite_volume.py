@@ -271,6 +271,14 @@ class FiniteVolume:
                                           C.byref(pa)), "ibis_residual_scratch")
         return g.value, lm.value, fx.value, bool(pa.value)
 
+    def set_precision(self, mode: str):
+        """"fp64" (default) or "fp32_flux": float face values, thermo closure
+        and flux calculator with FP64 accumulation."""
+        modes = {"fp64": 0, "fp32_flux": 1}
+        if mode not in modes:
+            raise IbisError(f"unknown precision mode {mode!r}; expected one of {sorted(modes)}")
+        check(lib().ibis_residual_set_precision(self._h, modes[mode]), "ibis_residual_set_precision")
+
     def set_option(self, key: str, value: int):
         """Implementation choice of the kernels (same results for every value)."""
         check(lib().ibis_residual_set_option(self._h, key.encode(), int(value)),

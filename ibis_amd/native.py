@@ -115,6 +115,7 @@ _SIGS = {
     "ibis_residual_set_option": (C.c_int, [_P, C.c_char_p, C.c_int64]),
     "ibis_residual_kernel_times": (C.c_int, [_P, _P, _P, C.c_int]),
     "ibis_residual_sizes": (C.c_int, [_P, _P]),
+    "ibis_residual_set_precision": (C.c_int, [_P, C.c_int]),
     "ibis_steady_matvec": (C.c_int, [_P, _P, _P, _P, C.c_double, C.c_int, _P]),
     "ibis_steady_rhs": (C.c_int, [_P, _P, _P, _P, C.c_int, _P]),
     "ibis_steady_op_matvec": (C.c_int, [_P, _P, _P, _P]),
@@ -128,6 +129,13 @@ _SIGS = {
     "ibis_jfnk_initialise": (C.c_int, [_P, _P, _P, _P, _P]),
     "ibis_jfnk_step": (C.c_int, [_P, _P, _P, C.c_int64, _P, _P, _P, _P]),
     "ibis_cfl_eval": (C.c_int, [_P, C.c_double, _P, _P, _P]),
+    "ibis_batch_create": (C.c_int, [_P, C.c_int, _P, _P, _P, C.POINTER(C.c_void_p)]),
+    "ibis_batch_destroy": (None, [_P]),
+    "ibis_batch_residual": (C.c_void_p, [_P]),
+    "ibis_batch_rk_init": (C.c_int, [_P, C.c_int, _P, _P, C.c_double, C.c_double, C.c_double,
+                                     _P, _P]),
+    "ibis_batch_rk_steps": (C.c_int, [_P, C.c_int, C.c_int]),
+    "ibis_batch_times": (C.c_int, [_P, _P, _P]),
     "ibis_last_error": (C.c_char_p, []),
 }
 

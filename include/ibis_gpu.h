@@ -46,6 +46,7 @@ typedef enum {
 #define IBIS_MAX_TAGS 16
 #define IBIS_MAX_ACT 4
 #define IBIS_NPAR 8
+#define IBIS_MAX_MESH_TAGS 4096 /* a mesh may hold more tags than a config (batches) */
 
 /* --------------------------------------------------------------------------
  * Host grid (replaces GridBlock construction, src/grid/grid/grid.h:89-167,
@@ -154,6 +155,16 @@ typedef struct ibis_residual ibis_residual;
 ibis_status ibis_residual_create(const ibis_mesh* m, const ibis_fv_desc* desc,
                                  ibis_residual** out);
 void ibis_residual_destroy(ibis_residual* r);
+
+/* Arithmetic of the residual.  IBIS_PREC_FP64 (default) is the reference's
+ * double arithmetic.  IBIS_PREC_FP32_FLUX stores the reconstructed face
+ * values as float and evaluates the thermo closure, frame rotation and flux
+ * calculator in float; gradients, limiters, the face-to-cell sum, the
+ * division by the volume, estimate_dt and every solver reduction stay FP64.
+ * Supported on the inviscid second-order residual without flux-override
+ * boundaries (ibis_compute_dudt fails otherwise).  No counterpart in the
+ * reference (its Ibis::real is double). */
+ibis_status ibis_residual_set_precision(ibis_residual* r, int mode);
 
 /* dim, n_cells, n_total, n_faces, n_ghost of the residual's mesh. */
 ibis_status ibis_residual_sizes(const ibis_residual* r, int64_t* sizes);
@@ -361,6 +372,40 @@ ibis_status ibis_jfnk_step(ibis_jfnk* j, double* cons, double* prim, int64_t ste
 /* CflSchedule::eval for a Jfnk's schedule (host only; for tests). */
 ibis_status ibis_cfl_eval(const ibis_jfnk_desc* desc, double t, double* previous_cfl,
                           double* previous_residual, double* cfl);
+
+/* ------------------------------------------------------------------------
+ * Batches: many independent instances (RungeKutta::take_step of each,
+ * src/solvers/solvers/runge_kutta.cpp:108-203) advanced together.  No
+ * counterpart in the reference, which steps one block per process.
+ * ------------------------------------------------------------------------ */
+typedef struct ibis_batch ibis_batch;
+
+/* m: the instances' grids concatenated (valid cells of instance 0, 1, ...,
+ * then their ghosts in the same order; tags of instance 0, then 1, ...).
+ * descs[i]: instance i's configuration (its ntags tags); all instances must
+ * share flux, reconstruction, limiter, gas and transport settings.
+ * cell_off / ghost_off (n_inst + 1): each instance's range of valid cells
+ * and of ghost cells (ghost g is cell n_cells + g). */
+ibis_status ibis_batch_create(const ibis_mesh* m, int n_inst, const ibis_fv_desc* descs,
+                              const int64_t* cell_off, const int64_t* ghost_off,
+                              ibis_batch** out);
+void ibis_batch_destroy(ibis_batch* b);
+/* The residual over all instances (owned by the batch). */
+ibis_residual* ibis_batch_residual(ibis_batch* b);
+
+/* Bind the device state (cons, prim over the concatenated mesh) and the
+ * tableau (as ibis_rk_step); every instance starts at t = 0 with
+ * dt_prev = dt_init.  Plotting clips are not applied. */
+ibis_status ibis_batch_rk_init(ibis_batch* b, int nstages, const double* a, const double* bw,
+                               double cfl, double max_time, double dt_init, double* cons,
+                               double* prim);
+/* nsteps explicit steps of every instance on the batch's own stream, each
+ * instance with its own dt; with use_graph the first call captures one step
+ * into a HIP graph that later steps replay.  Synchronises the device before
+ * and the batch's stream after. */
+ibis_status ibis_batch_rk_steps(ibis_batch* b, int nsteps, int use_graph);
+/* Per-instance time and last dt (n_inst each; either may be NULL). */
+ibis_status ibis_batch_times(ibis_batch* b, double* t, double* dt_prev);
 
 const char* ibis_last_error(void);
 

@@ -23,7 +23,7 @@ def _tangent(shape, seed, scale):
     return rng.standard_normal(shape) * scale[:, None]
 
 
-def _jvp_case(grid_io, cfg, prim_ref, seed, exact, allow_reconstruction=True):
+def _jvp_case(grid_io, cfg, prim_ref, seed, exact, allow_reconstruction=True, nan_ok=False):
     import torch
     og, ofv = oracle_pair(grid_io, cfg)
     n, nt = og.N, og.N + og.G
@@ -44,6 +44,13 @@ def _jvp_case(grid_io, cfg, prim_ref, seed, exact, allow_reconstruction=True):
     dd_gpu = gb.to_reference_order(ddot.cpu().numpy())
     p_gpu = gb.to_reference_order(prim.cpu().numpy())
     pd_gpu = gb.to_reference_order(pdot.cpu().numpy())
+    if nan_ok:
+        # NaN tangents where the oracle has them (same positions), the rest
+        # compared like any other tangent
+        bad = ~np.isfinite(dd_cpu[:, :n])
+        assert np.array_equal(bad, ~np.isfinite(dd_gpu[:, :n]))
+        dd_cpu, dd_gpu = np.where(np.isfinite(dd_cpu), dd_cpu, 0.0), np.where(
+            np.isfinite(dd_gpu), dd_gpu, 0.0)
     assert np.isfinite(dd_cpu[:, :n]).all() and np.abs(dd_cpu[:, :n]).max() > 0
     if exact:
         assert np.array_equal(d_gpu[:, :n], d_cpu[:, :n]), rel_err(d_gpu, d_cpu, n)
@@ -81,9 +88,14 @@ def test_wedge_jvp_first_order_preconditioner():
 
 
 def test_wedge_jvp_ldfss2():
+    """The reference's Dual pow (UT/dual.h:239-245) adds power.dual *
+    real * log(base): with a constant exponent that is 0 * log(base), NaN
+    for the negative base LDFSS2 passes it on subsonic faces
+    (ldfss2.cpp, (sqrt(0.5 (ML^2 + MR^2)) - 1)^2).  The tangent is NaN there
+    in the reference, the oracle and here alike."""
     g_io, cfg, fs = cases.wedge_case(48, 24, flux="ldfss2")
     prim = cases.perturbed_prim(g_io, fs, 48 * 24 + 2 * (48 + 24), seed=35)
-    _jvp_case(g_io, cfg, prim, seed=36, exact=False)
+    _jvp_case(g_io, cfg, prim, seed=36, exact=False, nan_ok=True)
 
 
 def test_hex_jvp():
