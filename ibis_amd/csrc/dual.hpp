@@ -27,55 +27,55 @@ struct Dual {
     __host__ __device__ Dual(double re, double du) : r(re), d(du) {}
 };
 
-__device__ __forceinline__ bool operator<(const Dual& a, const Dual& b) { return a.r < b.r; }
-__device__ __forceinline__ bool operator>(const Dual& a, const Dual& b) { return a.r > b.r; }
-__device__ __forceinline__ bool operator<=(const Dual& a, const Dual& b) { return a.r <= b.r; }
-__device__ __forceinline__ bool operator>=(const Dual& a, const Dual& b) { return a.r >= b.r; }
-__device__ __forceinline__ bool operator==(const Dual& a, const Dual& b) { return a.r == b.r; }
-__device__ __forceinline__ bool operator!=(const Dual& a, const Dual& b) { return a.r != b.r; }
+__host__ __device__ __forceinline__ bool operator<(const Dual& a, const Dual& b) { return a.r < b.r; }
+__host__ __device__ __forceinline__ bool operator>(const Dual& a, const Dual& b) { return a.r > b.r; }
+__host__ __device__ __forceinline__ bool operator<=(const Dual& a, const Dual& b) { return a.r <= b.r; }
+__host__ __device__ __forceinline__ bool operator>=(const Dual& a, const Dual& b) { return a.r >= b.r; }
+__host__ __device__ __forceinline__ bool operator==(const Dual& a, const Dual& b) { return a.r == b.r; }
+__host__ __device__ __forceinline__ bool operator!=(const Dual& a, const Dual& b) { return a.r != b.r; }
 
-__device__ __forceinline__ Dual operator+(const Dual& a, const Dual& b) { return Dual(a.r + b.r, a.d + b.d); }
-__device__ __forceinline__ Dual operator+(const Dual& a, double b) { return Dual(a.r + b, a.d); }
-__device__ __forceinline__ Dual operator+(double a, const Dual& b) { return Dual(a + b.r, b.d); }
-__device__ __forceinline__ Dual operator-(const Dual& a, const Dual& b) { return Dual(a.r - b.r, a.d - b.d); }
-__device__ __forceinline__ Dual operator-(const Dual& a, double b) { return Dual(a.r - b, a.d); }
-__device__ __forceinline__ Dual operator-(double a, const Dual& b) { return Dual(a - b.r, -b.d); }
-__device__ __forceinline__ Dual operator-(const Dual& a) { return Dual(-a.r, -a.d); }
-__device__ __forceinline__ Dual operator*(const Dual& a, const Dual& b) {
+__host__ __device__ __forceinline__ Dual operator+(const Dual& a, const Dual& b) { return Dual(a.r + b.r, a.d + b.d); }
+__host__ __device__ __forceinline__ Dual operator+(const Dual& a, double b) { return Dual(a.r + b, a.d); }
+__host__ __device__ __forceinline__ Dual operator+(double a, const Dual& b) { return Dual(a + b.r, b.d); }
+__host__ __device__ __forceinline__ Dual operator-(const Dual& a, const Dual& b) { return Dual(a.r - b.r, a.d - b.d); }
+__host__ __device__ __forceinline__ Dual operator-(const Dual& a, double b) { return Dual(a.r - b, a.d); }
+__host__ __device__ __forceinline__ Dual operator-(double a, const Dual& b) { return Dual(a - b.r, -b.d); }
+__host__ __device__ __forceinline__ Dual operator-(const Dual& a) { return Dual(-a.r, -a.d); }
+__host__ __device__ __forceinline__ Dual operator*(const Dual& a, const Dual& b) {
     return Dual(a.r * b.r, a.r * b.d + a.d * b.r);
 }
-__device__ __forceinline__ Dual operator*(const Dual& a, double b) { return Dual(a.r * b, a.d * b); }
-__device__ __forceinline__ Dual operator*(double a, const Dual& b) { return Dual(b.r * a, b.d * a); }
-__device__ __forceinline__ Dual operator/(const Dual& n, const Dual& m) {
+__host__ __device__ __forceinline__ Dual operator*(const Dual& a, double b) { return Dual(a.r * b, a.d * b); }
+__host__ __device__ __forceinline__ Dual operator*(double a, const Dual& b) { return Dual(b.r * a, b.d * a); }
+__host__ __device__ __forceinline__ Dual operator/(const Dual& n, const Dual& m) {
     return Dual(n.r / m.r, (n.d * m.r - n.r * m.d) / (m.r * m.r));
 }
-__device__ __forceinline__ Dual operator/(const Dual& n, double m) { return Dual(n.r / m, n.d / m); }
-__device__ __forceinline__ Dual operator/(double n, const Dual& m) {
+__host__ __device__ __forceinline__ Dual operator/(const Dual& n, double m) { return Dual(n.r / m, n.d / m); }
+__host__ __device__ __forceinline__ Dual operator/(double n, const Dual& m) {
     return Dual(n / m.r, -m.d * n / (m.r * m.r));
 }
-__device__ __forceinline__ Dual& operator+=(Dual& a, const Dual& b) {
+__host__ __device__ __forceinline__ Dual& operator+=(Dual& a, const Dual& b) {
     a.r += b.r;
     a.d += b.d;
     return a;
 }
-__device__ __forceinline__ Dual& operator-=(Dual& a, const Dual& b) {
+__host__ __device__ __forceinline__ Dual& operator-=(Dual& a, const Dual& b) {
     a.r -= b.r;
     a.d -= b.d;
     return a;
 }
 
-__device__ __forceinline__ Dual sqrt(const Dual& x) {
+__host__ __device__ __forceinline__ Dual sqrt(const Dual& x) {
     const double re = ::sqrt(x.r);
     return Dual(re, x.d / (2.0 * re));
 }
-__device__ __forceinline__ Dual fabs(const Dual& x) { return (x.r < 0.0) ? -x : x; }
-__device__ __forceinline__ Dual pow(const Dual& b, const Dual& p) {
+__host__ __device__ __forceinline__ Dual fabs(const Dual& x) { return (x.r < 0.0) ? -x : x; }
+__host__ __device__ __forceinline__ Dual pow(const Dual& b, const Dual& p) {
     const double re = ::pow(b.r, p.r);
     const double du = b.d * p.r * ::pow(b.r, p.r - 1.0) + p.d * re * ::log(b.r);
     return Dual(re, du);
 }
-__device__ __forceinline__ Dual floor(const Dual& x) { return Dual(::floor(x.r), ::floor(x.d)); }
-__device__ __forceinline__ Dual copysign(const Dual& m, const Dual& s) {
+__host__ __device__ __forceinline__ Dual floor(const Dual& x) { return Dual(::floor(x.r), ::floor(x.d)); }
+__host__ __device__ __forceinline__ Dual copysign(const Dual& m, const Dual& s) {
     const double re = ::copysign(m.r, s.r);
     const double du = (re - m.r) < 1e-15 ? m.d : -m.d;
     return Dual(re, du);
@@ -84,18 +84,18 @@ __device__ __forceinline__ Dual copysign(const Dual& m, const Dual& s) {
 // Kokkos::min / Kokkos::max on doubles return the first argument on ties;
 // the Dual overloads return the second.
 template <class S>
-__device__ __forceinline__ S kmin(S a, S b) { return (b < a) ? b : a; }
+__host__ __device__ __forceinline__ S kmin(S a, S b) { return (b < a) ? b : a; }
 template <class S>
-__device__ __forceinline__ S kmax(S a, S b) { return (a < b) ? b : a; }
-__device__ __forceinline__ Dual kmin(Dual a, Dual b) { return a.r < b.r ? a : b; }
-__device__ __forceinline__ Dual kmax(Dual a, Dual b) { return a.r > b.r ? a : b; }
-__device__ __forceinline__ Dual kmin(double a, Dual b) { return a < b.r ? Dual(a) : b; }
-__device__ __forceinline__ Dual kmax(double a, Dual b) { return a > b.r ? Dual(a) : b; }
-__device__ __forceinline__ Dual kmin(Dual a, double b) { return a.r < b ? a : Dual(b); }
-__device__ __forceinline__ Dual kmax(Dual a, double b) { return a.r > b ? a : Dual(b); }
+__host__ __device__ __forceinline__ S kmax(S a, S b) { return (a < b) ? b : a; }
+__host__ __device__ __forceinline__ Dual kmin(Dual a, Dual b) { return a.r < b.r ? a : b; }
+__host__ __device__ __forceinline__ Dual kmax(Dual a, Dual b) { return a.r > b.r ? a : b; }
+__host__ __device__ __forceinline__ Dual kmin(double a, Dual b) { return a < b.r ? Dual(a) : b; }
+__host__ __device__ __forceinline__ Dual kmax(double a, Dual b) { return a > b.r ? Dual(a) : b; }
+__host__ __device__ __forceinline__ Dual kmin(Dual a, double b) { return a.r < b ? a : Dual(b); }
+__host__ __device__ __forceinline__ Dual kmax(Dual a, double b) { return a.r > b ? a : Dual(b); }
 
-__device__ __forceinline__ double real_part(double x) { return x; }
-__device__ __forceinline__ double real_part(const Dual& x) { return x.r; }
+__host__ __device__ __forceinline__ double real_part(double x) { return x; }
+__host__ __device__ __forceinline__ double real_part(const Dual& x) { return x.r; }
 
 // Type of the literal constants inside an S-valued expression: a Dual
 // expression multiplies by plain doubles (as the reference's code does),

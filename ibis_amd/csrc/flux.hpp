@@ -25,7 +25,7 @@ struct GasConst {
 
 // Ausmdv::compute_flux, src/finite_volume/finite_volume/flux_calculators/ausmdv.cpp:9-131
 template <class S, bool THREE_D>
-__device__ __forceinline__ void flux_ausmdv(const S* L, const S* Rs, typename lit<S>::type R,
+__host__ __device__ __forceinline__ void flux_ausmdv(const S* L, const S* Rs, typename lit<S>::type R,
                                             typename lit<S>::type Cv,
                                             typename lit<S>::type gamma, S* out) {
     using K = typename lit<S>::type;
@@ -107,7 +107,7 @@ __device__ __forceinline__ void flux_ausmdv(const S* L, const S* Rs, typename li
 
 // Hanel::compute_flux, flux_calculators/hanel.cpp:8-64
 template <class S, bool THREE_D>
-__device__ __forceinline__ void flux_hanel(const S* L, const S* Rs, typename lit<S>::type R,
+__host__ __device__ __forceinline__ void flux_hanel(const S* L, const S* Rs, typename lit<S>::type R,
                                            typename lit<S>::type Cv,
                                            typename lit<S>::type gamma, S* out) {
     using K = typename lit<S>::type;
@@ -151,7 +151,7 @@ __device__ __forceinline__ void flux_hanel(const S* L, const S* Rs, typename lit
 
 // Ldfss2::compute_flux, flux_calculators/ldfss2.cpp:10-89
 template <class S, bool THREE_D>
-__device__ __forceinline__ void flux_ldfss2(const S* L, const S* Rs, typename lit<S>::type R,
+__host__ __device__ __forceinline__ void flux_ldfss2(const S* L, const S* Rs, typename lit<S>::type R,
                                             typename lit<S>::type Cv,
                                             typename lit<S>::type gamma,
                                             typename lit<S>::type delta, S* out) {
@@ -200,7 +200,7 @@ __device__ __forceinline__ void flux_ldfss2(const S* L, const S* Rs, typename li
 
 // Rusanov::compute_flux, flux_calculators/rusanov.cpp:10-52
 template <class S, bool THREE_D>
-__device__ __forceinline__ void flux_rusanov(const S* L, const S* Rs, typename lit<S>::type R,
+__host__ __device__ __forceinline__ void flux_rusanov(const S* L, const S* Rs, typename lit<S>::type R,
                                              typename lit<S>::type Cv,
                                              typename lit<S>::type gamma, S* out) {
     using K = typename lit<S>::type;
@@ -233,7 +233,7 @@ __device__ __forceinline__ void flux_rusanov(const S* L, const S* Rs, typename l
 
 // Thermodynamic closure after reconstruction (src/gas/gas/gas_model.h:91-116)
 template <class S>
-__device__ __forceinline__ void thermo_update(int mode, typename lit<S>::type R,
+__host__ __device__ __forceinline__ void thermo_update(int mode, typename lit<S>::type R,
                                               typename lit<S>::type Cv, S& rho, S& p, S& T,
                                               S& e) {
     switch (mode) {

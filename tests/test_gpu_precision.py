@@ -55,7 +55,7 @@ def test_hex_fp32_flux_drift(flux):
     print(f"hex24 {flux} drift (max_rel, rms_rel): {dr}", flush=True)
     for name, (mx, rms) in dr.items():
         assert 1e-12 < mx < 1e-3, (name, mx)
-        assert rms <= mx
+        assert 1e-12 < rms < 1e-3, (name, rms)
 
 
 def test_wedge_fp32_flux_drift():
