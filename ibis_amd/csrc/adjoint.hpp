@@ -1,0 +1,662 @@
+// adjoint.hpp -- discrete adjoint (vector-Jacobian product) of the residual.
+//
+// Included by residual.hip after the forward kernels, inside its anonymous
+// namespace (it uses DevMesh, VarSet, FaceParams, ViscParams).
+//
+// Given a cotangent db of dU/dt (rows [0, N)) and, optionally, a cotangent
+// on the face momentum fluxes of selected boundary tags (a wall force
+// functional), the kernels compute pb = (dR/dprim)^T db on the valid cells,
+// following FiniteVolume::compute_dudt (finite_volume.cpp:57-96) backwards:
+//
+//   integral (finite_volume.cpp:221-255)      flux bar of every face
+//   k_vjp_conv_face  flux calculator, frame rotation, thermo closure
+//                    -> bars of the reconstructed values of both sides
+//   k_vjp_visc_face  viscous flux (viscous_flux.cpp:133-184), Hasselbacher
+//                    face gradients and face averages -> bars of T, v of
+//                    both sides and of the two cells' viscous gradients
+//   k_vjp_cell       MUSCL reconstruction, limiter (active or frozen) and
+//                    the cell's own LSQ stencil
+//   k_vjp_ghost      ghost cells: their stencil contributions, then the
+//                    viscous and pre-reconstruction boundary actions
+//                    backwards onto the valid cell
+//   k_vjp_gather     per valid cell: its own terms plus what its
+//                    neighbours' stencils and its ghosts send back
+//
+// Nothing is stored by the forward pass beyond what the residual keeps
+// anyway (gradients, limiters, the state before and after the viscous
+// ghost fill): face states and fluxes are recomputed.  All sums are
+// gathers in a fixed order, so the result is deterministic.
+#ifndef IBIS_ADJOINT_HPP
+#define IBIS_ADJOINT_HPP
+
+// needs flux_adj.hpp, included by residual.hip at global scope
+
+struct AdjBufs {
+    // forward state
+    const double* prim_c;  // 7 x Nt, ghosts after the pre-reconstruction actions
+    const double* prim_v;  // 7 x Nt, ghosts after the viscous actions
+    const double* cgrad;   // NGRAD x 3 x N convective gradients
+    const double* clim;    // NGRAD x N limiters
+    const double* vgrad;   // NGRAD x 3 x N viscous gradients
+    // seeds
+    const double* db;      // (dim+2) x Nt
+    unsigned long long force_tags;
+    double force_dir[3];
+    // face cotangents
+    double* QLb;           // 5 x F: reconstructed values, left side
+    double* QRb;           // 5 x F
+    double* VLb;           // 4 x F: T, vx, vy, vz of the left cell (viscous)
+    double* VRb;           // 4 x F
+    double* GLb;           // 12 x F: viscous gradients of T, vx, vy, vz of the left cell
+    double* GRb;           // 12 x F
+    // cell cotangents
+    double* cself;         // 5 x N
+    double* cg;            // 5 x 3 x N
+    double* cext;          // 5 x N: bar of the limiter's extreme neighbour value
+    int* cextj;            // 5 x N: its slot in the cell's list (-1 none)
+    double* vself;         // 4 x N
+    double* vg;            // 4 x 3 x N
+    double* bcb;           // 7 x nb: what each ghost sends to its valid cell
+    double* pb;            // 7 x Nt output
+};
+
+// LSQ weights of cell c for a neighbour at centroid offset (dx, dy, dz)
+// (GR/gradient.h:66-116, the same expression as the forward kernel)
+template <int DIM>
+__device__ __forceinline__ void lsq_weights(const DevMesh& m, int c, double dx, double dy,
+                                            double dz, double* w) {
+    const int N = m.N;
+    const double r11 = m.lsq[c], r12 = m.lsq[N + c], r22 = m.lsq[2 * (size_t)N + c];
+    double r13 = 0.0, r23 = 0.0, r33 = 0.0;
+    if (DIM == 3) {
+        r13 = m.lsq[3 * (size_t)N + c];
+        r23 = m.lsq[4 * (size_t)N + c];
+        r33 = m.lsq[5 * (size_t)N + c];
+    }
+    const double beta = (r12 * r23 - r13 * r23) / (r11 * r22);
+    const double q12 = r12 / r11, q23 = r23 / r22;
+    const double a1 = dx / (r11 * r11);
+    const double a2 = 1.0 / (r22 * r22) * (dy - q12 * dx);
+    double a3 = 0.0;
+    if (DIM == 3) a3 = 1.0 / (r33 * r33) * (dz - q23 * dy + beta * dx);
+    w[0] = a1 - q12 * a2 + beta * a3;
+    w[1] = a2 - q23 * a3;
+    w[2] = a3;
+}
+
+// cotangent of the face flux (global frame, conserved rows) from dU/dt and
+// the force functional
+template <int DIM>
+__device__ __forceinline__ void face_flux_bar(const DevMesh& m, const AdjBufs& B, int f,
+                                              double* fb) {
+    constexpr int NC = DIM + 2;
+    const int N = m.N, Nt = m.Nt, F = m.F;
+    const int l = m.L[f], r = m.R[f];
+    const double A = m.geom[12 * (size_t)F + f];
+#pragma unroll
+    for (int q = 0; q < NC; q++) fb[q] = 0.0;
+    if (l < N) {
+        const double V = m.vol[l];
+#pragma unroll
+        for (int q = 0; q < NC; q++) fb[q] += B.db[(size_t)q * Nt + l] * (-A) / V;
+    }
+    if (r < N) {
+        const double V = m.vol[r];
+#pragma unroll
+        for (int q = 0; q < NC; q++) fb[q] += B.db[(size_t)q * Nt + r] * A / V;
+    }
+    if (B.force_tags && (l >= N || r >= N)) {
+        const int g = l >= N ? l : r;
+        const int b = m.b_of_ghost[g - N];
+        if ((B.force_tags >> m.b_tag[b]) & 1ull) {
+            // force on the boundary: momentum flux * area, oriented out of
+            // the valid cell
+            const double s = l < N ? A : -A;
+            for (int k = 0; k < DIM; k++) fb[1 + k] += B.force_dir[k] * s;
+        }
+    }
+}
+
+// Convective face flux backwards: bars of the reconstructed values
+template <int DIM, int FLUX>
+__global__ void __launch_bounds__(BLOCK)
+    k_vjp_conv_face(DevMesh m, FaceParams fp, VarSet vs, AdjBufs B) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int F = m.F, N = m.N, Nt = m.Nt;
+    if (f >= F) return;
+    constexpr int NC = DIM + 2;
+    constexpr bool T3 = DIM == 3;
+    double fb[NC];
+    face_flux_bar<DIM>(m, B, f, fb);
+    // forward: reconstruction of both sides (FV/convective_flux.cpp:243-420)
+    double q[2][5], st[2][NPRIM];
+    for (int side = 0; side < 2; side++) {
+        const int c = side ? m.R[f] : m.L[f];
+        const bool valid = c < N;
+        const double dx = m.geom[9 * (size_t)F + f] - m.cen[c];
+        const double dy = m.geom[10 * (size_t)F + f] - m.cen[Nt + c];
+        const double dz = m.geom[11 * (size_t)F + f] - m.cen[2 * (size_t)Nt + c];
+        for (int v = 0; v < 5; v++) {
+            const double u = B.prim_c[(size_t)vs.row[v] * Nt + c];
+            double gx = 0.0, gy = 0.0, gz = 0.0, L = 1.0;
+            if (valid) {
+                const double* g = B.cgrad + (size_t)vs.slot[v] * 3 * N;
+                gx = g[c];
+                gy = g[N + c];
+                gz = g[2 * (size_t)N + c];
+                if (fp.limited) L = B.clim[(size_t)vs.slot[v] * N + c];
+            }
+            q[side][v] = u + L * (gx * dx + gy * dy + gz * dz);
+        }
+        double rho = 0.0, p = 0.0, T = 0.0, e = 0.0;
+        const double a = q[side][0], b = q[side][1];
+        switch (fp.thermo) {
+            case IBIS_THERMO_RHO_P: p = a; rho = b; break;
+            case IBIS_THERMO_RHO_T: rho = a; T = b; break;
+            case IBIS_THERMO_RHO_U: rho = a; e = b; break;
+            default: p = a; T = b; break;
+        }
+        ibis::thermo_update<double>(fp.thermo, fp.R, fp.Cv, rho, p, T, e);
+        st[side][ibis::S_RHO] = rho;
+        st[side][ibis::S_P] = p;
+        st[side][ibis::S_T] = T;
+        st[side][ibis::S_E] = e;
+    }
+    const double nx = m.geom[f], ny = m.geom[(size_t)F + f], nz = m.geom[2 * (size_t)F + f];
+    const double t1x = m.geom[3 * (size_t)F + f], t1y = m.geom[4 * (size_t)F + f],
+                 t1z = m.geom[5 * (size_t)F + f];
+    const double t2x = m.geom[6 * (size_t)F + f], t2y = m.geom[7 * (size_t)F + f],
+                 t2z = m.geom[8 * (size_t)F + f];
+    for (int side = 0; side < 2; side++) {
+        const double a = q[side][2], b = q[side][3], c = q[side][4];
+        st[side][4] = a * nx + b * ny + c * nz;
+        st[side][5] = a * t1x + b * t1y + c * t1z;
+        st[side][6] = a * t2x + b * t2y + c * t2z;
+    }
+    // backward: rotation to the global frame (transpose), flux, rotation to
+    // the face frame (transpose), thermo closure
+    double ob[5];
+    ob[0] = fb[0];
+    ob[1] = nx * fb[1] + ny * fb[2] + (T3 ? nz * fb[3] : 0.0);
+    ob[2] = t1x * fb[1] + t1y * fb[2] + (T3 ? t1z * fb[3] : 0.0);
+    ob[3] = T3 ? t2x * fb[1] + t2y * fb[2] + t2z * fb[3] : 0.0;
+    ob[4] = fb[NC - 1];
+    double sb[2][NPRIM];
+    for (int side = 0; side < 2; side++)
+        for (int k = 0; k < NPRIM; k++) sb[side][k] = 0.0;
+    if (FLUX == IBIS_FLUX_AUSMDV)
+        ibis::flux_ausmdv_vjp<T3>(st[0], st[1], fp.R, fp.Cv, fp.gamma, ob, sb[0], sb[1]);
+    else if (FLUX == IBIS_FLUX_HANEL)
+        ibis::flux_hanel_vjp<T3>(st[0], st[1], fp.R, fp.Cv, fp.gamma, ob, sb[0], sb[1]);
+    else
+        ibis::flux_rusanov_vjp<T3>(st[0], st[1], fp.R, fp.Cv, fp.gamma, ob, sb[0], sb[1]);
+    for (int side = 0; side < 2; side++) {
+        const double s4 = sb[side][4], s5 = sb[side][5], s6 = sb[side][6];
+        double qb[5];
+        qb[2] = nx * s4 + t1x * s5 + t2x * s6;
+        qb[3] = ny * s4 + t1y * s5 + t2y * s6;
+        qb[4] = nz * s4 + t1z * s5 + t2z * s6;
+        ibis::thermo_vjp(fp.thermo, fp.R, fp.Cv, st[side][ibis::S_RHO], st[side][ibis::S_P],
+                         st[side][ibis::S_T], st[side][ibis::S_E], sb[side], &qb[0], &qb[1]);
+        double* out = side ? B.QRb : B.QLb;
+        for (int v = 0; v < 5; v++) out[(size_t)v * F + f] = qb[v];
+    }
+}
+
+// GAS/transport_properties.h:27-42: d mu / d T of Sutherland's law
+__device__ __forceinline__ double sutherland_dT(double T, double mu, const ViscParams& vp) {
+    return mu * (1.5 / T - 1.0 / (T + vp.Ts));
+}
+
+// Viscous face flux backwards (viscous_flux.cpp:16-184)
+template <int DIM>
+__global__ void __launch_bounds__(BLOCK)
+    k_vjp_visc_face(DevMesh m, ViscParams vp, AdjBufs B) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int F = m.F, N = m.N, Nt = m.Nt;
+    if (f >= F) return;
+    constexpr int NC = DIM + 2;
+    double fb[NC];
+    face_flux_bar<DIM>(m, B, f, fb);
+    const int l = m.L[f], r = m.R[f];
+    const bool lv = l < N, rv = r < N;
+    const double n[3] = {m.geom[f], m.geom[(size_t)F + f], m.geom[2 * (size_t)F + f]};
+    const double* P = B.prim_v;
+    // forward: face gradients of T, vx, vy, vz (var k at g[k][0..2])
+    const int slots[4] = {G_T, G_VX, G_VY, G_VZ};
+    const int rows[4] = {P_T, P_VX, P_VY, P_VZ};
+    double g[4][3];
+    double ehat[3] = {0.0, 0.0, 0.0}, len_e = 1.0, edn = 1.0;
+    const bool interior = lv && rv;
+    if (!interior) {
+        const int c = lv ? l : r;
+        for (int k = 0; k < 4; k++)
+            for (int d = 0; d < 3; d++) g[k][d] = B.vgrad[((size_t)slots[k] * 3 + d) * N + c];
+    } else {
+        const double ex = m.cen[r] - m.cen[l];
+        const double ey = m.cen[Nt + r] - m.cen[Nt + l];
+        const double ez = m.cen[2 * (size_t)Nt + r] - m.cen[2 * (size_t)Nt + l];
+        len_e = sqrt(ex * ex + ey * ey + ez * ez);
+        ehat[0] = ex / len_e;
+        ehat[1] = ey / len_e;
+        ehat[2] = ez / len_e;
+        edn = ehat[0] * n[0] + ehat[1] * n[1] + ehat[2] * n[2];
+        for (int k = 0; k < 4; k++) {
+            const double* gg = B.vgrad + (size_t)slots[k] * 3 * N;
+            const double ax = 0.5 * (gg[l] + gg[r]);
+            const double ay = 0.5 * (gg[N + l] + gg[N + r]);
+            const double az = 0.5 * (gg[2 * (size_t)N + l] + gg[2 * (size_t)N + r]);
+            const double vl = P[(size_t)rows[k] * Nt + l], vr = P[(size_t)rows[k] * Nt + r];
+            const double corr = ax * ehat[0] + ay * ehat[1] + az * ehat[2] - (vr - vl) / len_e;
+            g[k][0] = ax - corr * n[0] / edn;
+            g[k][1] = ay - corr * n[1] / edn;
+            g[k][2] = az - corr * n[2] / edn;
+        }
+    }
+    const double temp = 0.5 * (P[(size_t)P_T * Nt + l] + P[(size_t)P_T * Nt + r]);
+    const double vx = 0.5 * (P[(size_t)P_VX * Nt + l] + P[(size_t)P_VX * Nt + r]);
+    const double vy = 0.5 * (P[(size_t)P_VY * Nt + l] + P[(size_t)P_VY * Nt + r]);
+    const double vz = 0.5 * (P[(size_t)P_VZ * Nt + l] + P[(size_t)P_VZ * Nt + r]);
+    const double mu = sutherland<double>(temp, vp);
+    const double kc = vp.Cp * mu / vp.Pr;
+    const double lambda = -2.0 / 3.0 * mu;
+    const double (&gT)[3] = g[0];
+    const double (&gx)[3] = g[1];
+    const double (&gy)[3] = g[2];
+    const double (&gz)[3] = g[3];
+    const double div = gx[0] + gy[1] + gz[2];
+    const double bulk = lambda * div;
+    const double txx = 2.0 * mu * gx[0] + bulk, tyy = 2.0 * mu * gy[1] + bulk,
+                 tzz = 2.0 * mu * gz[2] + bulk;
+    const double txy = mu * (gx[1] + gy[0]), txz = mu * (gx[2] + gz[0]), tyz = mu * (gy[2] + gz[1]);
+    // backward: flux -= (n.tau_x, n.tau_y, [n.tau_z], n.theta)
+    const double Fxb = -fb[1], Fyb = -fb[2], Fzb = DIM == 3 ? -fb[3] : 0.0, Feb = -fb[NC - 1];
+    const double thxb = n[0] * Feb, thyb = n[1] * Feb, thzb = n[2] * Feb;
+    const double txxb = n[0] * Fxb + vx * thxb;
+    const double tyyb = n[1] * Fyb + vy * thyb;
+    const double tzzb = n[2] * Fzb + vz * thzb;
+    const double txyb = n[1] * Fxb + n[0] * Fyb + vy * thxb + vx * thyb;
+    const double txzb = n[2] * Fxb + n[0] * Fzb + vz * thxb + vx * thzb;
+    const double tyzb = n[2] * Fyb + n[1] * Fzb + vz * thyb + vy * thzb;
+    const double vxb = thxb * txx + thyb * txy + thzb * txz;
+    const double vyb = thxb * txy + thyb * tyy + thzb * tyz;
+    const double vzb = thxb * txz + thyb * tyz + thzb * tzz;
+    const double kcb = thxb * gT[0] + thyb * gT[1] + thzb * gT[2];
+    double gb[4][3];
+    gb[0][0] = thxb * kc;
+    gb[0][1] = thyb * kc;
+    gb[0][2] = thzb * kc;
+    const double bulkb = txxb + tyyb + tzzb;
+    const double divb = bulkb * lambda;
+    double mub = 2.0 * gx[0] * txxb + 2.0 * gy[1] * tyyb + 2.0 * gz[2] * tzzb +
+                 (gx[1] + gy[0]) * txyb + (gx[2] + gz[0]) * txzb + (gy[2] + gz[1]) * tyzb;
+    mub += bulkb * div * (-2.0 / 3.0) + kcb * vp.Cp / vp.Pr;
+    gb[1][0] = 2.0 * mu * txxb + divb;
+    gb[2][1] = 2.0 * mu * tyyb + divb;
+    gb[3][2] = 2.0 * mu * tzzb + divb;
+    gb[1][1] = mu * txyb;
+    gb[2][0] = mu * txyb;
+    gb[1][2] = mu * txzb;
+    gb[3][0] = mu * txzb;
+    gb[2][2] = mu * tyzb;
+    gb[3][1] = mu * tyzb;
+    const double tempb = mub * sutherland_dT(temp, mu, vp);
+    // face averages: half to each side
+    double Lv[4] = {0.5 * tempb, 0.5 * vxb, 0.5 * vyb, 0.5 * vzb};
+    double Rv[4] = {0.5 * tempb, 0.5 * vxb, 0.5 * vyb, 0.5 * vzb};
+    double GL[4][3], GR[4][3];
+    if (!interior) {  // the face gradient is the valid cell's gradient
+        for (int k = 0; k < 4; k++)
+            for (int d = 0; d < 3; d++) {
+                GL[k][d] = lv ? gb[k][d] : 0.0;
+                GR[k][d] = lv ? 0.0 : gb[k][d];
+            }
+    } else {  // Hasselbacher average
+        for (int k = 0; k < 4; k++) {
+            const double corrb = -(gb[k][0] * n[0] + gb[k][1] * n[1] + gb[k][2] * n[2]) / edn;
+            for (int d = 0; d < 3; d++) {
+                const double ab = gb[k][d] + corrb * ehat[d];
+                GL[k][d] = 0.5 * ab;
+                GR[k][d] = 0.5 * ab;
+            }
+            Rv[k] += -corrb / len_e;
+            Lv[k] += corrb / len_e;
+        }
+    }
+    for (int k = 0; k < 4; k++) {
+        B.VLb[(size_t)k * F + f] = Lv[k];
+        B.VRb[(size_t)k * F + f] = Rv[k];
+        for (int d = 0; d < 3; d++) {
+            B.GLb[(size_t)(k * 3 + d) * F + f] = GL[k][d];
+            B.GRb[(size_t)(k * 3 + d) * F + f] = GR[k][d];
+        }
+    }
+}
+
+// Per valid cell: reconstruction, limiter and own-stencil terms
+template <int DIM, int LIM, bool ACTIVE, bool VISC>
+__global__ void __launch_bounds__(BLOCK)
+    k_vjp_cell(DevMesh m, VarSet vs, double eps, double venkat_K, AdjBufs B) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int N = m.N, Nt = m.Nt, F = m.F;
+    if (i >= N) return;
+    const int nf = m.nfaces[i];
+    const double xi = m.cen[i], yi = m.cen[Nt + i], zi = m.cen[2 * (size_t)Nt + i];
+    const int ngrad = DIM == 3 ? 5 : 4;
+    for (int v = 0; v < 5; v++) {
+        const int row = vs.row[v], slot = vs.slot[v];
+        const double* u = B.prim_c + (size_t)row * Nt;
+        const double ui = u[i];
+        const double* gg = B.cgrad + (size_t)slot * 3 * N;
+        const double g[3] = {gg[i], gg[N + i], gg[2 * (size_t)N + i]};
+        const double phi = LIM != 0 ? B.clim[(size_t)slot * N + i] : 1.0;
+        double self = 0.0, phib = 0.0, gb[3] = {0.0, 0.0, 0.0};
+        // MUSCL: q = u + phi * (g . d) on each face of the cell
+        for (int j = 0; j < nf; j++) {
+            const int f = m.ell_face[(size_t)j * N + i];
+            const double qb = (m.ell_sign[(size_t)j * N + i] > 0 ? B.QLb : B.QRb)[(size_t)v * F + f];
+            const double d[3] = {m.geom[9 * (size_t)F + f] - xi, m.geom[10 * (size_t)F + f] - yi,
+                                 m.geom[11 * (size_t)F + f] - zi};
+            self += qb;
+            phib += qb * (g[0] * d[0] + g[1] * d[1] + g[2] * d[2]);
+            for (int k = 0; k < 3; k++) gb[k] += qb * phi * d[k];
+        }
+        double ext = 0.0;
+        int extj = -1;
+        if (LIM != 0 && ACTIVE && phib != 0.0) {
+            // re-run the limiter with the Dual branch choices (UT/dual.h):
+            // max/min keep the running value only when strictly better
+            double umax = ui, umin = ui;
+            int jmax = -1, jmin = -1;
+            for (int j = 0; j < nf; j++) {
+                const double un = u[m.ell_nbr[(size_t)j * N + i]];
+                if (!(umax > un)) {
+                    umax = un;
+                    jmax = j;
+                }
+                if (!(umin < un)) {
+                    umin = un;
+                    jmin = j;
+                }
+            }
+            double eps2 = 0.0;
+            if (LIM == 2) {
+                const double V = m.vol[i];
+                const double K3 = venkat_K * venkat_K * venkat_K;
+                eps2 = (DIM == 3) ? K3 * V : K3 * (V * sqrt(V));
+            }
+            double cur = 1.0;
+            int jstar = -1;
+            double d1s = 0.0, d2s = 0.0, dstar[3] = {0.0, 0.0, 0.0};
+            int sgs = 0;
+            for (int j = 0; j < nf; j++) {
+                const int f = m.ell_face[(size_t)j * N + i];
+                const double d[3] = {m.geom[9 * (size_t)F + f] - xi,
+                                     m.geom[10 * (size_t)F + f] - yi,
+                                     m.geom[11 * (size_t)F + f] - zi};
+                const double d2 = g[0] * d[0] + g[1] * d[1] + g[2] * d[2];
+                const int sg = (d2 > 0.0) - (d2 < 0.0);
+                if (sg == 0) continue;
+                const double d1 = sg > 0 ? umax - ui : umin - ui;
+                double val;
+                if (LIM == 1)
+                    val = d1 / (sg * (fabs(d2) + eps));
+                else
+                    val = (d1 * d1 + eps2 + 2.0 * (d1 * d2)) /
+                          (d1 * d1 + 2.0 * (d2 * d2) + d1 * d2 + eps2);
+                if (!(cur < val)) {  // Dual min: the new value on ties
+                    cur = val;
+                    jstar = j;
+                    d1s = d1;
+                    d2s = d2;
+                    sgs = sg;
+                    dstar[0] = d[0];
+                    dstar[1] = d[1];
+                    dstar[2] = d[2];
+                }
+            }
+            if (jstar >= 0) {
+                double d1b, d2b;
+                if (LIM == 1) {
+                    const double den = sgs * (fabs(d2s) + eps);
+                    d1b = phib / den;
+                    d2b = -phib * cur / den;  // d den / d d2 = 1 for either sign
+                } else {
+                    const double num = d1s * d1s + eps2 + 2.0 * (d1s * d2s);
+                    const double den = d1s * d1s + 2.0 * (d2s * d2s) + d1s * d2s + eps2;
+                    d1b = phib * ((2.0 * d1s + 2.0 * d2s) - num / den * (2.0 * d1s + d2s)) / den;
+                    d2b = phib * (2.0 * d1s - num / den * (4.0 * d2s + d1s)) / den;
+                }
+                // d1 = U_ext - u_i, U_ext the running max / min
+                const int je = sgs > 0 ? jmax : jmin;
+                if (je >= 0) {
+                    ext = d1b;
+                    extj = je;
+                } else {
+                    self += d1b;  // the extreme is u_i itself: d1 = 0 identically
+                }
+                self -= d1b;
+                for (int k = 0; k < 3; k++) gb[k] += d2b * dstar[k];
+            }
+        }
+        // own LSQ stencil: g = sum_j w_j (u_j - u_i)
+        if (v < ngrad) {
+            for (int j = 0; j < nf; j++) {
+                const int nb = m.ell_nbr[(size_t)j * N + i];
+                double w[3];
+                lsq_weights<DIM>(m, i, m.cen[nb] - xi, m.cen[Nt + nb] - yi,
+                                 m.cen[2 * (size_t)Nt + nb] - zi, w);
+                self -= w[0] * gb[0] + w[1] * gb[1] + w[2] * gb[2];
+            }
+        } else {
+            gb[0] = gb[1] = gb[2] = 0.0;  // 2D vz: no gradient is computed
+        }
+        B.cself[(size_t)v * N + i] = self;
+        for (int k = 0; k < 3; k++) B.cg[((size_t)v * 3 + k) * N + i] = gb[k];
+        B.cext[(size_t)v * N + i] = ext;
+        B.cextj[(size_t)v * N + i] = extj;
+    }
+    if (VISC) {
+        for (int k = 0; k < 4; k++) {
+            double self = 0.0, gb[3] = {0.0, 0.0, 0.0};
+            for (int j = 0; j < nf; j++) {
+                const int f = m.ell_face[(size_t)j * N + i];
+                const bool left = m.ell_sign[(size_t)j * N + i] > 0;
+                self += (left ? B.VLb : B.VRb)[(size_t)k * F + f];
+                for (int d = 0; d < 3; d++)
+                    gb[d] += (left ? B.GLb : B.GRb)[(size_t)(k * 3 + d) * F + f];
+            }
+            for (int j = 0; j < nf; j++) {
+                const int nb = m.ell_nbr[(size_t)j * N + i];
+                double w[3];
+                lsq_weights<DIM>(m, i, m.cen[nb] - xi, m.cen[Nt + nb] - yi,
+                                 m.cen[2 * (size_t)Nt + nb] - zi, w);
+                self -= w[0] * gb[0] + w[1] * gb[1] + w[2] * gb[2];
+            }
+            B.vself[(size_t)k * N + i] = self;
+            for (int d = 0; d < 3; d++) B.vg[((size_t)k * 3 + d) * N + i] = gb[d];
+        }
+    }
+}
+
+// slot of face f in the list of cell c
+__device__ __forceinline__ int slot_of(const DevMesh& m, int f, int c) {
+    return m.L[f] == c ? m.slotL[f] : m.slotR[f];
+}
+
+// reverse of one boundary action on the ghost bar gb (7 prim rows) and
+// the valid cell's bar vb (boundary.cpp:15-277)
+__device__ __forceinline__ void bc_action_vjp(const DevMesh& m, int kind, int f, double* gb,
+                                              double* vb) {
+    const int F = m.F;
+    switch (kind) {
+        case IBIS_BC_FLOW_STATE_COPY:
+            for (int k = 0; k < NPRIM; k++) gb[k] = 0.0;
+            break;
+        case IBIS_BC_INTERNAL_COPY:
+            for (int k = 0; k < NPRIM; k++) {
+                vb[k] += gb[k];
+                gb[k] = 0.0;
+            }
+            break;
+        case IBIS_BC_REFLECT_NORMAL: {
+            for (int k = 0; k < 4; k++) {  // rho, p, T, e
+                vb[k] += gb[k];
+                gb[k] = 0.0;
+            }
+            const double nx = m.geom[0 * F + f], ny = m.geom[1 * F + f], nz = m.geom[2 * F + f];
+            const double t1x = m.geom[3 * F + f], t1y = m.geom[4 * F + f], t1z = m.geom[5 * F + f];
+            const double t2x = m.geom[6 * F + f], t2y = m.geom[7 * F + f], t2z = m.geom[8 * F + f];
+            const double bx = gb[P_VX], by = gb[P_VY], bz = gb[P_VZ];
+            // v_g = xs n + ys t1 + zs t2, xs = -(v.n), ys = v.t1, zs = v.t2
+            const double xsb = bx * nx + by * ny + bz * nz;
+            const double ysb = bx * t1x + by * t1y + bz * t1z;
+            const double zsb = bx * t2x + by * t2y + bz * t2z;
+            vb[P_VX] += -xsb * nx + ysb * t1x + zsb * t2x;
+            vb[P_VY] += -xsb * ny + ysb * t1y + zsb * t2y;
+            vb[P_VZ] += -xsb * nz + ysb * t1z + zsb * t2z;
+            gb[P_VX] = gb[P_VY] = gb[P_VZ] = 0.0;
+            break;
+        }
+        case IBIS_BC_VEL_REFLECT:
+            for (int k = P_VX; k <= P_VZ; k++) {
+                vb[k] -= gb[k];
+                gb[k] = 0.0;
+            }
+            break;
+        case IBIS_BC_FIX_TEMPERATURE:
+            vb[P_T] -= gb[P_T];
+            gb[P_T] = 0.0;
+            break;
+        default:
+            break;
+    }
+}
+
+// Per boundary face: the ghost's bars, the boundary actions backwards
+template <int DIM, bool VISC>
+__global__ void __launch_bounds__(BLOCK)
+    k_vjp_ghost(DevMesh m, VarSet vs, const int* __restrict__ kinds, int ntab, AdjBufs B) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.nb) return;
+    const int N = m.N, Nt = m.Nt, F = m.F;
+    const int f = m.b_face[b], g = m.b_ghost[b], va = m.b_valid[b], t = m.b_tag[b];
+    const int ngrad = DIM == 3 ? 5 : 4;
+    const int j = slot_of(m, f, va);
+    double w[3];
+    lsq_weights<DIM>(m, va, m.cen[g] - m.cen[va], m.cen[Nt + g] - m.cen[Nt + va],
+                     m.cen[2 * (size_t)Nt + g] - m.cen[2 * (size_t)Nt + va], w);
+    double gb[NPRIM], vb[NPRIM];
+    for (int k = 0; k < NPRIM; k++) gb[k] = vb[k] = 0.0;
+    if (VISC) {  // viscous path reads the ghost after the viscous actions
+        const bool gl = m.L[f] == g;
+        const int rows[4] = {P_T, P_VX, P_VY, P_VZ};
+        for (int k = 0; k < 4; k++) {
+            double x = (gl ? B.VLb : B.VRb)[(size_t)k * F + f];
+            x += w[0] * B.vg[((size_t)k * 3 + 0) * N + va] + w[1] * B.vg[((size_t)k * 3 + 1) * N + va] +
+                 w[2] * B.vg[((size_t)k * 3 + 2) * N + va];
+            gb[rows[k]] += x;
+        }
+        for (int a = IBIS_MAX_ACT - 1; a >= 0; a--)
+            bc_action_vjp(m, kinds[ntab + t * IBIS_MAX_ACT + a], f, gb, vb);
+    }
+    {  // convective path reads the ghost after the pre-reconstruction actions
+        const bool gl = m.L[f] == g;
+        for (int v = 0; v < 5; v++) {
+            double x = (gl ? B.QLb : B.QRb)[(size_t)v * F + f];  // ghost side: plain copy
+            if (v < ngrad)
+                x += w[0] * B.cg[((size_t)v * 3 + 0) * N + va] +
+                     w[1] * B.cg[((size_t)v * 3 + 1) * N + va] +
+                     w[2] * B.cg[((size_t)v * 3 + 2) * N + va];
+            if (B.cextj[(size_t)v * N + va] == j) x += B.cext[(size_t)v * N + va];
+            gb[vs.row[v]] += x;
+        }
+    }
+    for (int a = IBIS_MAX_ACT - 1; a >= 0; a--) bc_action_vjp(m, kinds[t * IBIS_MAX_ACT + a], f, gb, vb);
+    for (int k = 0; k < NPRIM; k++) B.bcb[(size_t)k * m.nb + b] = vb[k];
+}
+
+// Per valid cell: own terms + neighbours' stencils + ghosts' actions
+template <int DIM, bool VISC>
+__global__ void __launch_bounds__(BLOCK) k_vjp_gather(DevMesh m, VarSet vs, AdjBufs B) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int N = m.N, Nt = m.Nt;
+    if (i >= N) return;
+    const int ngrad = DIM == 3 ? 5 : 4;
+    double out[NPRIM];
+    for (int k = 0; k < NPRIM; k++) out[k] = 0.0;
+    for (int v = 0; v < 5; v++) out[vs.row[v]] += B.cself[(size_t)v * N + i];
+    const int vrows[4] = {P_T, P_VX, P_VY, P_VZ};
+    if (VISC)
+        for (int k = 0; k < 4; k++) out[vrows[k]] += B.vself[(size_t)k * N + i];
+    const int nf = m.nfaces[i];
+    for (int j = 0; j < nf; j++) {
+        const int nb = m.ell_nbr[(size_t)j * N + i];
+        const int f = m.ell_face[(size_t)j * N + i];
+        if (nb >= N) {  // ghost: its actions' share for this cell
+            const int b = m.b_of_ghost[nb - N];
+            for (int k = 0; k < NPRIM; k++) out[k] += B.bcb[(size_t)k * m.nb + b];
+            continue;
+        }
+        // cell nb's stencil holds this cell at slot s
+        const int s = slot_of(m, f, nb);
+        double w[3];
+        lsq_weights<DIM>(m, nb, m.cen[i] - m.cen[nb], m.cen[Nt + i] - m.cen[Nt + nb],
+                         m.cen[2 * (size_t)Nt + i] - m.cen[2 * (size_t)Nt + nb], w);
+        for (int v = 0; v < 5; v++) {
+            double x = 0.0;
+            if (v < ngrad)
+                x = w[0] * B.cg[((size_t)v * 3 + 0) * N + nb] + w[1] * B.cg[((size_t)v * 3 + 1) * N + nb] +
+                    w[2] * B.cg[((size_t)v * 3 + 2) * N + nb];
+            if (B.cextj[(size_t)v * N + nb] == s) x += B.cext[(size_t)v * N + nb];
+            out[vs.row[v]] += x;
+        }
+        if (VISC)
+            for (int k = 0; k < 4; k++)
+                out[vrows[k]] += w[0] * B.vg[((size_t)k * 3 + 0) * N + nb] +
+                                 w[1] * B.vg[((size_t)k * 3 + 1) * N + nb] +
+                                 w[2] * B.vg[((size_t)k * 3 + 2) * N + nb];
+    }
+    for (int k = 0; k < NPRIM; k++) B.pb[(size_t)k * Nt + i] = out[k];
+}
+
+__global__ void __launch_bounds__(BLOCK) k_zero_rows(int rows, int N, int Nt, double* __restrict__ a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nt - N) return;
+    for (int k = 0; k < rows; k++) a[(size_t)k * Nt + N + i] = 0.0;
+}
+
+// conserved_to_primatives backwards (primative_conserved_conversion.cpp:
+// 5-30): prim bar -> cons bar, all n_total rows
+template <int DIM>
+__global__ void __launch_bounds__(BLOCK)
+    k_cons_to_prim_vjp(int Nt, double R, double Cv, const double* __restrict__ cq,
+                       const double* __restrict__ pb, double* __restrict__ cb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nt) return;
+    const double rho = cq[i];
+    const double mx = cq[(size_t)Nt + i], my = cq[2 * (size_t)Nt + i];
+    const double mz = DIM == 3 ? cq[3 * (size_t)Nt + i] : 0.0;
+    const double E = cq[(size_t)(DIM + 1) * Nt + i];
+    const double vx = mx / rho, vy = my / rho, vz = DIM == 3 ? mz / rho : 0.0;
+    const double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
+    const double u = E / rho - ke;
+    const double T = u / Cv;
+    // p = rho R T, T = u / Cv, u = E/rho - ke
+    double rhob = pb[(size_t)P_RHO * Nt + i] + pb[(size_t)P_P * Nt + i] * R * T;
+    const double Tb = pb[(size_t)P_T * Nt + i] + pb[(size_t)P_P * Nt + i] * rho * R;
+    const double ub = pb[(size_t)P_E * Nt + i] + Tb / Cv;
+    double vxb = pb[(size_t)P_VX * Nt + i] - ub * vx;
+    double vyb = pb[(size_t)P_VY * Nt + i] - ub * vy;
+    double vzb = DIM == 3 ? pb[(size_t)P_VZ * Nt + i] - ub * vz : 0.0;
+    const double Eb = ub / rho;
+    rhob -= ub * E / (rho * rho);
+    rhob -= (vxb * vx + vyb * vy + vzb * vz) / rho;
+    cb[i] = rhob;
+    cb[(size_t)Nt + i] = vxb / rho;
+    cb[2 * (size_t)Nt + i] = vyb / rho;
+    if (DIM == 3) cb[3 * (size_t)Nt + i] = vzb / rho;
+    cb[(size_t)(DIM + 1) * Nt + i] = Eb;
+}
+
+#endif

@@ -177,8 +177,9 @@ __host__ __device__ __forceinline__ void flux_ldfss2(const S* L, const S* Rs, ty
     S MmR = K(-0.25) * (MR - K(1.0)) * (MR - K(1.0));
     S alphaL = K(0.5) * (K(1.0) + copysign(S(1.0), ML));
     S alphaR = K(0.5) * (K(1.0) - copysign(S(1.0), MR));
-    S betaL = -kmax(K(0.0), K(1.0) - floor(fabs(ML)));
-    S betaR = -kmax(K(0.0), K(1.0) - floor(fabs(MR)));
+    // S(...): on the host side float floor/fabs may widen to double
+    S betaL = -kmax(K(0.0), S(K(1.0) - S(floor(fabs(ML)))));
+    S betaR = -kmax(K(0.0), S(K(1.0) - S(floor(fabs(MR)))));
     S PL = K(0.25) * (ML + K(1.0)) * (ML + K(1.0)) * (K(2.0) - ML);
     S PR = K(0.25) * (MR - K(1.0)) * (MR - K(1.0)) * (K(2.0) + MR);
     S DL = alphaL * (K(1.0) + betaL) - betaL * PL;
@@ -220,9 +221,9 @@ __host__ __device__ __forceinline__ void flux_rusanov(const S* L, const S* Rs, t
     S keR = K(0.5) * (uR * uR + vR * vR + wR * wR);
     S HR = eR + pR / rR + keR;
     S aR = sqrt(gamma * R * Rs[S_T]);
-    S S_plus = kmax(fabs(uL - aL), fabs(uR - aR));
-    S_plus = kmax(S_plus, fabs(uL + aL));
-    S_plus = kmax(S_plus, fabs(uR + aR));
+    S S_plus = kmax(S(fabs(uL - aL)), S(fabs(uR - aR)));
+    S_plus = kmax(S_plus, S(fabs(uL + aL)));
+    S_plus = kmax(S_plus, S(fabs(uR + aR)));
     out[0] = K(0.5) * (ruL + ruR) - K(0.5) * S_plus * (rR - rL);
     out[1] = K(0.5) * (ruL * uL + pL + ruR * uR + pR) - K(0.5) * S_plus * (rR * uR - rL * uL);
     out[2] = K(0.5) * (ruL * vL + ruR * vR) - K(0.5) * S_plus * (rR * vR - rL * vL);

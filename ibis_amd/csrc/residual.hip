@@ -36,9 +36,11 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "flux.hpp"
+#include "flux_adj.hpp"
 #include "ibis_internal.h"
 
 namespace {
@@ -78,6 +80,7 @@ struct DevMesh {
     const int* b_ghost;
     const int* b_valid;
     const int* b_tag;
+    const int* b_of_ghost;  // G: boundary-face index of each ghost cell
     // tiles of TILE consecutive cells and the faces touching them
     int ntiles, maxtf;
     const int* tile_face_off;         // ntiles + 1
@@ -1271,6 +1274,29 @@ __global__ void __launch_bounds__(BLOCK)
     if (i < n) u1[i] = u0[i] + k[i] * dt;
 }
 
+#include "adjoint.hpp"
+
+// zero the tangent of the limiters (frozen-limiter tangent residual)
+__global__ void __launch_bounds__(BLOCK) k_freeze_tangent(long n, ibis::Dual* __restrict__ a) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i].d = 0.0;
+}
+
+// per boundary face of the selected tags: momentum flux * area, oriented
+// out of the valid cell (the face force on the boundary)
+__global__ void __launch_bounds__(BLOCK)
+    k_force_terms(DevMesh m, const double* __restrict__ flux, unsigned long long tags,
+                  double* __restrict__ out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.nb) return;
+    const int f = m.b_face[b], F = m.F;
+    const bool on = (tags >> m.b_tag[b]) & 1ull;
+    const double A = m.geom[12 * (size_t)F + f];
+    const double s = m.L[f] < m.N ? A : -A;
+    for (int k = 0; k < 3; k++)
+        out[(size_t)k * m.nb + b] = (on && k < m.dim) ? flux[(size_t)(1 + k) * F + f] * s : 0.0;
+}
+
 inline unsigned nblocks(long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
 inline unsigned xcd_grid(long nlogical) {
     const long per = (nlogical + NXCD - 1) / NXCD;
@@ -1352,7 +1378,7 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
                                   "ibis_mesh_create: outsigns disagree with face left/right cells");
         }
     const int64_t nb = d->tag_off[d->ntags];
-    std::vector<int> b_face(nb), b_ghost(nb), b_valid(nb), b_tag(nb);
+    std::vector<int> b_face(nb), b_ghost(nb), b_valid(nb), b_tag(nb), b_of_ghost(G > 0 ? G : 1, -1);
     for (int t = 0; t < d->ntags; t++)
         for (int64_t k = d->tag_off[t]; k < d->tag_off[t + 1]; k++) {
             int64_t f = d->bfaces[k];
@@ -1364,6 +1390,9 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
             b_tag[k] = t;
             if (b_ghost[k] < N)
                 return ibis::fail(IBIS_ERR_INVALID, "boundary face without a ghost cell");
+            if (b_of_ghost[b_ghost[k] - N] != -1)
+                return ibis::fail(IBIS_ERR_INVALID, "a ghost cell on two boundary faces");
+            b_of_ghost[b_ghost[k] - N] = (int)k;
         }
     ibis_mesh* m = new ibis_mesh();
     m->ntags = d->ntags;
@@ -1466,6 +1495,7 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
     UP(b_ghost, b_ghost);
     UP(b_valid, b_valid);
     UP(b_tag, b_tag);
+    UP(b_of_ghost, b_of_ghost);
     UP(toff, tile_face_off);
     UP(tfaces, tile_faces);
     UP(clds, cell_lds);
@@ -1524,6 +1554,12 @@ struct ibis_residual {
     std::vector<double*> k;
     double* k_tmp = nullptr;
     double* flow_tmp = nullptr;
+    // adjoint (VJP) scratch, allocated on first use
+    std::vector<void*> adj;
+    double *adj_prim_c = nullptr, *adj_vgrad = nullptr, *adj_face = nullptr, *adj_cell = nullptr;
+    int* adj_extj = nullptr;
+    double *adj_bcb = nullptr, *adj_zero = nullptr, *adj_dudt = nullptr, *adj_force = nullptr;
+    int freeze_lim = 0;  // option freeze_limiter (tangent residual only)
     // tangent (JVP) state and scratch, allocated on first use
     ibis::Dual* dual_prim = nullptr;
     ibis::Dual* dual_dudt = nullptr;
@@ -1541,6 +1577,7 @@ void ibis_residual_destroy(ibis_residual* r) {
                     (void*)r->dual_lim, (void*)r->dual_flux})
         if (p) (void)hipFree(p);
     for (double* p : r->k) (void)hipFree(p);
+    for (void* p : r->adj) (void)hipFree(p);
     for (auto& e : r->pending) {
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
@@ -1914,6 +1951,12 @@ void run_general(ibis_residual* r, const GenBufs<S>& B, S* prim, S* dudt, bool r
             launch_cell_grad<S, 3>(D, vs, d.limiter, prim, B.grad, B.lim, d.lim_eps, d.venkat_K, st);
         else
             launch_cell_grad<S, 2>(D, vs, d.limiter, prim, B.grad, B.lim, d.lim_eps, d.venkat_K, st);
+        if constexpr (std::is_same<S, ibis::Dual>::value) {
+            if (r->freeze_lim && d.limiter != IBIS_LIM_UNLIMITED) {
+                const long nl = (long)NGRAD * D.N;
+                k_freeze_tangent<<<nblocks(nl), BLOCK, 0, st>>>(nl, B.lim);
+            }
+        }
         pf.end(1);
     }
     pf.beg();
@@ -2204,7 +2247,221 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
         r->opt_offsets = value ? 1 : 0;
         return IBIS_OK;
     }
+    if (k == "freeze_limiter") {
+        r->freeze_lim = value ? 1 : 0;
+        return IBIS_OK;
+    }
     return ibis::fail(IBIS_ERR_INVALID, ("unknown option " + k).c_str());
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint (vector-Jacobian product) of the residual
+// ---------------------------------------------------------------------------
+namespace {
+
+ibis_status ensure_adjoint(ibis_residual* r) {
+    if (r->adj_prim_c) return IBIS_OK;
+    const DevMesh& D = r->m->d;
+    const size_t N = D.N, Nt = D.Nt, F = D.F, nb = std::max(D.nb, 1);
+    const int nc = D.dim + 2;
+    auto al = [&](void** p, size_t bytes) -> hipError_t {
+        hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 8));
+        if (e == hipSuccess) e = hipMemset(*p, 0, std::max<size_t>(bytes, 8));
+        if (e == hipSuccess) r->adj.push_back(*p);
+        return e;
+    };
+    HIP_TRY(al((void**)&r->adj_prim_c, sizeof(double) * NPRIM * Nt));
+    HIP_TRY(al((void**)&r->adj_vgrad, sizeof(double) * NGRAD * 3 * N));
+    HIP_TRY(al((void**)&r->adj_face, sizeof(double) * (5 + 5 + 4 + 4 + 12 + 12) * F));
+    HIP_TRY(al((void**)&r->adj_cell, sizeof(double) * (5 + 15 + 5 + 4 + 12) * N));
+    HIP_TRY(al((void**)&r->adj_extj, sizeof(int) * 5 * N));
+    HIP_TRY(al((void**)&r->adj_bcb, sizeof(double) * NPRIM * nb));
+    HIP_TRY(al((void**)&r->adj_zero, sizeof(double) * nc * Nt));
+    HIP_TRY(al((void**)&r->adj_dudt, sizeof(double) * nc * Nt));
+    HIP_TRY(al((void**)&r->adj_force, sizeof(double) * 3 * nb));
+    return IBIS_OK;
+}
+
+ibis_status adjoint_supported(const ibis_residual* r) {
+    const ibis_fv_desc& d = r->desc;
+    if (d.order != 2)
+        return ibis::fail(IBIS_ERR_INVALID, "the adjoint needs the second-order residual");
+    if (d.flux == IBIS_FLUX_LDFSS2)
+        return ibis::fail(IBIS_ERR_INVALID, "the adjoint supports the AUSMDV, Hanel and Rusanov fluxes");
+    if (r->has_post)
+        return ibis::fail(IBIS_ERR_INVALID, "the adjoint does not support constant-flux boundaries");
+    if (r->prec != IBIS_PREC_FP64)
+        return ibis::fail(IBIS_ERR_INVALID, "the adjoint is FP64 only");
+    std::vector<int> kinds(2 * r->ntab);
+    if (hipMemcpy(kinds.data(), r->kinds, sizeof(int) * 2 * r->ntab, hipMemcpyDeviceToHost) != hipSuccess)
+        return ibis::fail(IBIS_ERR_HIP, "adjoint: reading the boundary tables failed");
+    for (int k : kinds)
+        if (k == IBIS_BC_SUBSONIC_INFLOW || k == IBIS_BC_SUBSONIC_OUTFLOW)
+            return ibis::fail(IBIS_ERR_INVALID, "the adjoint does not support subsonic boundaries");
+    return IBIS_OK;
+}
+
+template <int DIM>
+void launch_vjp(ibis_residual* r, const VarSet& vs, const FaceParams& fp, const AdjBufs& B,
+                bool active, hipStream_t st) {
+    const DevMesh& D = r->m->d;
+    const ibis_fv_desc& d = r->desc;
+    const bool visc = d.viscous != 0;
+    const unsigned nf = nblocks(D.F), nc = nblocks(D.N);
+    switch (d.flux) {
+        case IBIS_FLUX_AUSMDV: k_vjp_conv_face<DIM, IBIS_FLUX_AUSMDV><<<nf, BLOCK, 0, st>>>(D, fp, vs, B); break;
+        case IBIS_FLUX_HANEL: k_vjp_conv_face<DIM, IBIS_FLUX_HANEL><<<nf, BLOCK, 0, st>>>(D, fp, vs, B); break;
+        default: k_vjp_conv_face<DIM, IBIS_FLUX_RUSANOV><<<nf, BLOCK, 0, st>>>(D, fp, vs, B); break;
+    }
+    ViscParams vp{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr};
+    if (visc) k_vjp_visc_face<DIM><<<nf, BLOCK, 0, st>>>(D, vp, B);
+#define CELL(LIM, ACT, VIS) \
+    k_vjp_cell<DIM, LIM, ACT, VIS><<<nc, BLOCK, 0, st>>>(D, vs, d.lim_eps, d.venkat_K, B)
+    const int lim = d.limiter;
+    if (visc) {
+        if (lim == 0) CELL(0, false, true);
+        else if (lim == 1) { if (active) CELL(1, true, true); else CELL(1, false, true); }
+        else { if (active) CELL(2, true, true); else CELL(2, false, true); }
+    } else {
+        if (lim == 0) CELL(0, false, false);
+        else if (lim == 1) { if (active) CELL(1, true, false); else CELL(1, false, false); }
+        else { if (active) CELL(2, true, false); else CELL(2, false, false); }
+    }
+#undef CELL
+    if (D.nb > 0) {
+        if (visc)
+            k_vjp_ghost<DIM, true><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, r->kinds, r->ntab, B);
+        else
+            k_vjp_ghost<DIM, false><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, r->kinds, r->ntab, B);
+    }
+    if (visc)
+        k_vjp_gather<DIM, true><<<nc, BLOCK, 0, st>>>(D, vs, B);
+    else
+        k_vjp_gather<DIM, false><<<nc, BLOCK, 0, st>>>(D, vs, B);
+    if (D.G > 0) k_zero_rows<<<nblocks(D.G), BLOCK, 0, st>>>(NPRIM, D.N, D.Nt, B.pb);
+}
+
+}  // namespace
+
+ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* dudt_bar,
+                                  uint64_t force_tags, const double* force_dir, double* prim_bar,
+                                  int limiter_mode, ibis_stream_t stream) {
+    if (!r || !prim || !prim_bar || (force_tags && !force_dir))
+        return ibis::fail(IBIS_ERR_INVALID, "ibis_compute_dudt_vjp: null argument");
+    if (limiter_mode != IBIS_LIMITER_FROZEN && limiter_mode != IBIS_LIMITER_ACTIVE)
+        return ibis::fail(IBIS_ERR_INVALID, "ibis_compute_dudt_vjp: unknown limiter mode");
+    ibis_status s = adjoint_supported(r);
+    if (s != IBIS_OK) return s;
+    s = ensure_adjoint(r);
+    if (s != IBIS_OK) return s;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const DevMesh& D = r->m->d;
+    const ibis_fv_desc& d = r->desc;
+    const size_t N = D.N, F = D.F, Nt = D.Nt;
+    const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
+    FaceParams fp;
+    VarSet vs;
+    residual_params(r, &fp, &vs);
+    // forward: the residual's state, gradients and limiters (recomputed)
+    if (r->has_pre) k_bc<double><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, prim);
+    HIP_TRY(hipMemcpyAsync(r->adj_prim_c, prim, sizeof(double) * NPRIM * Nt,
+                           hipMemcpyDeviceToDevice, st));
+    if (D.dim == 3)
+        launch_cell_grad<double, 3>(D, vs, d.limiter, r->adj_prim_c, r->grad, r->lim, d.lim_eps,
+                                    d.venkat_K, st);
+    else
+        launch_cell_grad<double, 2>(D, vs, d.limiter, r->adj_prim_c, r->grad, r->lim, d.lim_eps,
+                                    d.venkat_K, st);
+    if (d.viscous) {
+        if (r->has_visc)
+            k_bc<double><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds + r->ntab,
+                                                          r->pars + (size_t)r->ntab * IBIS_NPAR, gp, prim);
+        VarSet vv;
+        vv.n = 4;
+        vv.ngrad = 4;
+        const int rows[5] = {P_T, P_VX, P_VY, P_VZ, P_VZ};
+        const int slots[5] = {G_T, G_VX, G_VY, G_VZ, G_VZ};
+        for (int q = 0; q < 5; q++) {
+            vv.row[q] = rows[q];
+            vv.slot[q] = slots[q];
+        }
+        if (D.dim == 3)
+            launch_cell_grad<double, 3>(D, vv, 0, prim, r->adj_vgrad, r->lim, 0.0, 0.0, st);
+        else
+            launch_cell_grad<double, 2>(D, vv, 0, prim, r->adj_vgrad, r->lim, 0.0, 0.0, st);
+    }
+    AdjBufs B;
+    B.prim_c = r->adj_prim_c;
+    B.prim_v = prim;
+    B.cgrad = r->grad;
+    B.clim = r->lim;
+    B.vgrad = r->adj_vgrad;
+    B.db = dudt_bar ? dudt_bar : r->adj_zero;
+    B.force_tags = force_tags;
+    for (int k = 0; k < 3; k++) B.force_dir[k] = force_tags ? force_dir[k] : 0.0;
+    double* fa = r->adj_face;
+    B.QLb = fa;
+    B.QRb = fa + 5 * F;
+    B.VLb = fa + 10 * F;
+    B.VRb = fa + 14 * F;
+    B.GLb = fa + 18 * F;
+    B.GRb = fa + 30 * F;
+    double* ca = r->adj_cell;
+    B.cself = ca;
+    B.cg = ca + 5 * N;
+    B.cext = ca + 20 * N;
+    B.vself = ca + 25 * N;
+    B.vg = ca + 29 * N;
+    B.cextj = r->adj_extj;
+    B.bcb = r->adj_bcb;
+    B.pb = prim_bar;
+    const bool active = limiter_mode == IBIS_LIMITER_ACTIVE;
+    if (D.dim == 3)
+        launch_vjp<3>(r, vs, fp, B, active, st);
+    else
+        launch_vjp<2>(r, vs, fp, B, active, st);
+    HIP_TRY(hipGetLastError());
+    return IBIS_OK;
+}
+
+ibis_status ibis_cons_to_prim_vjp(ibis_residual* r, const double* cons, const double* prim_bar,
+                                  double* cons_bar, ibis_stream_t stream) {
+    if (!r || !cons || !prim_bar || !cons_bar)
+        return ibis::fail(IBIS_ERR_INVALID, "ibis_cons_to_prim_vjp: null argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const DevMesh& D = r->m->d;
+    if (D.dim == 3)
+        k_cons_to_prim_vjp<3><<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, r->desc.R, r->desc.Cv, cons,
+                                                                prim_bar, cons_bar);
+    else
+        k_cons_to_prim_vjp<2><<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, r->desc.R, r->desc.Cv, cons,
+                                                                prim_bar, cons_bar);
+    HIP_TRY(hipGetLastError());
+    return IBIS_OK;
+}
+
+ibis_status ibis_wall_force(ibis_residual* r, double* prim, uint64_t force_tags, double* force,
+                            ibis_stream_t stream) {
+    if (!r || !prim || !force) return ibis::fail(IBIS_ERR_INVALID, "ibis_wall_force: null argument");
+    ibis_status s = ensure_adjoint(r);
+    if (s != IBIS_OK) return s;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const DevMesh& D = r->m->d;
+    if (r->prec != IBIS_PREC_FP64)
+        return ibis::fail(IBIS_ERR_INVALID, "ibis_wall_force is FP64 only");
+    Prof pf{r, st};
+    GenBufs<double> B{r->grad, r->lim, r->flux, r->face_rho, r->face_T};
+    run_general<double>(r, B, prim, r->adj_dudt, r->desc.order == 2, st, pf);
+    for (int k = 0; k < 3; k++) force[k] = 0.0;
+    if (D.nb == 0) return IBIS_OK;
+    k_force_terms<<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->flux, force_tags, r->adj_force);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> h(3 * (size_t)D.nb);
+    HIP_TRY(hipMemcpyAsync(h.data(), r->adj_force, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int k = 0; k < 3; k++)
+        for (int b = 0; b < D.nb; b++) force[k] += h[(size_t)k * D.nb + b];
+    return IBIS_OK;
 }
 
 ibis_status ibis_residual_profile(ibis_residual* r, int enable) {

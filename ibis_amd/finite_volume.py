@@ -238,6 +238,45 @@ class FiniteVolume:
                                           ptr(prim_dot), _stream_handle(stream)),
               "ibis_cons_to_prim_jvp")
 
+    def tag_mask(self, names) -> int:
+        """Bit mask of boundary tags (by name) for the force functional."""
+        tags = list(self.grid.tags)
+        mask = 0
+        for n in ([names] if isinstance(names, str) else names):
+            if n not in tags:
+                raise IbisError(f"unknown boundary tag {n!r}; tags are {tags}")
+            mask |= 1 << tags.index(n)
+        return mask
+
+    def compute_dudt_vjp(self, prim, prim_bar, dudt_bar=None, force_tags: int = 0,
+                         force_dir=(1.0, 0.0, 0.0), limiter: str = "active", stream=None):
+        """prim_bar = (dR/dprim)^T dudt_bar + (dJ/dprim)^T, J the force on the
+        boundaries in force_tags along force_dir (ibis_compute_dudt_vjp).
+        Ghost rows of prim are written as by compute_dudt."""
+        modes = {"frozen": 0, "active": 1}
+        if limiter not in modes:
+            raise IbisError(f"limiter must be 'frozen' or 'active', not {limiter!r}")
+        self._check_state(prim, NPRIM, self.grid.Nt, "prim")
+        self._check_state(prim_bar, NPRIM, self.grid.Nt, "prim_bar")
+        if dudt_bar is not None:
+            self._check_state(dudt_bar, self.ncons, self.grid.Nt, "dudt_bar")
+        fd = np.ascontiguousarray(np.asarray(force_dir, dtype=np.float64).reshape(3))
+        check(lib().ibis_compute_dudt_vjp(self._h, ptr(prim), ptr(dudt_bar), int(force_tags),
+                                          ptr(fd), ptr(prim_bar), modes[limiter],
+                                          _stream_handle(stream)), "ibis_compute_dudt_vjp")
+
+    def cons_to_prim_vjp(self, cons, prim_bar, cons_bar, stream=None):
+        check(lib().ibis_cons_to_prim_vjp(self._h, ptr(cons), ptr(prim_bar), ptr(cons_bar),
+                                          _stream_handle(stream)), "ibis_cons_to_prim_vjp")
+
+    def wall_force(self, prim, force_tags: int, stream=None) -> np.ndarray:
+        """Pressure + viscous force on the boundaries in force_tags (x, y, z)."""
+        self._check_state(prim, NPRIM, self.grid.Nt, "prim")
+        out = np.zeros(3)
+        check(lib().ibis_wall_force(self._h, ptr(prim), int(force_tags), ptr(out),
+                                    _stream_handle(stream)), "ibis_wall_force")
+        return out
+
     def estimate_dt(self, flow_state, grid=None, gas_model=None, trans_prop=None, stream=None):
         prim = flow_state.data if isinstance(flow_state, FlowStates) else flow_state
         self._check_state(prim, NPRIM, self.grid.Nt, "flow_state")

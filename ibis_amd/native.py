@@ -129,6 +129,9 @@ _SIGS = {
     "ibis_jfnk_initialise": (C.c_int, [_P, _P, _P, _P, _P]),
     "ibis_jfnk_step": (C.c_int, [_P, _P, _P, C.c_int64, _P, _P, _P, _P]),
     "ibis_cfl_eval": (C.c_int, [_P, C.c_double, _P, _P, _P]),
+    "ibis_compute_dudt_vjp": (C.c_int, [_P, _P, _P, C.c_uint64, _P, _P, C.c_int, _P]),
+    "ibis_cons_to_prim_vjp": (C.c_int, [_P, _P, _P, _P, _P]),
+    "ibis_wall_force": (C.c_int, [_P, _P, C.c_uint64, _P, _P]),
     "ibis_batch_create": (C.c_int, [_P, C.c_int, _P, _P, _P, C.POINTER(C.c_void_p)]),
     "ibis_batch_destroy": (None, [_P]),
     "ibis_batch_residual": (C.c_void_p, [_P]),

@@ -186,6 +186,37 @@ ibis_status ibis_compute_dudt_jvp(ibis_residual* r, double* prim, double* prim_d
                                   double* dudt_dot, int allow_reconstruction,
                                   ibis_stream_t stream);
 
+/* Discrete adjoint of the residual (no counterpart in the reference, which
+ * differentiates only forward, through Dual numbers).  With prim the state
+ * (ghost rows are written as by ibis_compute_dudt), computes
+ *   prim_bar = (d dU/dt / d prim)^T dudt_bar + (d J / d prim)^T
+ * on the valid cells (ghost rows of prim_bar are zero), where
+ *   J = sum over the boundary faces of the tags in force_tags (bit t = tag t)
+ *       of force_dir . (momentum flux * area, oriented out of the valid
+ *       cell): the pressure + viscous force on those boundaries.
+ * dudt_bar ((dim+2) x n_total, rows [0, n_cells)) may be NULL; force_tags may
+ * be 0.  limiter_mode: IBIS_LIMITER_ACTIVE differentiates through the
+ * limiter (the branch the Dual residual takes), IBIS_LIMITER_FROZEN holds it
+ * constant (match it with the option "freeze_limiter" of the tangent
+ * residual).  Second-order residual, AUSMDV / Hanel / Rusanov, FP64, no
+ * subsonic or constant-flux boundaries.  Reconstructed states are
+ * recomputed, not stored.  Asynchronous. */
+enum { IBIS_LIMITER_FROZEN = 0, IBIS_LIMITER_ACTIVE = 1 };
+ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* dudt_bar,
+                                  uint64_t force_tags, const double* force_dir, double* prim_bar,
+                                  int limiter_mode, ibis_stream_t stream);
+
+/* conserved_to_primatives backwards: cons_bar = (d prim / d cons)^T prim_bar,
+ * all n_total rows. */
+ibis_status ibis_cons_to_prim_vjp(ibis_residual* r, const double* cons, const double* prim_bar,
+                                  double* cons_bar, ibis_stream_t stream);
+
+/* The force functional J above (x, y, z components, force_dir = e_k) for
+ * the state prim: runs the residual and sums its boundary face fluxes.
+ * Synchronises. */
+ibis_status ibis_wall_force(ibis_residual* r, double* prim, uint64_t force_tags, double* force,
+                            ibis_stream_t stream);
+
 /* conserved_to_primatives on Dual values (primative_conserved_conversion.cpp:
  * 5-30), all n_total rows: value and tangent of the state from those of the
  * conserved quantities. */
