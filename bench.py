@@ -85,6 +85,127 @@ def cpu_baseline(g_io, cfg, prim, n_res, threads):
     return og.N / dt / 1e6, dt
 
 
+def time_calls(fn, n, stream):
+    """Average device time (us) of n calls of fn on stream (HIP events)."""
+    import torch
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(n):
+        fn()
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) * 1e3 / n
+
+
+def adjoint_leg(steps):
+    """Adjoint/primal time ratio on the 1024 x 512 viscous flat plate: the VJP
+    of the wall force (x) plus a dU/dt cotangent, limiter active and frozen,
+    against one primal residual."""
+    import torch
+    from ibis_amd import cases
+    from ibis_amd.finite_volume import FiniteVolume, GridBlock
+    g_io, cfg, fs = cases.plate_case(1024, 512)
+    gb = GridBlock(g_io, cfg, reorder=True)
+    fv = FiniteVolume(gb, cfg)
+    prim0 = torch.from_numpy(gb.to_grid_order(
+        cases.perturbed_prim(g_io, fs, gb.Nt, seed=2))).to("cuda")
+    prim = prim0.clone()
+    dudt = torch.zeros((gb.dim + 2, gb.Nt), dtype=torch.float64, device="cuda")
+    db = torch.ones_like(dudt)
+    db[:, gb.N:] = 0.0
+    pb = torch.zeros_like(prim)
+    tags = fv.tag_mask("wall")
+    stream = torch.cuda.current_stream()
+    for _ in range(3):
+        fv.compute_dudt(prim, gb, dudt)
+        fv.compute_dudt_vjp(prim, pb, dudt_bar=db, force_tags=tags)
+    t_p = time_calls(lambda: fv.compute_dudt(prim, gb, dudt), steps, stream)
+    out = {"workload": "flat plate 1024x512 viscous, AUSMDV, Barth-Jespersen, rho_u (4.2e5 wall "
+                       "force + dU/dt cotangent)",
+           "primal_us": round(t_p, 1), "primal_Mcells_per_s": round(gb.N / t_p, 1)}
+    for mode in ("active", "frozen"):
+        t_a = time_calls(lambda: fv.compute_dudt_vjp(prim, pb, dudt_bar=db, force_tags=tags,
+                                                     limiter=mode), steps, stream)
+        out[f"vjp_{mode}_us"] = round(t_a, 1)
+        out[f"ratio_{mode}"] = round(t_a / t_p, 3)
+    print(f"[bench] adjoint: {out}", file=sys.stderr, flush=True)
+    fv.close()
+    gb.close()
+    return out
+
+
+def batch_leg(nsteps, n_inst=64):
+    """64 seeded wedge/ramp instances (10k-400k cells, seed 7), ssp-rk3 at
+    CFL 0.5: one at a time on the plain path vs all together in one HIP
+    graph per step.  Both do one untimed step first."""
+    import torch
+    from ibis_amd import cases
+    from ibis_amd.batch import Batch
+    from ibis_amd.finite_volume import FiniteVolume, RungeKutta
+    specs = cases.batch_instances(n=n_inst, seed=7)
+    inst = [cases.batch_case(s) for s in specs]
+    t0 = time.perf_counter()
+    bt = Batch([(g, c) for g, c, _ in inst], device_blocks=True)
+    print(f"[bench] batch of {bt.n}: {bt.N} cells, grids built in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    prims = [cases.uniform_prim(f, b.N, b.Nt) for (g, c, f), b in zip(inst, bt.blocks)]
+    # one at a time
+    seq = []
+    for (g, c, f), b, p in zip(inst, bt.blocks, prims):
+        fv = FiniteVolume(b, c)
+        dp = torch.from_numpy(p).to("cuda")
+        dc = torch.zeros((b.dim + 2, b.Nt), dtype=torch.float64, device="cuda")
+        fv.prim_to_cons(dp, dc)
+        rk = RungeKutta(fv, "ssp-rk3", cfl=0.5, max_time=1e30)
+        rk.take_step(dc, dp)
+        seq.append((fv, rk, dc, dp))
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    for fv, rk, dc, dp in seq:
+        for _ in range(nsteps):
+            rk.take_step(dc, dp)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - w0
+    # batched
+    d_prim = torch.from_numpy(bt.gather(prims)).to("cuda")
+    d_cons = torch.zeros((bt.ncons, bt.Nt), dtype=torch.float64, device="cuda")
+    from ibis_amd.native import check, lib, ptr
+    check(lib().ibis_prim_to_cons(bt.residual_handle(), ptr(d_prim), ptr(d_cons), None),
+          "ibis_prim_to_cons")
+    torch.cuda.synchronize()
+    # autotune the face-state layout of the batch's residual on a throwaway copy
+    tune = {}
+    for layout in (0, 1, 2):
+        bt.set_option("face_state_layout", layout)
+        c2, p2 = d_cons.clone(), d_prim.clone()
+        bt.rk_init(c2, p2, cfl=0.5, max_time=1e30)
+        bt.rk_steps(1, use_graph=True)
+        w = time.perf_counter()
+        bt.rk_steps(2, use_graph=True)
+        tune[layout] = (time.perf_counter() - w) / 2
+    best = min(tune, key=tune.get)
+    bt.set_option("face_state_layout", best)
+    bt.rk_init(d_cons, d_prim, cfl=0.5, max_time=1e30)
+    bt.rk_steps(1, use_graph=True)
+    w0 = time.perf_counter()
+    bt.rk_steps(nsteps, use_graph=True)
+    t_bat = time.perf_counter() - w0
+    cons = d_cons.cpu().numpy()
+    identical = all(
+        np.array_equal(bt.scatter(cons, i)[:, :b.N], dc.cpu().numpy()[:, :b.N])
+        for i, (b, (fv, rk, dc, dp)) in enumerate(zip(bt.blocks, seq)))
+    out = {"instances": bt.n, "cells": int(bt.N), "steps": nsteps,
+           "sequential_s": round(t_seq, 4), "batched_s": round(t_bat, 4),
+           "speedup": round(t_seq / t_bat, 3), "identical_fp64": bool(identical),
+           "autotune_step_s": {f"layout{k}": round(v, 5) for k, v in tune.items()},
+           "layout": best}
+    print(f"[bench] batch: {out}", file=sys.stderr, flush=True)
+    for fv, rk, dc, dp in seq:
+        fv.close()
+    bt.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +218,9 @@ def main():
                     help="fix the face-state layout instead of autotuning")
     ap.add_argument("--offsets", type=int, default=None, choices=[0, 1],
                     help="fix the precomputed-offsets option instead of autotuning")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/primal leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-stepping leg")
+    ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--no-fp32", action="store_true",
                     help="skip the FP32-flux / FP64-accumulate comparison leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -217,6 +341,9 @@ def main():
         print(f"[bench] fp32_flux: {us32:.1f} us/residual, drift {drift}", file=sys.stderr,
               flush=True)
 
+    adj = None if args.no_adjoint else adjoint_leg(max(10, args.steps // 2))
+    bat = None if args.no_batch else batch_leg(args.batch_steps)
+
     N, F, Nt = gb.N, gb.F, gb.Nt
     ncons = gb.dim + 2
     B = algorithmic_bytes(N, F, Nt, gb.dim, ncons, fpc)
@@ -265,6 +392,8 @@ def main():
                      "sum_kernels_us": round(residual_kernel_us, 2)},
         "cpu_baseline": cpu,
         "fp32_flux": fp32,
+        "adjoint": adj,
+        "batch": bat,
     }
     print(json.dumps(out), flush=True)
 
