@@ -260,26 +260,31 @@ def main():
     # autotune the implementation choices (all give identical results)
     variants = {}
     best = None
+    choices = []
     for layout in ((0, 1, 2) if args.layout is None else (args.layout,)):
         for offs in ((0, 1) if args.offsets is None else (args.offsets,)):
-            fv.set_option("face_state_layout", layout)
-            fv.set_option("precomputed_offsets", offs)
-            for _ in range(2):
-                fv.compute_dudt(prim, gb, dudt)
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record(stream)
-            for _ in range(5):
-                fv.compute_dudt(prim, gb, dudt)
-            b.record(stream)
-            torch.cuda.synchronize()
-            us = a.elapsed_time(b) * 1e3 / 5
-            key = f"layout{layout}_offsets{offs}"
-            variants[key] = round(us, 1)
-            print(f"[bench] variant {key}: {us:.1f} us/residual", file=sys.stderr, flush=True)
-            if best is None or us < best[0]:
-                best = (us, layout, offs)
-    fv.set_option("face_state_layout", best[1])
-    fv.set_option("precomputed_offsets", best[2])
+            for pack in ((1, 0) if layout == 2 else (1,)):
+                choices.append({"face_state_layout": layout, "precomputed_offsets": offs,
+                                "pack_cells": pack})
+    for opt in choices:
+        for k, v in opt.items():
+            fv.set_option(k, v)
+        for _ in range(2):
+            fv.compute_dudt(prim, gb, dudt)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(5):
+            fv.compute_dudt(prim, gb, dudt)
+        b.record(stream)
+        torch.cuda.synchronize()
+        us = a.elapsed_time(b) * 1e3 / 5
+        key = "layout{face_state_layout}_offsets{precomputed_offsets}_pack{pack_cells}".format(**opt)
+        variants[key] = round(us, 1)
+        print(f"[bench] variant {key}: {us:.1f} us/residual", file=sys.stderr, flush=True)
+        if best is None or us < best[0]:
+            best = (us, opt)
+    for k, v in best[1].items():
+        fv.set_option(k, v)
 
     for _ in range(args.warmup):
         fv.compute_dudt(prim, gb, dudt)
@@ -381,7 +386,7 @@ def main():
                                 "wedge": "wedge 192x96 inviscid Mach 3, AUSMDV, Barth-Jespersen"
                                 }[args.workload],
                    "cells": N, "faces": F, "ghosts": Nt - N,
-                   "variant": {"face_state_layout": best[1], "precomputed_offsets": best[2]},
+                   "variant": best[1],
                    "autotune_us": variants},
         "roofline": {"bound": "hbm", "kernel": "residual (all kernels of one compute_dudt)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -617,33 +617,51 @@ __global__ void __launch_bounds__(BLOCK) k_offsets(DevMesh m, double* __restrict
 
 // W is the storage type of the face values (float in the reduced-precision
 // mode, layout 2 only); gradients and limiters are FP64 in both.
-template <int DIM, int LIM, int LAYOUT, bool OFFS, class W = double>
+// values (A, B, vx, vy, vz) and centroid of cell c: from the packed
+// 64-byte record, or (PACKED = false) straight from the prim rows and the
+// centroid rows
+template <bool PACKED>
+__device__ __forceinline__ void load_cell(const DevMesh& m, const VarSet& vs,
+                                          const double* __restrict__ pack,
+                                          const double* __restrict__ prim, int c, double* u,
+                                          double& x, double& y, double& z) {
+    if (PACKED) {
+        const double2* P = reinterpret_cast<const double2*>(pack);
+        const double2 a0 = P[(size_t)c * 4 + 0], a1 = P[(size_t)c * 4 + 1];
+        const double2 a2 = P[(size_t)c * 4 + 2], a3 = P[(size_t)c * 4 + 3];
+        u[0] = a0.x;
+        u[1] = a0.y;
+        u[2] = a1.x;
+        u[3] = a1.y;
+        u[4] = a2.x;
+        x = a2.y;
+        y = a3.x;
+        z = a3.y;
+    } else {
+        const int Nt = m.Nt;
+#pragma unroll
+        for (int v = 0; v < 5; v++) u[v] = prim[(size_t)vs.row[v] * Nt + c];
+        x = m.cen[c];
+        y = m.cen[Nt + c];
+        z = m.cen[2 * (size_t)Nt + c];
+    }
+}
+
+template <int DIM, int LIM, int LAYOUT, bool OFFS, class W = double, bool PACKED = true>
 __global__ void __launch_bounds__(BLOCK)
     k_cell_fs(DevMesh m, VarSet vs, const double* __restrict__ pack, double* __restrict__ grad,
               double* __restrict__ lim, double eps, double venkat_K, int nlogical,
               int write_slots, W* __restrict__ FL, W* __restrict__ FR,
-              const double* __restrict__ offs) {
+              const double* __restrict__ offs, const double* __restrict__ prim) {
     const int lb = xcd_block(nlogical);
     if (lb >= nlogical) return;
     const int i = lb * BLOCK + threadIdx.x;
     const int N = m.N, F = m.F;
     if (i >= N) return;
     constexpr int NG = DIM == 3 ? 5 : 4;  // variables whose gradient is computed here
-    const double2* P = reinterpret_cast<const double2*>(pack);
     double ui[5];
     double xi, yi, zi;
-    {
-        const double2 a0 = P[(size_t)i * 4 + 0], a1 = P[(size_t)i * 4 + 1];
-        const double2 a2 = P[(size_t)i * 4 + 2], a3 = P[(size_t)i * 4 + 3];
-        ui[0] = a0.x;
-        ui[1] = a0.y;
-        ui[2] = a1.x;
-        ui[3] = a1.y;
-        ui[4] = a2.x;
-        xi = a2.y;
-        yi = a3.x;
-        zi = a3.y;
-    }
+    load_cell<PACKED>(m, vs, pack, prim, i, ui, xi, yi, zi);
     const int nf = m.nfaces[i];
     const double r11 = m.lsq[i], r12 = m.lsq[N + i], r22 = m.lsq[2 * (size_t)N + i];
     double r13 = 0.0, r23 = 0.0, r33 = 0.0;
@@ -666,12 +684,11 @@ __global__ void __launch_bounds__(BLOCK)
     }
     for (int j = 0; j < nf; j++) {
         const int n = m.ell_nbr[(size_t)j * N + i];
-        const double2 b0 = P[(size_t)n * 4 + 0], b1 = P[(size_t)n * 4 + 1];
-        const double2 b2 = P[(size_t)n * 4 + 2], b3 = P[(size_t)n * 4 + 3];
-        const double un[5] = {b0.x, b0.y, b1.x, b1.y, b2.x};
-        const double dx = b2.y - xi;
-        const double dy = b3.x - yi;
-        const double dz = b3.y - zi;
+        double un[5], xn, yn, zn;
+        load_cell<PACKED>(m, vs, pack, prim, n, un, xn, yn, zn);
+        const double dx = xn - xi;
+        const double dy = yn - yi;
+        const double dz = zn - zi;
         const double alpha_1 = dx / r11s;
         const double alpha_2 = i22 * (dy - q12 * dx);
         double alpha_3 = 0.0;
@@ -1532,6 +1549,7 @@ struct ibis_residual {
     int ntab = 0;               // tag x action entries of each boundary table
     bool has_pre = false, has_visc = false, has_post = false;
     int opt_offsets = 0;     // option precomputed_offsets
+    int opt_pack = 1;        // option pack_cells
     double* face_rho = nullptr;
     double* face_T = nullptr;
     int* kinds = nullptr;    // pre, visc, post
@@ -1787,40 +1805,47 @@ void launch_face_flux(const DevMesh& D, int flux_kind, const FaceParams& fp, boo
 
 template <int DIM, int LIM, int LAYOUT, bool OFFS>
 void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d, ibis_residual* r,
-                      int nlog, hipStream_t st) {
+                      int nlog, const double* prim, hipStream_t st) {
     if (r->prec == IBIS_PREC_FP32_FLUX) {
         float* fl = reinterpret_cast<float*>(r->FL);
         k_cell_fs<DIM, LIM, 2, OFFS, float><<<xcd_grid(nlog), BLOCK, 0, st>>>(
-            D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, fl, fl, r->offs);
+            D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, fl, fl, r->offs, prim);
+        return;
+    }
+    if (!r->opt_pack && LAYOUT == 2) {
+        k_cell_fs<DIM, LIM, 2, OFFS, double, false><<<xcd_grid(nlog), BLOCK, 0, st>>>(
+            D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR, r->offs,
+            prim);
         return;
     }
     k_cell_fs<DIM, LIM, LAYOUT, OFFS><<<xcd_grid(nlog), BLOCK, 0, st>>>(
-        D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR, r->offs);
+        D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR, r->offs,
+        prim);
 }
 
 template <int DIM, int LIM>
 void launch_cell_fs_l(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d, ibis_residual* r,
-                      int nlog, hipStream_t st) {
+                      int nlog, const double* prim, hipStream_t st) {
     const int sel = r->opt_layout * 2 + (r->opt_offsets ? 1 : 0);
     switch (sel) {
-        case 0: launch_cell_fs_t<DIM, LIM, 0, false>(D, vs, d, r, nlog, st); break;
-        case 1: launch_cell_fs_t<DIM, LIM, 0, true>(D, vs, d, r, nlog, st); break;
-        case 2: launch_cell_fs_t<DIM, LIM, 1, false>(D, vs, d, r, nlog, st); break;
-        case 3: launch_cell_fs_t<DIM, LIM, 1, true>(D, vs, d, r, nlog, st); break;
-        case 4: launch_cell_fs_t<DIM, LIM, 2, false>(D, vs, d, r, nlog, st); break;
-        default: launch_cell_fs_t<DIM, LIM, 2, true>(D, vs, d, r, nlog, st); break;
+        case 0: launch_cell_fs_t<DIM, LIM, 0, false>(D, vs, d, r, nlog, prim, st); break;
+        case 1: launch_cell_fs_t<DIM, LIM, 0, true>(D, vs, d, r, nlog, prim, st); break;
+        case 2: launch_cell_fs_t<DIM, LIM, 1, false>(D, vs, d, r, nlog, prim, st); break;
+        case 3: launch_cell_fs_t<DIM, LIM, 1, true>(D, vs, d, r, nlog, prim, st); break;
+        case 4: launch_cell_fs_t<DIM, LIM, 2, false>(D, vs, d, r, nlog, prim, st); break;
+        default: launch_cell_fs_t<DIM, LIM, 2, true>(D, vs, d, r, nlog, prim, st); break;
     }
 }
 
 template <int DIM>
 void launch_cell_fs(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d, ibis_residual* r,
-                    int nlog, const double* /*prim*/, hipStream_t st) {
+                    int nlog, const double* prim, hipStream_t st) {
     if (d.limiter == 0)
-        launch_cell_fs_l<DIM, 0>(D, vs, d, r, nlog, st);
+        launch_cell_fs_l<DIM, 0>(D, vs, d, r, nlog, prim, st);
     else if (d.limiter == 1)
-        launch_cell_fs_l<DIM, 1>(D, vs, d, r, nlog, st);
+        launch_cell_fs_l<DIM, 1>(D, vs, d, r, nlog, prim, st);
     else
-        launch_cell_fs_l<DIM, 2>(D, vs, d, r, nlog, st);
+        launch_cell_fs_l<DIM, 2>(D, vs, d, r, nlog, prim, st);
 }
 
 template <int DIM, int FLUX>
@@ -2049,7 +2074,8 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
             // face-state path: pack, ghost face states, cell reconstruction,
             // fused face flux + integral
             pf.beg();
-            k_pack<<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, vs, prim, r->pack);
+            const bool packed = r->opt_pack || r->opt_layout != 2 || r->prec == IBIS_PREC_FP32_FLUX;
+            if (packed) k_pack<<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, vs, prim, r->pack);
             if (D.nb > 0) {
                 if (r->prec == IBIS_PREC_FP32_FLUX) {
                     float* fl = reinterpret_cast<float*>(r->FL);
@@ -2245,6 +2271,10 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
             HIP_TRY(hipDeviceSynchronize());
         }
         r->opt_offsets = value ? 1 : 0;
+        return IBIS_OK;
+    }
+    if (k == "pack_cells") {
+        r->opt_pack = value ? 1 : 0;
         return IBIS_OK;
     }
     if (k == "freeze_limiter") {
