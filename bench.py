@@ -175,16 +175,18 @@ def batch_leg(nsteps, n_inst=64):
     torch.cuda.synchronize()
     # autotune the face-state layout of the batch's residual on a throwaway copy
     tune = {}
-    for layout in (0, 1, 2):
+    for layout, fk in ((0, 0), (2, 0), (2, 64), (2, 128), (2, 256)):
         bt.set_option("face_state_layout", layout)
+        bt.set_option("face_kernel", fk)
         c2, p2 = d_cons.clone(), d_prim.clone()
         bt.rk_init(c2, p2, cfl=0.5, max_time=1e30)
         bt.rk_steps(1, use_graph=True)
         w = time.perf_counter()
         bt.rk_steps(2, use_graph=True)
-        tune[layout] = (time.perf_counter() - w) / 2
+        tune[(layout, fk)] = (time.perf_counter() - w) / 2
     best = min(tune, key=tune.get)
-    bt.set_option("face_state_layout", best)
+    bt.set_option("face_state_layout", best[0])
+    bt.set_option("face_kernel", best[1])
     bt.rk_init(d_cons, d_prim, cfl=0.5, max_time=1e30)
     bt.rk_steps(1, use_graph=True)
     w0 = time.perf_counter()
@@ -197,7 +199,7 @@ def batch_leg(nsteps, n_inst=64):
     out = {"instances": bt.n, "cells": int(bt.N), "steps": nsteps,
            "sequential_s": round(t_seq, 4), "batched_s": round(t_bat, 4),
            "speedup": round(t_seq / t_bat, 3), "identical_fp64": bool(identical),
-           "autotune_step_s": {f"layout{k}": round(v, 5) for k, v in tune.items()},
+           "autotune_step_s": {f"layout{k[0]}_fk{k[1]}": round(v, 5) for k, v in tune.items()},
            "layout": best}
     print(f"[bench] batch: {out}", file=sys.stderr, flush=True)
     for fv, rk, dc, dp in seq:
@@ -264,8 +266,9 @@ def main():
     for layout in ((0, 1, 2) if args.layout is None else (args.layout,)):
         for offs in ((0, 1) if args.offsets is None else (args.offsets,)):
             for pack in ((1, 0) if layout == 2 else (1,)):
-                choices.append({"face_state_layout": layout, "precomputed_offsets": offs,
-                                "pack_cells": pack})
+                for fk in ((0, 64, 128, 256) if layout == 2 else (0,)):
+                    choices.append({"face_state_layout": layout, "precomputed_offsets": offs,
+                                    "pack_cells": pack, "face_kernel": fk})
     for opt in choices:
         for k, v in opt.items():
             fv.set_option(k, v)
@@ -278,7 +281,8 @@ def main():
         b.record(stream)
         torch.cuda.synchronize()
         us = a.elapsed_time(b) * 1e3 / 5
-        key = "layout{face_state_layout}_offsets{precomputed_offsets}_pack{pack_cells}".format(**opt)
+        key = ("layout{face_state_layout}_offsets{precomputed_offsets}_pack{pack_cells}"
+               "_fk{face_kernel}").format(**opt)
         variants[key] = round(us, 1)
         print(f"[bench] variant {key}: {us:.1f} us/residual", file=sys.stderr, flush=True)
         if best is None or us < best[0]:

@@ -799,10 +799,90 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
-// Fused face flux + surface integral from the face states.
-// R is the arithmetic of the thermo closure, frame rotation and flux:
-// double, or float for the reduced-precision mode (face states stored as
-// float); the face-to-cell sum and the division by the volume stay FP64.
+// Flux * area of face f from the two sides' reconstructed values (A, B,
+// vx, vy, vz): thermo closure, rotation to the face frame, flux calculator,
+// rotation back (FV/convective_flux.cpp:85-178).  R is the arithmetic of
+// these steps: double, or float in the reduced-precision mode; the product
+// with the area is FP64.
+template <int DIM, int FLUX, class R>
+__device__ __forceinline__ void face_flux_values(const DevMesh& m, const FaceParams& fp, int f,
+                                                 const R* lv, const R* rv, double* res_a) {
+    constexpr int NC = DIM + 2;
+    constexpr bool T3 = DIM == 3;
+    const int F = m.F;
+    R Ls[NPRIM], Rs[NPRIM];
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+        const R* rec = side ? rv : lv;
+        R* st = side ? Rs : Ls;
+        const R a = rec[0], b = rec[1];
+        R rho = R(0.0), p = R(0.0), T = R(0.0), e = R(0.0);
+        switch (fp.thermo) {
+            case IBIS_THERMO_RHO_P: p = a; rho = b; break;
+            case IBIS_THERMO_RHO_T: rho = a; T = b; break;
+            case IBIS_THERMO_RHO_U: rho = a; e = b; break;
+            default: p = a; T = b; break;
+        }
+        ibis::thermo_update<R>(fp.thermo, R(fp.R), R(fp.Cv), rho, p, T, e);
+        st[ibis::S_RHO] = rho;
+        st[ibis::S_P] = p;
+        st[ibis::S_T] = T;
+        st[ibis::S_E] = e;
+        st[4] = rec[2];
+        st[5] = rec[3];
+        st[6] = rec[4];
+    }
+    const R t1x = R(m.geom[3 * (size_t)F + f]), t1y = R(m.geom[4 * (size_t)F + f]),
+            t1z = R(m.geom[5 * (size_t)F + f]);
+    const R t2x = R(m.geom[6 * (size_t)F + f]), t2y = R(m.geom[7 * (size_t)F + f]),
+            t2z = R(m.geom[8 * (size_t)F + f]);
+    R nx, ny, nz;
+    if (m.n_from_t) {  // cross(tan1, tan2), GR/interface.h:249
+        nx = t1y * t2z - t1z * t2y;
+        ny = t1z * t2x - t1x * t2z;
+        nz = t1x * t2y - t1y * t2x;
+    } else {
+        nx = R(m.geom[f]);
+        ny = R(m.geom[(size_t)F + f]);
+        nz = R(m.geom[2 * (size_t)F + f]);
+    }
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+        R* s = side ? Rs : Ls;
+        const R a = s[4], b = s[5], c = s[6];
+        s[4] = a * nx + b * ny + c * nz;
+        s[5] = a * t1x + b * t1y + c * t1z;
+        s[6] = a * t2x + b * t2y + c * t2z;
+    }
+    R out[5];
+    if (FLUX == IBIS_FLUX_AUSMDV)
+        ibis::flux_ausmdv<R, T3>(Ls, Rs, R(fp.R), R(fp.Cv), R(fp.gamma), out);
+    else if (FLUX == IBIS_FLUX_HANEL)
+        ibis::flux_hanel<R, T3>(Ls, Rs, R(fp.R), R(fp.Cv), R(fp.gamma), out);
+    else if (FLUX == IBIS_FLUX_LDFSS2)
+        ibis::flux_ldfss2<R, T3>(Ls, Rs, R(fp.R), R(fp.Cv), R(fp.gamma), R(fp.delta), out);
+    else
+        ibis::flux_rusanov<R, T3>(Ls, Rs, R(fp.R), R(fp.Cv), R(fp.gamma), out);
+    const R px = out[1], py = out[2];
+    R pz = R(0.0);
+    if (T3) pz = out[3];
+    R res[NC];
+    res[0] = out[0];
+    res[1] = px * nx + py * t1x + pz * t2x;
+    res[2] = px * ny + py * t1y + pz * t2y;
+    if (T3) {
+        res[3] = px * nz + py * t1z + pz * t2z;
+        res[4] = out[4];
+    } else {
+        res[3] = out[4];
+    }
+    const double A = m.geom[12 * (size_t)F + f];
+#pragma unroll
+    for (int q = 0; q < NC; q++) res_a[q] = (double)res[q] * A;
+}
+
+// Fused face flux + surface integral from the face states, one tile of
+// TILE cells per block, driven by the tile's face list.
 template <int DIM, int FLUX, int LAYOUT, class R = double>
 __global__ void __launch_bounds__(TILE)
     k_face_tile_fs(DevMesh m, FaceParams fp, const R* __restrict__ FL,
@@ -811,97 +891,34 @@ __global__ void __launch_bounds__(TILE)
     const int t = xcd_block(m.ntiles);
     if (t >= m.ntiles) return;
     constexpr int NC = DIM + 2;
-    constexpr bool T3 = DIM == 3;
-    const int F = m.F, N = m.N, Nt = m.Nt, S = m.maxtf;
+    const int N = m.N, Nt = m.Nt, S = m.maxtf;
     const int f0 = m.tile_face_off[t];
     const int nfl = m.tile_face_off[t + 1] - f0;
     for (int k = threadIdx.x; k < nfl; k += TILE) {
         const int f = m.tile_faces[f0 + k];
-        R Ls[NPRIM], Rs[NPRIM];
+        R v[2][5];
 #pragma unroll
         for (int side = 0; side < 2; side++) {
-            R rv[5];
             if (LAYOUT == 0) {
                 const R* rec = (side ? FR : FL) + (size_t)f * 5;
 #pragma unroll
-                for (int v = 0; v < 5; v++) rv[v] = rec[v];
+                for (int q = 0; q < 5; q++) v[side][q] = rec[q];
             } else if (LAYOUT == 1) {
                 const R* rec =
                     FL + (side ? rec_index(m, m.R[f], m.slotR[f]) : rec_index(m, m.L[f], m.slotL[f])) * 5;
 #pragma unroll
-                for (int v = 0; v < 5; v++) rv[v] = rec[v];
+                for (int q = 0; q < 5; q++) v[side][q] = rec[q];
             } else {
                 const int c = side ? m.R[f] : m.L[f];
                 const int sl = side ? m.slotR[f] : m.slotL[f];
 #pragma unroll
-                for (int v = 0; v < 5; v++) rv[v] = FL[soa_index(m, c, sl, v)];
+                for (int q = 0; q < 5; q++) v[side][q] = FL[soa_index(m, c, sl, q)];
             }
-            const R* rec = rv;
-            R* st = side ? Rs : Ls;
-            const R a = rec[0], b = rec[1];
-            R rho = R(0.0), p = R(0.0), T = R(0.0), e = R(0.0);
-            switch (fp.thermo) {
-                case IBIS_THERMO_RHO_P: p = a; rho = b; break;
-                case IBIS_THERMO_RHO_T: rho = a; T = b; break;
-                case IBIS_THERMO_RHO_U: rho = a; e = b; break;
-                default: p = a; T = b; break;
-            }
-            ibis::thermo_update<R>(fp.thermo, R(fp.R), R(fp.Cv), rho, p, T, e);
-            st[ibis::S_RHO] = rho;
-            st[ibis::S_P] = p;
-            st[ibis::S_T] = T;
-            st[ibis::S_E] = e;
-            st[4] = rec[2];
-            st[5] = rec[3];
-            st[6] = rec[4];
         }
-        const R t1x = R(m.geom[3 * (size_t)F + f]), t1y = R(m.geom[4 * (size_t)F + f]),
-                t1z = R(m.geom[5 * (size_t)F + f]);
-        const R t2x = R(m.geom[6 * (size_t)F + f]), t2y = R(m.geom[7 * (size_t)F + f]),
-                t2z = R(m.geom[8 * (size_t)F + f]);
-        R nx, ny, nz;
-        if (m.n_from_t) {  // cross(tan1, tan2), GR/interface.h:249
-            nx = t1y * t2z - t1z * t2y;
-            ny = t1z * t2x - t1x * t2z;
-            nz = t1x * t2y - t1y * t2x;
-        } else {
-            nx = R(m.geom[f]);
-            ny = R(m.geom[(size_t)F + f]);
-            nz = R(m.geom[2 * (size_t)F + f]);
-        }
+        double res[NC];
+        face_flux_values<DIM, FLUX, R>(m, fp, f, v[0], v[1], res);
 #pragma unroll
-        for (int side = 0; side < 2; side++) {
-            R* s = side ? Rs : Ls;
-            const R a = s[4], b = s[5], c = s[6];
-            s[4] = a * nx + b * ny + c * nz;
-            s[5] = a * t1x + b * t1y + c * t1z;
-            s[6] = a * t2x + b * t2y + c * t2z;
-        }
-        R out[5];
-        if (FLUX == IBIS_FLUX_AUSMDV)
-            ibis::flux_ausmdv<R, T3>(Ls, Rs, R(fp.R), R(fp.Cv), R(fp.gamma), out);
-        else if (FLUX == IBIS_FLUX_HANEL)
-            ibis::flux_hanel<R, T3>(Ls, Rs, R(fp.R), R(fp.Cv), R(fp.gamma), out);
-        else if (FLUX == IBIS_FLUX_LDFSS2)
-            ibis::flux_ldfss2<R, T3>(Ls, Rs, R(fp.R), R(fp.Cv), R(fp.gamma), R(fp.delta), out);
-        else
-            ibis::flux_rusanov<R, T3>(Ls, Rs, R(fp.R), R(fp.Cv), R(fp.gamma), out);
-        const R px = out[1], py = out[2];
-        R pz = R(0.0);
-        if (T3) pz = out[3];
-        R res[NC];
-        res[0] = out[0];
-        res[1] = px * nx + py * t1x + pz * t2x;
-        res[2] = px * ny + py * t1y + pz * t2y;
-        if (T3) {
-            res[3] = px * nz + py * t1z + pz * t2z;
-            res[4] = out[4];
-        } else {
-            res[3] = out[4];
-        }
-        const double A = m.geom[12 * (size_t)F + f];
-#pragma unroll
-        for (int q = 0; q < NC; q++) sflux[q * S + k] = (double)res[q] * A;
+        for (int q = 0; q < NC; q++) sflux[q * S + k] = res[q];
     }
     __syncthreads();
     const int c = t * TILE + threadIdx.x;
@@ -916,6 +933,73 @@ __global__ void __launch_bounds__(TILE)
 #pragma unroll
         for (int q = 0; q < NC; q++) {
             const double fa = sflux[q * S + k];
+            d[q] += (s > 0) ? -fa : fa;  // == flux * (-area * outsign), exactly
+        }
+    }
+    const double V = m.vol[c];
+#pragma unroll
+    for (int q = 0; q < NC; q++) dudt[(size_t)q * Nt + c] = d[q] / V;
+}
+
+// Fused face flux + surface integral, slot-major variant for face values in
+// (slot, variable) planes (layout 2).  Block t owns cells [t*TS, t*TS + TS).
+// The thread of cell c evaluates, for each slot j, the face f = ell_face[j][c]
+// when c is its left cell, or when c is its right cell and the left cell
+// lies outside the block: its own side's values are one coalesced load per
+// plane across the wavefront, only the other side's are gathered.  The
+// flux * area goes to LDS at (j, c); each cell then sums its faces from LDS
+// in its own face order (a face both of whose cells are in the block is
+// read from its left cell's slot).  No face lists are needed, so TS is
+// free: 64, 128 or 256.
+template <int DIM, int FLUX, int TS, class R = double>
+__global__ void __launch_bounds__(TS)
+    k_face_slot_fs(DevMesh m, FaceParams fp, int nlogical, const R* __restrict__ FL,
+                   double* __restrict__ dudt) {
+    extern __shared__ double sflux[];  // NC x maxf x TS
+    const int t = xcd_block(nlogical);
+    if (t >= nlogical) return;
+    constexpr int NC = DIM + 2;
+    const int N = m.N, Nt = m.Nt, MF = m.maxf;
+    const int c0 = t * TS;
+    const int cend = min(c0 + TS, N);  // valid cells of the block (ghost ids are >= N)
+    const int c = c0 + threadIdx.x;
+    const int nf = c < N ? m.nfaces[c] : 0;
+    for (int j = 0; j < nf; j++) {
+        const int f = m.ell_face[(size_t)j * N + c];
+        const int l = m.L[f];
+        const bool left = l == c;
+        if (!left && l >= c0 && l < cend) continue;  // evaluated by its left cell
+        const int o = left ? m.R[f] : l;
+        const int so = left ? m.slotR[f] : m.slotL[f];
+        R own[5], oth[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            own[q] = FL[((size_t)j * 5 + q) * N + c];
+            oth[q] = FL[soa_index(m, o, so, q)];
+        }
+        double res[NC];
+        if (left)
+            face_flux_values<DIM, FLUX, R>(m, fp, f, own, oth, res);
+        else
+            face_flux_values<DIM, FLUX, R>(m, fp, f, oth, own, res);
+#pragma unroll
+        for (int q = 0; q < NC; q++) sflux[((size_t)q * MF + j) * TS + threadIdx.x] = res[q];
+    }
+    __syncthreads();
+    if (c >= N) return;
+    double d[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) d[q] = 0.0;
+    for (int j = 0; j < nf; j++) {
+        const int f = m.ell_face[(size_t)j * N + c];
+        const int l = m.L[f];
+        const int s = m.ell_sign[(size_t)j * N + c];
+        const bool mine = l == c || l < c0 || l >= cend;
+        const int jj = mine ? j : m.slotL[f];
+        const int tt = mine ? threadIdx.x : l - c0;
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+            const double fa = sflux[((size_t)q * MF + jj) * TS + tt];
             d[q] += (s > 0) ? -fa : fa;  // == flux * (-area * outsign), exactly
         }
     }
@@ -1550,6 +1634,7 @@ struct ibis_residual {
     bool has_pre = false, has_visc = false, has_post = false;
     int opt_offsets = 0;     // option precomputed_offsets
     int opt_pack = 1;        // option pack_cells
+    int opt_face_kernel = 0; // option face_kernel: 0 tile face lists, 64/128/256 slot-major
     double* face_rho = nullptr;
     double* face_T = nullptr;
     int* kinds = nullptr;    // pre, visc, post
@@ -1848,14 +1933,40 @@ void launch_cell_fs(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d, i
         launch_cell_fs_l<DIM, 2>(D, vs, d, r, nlog, prim, st);
 }
 
+template <int DIM, int FLUX, int TS, class R>
+void launch_face_slot(const DevMesh& D, const FaceParams& fp, const R* FL, double* dudt,
+                      hipStream_t st) {
+    const int nlog = (D.N + TS - 1) / TS;
+    const size_t shm = (size_t)(DIM + 2) * D.maxf * TS * sizeof(double);
+    k_face_slot_fs<DIM, FLUX, TS, R><<<xcd_grid(nlog), TS, shm, st>>>(D, fp, nlog, FL, dudt);
+}
+
+template <int DIM, int FLUX, class R>
+void launch_face_slot_ts(const DevMesh& D, const FaceParams& fp, int ts, const R* FL, double* dudt,
+                         hipStream_t st) {
+    if (ts == 64)
+        launch_face_slot<DIM, FLUX, 64, R>(D, fp, FL, dudt, st);
+    else if (ts == 128)
+        launch_face_slot<DIM, FLUX, 128, R>(D, fp, FL, dudt, st);
+    else
+        launch_face_slot<DIM, FLUX, 256, R>(D, fp, FL, dudt, st);
+}
+
+// layout 3 = layout 2 holding float values; face_kernel 0 = tile face
+// lists, otherwise the slot-major kernel with that many cells per block
 template <int DIM, int FLUX>
-void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, const double* FL,
-                      const double* FR, double* dudt, hipStream_t st) {
+void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, int face_kernel,
+                      const double* FL, const double* FR, double* dudt, hipStream_t st) {
     const unsigned g = xcd_grid(D.ntiles);
     const size_t shm = (size_t)(DIM + 2) * D.maxtf * sizeof(double);
-    if (layout == 3) {  // layout 2 holding float values
+    if (layout == 3) {
         const float* fl = reinterpret_cast<const float*>(FL);
-        k_face_tile_fs<DIM, FLUX, 2, float><<<g, TILE, shm, st>>>(D, fp, fl, fl, dudt);
+        if (face_kernel)
+            launch_face_slot_ts<DIM, FLUX, float>(D, fp, face_kernel, fl, dudt, st);
+        else
+            k_face_tile_fs<DIM, FLUX, 2, float><<<g, TILE, shm, st>>>(D, fp, fl, fl, dudt);
+    } else if (layout == 2 && face_kernel) {
+        launch_face_slot_ts<DIM, FLUX, double>(D, fp, face_kernel, FL, dudt, st);
     } else if (layout == 0)
         k_face_tile_fs<DIM, FLUX, 0><<<g, TILE, shm, st>>>(D, fp, FL, FR, dudt);
     else if (layout == 1)
@@ -1865,13 +1976,13 @@ void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, const 
 }
 
 template <int DIM>
-void launch_face_fs(const DevMesh& D, const FaceParams& fp, int flux, int layout,
+void launch_face_fs(const DevMesh& D, const FaceParams& fp, int flux, int layout, int face_kernel,
                     const double* FL, const double* FR, double* dudt, hipStream_t st) {
     switch (flux) {
-        case IBIS_FLUX_HANEL: launch_face_fs_f<DIM, IBIS_FLUX_HANEL>(D, fp, layout, FL, FR, dudt, st); break;
-        case IBIS_FLUX_AUSMDV: launch_face_fs_f<DIM, IBIS_FLUX_AUSMDV>(D, fp, layout, FL, FR, dudt, st); break;
-        case IBIS_FLUX_LDFSS2: launch_face_fs_f<DIM, IBIS_FLUX_LDFSS2>(D, fp, layout, FL, FR, dudt, st); break;
-        default: launch_face_fs_f<DIM, IBIS_FLUX_RUSANOV>(D, fp, layout, FL, FR, dudt, st); break;
+        case IBIS_FLUX_HANEL: launch_face_fs_f<DIM, IBIS_FLUX_HANEL>(D, fp, layout, face_kernel, FL, FR, dudt, st); break;
+        case IBIS_FLUX_AUSMDV: launch_face_fs_f<DIM, IBIS_FLUX_AUSMDV>(D, fp, layout, face_kernel, FL, FR, dudt, st); break;
+        case IBIS_FLUX_LDFSS2: launch_face_fs_f<DIM, IBIS_FLUX_LDFSS2>(D, fp, layout, face_kernel, FL, FR, dudt, st); break;
+        default: launch_face_fs_f<DIM, IBIS_FLUX_RUSANOV>(D, fp, layout, face_kernel, FL, FR, dudt, st); break;
     }
 }
 
@@ -2097,10 +2208,11 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
             pf.end(1);
             pf.beg();
             const int lay = r->prec == IBIS_PREC_FP32_FLUX ? 3 : r->opt_layout;
+            const int fk = (lay >= 2) ? r->opt_face_kernel : 0;
             if (D.dim == 3)
-                launch_face_fs<3>(D, fp, d.flux, lay, r->FL, r->FR, dudt, st);
+                launch_face_fs<3>(D, fp, d.flux, lay, fk, r->FL, r->FR, dudt, st);
             else
-                launch_face_fs<2>(D, fp, d.flux, lay, r->FL, r->FR, dudt, st);
+                launch_face_fs<2>(D, fp, d.flux, lay, fk, r->FL, r->FR, dudt, st);
             pf.end(2);
         } else {
             // first order: face flux and surface integral as one tiled kernel
@@ -2271,6 +2383,15 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
             HIP_TRY(hipDeviceSynchronize());
         }
         r->opt_offsets = value ? 1 : 0;
+        return IBIS_OK;
+    }
+    if (k == "face_kernel") {
+        if (value != 0 && value != 64 && value != 128 && value != 256)
+            return ibis::fail(IBIS_ERR_INVALID, "face_kernel must be 0, 64, 128 or 256");
+        const DevMesh& D = r->m->d;
+        if (value && (size_t)(D.dim + 2) * D.maxf * value * sizeof(double) > 65536)
+            return ibis::fail(IBIS_ERR_INVALID, "face_kernel: block does not fit in LDS");
+        r->opt_face_kernel = (int)value;
         return IBIS_OK;
     }
     if (k == "pack_cells") {

@@ -38,11 +38,17 @@ def _pair(grid_io, cfg, prim_ref):
     fv.compute_dudt(prim, gb, d64)
     fv.set_precision("fp32_flux")
     fv.compute_dudt(prim, gb, d32)
+    # the slot-major face kernel gives the same float results
+    fv.set_option("face_kernel", 128)
+    d32b = torch.zeros_like(d64)
+    fv.compute_dudt(prim, gb, d32b)
+    fv.set_option("face_kernel", 0)
     fv.set_precision("fp64")
     d64b = torch.zeros_like(d64)
     fv.compute_dudt(prim, gb, d64b)
     torch.cuda.synchronize()
     assert torch.equal(d64, d64b)  # switching back restores the FP64 bits
+    assert torch.equal(d32, d32b)
     return gb, d32.cpu().numpy(), d64.cpu().numpy()
 
 
