@@ -804,9 +804,31 @@ __global__ void __launch_bounds__(BLOCK)
 // rotation back (FV/convective_flux.cpp:85-178).  R is the arithmetic of
 // these steps: double, or float in the reduced-precision mode; the product
 // with the area is FP64.
+// t1, t2 and area of face f (geom rows 3-8 and 12)
+__device__ __forceinline__ void load_face_geom(const DevMesh& m, int f, double* g) {
+    const int F = m.F;
+#pragma unroll
+    for (int k = 0; k < 6; k++) g[k] = m.geom[(3 + k) * (size_t)F + f];
+    g[6] = m.geom[12 * (size_t)F + f];
+}
+
+template <int DIM, int FLUX, class R>
+__device__ __forceinline__ void face_flux_values_g(const DevMesh& m, const FaceParams& fp, int f,
+                                                   const double* g, const R* lv, const R* rv,
+                                                   double* res_a);
+
 template <int DIM, int FLUX, class R>
 __device__ __forceinline__ void face_flux_values(const DevMesh& m, const FaceParams& fp, int f,
                                                  const R* lv, const R* rv, double* res_a) {
+    double g[7];
+    load_face_geom(m, f, g);
+    face_flux_values_g<DIM, FLUX, R>(m, fp, f, g, lv, rv, res_a);
+}
+
+template <int DIM, int FLUX, class R>
+__device__ __forceinline__ void face_flux_values_g(const DevMesh& m, const FaceParams& fp, int f,
+                                                   const double* g, const R* lv, const R* rv,
+                                                   double* res_a) {
     constexpr int NC = DIM + 2;
     constexpr bool T3 = DIM == 3;
     const int F = m.F;
@@ -832,10 +854,8 @@ __device__ __forceinline__ void face_flux_values(const DevMesh& m, const FacePar
         st[5] = rec[3];
         st[6] = rec[4];
     }
-    const R t1x = R(m.geom[3 * (size_t)F + f]), t1y = R(m.geom[4 * (size_t)F + f]),
-            t1z = R(m.geom[5 * (size_t)F + f]);
-    const R t2x = R(m.geom[6 * (size_t)F + f]), t2y = R(m.geom[7 * (size_t)F + f]),
-            t2z = R(m.geom[8 * (size_t)F + f]);
+    const R t1x = R(g[0]), t1y = R(g[1]), t1z = R(g[2]);
+    const R t2x = R(g[3]), t2y = R(g[4]), t2z = R(g[5]);
     R nx, ny, nz;
     if (m.n_from_t) {  // cross(tan1, tan2), GR/interface.h:249
         nx = t1y * t2z - t1z * t2y;
@@ -876,7 +896,7 @@ __device__ __forceinline__ void face_flux_values(const DevMesh& m, const FacePar
     } else {
         res[3] = out[4];
     }
-    const double A = m.geom[12 * (size_t)F + f];
+    const double A = g[6];
 #pragma unroll
     for (int q = 0; q < NC; q++) res_a[q] = (double)res[q] * A;
 }
@@ -894,9 +914,10 @@ __global__ void __launch_bounds__(TILE)
     const int N = m.N, Nt = m.Nt, S = m.maxtf;
     const int f0 = m.tile_face_off[t];
     const int nfl = m.tile_face_off[t + 1] - f0;
-    for (int k = threadIdx.x; k < nfl; k += TILE) {
-        const int f = m.tile_faces[f0 + k];
-        R v[2][5];
+    // the inputs of this thread's next face are loaded before the current
+    // face is evaluated, so their latency overlaps the flux arithmetic
+    auto load = [&](int k, int& f, R (&v)[2][5], double (&g)[7]) {
+        f = m.tile_faces[f0 + k];
 #pragma unroll
         for (int side = 0; side < 2; side++) {
             if (LAYOUT == 0) {
@@ -915,10 +936,30 @@ __global__ void __launch_bounds__(TILE)
                 for (int q = 0; q < 5; q++) v[side][q] = FL[soa_index(m, c, sl, q)];
             }
         }
+        load_face_geom(m, f, g);
+    };
+    int k = threadIdx.x;
+    int fn = 0;
+    R vn[2][5];
+    double gn[7];
+    if (k < nfl) load(k, fn, vn, gn);
+    while (k < nfl) {
+        const int f = fn;
+        R v[2][5];
+        double g[7];
+#pragma unroll
+        for (int side = 0; side < 2; side++)
+#pragma unroll
+            for (int q = 0; q < 5; q++) v[side][q] = vn[side][q];
+#pragma unroll
+        for (int q = 0; q < 7; q++) g[q] = gn[q];
+        const int kn = k + TILE;
+        if (kn < nfl) load(kn, fn, vn, gn);
         double res[NC];
-        face_flux_values<DIM, FLUX, R>(m, fp, f, v[0], v[1], res);
+        face_flux_values_g<DIM, FLUX, R>(m, fp, f, g, v[0], v[1], res);
 #pragma unroll
         for (int q = 0; q < NC; q++) sflux[q * S + k] = res[q];
+        k = kn;
     }
     __syncthreads();
     const int c = t * TILE + threadIdx.x;
