@@ -220,6 +220,10 @@ def main():
                     help="fix the face-state layout instead of autotuning")
     ap.add_argument("--offsets", type=int, default=None, choices=[0, 1],
                     help="fix the precomputed-offsets option instead of autotuning")
+    ap.add_argument("--pack", type=int, default=None, choices=[0, 1],
+                    help="fix the pack_cells option instead of autotuning")
+    ap.add_argument("--face-kernel", type=int, default=None, choices=[0, 64, 128, 256],
+                    help="fix the face_kernel option instead of autotuning")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/primal leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-stepping leg")
     ap.add_argument("--batch-steps", type=int, default=10)
@@ -265,8 +269,10 @@ def main():
     choices = []
     for layout in ((0, 1, 2) if args.layout is None else (args.layout,)):
         for offs in ((0, 1) if args.offsets is None else (args.offsets,)):
-            for pack in ((1, 0) if layout == 2 else (1,)):
-                for fk in ((0, 64, 128, 256) if layout == 2 else (0,)):
+            packs = (1, 0) if layout == 2 else (1,)
+            fks = (0, 64, 128, 256) if layout == 2 else (0,)
+            for pack in (packs if args.pack is None else (args.pack,)):
+                for fk in (fks if args.face_kernel is None else (args.face_kernel,)):
                     choices.append({"face_state_layout": layout, "precomputed_offsets": offs,
                                     "pack_cells": pack, "face_kernel": fk})
     for opt in choices:
