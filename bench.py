@@ -224,6 +224,8 @@ def main():
                     help="fix the pack_cells option instead of autotuning")
     ap.add_argument("--face-kernel", type=int, default=None, choices=[0, 64, 128, 256],
                     help="fix the face_kernel option instead of autotuning")
+    ap.add_argument("--stage", type=int, default=None, choices=[0, 1],
+                    help="fix the stage_offsets option instead of autotuning")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/primal leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-stepping leg")
     ap.add_argument("--batch-steps", type=int, default=10)
@@ -271,10 +273,13 @@ def main():
         for offs in ((0, 1) if args.offsets is None else (args.offsets,)):
             packs = (1, 0) if layout == 2 else (1,)
             fks = (0, 64, 128, 256) if layout == 2 else (0,)
+            stages = (1, 0) if layout == 2 else (1,)
             for pack in (packs if args.pack is None else (args.pack,)):
                 for fk in (fks if args.face_kernel is None else (args.face_kernel,)):
-                    choices.append({"face_state_layout": layout, "precomputed_offsets": offs,
-                                    "pack_cells": pack, "face_kernel": fk})
+                    for stage in (stages if args.stage is None else (args.stage,)):
+                        choices.append({"face_state_layout": layout, "precomputed_offsets": offs,
+                                        "pack_cells": pack, "face_kernel": fk,
+                                        "stage_offsets": stage})
     for opt in choices:
         for k, v in opt.items():
             fv.set_option(k, v)
@@ -288,7 +293,7 @@ def main():
         torch.cuda.synchronize()
         us = a.elapsed_time(b) * 1e3 / 5
         key = ("layout{face_state_layout}_offsets{precomputed_offsets}_pack{pack_cells}"
-               "_fk{face_kernel}").format(**opt)
+               "_fk{face_kernel}_stage{stage_offsets}").format(**opt)
         variants[key] = round(us, 1)
         print(f"[bench] variant {key}: {us:.1f} us/residual", file=sys.stderr, flush=True)
         if best is None or us < best[0]:

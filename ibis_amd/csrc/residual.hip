@@ -652,7 +652,11 @@ __global__ void __launch_bounds__(BLOCK)
     k_cell_fs(DevMesh m, VarSet vs, const double* __restrict__ pack, double* __restrict__ grad,
               double* __restrict__ lim, double eps, double venkat_K, int nlogical,
               int write_slots, W* __restrict__ FL, W* __restrict__ FR,
-              const double* __restrict__ offs, const double* __restrict__ prim) {
+              const double* __restrict__ offs, const double* __restrict__ prim, int stage) {
+    // stage != 0: the limiter pass keeps each face offset in LDS
+    // (maxf x 3 doubles per thread) for the reconstruction pass, so the
+    // offsets are read from memory once
+    extern __shared__ double soff[];
     const int lb = xcd_block(nlogical);
     if (lb >= nlogical) return;
     const int i = lb * BLOCK + threadIdx.x;
@@ -743,6 +747,11 @@ __global__ void __launch_bounds__(BLOCK)
                 dy = m.geom[10 * (size_t)F + f] - yi;
                 dz = m.geom[11 * (size_t)F + f] - zi;
             }
+            if (stage) {
+                soff[(j * 3 + 0) * BLOCK + threadIdx.x] = dx;
+                soff[(j * 3 + 1) * BLOCK + threadIdx.x] = dy;
+                soff[(j * 3 + 2) * BLOCK + threadIdx.x] = dz;
+            }
 #pragma unroll
             for (int v = 0; v < 5; v++) {
                 double d2 = gx[v] * dx + gy[v] * dy + gz[v] * dz;
@@ -777,7 +786,11 @@ __global__ void __launch_bounds__(BLOCK)
     for (int j = 0; j < nf; j++) {
         const int f = m.ell_face[(size_t)j * N + i];
         double dx, dy, dz;
-        if (OFFS) {
+        if (LIM != 0 && stage) {
+            dx = soff[(j * 3 + 0) * BLOCK + threadIdx.x];
+            dy = soff[(j * 3 + 1) * BLOCK + threadIdx.x];
+            dz = soff[(j * 3 + 2) * BLOCK + threadIdx.x];
+        } else if (OFFS) {
             dx = offs[(size_t)(j * 3 + 0) * N + i];
             dy = offs[(size_t)(j * 3 + 1) * N + i];
             dz = offs[(size_t)(j * 3 + 2) * N + i];
@@ -1675,6 +1688,7 @@ struct ibis_residual {
     bool has_pre = false, has_visc = false, has_post = false;
     int opt_offsets = 0;     // option precomputed_offsets
     int opt_pack = 1;        // option pack_cells
+    int opt_stage = 1;       // option stage_offsets
     int opt_face_kernel = 0; // option face_kernel: 0 tile face lists, 64/128/256 slot-major
     double* face_rho = nullptr;
     double* face_T = nullptr;
@@ -1929,24 +1943,35 @@ void launch_face_flux(const DevMesh& D, int flux_kind, const FaceParams& fp, boo
     }
 }
 
+// LDS bytes of the cell kernel's offset staging: maxf x 3 doubles per
+// thread, only with a limiter (two passes over the faces) and only while
+// the block's share stays within 64 KiB (maxf <= 10); 0 = not staged
+size_t cell_stage_bytes(const DevMesh& D, int lim, int opt) {
+    const size_t b = (size_t)D.maxf * 3 * BLOCK * sizeof(double);
+    return (opt && lim != 0 && b <= 65536) ? b : 0;
+}
+
 template <int DIM, int LIM, int LAYOUT, bool OFFS>
 void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d, ibis_residual* r,
                       int nlog, const double* prim, hipStream_t st) {
+    const size_t shm = cell_stage_bytes(D, LIM, r->opt_stage);
+    const int stage = shm ? 1 : 0;
     if (r->prec == IBIS_PREC_FP32_FLUX) {
         float* fl = reinterpret_cast<float*>(r->FL);
-        k_cell_fs<DIM, LIM, 2, OFFS, float><<<xcd_grid(nlog), BLOCK, 0, st>>>(
-            D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, fl, fl, r->offs, prim);
+        k_cell_fs<DIM, LIM, 2, OFFS, float><<<xcd_grid(nlog), BLOCK, shm, st>>>(
+            D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, fl, fl, r->offs, prim,
+            stage);
         return;
     }
     if (!r->opt_pack && LAYOUT == 2) {
-        k_cell_fs<DIM, LIM, 2, OFFS, double, false><<<xcd_grid(nlog), BLOCK, 0, st>>>(
+        k_cell_fs<DIM, LIM, 2, OFFS, double, false><<<xcd_grid(nlog), BLOCK, shm, st>>>(
             D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR, r->offs,
-            prim);
+            prim, stage);
         return;
     }
-    k_cell_fs<DIM, LIM, LAYOUT, OFFS><<<xcd_grid(nlog), BLOCK, 0, st>>>(
+    k_cell_fs<DIM, LIM, LAYOUT, OFFS><<<xcd_grid(nlog), BLOCK, shm, st>>>(
         D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR, r->offs,
-        prim);
+        prim, stage);
 }
 
 template <int DIM, int LIM>
@@ -2441,6 +2466,10 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
     }
     if (k == "freeze_limiter") {
         r->freeze_lim = value ? 1 : 0;
+        return IBIS_OK;
+    }
+    if (k == "stage_offsets") {
+        r->opt_stage = value ? 1 : 0;
         return IBIS_OK;
     }
     return ibis::fail(IBIS_ERR_INVALID, ("unknown option " + k).c_str());

@@ -268,6 +268,18 @@ ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** lim
  *                         2: per (face slot, variable) planes over cells
  *   "precomputed_offsets" 0: x_face - x_cell computed from face centres
  *                         1: precomputed once per (cell, face slot)
+ *   "pack_cells"          1: cell values + centroid packed into 64-byte
+ *                            records before the cell kernel
+ *                         0: gathered from the state rows (layout 2 only)
+ *   "face_kernel"         0: face flux + integral over per-tile face lists
+ *                         64, 128, 256: slot-major kernel with that many
+ *                            cells per block (layout 2 only)
+ *   "stage_offsets"       1: the cell kernel keeps the face offsets of its
+ *                            limiter pass in LDS for the reconstruction
+ *                            pass (limited schemes, at most 10 faces/cell)
+ *                         0: both passes read them from memory
+ *   "freeze_limiter"      1: the tangent (JVP) treats the limiter values
+ *                            as constants; 0 (default) differentiates them
  * Setting precomputed_offsets allocates and synchronises once. */
 ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t value);
 
