@@ -173,20 +173,27 @@ def batch_leg(nsteps, n_inst=64):
     check(lib().ibis_prim_to_cons(bt.residual_handle(), ptr(d_prim), ptr(d_cons), None),
           "ibis_prim_to_cons")
     torch.cuda.synchronize()
-    # autotune the face-state layout of the batch's residual on a throwaway copy
-    tune = {}
-    for layout, fk in ((0, 0), (2, 0), (2, 64), (2, 128), (2, 256)):
-        bt.set_option("face_state_layout", layout)
-        bt.set_option("face_kernel", fk)
+    # autotune the implementation options of the batch's residual (all give
+    # the same bits) on a throwaway copy
+    tune, opts = {}, {}
+    for layout, fk, pack, offs in ((0, 0, 1, 0), (2, 0, 1, 1), (2, 0, 0, 0), (2, 0, 0, 1),
+                                   (2, 64, 0, 1), (2, 128, 0, 1), (2, 256, 0, 1)):
+        opt = {"face_state_layout": layout, "face_kernel": fk, "pack_cells": pack,
+               "precomputed_offsets": offs}
+        for k, v in opt.items():
+            bt.set_option(k, v)
         c2, p2 = d_cons.clone(), d_prim.clone()
         bt.rk_init(c2, p2, cfl=0.5, max_time=1e30)
         bt.rk_steps(1, use_graph=True)
         w = time.perf_counter()
         bt.rk_steps(2, use_graph=True)
-        tune[(layout, fk)] = (time.perf_counter() - w) / 2
+        key = f"layout{layout}_fk{fk}_pack{pack}_offsets{offs}"
+        tune[key] = (time.perf_counter() - w) / 2
+        opts[key] = opt
+        del c2, p2
     best = min(tune, key=tune.get)
-    bt.set_option("face_state_layout", best[0])
-    bt.set_option("face_kernel", best[1])
+    for k, v in opts[best].items():
+        bt.set_option(k, v)
     bt.rk_init(d_cons, d_prim, cfl=0.5, max_time=1e30)
     bt.rk_steps(1, use_graph=True)
     w0 = time.perf_counter()
@@ -199,8 +206,7 @@ def batch_leg(nsteps, n_inst=64):
     out = {"instances": bt.n, "cells": int(bt.N), "steps": nsteps,
            "sequential_s": round(t_seq, 4), "batched_s": round(t_bat, 4),
            "speedup": round(t_seq / t_bat, 3), "identical_fp64": bool(identical),
-           "autotune_step_s": {f"layout{k[0]}_fk{k[1]}": round(v, 5) for k, v in tune.items()},
-           "layout": best}
+           "autotune_step_s": {k: round(v, 5) for k, v in tune.items()}, "options": opts[best]}
     print(f"[bench] batch: {out}", file=sys.stderr, flush=True)
     for fv, rk, dc, dp in seq:
         fv.close()

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "ibis_internal.h"
+#include "state.hpp"
 
 namespace {
 
@@ -50,22 +51,39 @@ inline unsigned nblk(int64_t n) { return (unsigned)((n + BB - 1) / BB); }
 
 __device__ __forceinline__ double kmin(double a, double b) { return (b < a) ? b : a; }
 
-// stable_dt[i] = min over the valid cells of instance i
+constexpr double DMAX = 1.79769313486231570815e+308;
+
+// stable_dt[i] = min over the valid cells of instance i, in two passes:
+// every chunk (at most CHUNK cells of one instance) reduces to partial[k],
+// then each instance reduces its chunks chunk_off[i] .. chunk_off[i+1].
+constexpr int CHUNK = 8192;
+
 __global__ void __launch_bounds__(BB)
-    k_seg_min(const int64_t* __restrict__ cell_off, const double* __restrict__ dt_cells,
-              double* __restrict__ stable) {
+    k_chunk_min(const int64_t* __restrict__ chunk_lo, const double* __restrict__ dt_cells,
+                double* __restrict__ partial) {
     __shared__ double sm[BB / 64];
-    const int inst = blockIdx.x;
-    double v = 1.79769313486231570815e+308;
-    for (int64_t c = cell_off[inst] + threadIdx.x; c < cell_off[inst + 1]; c += BB)
-        v = kmin(dt_cells[c], v);
+    const int k = blockIdx.x;
+    const int64_t lo = chunk_lo[k], hi = chunk_lo[k + 1];
+    double v = DMAX;
+    for (int64_t c = lo + threadIdx.x; c < hi; c += BB) v = kmin(dt_cells[c], v);
     for (int o = 32; o > 0; o >>= 1) v = kmin(__shfl_down(v, o, 64), v);
     if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int k = 1; k < BB / 64; k++) v = kmin(sm[k], v);
-        stable[inst] = v;
+        for (int w = 1; w < BB / 64; w++) v = kmin(sm[w], v);
+        partial[k] = v;
     }
+}
+
+__global__ void __launch_bounds__(64)
+    k_seg_min(const int32_t* __restrict__ chunk_off, const double* __restrict__ partial,
+              double* __restrict__ stable) {
+    const int inst = blockIdx.x;
+    double v = DMAX;
+    for (int k = chunk_off[inst] + threadIdx.x; k < chunk_off[inst + 1]; k += 64)
+        v = kmin(partial[k], v);
+    for (int o = 32; o > 0; o >>= 1) v = kmin(__shfl_down(v, o, 64), v);
+    if (threadIdx.x == 0) stable[inst] = v;
 }
 
 // RungeKutta::estimate_dt (runge_kutta.cpp:108-119) per instance
@@ -92,18 +110,33 @@ struct StageArgs {
     int nk;
 };
 
-// out = u + k_0 * (c_0 * dt) + k_1 * (c_1 * dt) + ..., evaluated left to
-// right as successive apply_time_derivative passes do
-// (conserved_quantities.cpp:80-94, runge_kutta.cpp:151-191)
+// Stage value u + k_0 * (c_0 * dt) + k_1 * (c_1 * dt) + ... of every
+// conserved row of cell i, evaluated left to right as successive
+// apply_time_derivative passes do (conserved_quantities.cpp:80-94,
+// runge_kutta.cpp:151-191), then converted to the primitive rows by the
+// residual's cons_to_prim expressions without storing the intermediate
+// conserved state; cons_out (may alias u) receives it when not null.
+template <int DIM>
 __global__ void __launch_bounds__(BB)
-    k_stage(int64_t n, int Nt, const int32_t* __restrict__ inst_of_cell,
-            const double* __restrict__ dt, const double* u, StageArgs sa, double* out) {
-    const int64_t i = (int64_t)blockIdx.x * BB + threadIdx.x;
-    if (i >= n) return;
-    const double d = dt[inst_of_cell[i % Nt]];
-    double v = u[i];
-    for (int j = 0; j < sa.nk; j++) v = v + sa.k[j][i] * (sa.c[j] * d);
-    out[i] = v;
+    k_stage_prim(int Nt, const int32_t* __restrict__ inst_of_cell, const double* __restrict__ dt,
+                 const double* u, StageArgs sa, double* cons_out, double R, double Cv,
+                 double* __restrict__ prim) {
+    const int i = blockIdx.x * BB + threadIdx.x;
+    if (i >= Nt) return;
+    constexpr int NC = DIM + 2;
+    const double d = dt[inst_of_cell[i]];
+    double q[NC], out[ibis::NPRIM];
+#pragma unroll
+    for (int r = 0; r < NC; r++) {
+        const size_t e = (size_t)r * Nt + i;
+        double v = u[e];
+        for (int j = 0; j < sa.nk; j++) v = v + sa.k[j][e] * (sa.c[j] * d);
+        q[r] = v;
+        if (cons_out) cons_out[e] = v;
+    }
+    ibis::cons_to_prim_cell<double, DIM>(q, R, Cv, out);
+#pragma unroll
+    for (int r = 0; r < ibis::NPRIM; r++) prim[(size_t)r * Nt + i] = out[r];
 }
 
 }  // namespace
@@ -114,6 +147,11 @@ struct ibis_batch {
     int dim = 0, N = 0, Nt = 0, nc = 0;
     int64_t* cell_off = nullptr;    // device, n_inst + 1
     int32_t* inst_of_cell = nullptr;  // device, Nt
+    int nchunks = 0;
+    int64_t* chunk_lo = nullptr;    // device, nchunks + 1: cell range of each chunk
+    int32_t* chunk_off = nullptr;   // device, n_inst + 1: chunk range of each instance
+    double* partial = nullptr;      // device, nchunks
+    double R = 0.0, Cv = 0.0;
     double* dt_cells = nullptr;
     double* stable = nullptr;
     double* dt = nullptr;
@@ -127,7 +165,6 @@ struct ibis_batch {
     double* cons = nullptr;
     double* prim = nullptr;
     std::vector<double*> k;
-    double* k_tmp = nullptr;
     double* flow_tmp = nullptr;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -139,9 +176,10 @@ void ibis_batch_destroy(ibis_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->stream) (void)hipStreamDestroy(b->stream);
-    for (void* p : {(void*)b->cell_off, (void*)b->inst_of_cell, (void*)b->dt_cells,
+    for (void* p : {(void*)b->cell_off, (void*)b->inst_of_cell, (void*)b->chunk_lo,
+                    (void*)b->chunk_off, (void*)b->partial, (void*)b->dt_cells,
                     (void*)b->stable, (void*)b->dt, (void*)b->dt_prev, (void*)b->t,
-                    (void*)b->k_tmp, (void*)b->flow_tmp})
+                    (void*)b->flow_tmp})
         if (p) (void)hipFree(p);
     for (double* p : b->k) (void)hipFree(p);
     ibis_residual_destroy(b->r);
@@ -214,7 +252,11 @@ ibis_status ibis_batch_create(const ibis_mesh* m, int n_inst, const ibis_fv_desc
         ibis_batch_destroy(b);
         return ibis::fail(IBIS_ERR_INVALID, "ibis_batch_create: instance ranges do not cover the mesh");
     }
+    b->R = d0.R;
+    b->Cv = d0.Cv;
     std::vector<int32_t> inst(b->Nt, -1);
+    std::vector<int64_t> clo;
+    std::vector<int32_t> coff(n_inst + 1, 0);
     for (int i = 0; i < n_inst; i++) {
         if (cell_off[i + 1] < cell_off[i] || ghost_off[i + 1] < ghost_off[i]) {
             ibis_batch_destroy(b);
@@ -222,11 +264,20 @@ ibis_status ibis_batch_create(const ibis_mesh* m, int n_inst, const ibis_fv_desc
         }
         for (int64_t c = cell_off[i]; c < cell_off[i + 1]; c++) inst[c] = i;
         for (int64_t g = ghost_off[i]; g < ghost_off[i + 1]; g++) inst[b->N + g] = i;
+        for (int64_t c = cell_off[i]; c < cell_off[i + 1]; c += CHUNK) clo.push_back(c);
+        coff[i + 1] = (int32_t)clo.size();
     }
+    b->nchunks = (int)clo.size();
+    clo.push_back(b->N);
     hipError_t e = hipMalloc((void**)&b->cell_off, sizeof(int64_t) * (n_inst + 1));
     if (e == hipSuccess) e = hipMemcpy(b->cell_off, cell_off, sizeof(int64_t) * (n_inst + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc((void**)&b->inst_of_cell, sizeof(int32_t) * b->Nt);
     if (e == hipSuccess) e = hipMemcpy(b->inst_of_cell, inst.data(), sizeof(int32_t) * b->Nt, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->chunk_lo, sizeof(int64_t) * clo.size());
+    if (e == hipSuccess) e = hipMemcpy(b->chunk_lo, clo.data(), sizeof(int64_t) * clo.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->chunk_off, sizeof(int32_t) * (n_inst + 1));
+    if (e == hipSuccess) e = hipMemcpy(b->chunk_off, coff.data(), sizeof(int32_t) * (n_inst + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->partial, sizeof(double) * std::max(b->nchunks, 1));
     if (e == hipSuccess) e = hipMalloc((void**)&b->dt_cells, sizeof(double) * b->N);
     for (double** p : {&b->stable, &b->dt, &b->dt_prev, &b->t})
         if (e == hipSuccess) e = hipMalloc((void**)p, sizeof(double) * n_inst);
@@ -258,9 +309,7 @@ ibis_status ibis_batch_rk_init(ibis_batch* b, int nstages, const double* a, cons
         HIP_TRY(hipMemset(p, 0, ncv * sizeof(double)));
         b->k.push_back(p);
     }
-    if (!b->k_tmp) {
-        HIP_TRY(hipMalloc(&b->k_tmp, ncv * sizeof(double)));
-        HIP_TRY(hipMemset(b->k_tmp, 0, ncv * sizeof(double)));
+    if (!b->flow_tmp) {
         HIP_TRY(hipMalloc(&b->flow_tmp, (size_t)7 * b->Nt * sizeof(double)));
         HIP_TRY(hipMemset(b->flow_tmp, 0, (size_t)7 * b->Nt * sizeof(double)));
     }
@@ -275,15 +324,27 @@ ibis_status ibis_batch_rk_init(ibis_batch* b, int nstages, const double* a, cons
     return IBIS_OK;
 }
 
+// stage values u + sum_j k_j (c_j dt) of every cell -> primitive rows
+static ibis_status stage_prim(ibis_batch* b, const StageArgs& sa, double* cons_out, double* prim) {
+    if (b->dim == 3)
+        k_stage_prim<3><<<nblk(b->Nt), BB, 0, b->stream>>>(b->Nt, b->inst_of_cell, b->dt, b->cons,
+                                                            sa, cons_out, b->R, b->Cv, prim);
+    else
+        k_stage_prim<2><<<nblk(b->Nt), BB, 0, b->stream>>>(b->Nt, b->inst_of_cell, b->dt, b->cons,
+                                                            sa, cons_out, b->R, b->Cv, prim);
+    HIP_TRY(hipGetLastError());
+    return IBIS_OK;
+}
+
 // one step of every instance on b->stream
 static ibis_status batch_step(ibis_batch* b) {
     hipStream_t st = b->stream;
     ibis_stream_t sh = reinterpret_cast<ibis_stream_t>(st);
     const int s = b->s;
-    const int64_t ncv = (int64_t)b->nc * b->Nt;
     IBIS_TRY(ibis_compute_dudt(b->r, b->prim, b->k[0], 1, sh));
     IBIS_TRY(ibis_residual_cell_dt(b->r, b->prim, b->dt_cells, sh));
-    k_seg_min<<<b->n_inst, BB, 0, st>>>(b->cell_off, b->dt_cells, b->stable);
+    if (b->nchunks) k_chunk_min<<<b->nchunks, BB, 0, st>>>(b->chunk_lo, b->dt_cells, b->partial);
+    k_seg_min<<<b->n_inst, 64, 0, st>>>(b->chunk_off, b->partial, b->stable);
     k_dt_rules<<<nblk(b->n_inst), BB, 0, st>>>(b->n_inst, b->stable, b->cfl, b->dt_prev, b->t,
                                               b->max_time, b->dt);
     for (int i = 1; i < s; i++) {
@@ -297,8 +358,7 @@ static ibis_status batch_step(ibis_batch* b) {
             sa.c[sa.nk] = b->a[i * s + j];
             sa.nk++;
         }
-        k_stage<<<nblk(ncv), BB, 0, st>>>(ncv, b->Nt, b->inst_of_cell, b->dt, b->cons, sa, b->k_tmp);
-        IBIS_TRY(ibis_cons_to_prim(b->r, b->k_tmp, b->flow_tmp, sh));
+        IBIS_TRY(stage_prim(b, sa, nullptr, b->flow_tmp));
         IBIS_TRY(ibis_compute_dudt(b->r, b->flow_tmp, b->k[i], 1, sh));
     }
     StageArgs sa{};
@@ -307,8 +367,7 @@ static ibis_status batch_step(ibis_batch* b) {
         sa.c[i] = b->bw[i];
     }
     sa.nk = s;
-    k_stage<<<nblk(ncv), BB, 0, st>>>(ncv, b->Nt, b->inst_of_cell, b->dt, b->cons, sa, b->cons);
-    IBIS_TRY(ibis_cons_to_prim(b->r, b->cons, b->prim, sh));
+    IBIS_TRY(stage_prim(b, sa, b->cons, b->prim));
     k_advance<<<nblk(b->n_inst), BB, 0, st>>>(b->n_inst, b->dt, b->dt_prev, b->t);
     HIP_TRY(hipGetLastError());
     return IBIS_OK;

@@ -40,6 +40,7 @@
 #include <vector>
 
 #include "flux.hpp"
+#include "state.hpp"
 #include "flux_adj.hpp"
 #include "ibis_internal.h"
 
@@ -56,8 +57,15 @@ thread_local std::string g_last_error;
         }                                                                               \
     } while (0)
 
-// primitive rows and gradient/limiter slots
-enum { P_RHO = 0, P_P = 1, P_T = 2, P_E = 3, P_VX = 4, P_VY = 5, P_VZ = 6, NPRIM = 7 };
+// primitive rows (ibis::P_*, state.hpp) and gradient/limiter slots
+using ibis::P_RHO;
+using ibis::P_P;
+using ibis::P_T;
+using ibis::P_E;
+using ibis::P_VX;
+using ibis::P_VY;
+using ibis::P_VZ;
+using ibis::NPRIM;
 enum { G_P = 0, G_RHO = 1, G_T = 2, G_U = 3, G_VX = 4, G_VY = 5, G_VZ = 6, NGRAD = 7 };
 
 constexpr int BLOCK = 256;
@@ -1333,20 +1341,12 @@ __global__ void __launch_bounds__(BLOCK)
     k_cons_to_prim(int Nt, double R, double Cv, const S* __restrict__ cq, S* __restrict__ fs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Nt) return;
-    const S rho = cq[i];
-    const S vx = cq[(size_t)Nt + i] / rho;
-    const S vy = cq[2 * (size_t)Nt + i] / rho;
-    const S vz = DIM == 3 ? cq[3 * (size_t)Nt + i] / rho : S(0.0);
-    const S ke = 0.5 * (vx * vx + vy * vy + vz * vz);
-    const S u = cq[(size_t)(DIM + 1) * Nt + i] / rho - ke;
-    const S temp = u / Cv;
-    fs[(size_t)P_RHO * Nt + i] = rho;
-    fs[(size_t)P_VX * Nt + i] = vx;
-    fs[(size_t)P_VY * Nt + i] = vy;
-    fs[(size_t)P_VZ * Nt + i] = vz;
-    fs[(size_t)P_E * Nt + i] = u;
-    fs[(size_t)P_T * Nt + i] = temp;
-    fs[(size_t)P_P * Nt + i] = rho * R * temp;
+    S q[DIM + 2], out[NPRIM];
+#pragma unroll
+    for (int k = 0; k < DIM + 2; k++) q[k] = cq[(size_t)k * Nt + i];
+    ibis::cons_to_prim_cell<S, DIM>(q, R, Cv, out);
+#pragma unroll
+    for (int k = 0; k < NPRIM; k++) fs[(size_t)k * Nt + i] = out[k];
 }
 
 template <int DIM>
