@@ -177,7 +177,7 @@ def batch_leg(nsteps, n_inst=64):
     # the same bits) on a throwaway copy
     tune, opts = {}, {}
     for layout, fk, pack, offs in ((0, 0, 1, 0), (2, 0, 1, 1), (2, 0, 0, 0), (2, 0, 0, 1),
-                                   (2, 64, 0, 1), (2, 128, 0, 1), (2, 256, 0, 1)):
+                                   (2, 1, 0, 1), (2, 64, 0, 1), (2, 128, 0, 1), (2, 256, 0, 1)):
         opt = {"face_state_layout": layout, "face_kernel": fk, "pack_cells": pack,
                "precomputed_offsets": offs}
         for k, v in opt.items():
@@ -228,7 +228,7 @@ def main():
                     help="fix the precomputed-offsets option instead of autotuning")
     ap.add_argument("--pack", type=int, default=None, choices=[0, 1],
                     help="fix the pack_cells option instead of autotuning")
-    ap.add_argument("--face-kernel", type=int, default=None, choices=[0, 64, 128, 256],
+    ap.add_argument("--face-kernel", type=int, default=None, choices=[0, 1, 64, 128, 256],
                     help="fix the face_kernel option instead of autotuning")
     ap.add_argument("--stage", type=int, default=None, choices=[0, 1],
                     help="fix the stage_offsets option instead of autotuning")
@@ -278,7 +278,7 @@ def main():
     for layout in ((0, 1, 2) if args.layout is None else (args.layout,)):
         for offs in ((0, 1) if args.offsets is None else (args.offsets,)):
             packs = (1, 0) if layout == 2 else (1,)
-            fks = (0, 64, 128, 256) if layout == 2 else (0,)
+            fks = (0, 1, 64, 128, 256) if layout == 2 else (0,)
             stages = (1, 0) if layout == 2 else (1,)
             for pack in (packs if args.pack is None else (args.pack,)):
                 for fk in (fks if args.face_kernel is None else (args.face_kernel,)):

@@ -90,7 +90,7 @@ struct DevMesh {
     const int* b_tag;
     const int* b_of_ghost;  // G: boundary-face index of each ghost cell
     // tiles of TILE consecutive cells and the faces touching them
-    int ntiles, maxtf;
+    int ntiles, maxtf, ntf;           // ntf: entries of all tile lists
     const int* tile_face_off;         // ntiles + 1
     const int* tile_faces;            // face ids, ascending within a tile
     const unsigned short* cell_lds;   // maxf x N: position of a cell's face in its tile list
@@ -836,7 +836,7 @@ __device__ __forceinline__ void load_face_geom(const DevMesh& m, int f, double* 
 template <int DIM, int FLUX, class R>
 __device__ __forceinline__ void face_flux_values_g(const DevMesh& m, const FaceParams& fp, int f,
                                                    const double* g, const R* lv, const R* rv,
-                                                   double* res_a);
+                                                   double* res_a, const double* gn = nullptr);
 
 template <int DIM, int FLUX, class R>
 __device__ __forceinline__ void face_flux_values(const DevMesh& m, const FaceParams& fp, int f,
@@ -846,10 +846,12 @@ __device__ __forceinline__ void face_flux_values(const DevMesh& m, const FacePar
     face_flux_values_g<DIM, FLUX, R>(m, fp, f, g, lv, rv, res_a);
 }
 
+// gn: the face normal when the caller has loaded it (else read by face id);
+// unused when the normal is cross(t1, t2)
 template <int DIM, int FLUX, class R>
 __device__ __forceinline__ void face_flux_values_g(const DevMesh& m, const FaceParams& fp, int f,
                                                    const double* g, const R* lv, const R* rv,
-                                                   double* res_a) {
+                                                   double* res_a, const double* gn) {
     constexpr int NC = DIM + 2;
     constexpr bool T3 = DIM == 3;
     const int F = m.F;
@@ -882,6 +884,10 @@ __device__ __forceinline__ void face_flux_values_g(const DevMesh& m, const FaceP
         nx = t1y * t2z - t1z * t2y;
         ny = t1z * t2x - t1x * t2z;
         nz = t1x * t2y - t1y * t2x;
+    } else if (gn) {
+        nx = R(gn[0]);
+        ny = R(gn[1]);
+        nz = R(gn[2]);
     } else {
         nx = R(m.geom[f]);
         ny = R(m.geom[(size_t)F + f]);
@@ -978,6 +984,99 @@ __global__ void __launch_bounds__(TILE)
         if (kn < nfl) load(kn, fn, vn, gn);
         double res[NC];
         face_flux_values_g<DIM, FLUX, R>(m, fp, f, g, v[0], v[1], res);
+#pragma unroll
+        for (int q = 0; q < NC; q++) sflux[q * S + k] = res[q];
+        k = kn;
+    }
+    __syncthreads();
+    const int c = t * TILE + threadIdx.x;
+    if (c >= N) return;
+    double d[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) d[q] = 0.0;
+    const int nf = m.nfaces[c];
+    for (int j = 0; j < nf; j++) {
+        const int k = m.cell_lds[(size_t)j * N + c];
+        const int s = m.ell_sign[(size_t)j * N + c];
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+            const double fa = sflux[q * S + k];
+            d[q] += (s > 0) ? -fa : fa;  // == flux * (-area * outsign), exactly
+        }
+    }
+    const double V = m.vol[c];
+#pragma unroll
+    for (int q = 0; q < NC; q++) dudt[(size_t)q * Nt + c] = d[q] / V;
+}
+
+// Per-entry copies of the tile lists' face data (face_kernel 1): for list
+// entry e (face f = tile_faces[e]) the start of each side's face values in
+// the layout-2 array (values at start + q * stride, stride N for a valid
+// cell's record and G for a ghost's, told apart by start >= maxf*5*N), and
+// t1, t2, area (+ the normal when it is not cross(t1, t2)) as planes of
+// length ntf.  A face's inputs are then contiguous loads at e plus the
+// gathers of the two sides' values.
+__global__ void __launch_bounds__(BLOCK)
+    k_entries(DevMesh m, int with_n, int* __restrict__ eidx, double* __restrict__ egeom) {
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    const int NE = m.ntf, F = m.F;
+    if (e >= NE) return;
+    const int f = m.tile_faces[e];
+    eidx[e] = (int)soa_index(m, m.L[f], m.slotL[f], 0);
+    eidx[(size_t)NE + e] = (int)soa_index(m, m.R[f], m.slotR[f], 0);
+#pragma unroll
+    for (int k = 0; k < 6; k++) egeom[(size_t)k * NE + e] = m.geom[(3 + k) * (size_t)F + f];
+    egeom[(size_t)6 * NE + e] = m.geom[12 * (size_t)F + f];
+    if (with_n)
+#pragma unroll
+        for (int k = 0; k < 3; k++) egeom[(size_t)(7 + k) * NE + e] = m.geom[(size_t)k * F + f];
+}
+
+// k_face_tile_fs over the per-entry face data (layout 2 values); NRM: the
+// normals are stored (they are not cross(t1, t2))
+template <int DIM, int FLUX, bool NRM, class R = double>
+__global__ void __launch_bounds__(TILE)
+    k_face_entry_fs(DevMesh m, FaceParams fp, const int* __restrict__ eidx,
+                    const double* __restrict__ egeom, const R* __restrict__ FL,
+                    double* __restrict__ dudt) {
+    extern __shared__ double sflux[];  // NC x maxtf
+    const int t = xcd_block(m.ntiles);
+    if (t >= m.ntiles) return;
+    constexpr int NC = DIM + 2;
+    const int N = m.N, Nt = m.Nt, S = m.maxtf, NE = m.ntf;
+    const int f0 = m.tile_face_off[t];
+    const int nfl = m.tile_face_off[t + 1] - f0;
+    const int gbase = m.maxf * 5 * N;
+    constexpr int NG = NRM ? 10 : 7;
+    auto load = [&](int k, R (&v)[2][5], double (&g)[NG]) {
+        const int e = f0 + k;
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const int b = eidx[(size_t)side * NE + e];
+            const size_t stride = b >= gbase ? (size_t)m.G : (size_t)N;
+#pragma unroll
+            for (int q = 0; q < 5; q++) v[side][q] = FL[(size_t)b + q * stride];
+        }
+#pragma unroll
+        for (int q = 0; q < NG; q++) g[q] = egeom[(size_t)q * NE + e];
+    };
+    int k = threadIdx.x;
+    R vn[2][5];
+    double gn[NG];
+    if (k < nfl) load(k, vn, gn);
+    while (k < nfl) {
+        R v[2][5];
+        double g[NG];
+#pragma unroll
+        for (int side = 0; side < 2; side++)
+#pragma unroll
+            for (int q = 0; q < 5; q++) v[side][q] = vn[side][q];
+#pragma unroll
+        for (int q = 0; q < NG; q++) g[q] = gn[q];
+        const int kn = k + TILE;
+        if (kn < nfl) load(kn, vn, gn);
+        double res[NC];
+        face_flux_values_g<DIM, FLUX, R>(m, fp, 0, g, v[0], v[1], res, NRM ? g + 7 : nullptr);
 #pragma unroll
         for (int q = 0; q < NC; q++) sflux[q * S + k] = res[q];
         k = kn;
@@ -1660,6 +1759,7 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
     D.n_from_t = n_from_t;
     D.ntiles = ntiles;
     D.maxtf = maxtf;
+    D.ntf = toff[ntiles];
     *out = m;
     return IBIS_OK;
 }
@@ -1689,7 +1789,10 @@ struct ibis_residual {
     int opt_offsets = 0;     // option precomputed_offsets
     int opt_pack = 1;        // option pack_cells
     int opt_stage = 1;       // option stage_offsets
-    int opt_face_kernel = 0; // option face_kernel: 0 tile face lists, 64/128/256 slot-major
+    int opt_face_kernel = 0; // option face_kernel: 0 tile face lists, 1 the same with
+                             // per-entry face data, 64/128/256 slot-major
+    int* ent_idx = nullptr;      // per tile-list entry face data (face_kernel 1)
+    double* ent_geom = nullptr;
     double* face_rho = nullptr;
     double* face_T = nullptr;
     int* kinds = nullptr;    // pre, visc, post
@@ -1728,7 +1831,7 @@ struct ibis_residual {
 
 void ibis_residual_destroy(ibis_residual* r) {
     if (!r) return;
-    for (void* p : {(void*)r->grad, (void*)r->lim, (void*)r->flux, (void*)r->pack, (void*)r->FL, (void*)r->FR, (void*)r->offs, (void*)r->face_rho,
+    for (void* p : {(void*)r->grad, (void*)r->lim, (void*)r->flux, (void*)r->pack, (void*)r->FL, (void*)r->FR, (void*)r->offs, (void*)r->ent_idx, (void*)r->ent_geom, (void*)r->face_rho,
                     (void*)r->face_T, (void*)r->kinds, (void*)r->pars, (void*)r->partial,
                     (void*)r->dt_dev, (void*)r->count_dev, (void*)r->k_tmp, (void*)r->flow_tmp,
                     (void*)r->dual_prim, (void*)r->dual_dudt, (void*)r->dual_grad,
@@ -2022,9 +2125,22 @@ void launch_face_slot_ts(const DevMesh& D, const FaceParams& fp, int ts, const R
 // lists, otherwise the slot-major kernel with that many cells per block
 template <int DIM, int FLUX>
 void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, int face_kernel,
-                      const double* FL, const double* FR, double* dudt, hipStream_t st) {
+                      const int* eidx, const double* egeom, const double* FL, const double* FR,
+                      double* dudt, hipStream_t st) {
     const unsigned g = xcd_grid(D.ntiles);
     const size_t shm = (size_t)(DIM + 2) * D.maxtf * sizeof(double);
+    if (face_kernel == 1 && layout >= 2) {
+        const float* fl = reinterpret_cast<const float*>(FL);
+        if (layout == 3 && D.n_from_t)
+            k_face_entry_fs<DIM, FLUX, false, float><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, fl, dudt);
+        else if (layout == 3)
+            k_face_entry_fs<DIM, FLUX, true, float><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, fl, dudt);
+        else if (D.n_from_t)
+            k_face_entry_fs<DIM, FLUX, false><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, FL, dudt);
+        else
+            k_face_entry_fs<DIM, FLUX, true><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, FL, dudt);
+        return;
+    }
     if (layout == 3) {
         const float* fl = reinterpret_cast<const float*>(FL);
         if (face_kernel)
@@ -2043,13 +2159,16 @@ void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, int fa
 
 template <int DIM>
 void launch_face_fs(const DevMesh& D, const FaceParams& fp, int flux, int layout, int face_kernel,
-                    const double* FL, const double* FR, double* dudt, hipStream_t st) {
+                    const int* eidx, const double* egeom, const double* FL, const double* FR,
+                    double* dudt, hipStream_t st) {
+#define IBIS_FF(K) launch_face_fs_f<DIM, K>(D, fp, layout, face_kernel, eidx, egeom, FL, FR, dudt, st)
     switch (flux) {
-        case IBIS_FLUX_HANEL: launch_face_fs_f<DIM, IBIS_FLUX_HANEL>(D, fp, layout, face_kernel, FL, FR, dudt, st); break;
-        case IBIS_FLUX_AUSMDV: launch_face_fs_f<DIM, IBIS_FLUX_AUSMDV>(D, fp, layout, face_kernel, FL, FR, dudt, st); break;
-        case IBIS_FLUX_LDFSS2: launch_face_fs_f<DIM, IBIS_FLUX_LDFSS2>(D, fp, layout, face_kernel, FL, FR, dudt, st); break;
-        default: launch_face_fs_f<DIM, IBIS_FLUX_RUSANOV>(D, fp, layout, face_kernel, FL, FR, dudt, st); break;
+        case IBIS_FLUX_HANEL: IBIS_FF(IBIS_FLUX_HANEL); break;
+        case IBIS_FLUX_AUSMDV: IBIS_FF(IBIS_FLUX_AUSMDV); break;
+        case IBIS_FLUX_LDFSS2: IBIS_FF(IBIS_FLUX_LDFSS2); break;
+        default: IBIS_FF(IBIS_FLUX_RUSANOV); break;
     }
+#undef IBIS_FF
 }
 
 // thermo reconstruction pair: (slot, prim row) of the two variables,
@@ -2276,9 +2395,11 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
             const int lay = r->prec == IBIS_PREC_FP32_FLUX ? 3 : r->opt_layout;
             const int fk = (lay >= 2) ? r->opt_face_kernel : 0;
             if (D.dim == 3)
-                launch_face_fs<3>(D, fp, d.flux, lay, fk, r->FL, r->FR, dudt, st);
+                launch_face_fs<3>(D, fp, d.flux, lay, fk, r->ent_idx, r->ent_geom, r->FL, r->FR,
+                                  dudt, st);
             else
-                launch_face_fs<2>(D, fp, d.flux, lay, fk, r->FL, r->FR, dudt, st);
+                launch_face_fs<2>(D, fp, d.flux, lay, fk, r->ent_idx, r->ent_geom, r->FL, r->FR,
+                                  dudt, st);
             pf.end(2);
         } else {
             // first order: face flux and surface integral as one tiled kernel
@@ -2452,11 +2573,22 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
         return IBIS_OK;
     }
     if (k == "face_kernel") {
-        if (value != 0 && value != 64 && value != 128 && value != 256)
-            return ibis::fail(IBIS_ERR_INVALID, "face_kernel must be 0, 64, 128 or 256");
+        if (value != 0 && value != 1 && value != 64 && value != 128 && value != 256)
+            return ibis::fail(IBIS_ERR_INVALID, "face_kernel must be 0, 1, 64, 128 or 256");
         const DevMesh& D = r->m->d;
-        if (value && (size_t)(D.dim + 2) * D.maxf * value * sizeof(double) > 65536)
+        if (value > 1 && (size_t)(D.dim + 2) * D.maxf * value * sizeof(double) > 65536)
             return ibis::fail(IBIS_ERR_INVALID, "face_kernel: block does not fit in LDS");
+        if (value == 1 && !r->ent_idx) {
+            if ((int64_t)D.maxf * 5 * D.N + 5 * (int64_t)D.G >= (int64_t)INT32_MAX)
+                return ibis::fail(IBIS_ERR_INVALID, "face_kernel 1: mesh too large for 32-bit value indices");
+            const size_t ne = std::max(D.ntf, 1);
+            const int with_n = D.n_from_t ? 0 : 1;
+            HIP_TRY(hipMalloc(&r->ent_idx, sizeof(int) * 2 * ne));
+            HIP_TRY(hipMalloc(&r->ent_geom, sizeof(double) * (with_n ? 10 : 7) * ne));
+            if (D.ntf > 0) k_entries<<<nblocks(D.ntf), BLOCK>>>(D, with_n, r->ent_idx, r->ent_geom);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipDeviceSynchronize());
+        }
         r->opt_face_kernel = (int)value;
         return IBIS_OK;
     }

@@ -272,6 +272,8 @@ ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** lim
  *                            records before the cell kernel
  *                         0: gathered from the state rows (layout 2 only)
  *   "face_kernel"         0: face flux + integral over per-tile face lists
+ *                         1: the same, with each list entry's face data
+ *                            (value indices, geometry) stored per entry
  *                         64, 128, 256: slot-major kernel with that many
  *                            cells per block (layout 2 only)
  *   "stage_offsets"       1: the cell kernel keeps the face offsets of its

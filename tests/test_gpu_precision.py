@@ -42,6 +42,10 @@ def _pair(grid_io, cfg, prim_ref):
     fv.set_option("face_kernel", 128)
     d32b = torch.zeros_like(d64)
     fv.compute_dudt(prim, gb, d32b)
+    # and so does the per-entry tile kernel
+    fv.set_option("face_kernel", 1)
+    d32c = torch.zeros_like(d64)
+    fv.compute_dudt(prim, gb, d32c)
     fv.set_option("face_kernel", 0)
     fv.set_precision("fp64")
     d64b = torch.zeros_like(d64)
@@ -49,6 +53,7 @@ def _pair(grid_io, cfg, prim_ref):
     torch.cuda.synchronize()
     assert torch.equal(d64, d64b)  # switching back restores the FP64 bits
     assert torch.equal(d32, d32b)
+    assert torch.equal(d32, d32c)
     return gb, d32.cpu().numpy(), d64.cpu().numpy()
 
 

@@ -97,6 +97,7 @@ def test_hex_jittered_permuted(flux, limiter):
 
 @pytest.mark.parametrize("layout,pack,fk,stage", [(0, 1, 0, 1), (1, 1, 0, 0), (2, 1, 0, 1),
                                                    (2, 0, 0, 1), (2, 0, 0, 0), (2, 1, 64, 0),
+                                                   (2, 0, 1, 1), (2, 1, 1, 0),
                                                    (2, 0, 128, 1), (2, 1, 256, 1)])
 @pytest.mark.parametrize("offsets", [0, 1])
 def test_implementation_options_give_identical_bits(layout, pack, fk, stage, offsets):
