@@ -134,6 +134,47 @@ def adjoint_leg(steps):
     return out
 
 
+# implementation options of the residual, each with its candidate values
+# (the first is the starting point); every choice gives the same bits
+OPTION_SPACE = (("face_state_layout", (2, 1, 0)), ("precomputed_offsets", (1, 0)),
+                ("pack_cells", (0, 1)), ("face_kernel", (1, 0, 64, 128, 256)),
+                ("stage_offsets", (1, 0)), ("cell_prefetch", (0, 1)))
+SHORT = {"face_state_layout": "layout", "precomputed_offsets": "offsets", "pack_cells": "pack",
+         "face_kernel": "fk", "stage_offsets": "stage", "cell_prefetch": "pf"}
+
+
+def autotune(set_option, measure, fixed, log):
+    """Coordinate descent over OPTION_SPACE: two sweeps, each trying every
+    value of one option with the others at their best so far.  `measure`
+    times the workload under the options set; `fixed` pins options.  Returns
+    the best options, their time and every measured variant."""
+    cur = {k: fixed.get(k, vals[0]) for k, vals in OPTION_SPACE}
+    seen = {}
+
+    def score(opt):
+        key = "_".join(f"{SHORT[k]}{opt[k]}" for k, _ in OPTION_SPACE)
+        if key not in seen:
+            for k, v in opt.items():
+                set_option(k, v)
+            seen[key] = measure()
+            log(key, seen[key])
+        return seen[key]
+
+    best = score(cur)
+    for _ in range(2):
+        for k, vals in OPTION_SPACE:
+            if k in fixed:
+                continue
+            for v in vals:
+                cand = dict(cur, **{k: v})
+                t = score(cand)
+                if t < best:
+                    best, cur = t, cand
+    for k, v in cur.items():
+        set_option(k, v)
+    return cur, best, seen
+
+
 def batch_leg(nsteps, n_inst=64):
     """64 seeded wedge/ramp instances (10k-400k cells, seed 7), ssp-rk3 at
     CFL 0.5: one at a time on the plain path vs all together in one HIP
@@ -175,25 +216,15 @@ def batch_leg(nsteps, n_inst=64):
     torch.cuda.synchronize()
     # autotune the implementation options of the batch's residual (all give
     # the same bits) on a throwaway copy
-    tune, opts = {}, {}
-    for layout, fk, pack, offs in ((0, 0, 1, 0), (2, 0, 1, 1), (2, 0, 0, 0), (2, 0, 0, 1),
-                                   (2, 1, 0, 1), (2, 64, 0, 1), (2, 128, 0, 1), (2, 256, 0, 1)):
-        opt = {"face_state_layout": layout, "face_kernel": fk, "pack_cells": pack,
-               "precomputed_offsets": offs}
-        for k, v in opt.items():
-            bt.set_option(k, v)
+    def measure():
         c2, p2 = d_cons.clone(), d_prim.clone()
         bt.rk_init(c2, p2, cfl=0.5, max_time=1e30)
         bt.rk_steps(1, use_graph=True)
         w = time.perf_counter()
         bt.rk_steps(2, use_graph=True)
-        key = f"layout{layout}_fk{fk}_pack{pack}_offsets{offs}"
-        tune[key] = (time.perf_counter() - w) / 2
-        opts[key] = opt
-        del c2, p2
-    best = min(tune, key=tune.get)
-    for k, v in opts[best].items():
-        bt.set_option(k, v)
+        return (time.perf_counter() - w) / 2
+
+    best_opt, _, tune = autotune(bt.set_option, measure, {}, lambda key, t: None)
     bt.rk_init(d_cons, d_prim, cfl=0.5, max_time=1e30)
     bt.rk_steps(1, use_graph=True)
     w0 = time.perf_counter()
@@ -206,7 +237,7 @@ def batch_leg(nsteps, n_inst=64):
     out = {"instances": bt.n, "cells": int(bt.N), "steps": nsteps,
            "sequential_s": round(t_seq, 4), "batched_s": round(t_bat, 4),
            "speedup": round(t_seq / t_bat, 3), "identical_fp64": bool(identical),
-           "autotune_step_s": {k: round(v, 5) for k, v in tune.items()}, "options": opts[best]}
+           "autotune_step_s": {k: round(v, 5) for k, v in tune.items()}, "options": best_opt}
     print(f"[bench] batch: {out}", file=sys.stderr, flush=True)
     for fv, rk, dc, dp in seq:
         fv.close()
@@ -232,6 +263,8 @@ def main():
                     help="fix the face_kernel option instead of autotuning")
     ap.add_argument("--stage", type=int, default=None, choices=[0, 1],
                     help="fix the stage_offsets option instead of autotuning")
+    ap.add_argument("--cell-prefetch", type=int, default=None, choices=[0, 1],
+                    help="fix the cell_prefetch option instead of autotuning")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/primal leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-stepping leg")
     ap.add_argument("--batch-steps", type=int, default=10)
@@ -271,24 +304,33 @@ def main():
     dudt = torch.zeros((gb.dim + 2, gb.Nt), dtype=torch.float64, device="cuda")
     stream = torch.cuda.current_stream()
 
+    # device copy bandwidth (1 GiB read + 1 GiB written per copy), the
+    # practical ceiling beside the 8 TB/s peak
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(10):
+        dst.copy_(src)
+    b.record(stream)
+    torch.cuda.synchronize()
+    copy_gbs = 2 * src.numel() * 8 * 10 / (a.elapsed_time(b) * 1e-3) / 1e9
+    del src, dst
+    print(f"[bench] device copy: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    for _ in range(10):  # warm the residual before timing variants
+        fv.compute_dudt(prim, gb, dudt)
+    torch.cuda.synchronize()
+
     # autotune the implementation choices (all give identical results)
-    variants = {}
-    best = None
-    choices = []
-    for layout in ((0, 1, 2) if args.layout is None else (args.layout,)):
-        for offs in ((0, 1) if args.offsets is None else (args.offsets,)):
-            packs = (1, 0) if layout == 2 else (1,)
-            fks = (0, 1, 64, 128, 256) if layout == 2 else (0,)
-            stages = (1, 0) if layout == 2 else (1,)
-            for pack in (packs if args.pack is None else (args.pack,)):
-                for fk in (fks if args.face_kernel is None else (args.face_kernel,)):
-                    for stage in (stages if args.stage is None else (args.stage,)):
-                        choices.append({"face_state_layout": layout, "precomputed_offsets": offs,
-                                        "pack_cells": pack, "face_kernel": fk,
-                                        "stage_offsets": stage})
-    for opt in choices:
-        for k, v in opt.items():
-            fv.set_option(k, v)
+    fixed = {k: v for k, v in (("face_state_layout", args.layout),
+                               ("precomputed_offsets", args.offsets),
+                               ("pack_cells", args.pack), ("face_kernel", args.face_kernel),
+                               ("stage_offsets", args.stage),
+                               ("cell_prefetch", args.cell_prefetch)) if v is not None}
+
+    def measure():
         for _ in range(2):
             fv.compute_dudt(prim, gb, dudt)
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -297,15 +339,14 @@ def main():
             fv.compute_dudt(prim, gb, dudt)
         b.record(stream)
         torch.cuda.synchronize()
-        us = a.elapsed_time(b) * 1e3 / 5
-        key = ("layout{face_state_layout}_offsets{precomputed_offsets}_pack{pack_cells}"
-               "_fk{face_kernel}_stage{stage_offsets}").format(**opt)
-        variants[key] = round(us, 1)
-        print(f"[bench] variant {key}: {us:.1f} us/residual", file=sys.stderr, flush=True)
-        if best is None or us < best[0]:
-            best = (us, opt)
-    for k, v in best[1].items():
-        fv.set_option(k, v)
+        return a.elapsed_time(b) * 1e3 / 5
+
+    best_opt, _, variants = autotune(
+        fv.set_option, measure, fixed,
+        lambda key, us: print(f"[bench] variant {key}: {us:.1f} us/residual", file=sys.stderr,
+                              flush=True))
+    variants = {k: round(v, 1) for k, v in variants.items()}
+    best = (None, best_opt)
 
     for _ in range(args.warmup):
         fv.compute_dudt(prim, gb, dudt)
@@ -409,10 +450,14 @@ def main():
                    "cells": N, "faces": F, "ghosts": Nt - N,
                    "variant": best[1],
                    "autotune_us": variants},
-        "roofline": {"bound": "hbm", "kernel": "residual (all kernels of one compute_dudt)",
+        "roofline": {"bound": "hbm", "kernel": "residual (all kernels of one compute_dudt: "
+                                               "boundary = k_bc + k_ghost_fs, cell_stencil = "
+                                               "k_cell_fs, face_flux = k_face_*_fs)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes": B,
+                     "copy_GBs": round(copy_gbs, 1),
+                     "frac_of_copy": round(achieved / copy_gbs, 4),
                      "dominant_kernel": dominant,
                      "kernels_us": {k: round(v, 2) for k, v in kernels_us.items()},
                      "sum_kernels_us": round(residual_kernel_us, 2)},

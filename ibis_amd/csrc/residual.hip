@@ -655,7 +655,8 @@ __device__ __forceinline__ void load_cell(const DevMesh& m, const VarSet& vs,
     }
 }
 
-template <int DIM, int LIM, int LAYOUT, bool OFFS, class W = double, bool PACKED = true>
+template <int DIM, int LIM, int LAYOUT, bool OFFS, class W = double, bool PACKED = true,
+          bool PF = false>
 __global__ void __launch_bounds__(BLOCK)
     k_cell_fs(DevMesh m, VarSet vs, const double* __restrict__ pack, double* __restrict__ grad,
               double* __restrict__ lim, double eps, double venkat_K, int nlogical,
@@ -694,10 +695,8 @@ __global__ void __launch_bounds__(BLOCK)
         gx[v] = gy[v] = gz[v] = 0.0;
         umin[v] = umax[v] = ui[v];
     }
-    for (int j = 0; j < nf; j++) {
-        const int n = m.ell_nbr[(size_t)j * N + i];
-        double un[5], xn, yn, zn;
-        load_cell<PACKED>(m, vs, pack, prim, n, un, xn, yn, zn);
+    // one neighbour's contribution to the LSQ gradient and the min / max
+    auto accumulate = [&](const double* un, double xn, double yn, double zn) {
         const double dx = xn - xi;
         const double dy = yn - yi;
         const double dz = zn - zi;
@@ -718,6 +717,35 @@ __global__ void __launch_bounds__(BLOCK)
             }
             umin[v] = ibis::kmin(umin[v], un[v]);
             umax[v] = ibis::kmax(umax[v], un[v]);
+        }
+    };
+    if (PF) {
+        // every neighbour index first, then each neighbour's values loaded
+        // while the previous neighbour is accumulated (same order of sums)
+        int nbr[MAXF];
+#pragma unroll
+        for (int j = 0; j < MAXF; j++) nbr[j] = j < nf ? m.ell_nbr[(size_t)j * N + i] : i;
+        double un[5], xn, yn, zn;
+        load_cell<PACKED>(m, vs, pack, prim, nbr[0], un, xn, yn, zn);
+#pragma unroll
+        for (int j = 0; j < MAXF; j++) {
+            if (j < nf) {
+                double cu[5];
+#pragma unroll
+                for (int v = 0; v < 5; v++) cu[v] = un[v];
+                const double cx = xn, cy = yn, cz = zn;
+                if (j + 1 < MAXF && j + 1 < nf)
+                    load_cell<PACKED>(m, vs, pack, prim, nbr[j + 1 < MAXF ? j + 1 : j], un, xn, yn,
+                                      zn);
+                accumulate(cu, cx, cy, cz);
+            }
+        }
+    } else {
+        for (int j = 0; j < nf; j++) {
+            const int n = m.ell_nbr[(size_t)j * N + i];
+            double un[5], xn, yn, zn;
+            load_cell<PACKED>(m, vs, pack, prim, n, un, xn, yn, zn);
+            accumulate(un, xn, yn, zn);
         }
     }
 #pragma unroll
@@ -1789,6 +1817,7 @@ struct ibis_residual {
     int opt_offsets = 0;     // option precomputed_offsets
     int opt_pack = 1;        // option pack_cells
     int opt_stage = 1;       // option stage_offsets
+    int opt_cell_prefetch = 0;  // option cell_prefetch (layout 2, pack_cells 0)
     int opt_face_kernel = 0; // option face_kernel: 0 tile face lists, 1 the same with
                              // per-entry face data, 64/128/256 slot-major
     int* ent_idx = nullptr;      // per tile-list entry face data (face_kernel 1)
@@ -2067,9 +2096,14 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
         return;
     }
     if (!r->opt_pack && LAYOUT == 2) {
-        k_cell_fs<DIM, LIM, 2, OFFS, double, false><<<xcd_grid(nlog), BLOCK, shm, st>>>(
-            D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR, r->offs,
-            prim, stage);
+        if (r->opt_cell_prefetch)
+            k_cell_fs<DIM, LIM, 2, OFFS, double, false, true><<<xcd_grid(nlog), BLOCK, shm, st>>>(
+                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR,
+                r->offs, prim, stage);
+        else
+            k_cell_fs<DIM, LIM, 2, OFFS, double, false><<<xcd_grid(nlog), BLOCK, shm, st>>>(
+                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR,
+                r->offs, prim, stage);
         return;
     }
     k_cell_fs<DIM, LIM, LAYOUT, OFFS><<<xcd_grid(nlog), BLOCK, shm, st>>>(
@@ -2602,6 +2636,10 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
     }
     if (k == "stage_offsets") {
         r->opt_stage = value ? 1 : 0;
+        return IBIS_OK;
+    }
+    if (k == "cell_prefetch") {
+        r->opt_cell_prefetch = value ? 1 : 0;
         return IBIS_OK;
     }
     return ibis::fail(IBIS_ERR_INVALID, ("unknown option " + k).c_str());

@@ -14,8 +14,13 @@ from . import build as _build
 
 MAX_TAGS, MAX_ACT, NPAR = 16, 4, 8
 NKERNELS = 8
-KERNEL_NAMES = ("k_bc", "k_cell_grad", "k_face_flux", "k_post_flux", "k_bc_visc",
-                "k_cell_grad_visc", "k_visc_flux", "k_integral")
+# timing slots of ibis_residual_kernel_times.  On the fused second-order
+# path: boundary = k_bc (+ k_pack) + k_ghost_fs, cell_stencil = k_cell_fs
+# (LSQ gradients, limiter, face values), face_flux = k_face_*_fs (flux and
+# surface integral).  On the general path: k_bc, k_cell_grad_var,
+# k_face_flux, k_post_flux, the viscous kernels and k_integral.
+KERNEL_NAMES = ("boundary", "cell_stencil", "face_flux", "post_flux", "boundary_viscous",
+                "cell_gradient_viscous", "viscous_flux", "surface_integral")
 
 HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
                       "ibis_gpu.h")

@@ -280,6 +280,9 @@ ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** lim
  *                            limiter pass in LDS for the reconstruction
  *                            pass (limited schemes, at most 10 faces/cell)
  *                         0: both passes read them from memory
+ *   "cell_prefetch"       1: the cell kernel loads all neighbour indices
+ *                            first and each neighbour's values while the
+ *                            previous one is summed (layout 2, pack_cells 0)
  *   "freeze_limiter"      1: the tangent (JVP) treats the limiter values
  *                            as constants; 0 (default) differentiates them
  * Setting precomputed_offsets allocates and synchronises once. */
@@ -287,9 +290,14 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
 
 /* Per-kernel timing with HIP events recorded around every launch of
  * ibis_compute_dudt on its stream.  Kernel slots:
- *   0 k_bc (pre-reconstruction)  1 k_cell_grad  2 k_face_flux  3 k_post_flux
- *   4 k_bc (pre-viscous)         5 k_cell_grad (viscous)  6 k_visc_flux
- *   7 k_integral
+ *   0 boundary actions before reconstruction (k_bc; on the fused path also
+ *     k_pack and k_ghost_fs)
+ *   1 cell stencil: k_cell_grad_var, or k_cell_fs on the fused path
+ *     (gradients, limiter and face values)
+ *   2 face flux: k_face_flux, or k_face_*_fs on the fused path (flux and
+ *     surface integral)
+ *   3 k_post_flux   4 k_bc (pre-viscous)   5 k_cell_grad_var (viscous)
+ *   6 k_visc_flux   7 k_integral
  * ibis_residual_kernel_times synchronises, adds the elapsed times of all
  * completed launches into total_ms[slot] / counts[slot] (IBIS_NKERNELS
  * entries each) and, with reset != 0, clears the accumulators. */
