@@ -29,6 +29,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from ibis_amd.autotune import coordinate_descent, tune_residual  # noqa: E402
+
 HBM_PEAK_GBS = 8000.0
 
 
@@ -134,51 +136,11 @@ def adjoint_leg(steps):
     return out
 
 
-# implementation options of the residual, each with its candidate values
-# (the first is the starting point); every choice gives the same bits
-OPTION_SPACE = (("face_state_layout", (2, 1, 0)), ("precomputed_offsets", (1, 0)),
-                ("pack_cells", (0, 1)), ("face_kernel", (1, 0, 64, 128, 256)),
-                ("stage_offsets", (1, 0)), ("cell_prefetch", (0, 1)))
-SHORT = {"face_state_layout": "layout", "precomputed_offsets": "offsets", "pack_cells": "pack",
-         "face_kernel": "fk", "stage_offsets": "stage", "cell_prefetch": "pf"}
-
-
-def autotune(set_option, measure, fixed, log):
-    """Coordinate descent over OPTION_SPACE: two sweeps, each trying every
-    value of one option with the others at their best so far.  `measure`
-    times the workload under the options set; `fixed` pins options.  Returns
-    the best options, their time and every measured variant."""
-    cur = {k: fixed.get(k, vals[0]) for k, vals in OPTION_SPACE}
-    seen = {}
-
-    def score(opt):
-        key = "_".join(f"{SHORT[k]}{opt[k]}" for k, _ in OPTION_SPACE)
-        if key not in seen:
-            for k, v in opt.items():
-                set_option(k, v)
-            seen[key] = measure()
-            log(key, seen[key])
-        return seen[key]
-
-    best = score(cur)
-    for _ in range(2):
-        for k, vals in OPTION_SPACE:
-            if k in fixed:
-                continue
-            for v in vals:
-                cand = dict(cur, **{k: v})
-                t = score(cand)
-                if t < best:
-                    best, cur = t, cand
-    for k, v in cur.items():
-        set_option(k, v)
-    return cur, best, seen
-
-
 def batch_leg(nsteps, n_inst=64):
     """64 seeded wedge/ramp instances (10k-400k cells, seed 7), ssp-rk3 at
     CFL 0.5: one at a time on the plain path vs all together in one HIP
-    graph per step.  Both do one untimed step first."""
+    graph per step, both with the options autotuned for the batch.  Both do
+    one untimed step first."""
     import torch
     from ibis_amd import cases
     from ibis_amd.batch import Batch
@@ -190,30 +152,13 @@ def batch_leg(nsteps, n_inst=64):
     print(f"[bench] batch of {bt.n}: {bt.N} cells, grids built in {time.perf_counter() - t0:.1f}s",
           file=sys.stderr, flush=True)
     prims = [cases.uniform_prim(f, b.N, b.Nt) for (g, c, f), b in zip(inst, bt.blocks)]
-    # one at a time
-    seq = []
-    for (g, c, f), b, p in zip(inst, bt.blocks, prims):
-        fv = FiniteVolume(b, c)
-        dp = torch.from_numpy(p).to("cuda")
-        dc = torch.zeros((b.dim + 2, b.Nt), dtype=torch.float64, device="cuda")
-        fv.prim_to_cons(dp, dc)
-        rk = RungeKutta(fv, "ssp-rk3", cfl=0.5, max_time=1e30)
-        rk.take_step(dc, dp)
-        seq.append((fv, rk, dc, dp))
-    torch.cuda.synchronize()
-    w0 = time.perf_counter()
-    for fv, rk, dc, dp in seq:
-        for _ in range(nsteps):
-            rk.take_step(dc, dp)
-    torch.cuda.synchronize()
-    t_seq = time.perf_counter() - w0
-    # batched
     d_prim = torch.from_numpy(bt.gather(prims)).to("cuda")
     d_cons = torch.zeros((bt.ncons, bt.Nt), dtype=torch.float64, device="cuda")
     from ibis_amd.native import check, lib, ptr
     check(lib().ibis_prim_to_cons(bt.residual_handle(), ptr(d_prim), ptr(d_cons), None),
           "ibis_prim_to_cons")
     torch.cuda.synchronize()
+
     # autotune the implementation options of the batch's residual (all give
     # the same bits) on a throwaway copy
     def measure():
@@ -224,7 +169,37 @@ def batch_leg(nsteps, n_inst=64):
         bt.rk_steps(2, use_graph=True)
         return (time.perf_counter() - w) / 2
 
-    best_opt, _, tune = autotune(bt.set_option, measure, {}, lambda key, t: None)
+    best_opt, _, tune = coordinate_descent(bt.set_option, measure, {}, lambda key, t: None)
+
+    # one at a time on the plain path: each instance's own FiniteVolume and
+    # RungeKutta, with the batch's options (and, for reference, with the
+    # library defaults)
+    def sequential(options):
+        seq = []
+        for (g, c, f), b, p in zip(inst, bt.blocks, prims):
+            fv = FiniteVolume(b, c)
+            for k, v in options.items():
+                fv.set_option(k, v)
+            dp = torch.from_numpy(p).to("cuda")
+            dc = torch.zeros((b.dim + 2, b.Nt), dtype=torch.float64, device="cuda")
+            fv.prim_to_cons(dp, dc)
+            rk = RungeKutta(fv, "ssp-rk3", cfl=0.5, max_time=1e30)
+            rk.take_step(dc, dp)
+            seq.append((fv, rk, dc, dp))
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        for fv, rk, dc, dp in seq:
+            for _ in range(nsteps):
+                rk.take_step(dc, dp)
+        torch.cuda.synchronize()
+        return time.perf_counter() - w0, seq
+
+    t_seq_default, seq_default = sequential({})
+    for fv, rk, dc, dp in seq_default:
+        fv.close()
+    del seq_default
+    t_seq, seq = sequential(best_opt)
+
     bt.rk_init(d_cons, d_prim, cfl=0.5, max_time=1e30)
     bt.rk_steps(1, use_graph=True)
     w0 = time.perf_counter()
@@ -237,6 +212,8 @@ def batch_leg(nsteps, n_inst=64):
     out = {"instances": bt.n, "cells": int(bt.N), "steps": nsteps,
            "sequential_s": round(t_seq, 4), "batched_s": round(t_bat, 4),
            "speedup": round(t_seq / t_bat, 3), "identical_fp64": bool(identical),
+           "sequential_default_options_s": round(t_seq_default, 4),
+           "speedup_vs_default_options": round(t_seq_default / t_bat, 3),
            "autotune_step_s": {k: round(v, 5) for k, v in tune.items()}, "options": best_opt}
     print(f"[bench] batch: {out}", file=sys.stderr, flush=True)
     for fv, rk, dc, dp in seq:
@@ -330,21 +307,10 @@ def main():
                                ("stage_offsets", args.stage),
                                ("cell_prefetch", args.cell_prefetch)) if v is not None}
 
-    def measure():
-        for _ in range(2):
-            fv.compute_dudt(prim, gb, dudt)
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        for _ in range(5):
-            fv.compute_dudt(prim, gb, dudt)
-        b.record(stream)
-        torch.cuda.synchronize()
-        return a.elapsed_time(b) * 1e3 / 5
-
-    best_opt, _, variants = autotune(
-        fv.set_option, measure, fixed,
-        lambda key, us: print(f"[bench] variant {key}: {us:.1f} us/residual", file=sys.stderr,
-                              flush=True))
+    best_opt, _, variants = tune_residual(
+        fv, prim, gb, dudt, fixed=fixed,
+        log=lambda key, us: print(f"[bench] variant {key}: {us:.1f} us/residual",
+                                  file=sys.stderr, flush=True))
     variants = {k: round(v, 1) for k, v in variants.items()}
     best = (None, best_opt)
 

@@ -1,0 +1,36 @@
+"""The coordinate-descent autotuner (host logic only, no GPU)."""
+from ibis_amd.autotune import OPTION_SPACE, SHORT, coordinate_descent, variant_key
+
+
+def test_finds_the_separable_minimum_and_leaves_it_set():
+    # a cost that is a sum over options: coordinate descent finds its minimum
+    best_val = {"face_state_layout": 1, "precomputed_offsets": 0, "pack_cells": 1,
+                "face_kernel": 128, "stage_offsets": 0, "cell_prefetch": 1}
+    state = {}
+
+    def measure():
+        return sum(0 if state[k] == v else 1 + i for i, (k, v) in enumerate(best_val.items()))
+
+    opt, t, seen = coordinate_descent(lambda k, v: state.__setitem__(k, v), measure)
+    assert opt == best_val and t == 0
+    assert state == best_val
+    assert variant_key(best_val) in seen
+    # every option was tried at every value at least once
+    for k, vals in OPTION_SPACE:
+        for v in vals:
+            assert any(f"{SHORT[k]}{v}" in key.split("_") for key in seen), (k, v)
+
+
+def test_fixed_options_are_not_changed():
+    state = {}
+    calls = []
+
+    def setter(k, v):
+        if k == "face_kernel":
+            calls.append(v)
+        state[k] = v
+
+    opt, _, _ = coordinate_descent(setter, lambda: float(state["face_state_layout"]),
+                                   fixed={"face_kernel": 64})
+    assert set(calls) == {64}
+    assert opt["face_kernel"] == 64 and opt["face_state_layout"] == 0
