@@ -35,6 +35,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <numeric>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -126,12 +127,11 @@ struct GasParams {
 
 // S is double for the residual and ibis::Dual for its tangent (JVP); the
 // geometry and the action parameters are plain doubles in both.
+// the actions of boundary face b, in the order of its tag's list
 template <class S>
-__global__ void __launch_bounds__(BLOCK) k_bc(DevMesh m, const int* __restrict__ kinds,
-                                              const double* __restrict__ pars, GasParams gp,
-                                              S* __restrict__ prim) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= m.nb) return;
+__device__ __forceinline__ void bc_actions(const DevMesh& m, const int* __restrict__ kinds,
+                                           const double* __restrict__ pars, const GasParams& gp,
+                                           S* __restrict__ prim, int b) {
     const int f = m.b_face[b], gh = m.b_ghost[b], va = m.b_valid[b], t = m.b_tag[b];
     const int Nt = m.Nt, F = m.F;
     S* rho = prim + (size_t)P_RHO * Nt;
@@ -234,6 +234,15 @@ __global__ void __launch_bounds__(BLOCK) k_bc(DevMesh m, const int* __restrict__
                 break;
         }
     }
+}
+
+template <class S>
+__global__ void __launch_bounds__(BLOCK) k_bc(DevMesh m, const int* __restrict__ kinds,
+                                              const double* __restrict__ pars, GasParams gp,
+                                              S* __restrict__ prim) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.nb) return;
+    bc_actions<S>(m, kinds, pars, gp, prim, b);
 }
 
 // ---------------------------------------------------------------------------
@@ -585,12 +594,10 @@ __device__ __forceinline__ size_t soa_index(const DevMesh& m, int c, int slot, i
                    : (size_t)m.maxf * 5 * m.N + (size_t)v * m.G + (c - m.N);
 }
 
-template <int LAYOUT, class W = double>
-__global__ void __launch_bounds__(BLOCK)
-    k_ghost_fs(DevMesh m, VarSet vs, const double* __restrict__ prim, W* __restrict__ FL,
-               W* __restrict__ FR) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= m.nb) return;
+template <int LAYOUT, class W>
+__device__ __forceinline__ void ghost_face_values(const DevMesh& m, const VarSet& vs,
+                                                  const double* __restrict__ prim,
+                                                  W* __restrict__ FL, W* __restrict__ FR, int b) {
     const int f = m.b_face[b], g = m.b_ghost[b];
     const int F = m.F, Nt = m.Nt;
     const double dx = m.geom[9 * (size_t)F + f] - m.cen[g];
@@ -606,6 +613,29 @@ __global__ void __launch_bounds__(BLOCK)
         else
             rec[v] = W(val);
     }
+}
+
+template <int LAYOUT, class W = double>
+__global__ void __launch_bounds__(BLOCK)
+    k_ghost_fs(DevMesh m, VarSet vs, const double* __restrict__ prim, W* __restrict__ FL,
+               W* __restrict__ FR) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.nb) return;
+    ghost_face_values<LAYOUT, W>(m, vs, prim, FL, FR, b);
+}
+
+// k_bc then k_ghost_fs for each boundary face in one thread: the ghost's
+// face values are computed from the state its actions have just written
+// (every ghost belongs to exactly one boundary face)
+template <int LAYOUT, class W = double>
+__global__ void __launch_bounds__(BLOCK)
+    k_bc_ghost_fs(DevMesh m, const int* __restrict__ kinds, const double* __restrict__ pars,
+                  GasParams gp, VarSet vs, double* __restrict__ prim, W* __restrict__ FL,
+                  W* __restrict__ FR) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.nb) return;
+    bc_actions<double>(m, kinds, pars, gp, prim, b);
+    ghost_face_values<LAYOUT, W>(m, vs, prim, FL, FR, b);
 }
 
 // Offsets x_face - x_cell per (cell, face slot), column-major like the other
@@ -1602,6 +1632,7 @@ extern "C" const char* ibis_last_error(void) { return g_last_error.c_str(); }
 struct ibis_mesh {
     DevMesh d;
     int ntags = 0;
+    std::vector<int> b_in_tag_order;  // device index of the k-th boundary face in tag order
     std::vector<void*> allocs;
 };
 
@@ -1676,8 +1707,32 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
                 return ibis::fail(IBIS_ERR_INVALID, "a ghost cell on two boundary faces");
             b_of_ghost[b_ghost[k] - N] = (int)k;
         }
+    // device order of the boundary faces: by ghost id, so that neighbouring
+    // threads of the boundary kernels write neighbouring ghost rows and read
+    // nearby valid cells (the locality renumbering places each ghost after
+    // its valid neighbour); b_in_tag_order keeps the tag order for sums
+    std::vector<int> ord(nb), b_in_tag_order(nb);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::sort(ord.begin(), ord.end(), [&](int a, int b) { return b_ghost[a] < b_ghost[b]; });
+    {
+        std::vector<int> pf(nb), pg(nb), pv(nb), pt(nb);
+        for (int64_t i = 0; i < nb; i++) {
+            const int k = ord[i];
+            pf[i] = b_face[k];
+            pg[i] = b_ghost[k];
+            pv[i] = b_valid[k];
+            pt[i] = b_tag[k];
+            b_of_ghost[pg[i] - N] = (int)i;
+            b_in_tag_order[k] = (int)i;
+        }
+        b_face.swap(pf);
+        b_ghost.swap(pg);
+        b_valid.swap(pv);
+        b_tag.swap(pt);
+    }
     ibis_mesh* m = new ibis_mesh();
     m->ntags = d->ntags;
+    m->b_in_tag_order = b_in_tag_order;
     DevMesh& D = m->d;
     D.dim = d->dim;
     D.nl = d->dim == 2 ? 3 : 6;
@@ -2395,28 +2450,44 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
                           "flux-override boundaries");
     if (fused) {
         const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
-        if (r->has_pre) {
+        if (r->has_pre && !recon) {
             pf.beg();
             k_bc<double><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, prim);
             pf.end(0);
         }
         if (recon) {
-            // face-state path: pack, ghost face states, cell reconstruction,
-            // fused face flux + integral
+            // face-state path: boundary actions + ghost face states, pack,
+            // cell reconstruction, fused face flux + integral
             pf.beg();
-            const bool packed = r->opt_pack || r->opt_layout != 2 || r->prec == IBIS_PREC_FP32_FLUX;
-            if (packed) k_pack<<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, vs, prim, r->pack);
             if (D.nb > 0) {
+                const unsigned g = nblocks(D.nb);
+                const int* kd = r->kinds;
+                const double* pr = r->pars;
                 if (r->prec == IBIS_PREC_FP32_FLUX) {
                     float* fl = reinterpret_cast<float*>(r->FL);
-                    k_ghost_fs<2, float><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, fl, fl);
-                } else if (r->opt_layout == 0)
-                    k_ghost_fs<0><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
-                else if (r->opt_layout == 1)
-                    k_ghost_fs<1><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
-                else
-                    k_ghost_fs<2><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
+                    if (r->has_pre)
+                        k_bc_ghost_fs<2, float><<<g, BLOCK, 0, st>>>(D, kd, pr, gp, vs, prim, fl, fl);
+                    else
+                        k_ghost_fs<2, float><<<g, BLOCK, 0, st>>>(D, vs, prim, fl, fl);
+                } else if (r->opt_layout == 0) {
+                    if (r->has_pre)
+                        k_bc_ghost_fs<0><<<g, BLOCK, 0, st>>>(D, kd, pr, gp, vs, prim, r->FL, r->FR);
+                    else
+                        k_ghost_fs<0><<<g, BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
+                } else if (r->opt_layout == 1) {
+                    if (r->has_pre)
+                        k_bc_ghost_fs<1><<<g, BLOCK, 0, st>>>(D, kd, pr, gp, vs, prim, r->FL, r->FR);
+                    else
+                        k_ghost_fs<1><<<g, BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
+                } else {
+                    if (r->has_pre)
+                        k_bc_ghost_fs<2><<<g, BLOCK, 0, st>>>(D, kd, pr, gp, vs, prim, r->FL, r->FR);
+                    else
+                        k_ghost_fs<2><<<g, BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
+                }
             }
+            const bool packed = r->opt_pack || r->opt_layout != 2 || r->prec == IBIS_PREC_FP32_FLUX;
+            if (packed) k_pack<<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, vs, prim, r->pack);
             pf.end(0);
             const int nlog = (D.N + BLOCK - 1) / BLOCK;
             pf.beg();
@@ -2850,8 +2921,9 @@ ibis_status ibis_wall_force(ibis_residual* r, double* prim, uint64_t force_tags,
     std::vector<double> h(3 * (size_t)D.nb);
     HIP_TRY(hipMemcpyAsync(h.data(), r->adj_force, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    const std::vector<int>& order = r->m->b_in_tag_order;
     for (int k = 0; k < 3; k++)
-        for (int b = 0; b < D.nb; b++) force[k] += h[(size_t)k * D.nb + b];
+        for (int b = 0; b < D.nb; b++) force[k] += h[(size_t)k * D.nb + order[b]];
     return IBIS_OK;
 }
 

@@ -65,6 +65,18 @@ def test_wedge_variants(kw):
     _check(g_io, cfg, prim, exact=True)
 
 
+@pytest.mark.parametrize("layout", [0, 2])
+def test_wedge_subsonic_boundaries(layout):
+    """Subsonic inflow / outflow and slip-wall actions on the second-order
+    inviscid path (boundary actions fused with the ghost face values)."""
+    from ibis_amd import config as C
+    g_io, cfg, fs = cases.wedge_case(48, 24, mach=0.5)
+    cfg.boundaries = {"inflow": C.subsonic_inflow(fs), "outflow": C.subsonic_outflow(fs.p * 0.98),
+                      "wall": C.slip_wall()}
+    prim = cases.perturbed_prim(g_io, fs, g_io.num_cells + 2 * (48 + 24), seed=15)
+    _check(g_io, cfg, prim, exact=True, options={"face_state_layout": layout})
+
+
 def test_wedge_ldfss2_tolerance():
     g_io, cfg, fs = cases.wedge_case(48, 24, flux="ldfss2")
     prim = cases.perturbed_prim(g_io, fs, g_io.num_cells + 2 * (48 + 24), seed=12)
