@@ -344,37 +344,42 @@ def main():
     residual_kernel_us = sum(kernels_us.values())
     dominant = max(kernels_us, key=kernels_us.get)
 
-    # FP32 reconstruction/flux with FP64 accumulation: time and drift per
-    # conserved variable against the FP64 residual (itself equal to the oracle)
+    # reduced precision with FP64 accumulation (fp32_flux: float face values
+    # and flux; fp32_recon: also gradients, limiter and reconstruction in
+    # float): time and drift per conserved variable against the FP64
+    # residual (itself equal to the oracle)
     fp32 = None
     if not args.no_fp32 and args.workload != "plate":
         fv.compute_dudt(prim, gb, dudt)
         d64 = dudt.clone()
-        fv.set_precision("fp32_flux")
-        d32 = torch.zeros_like(dudt)
-        for _ in range(2):
-            fv.compute_dudt(prim, gb, d32)
-        n32 = max(5, args.steps // 2)
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        for _ in range(n32):
-            fv.compute_dudt(prim, gb, d32)
-        b.record(stream)
-        torch.cuda.synchronize()
-        fv.set_precision("fp64")
-        us32 = a.elapsed_time(b) * 1e3 / n32
         names = (["mass", "momentum_x", "momentum_y", "momentum_z", "energy"] if gb.dim == 3
                  else ["mass", "momentum_x", "momentum_y", "energy"])
-        x32, x64 = d32[:, :gb.N].double(), d64[:, :gb.N]
-        drift = {}
-        for q, nm in enumerate(names):
-            diff = x32[q] - x64[q]
-            drift[nm] = {"max_rel": float(diff.abs().max() / x64[q].abs().max()),
-                         "rms_rel": float(diff.pow(2).mean().sqrt() / x64[q].pow(2).mean().sqrt())}
-        fp32 = {"us_per_residual": round(us32, 1), "Mcells_per_s": round(gb.N / us32, 2),
-                "speedup_vs_fp64": round(ms_per_step * 1e3 / us32, 3), "drift_vs_fp64": drift}
-        print(f"[bench] fp32_flux: {us32:.1f} us/residual, drift {drift}", file=sys.stderr,
-              flush=True)
+        fp32 = {}
+        for mode in ("fp32_flux", "fp32_recon"):
+            fv.set_precision(mode)
+            d32 = torch.zeros_like(dudt)
+            for _ in range(2):
+                fv.compute_dudt(prim, gb, d32)
+            n32 = max(5, args.steps // 2)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            for _ in range(n32):
+                fv.compute_dudt(prim, gb, d32)
+            b.record(stream)
+            torch.cuda.synchronize()
+            fv.set_precision("fp64")
+            us32 = a.elapsed_time(b) * 1e3 / n32
+            x32, x64 = d32[:, :gb.N].double(), d64[:, :gb.N]
+            drift = {}
+            for q, nm in enumerate(names):
+                diff = x32[q] - x64[q]
+                drift[nm] = {"max_rel": float(diff.abs().max() / x64[q].abs().max()),
+                             "rms_rel": float(diff.pow(2).mean().sqrt() / x64[q].pow(2).mean().sqrt())}
+            fp32[mode] = {"us_per_residual": round(us32, 1), "Mcells_per_s": round(gb.N / us32, 2),
+                          "speedup_vs_fp64": round(ms_per_step * 1e3 / us32, 3),
+                          "drift_vs_fp64": drift}
+            print(f"[bench] {mode}: {us32:.1f} us/residual, drift {drift}", file=sys.stderr,
+                  flush=True)
 
     adj = None if args.no_adjoint else adjoint_leg(max(10, args.steps // 2))
     bat = None if args.no_batch else batch_leg(args.batch_steps)
@@ -430,7 +435,7 @@ def main():
                      "kernels_us": {k: round(v, 2) for k, v in kernels_us.items()},
                      "sum_kernels_us": round(residual_kernel_us, 2)},
         "cpu_baseline": cpu,
-        "fp32_flux": fp32,
+        "fp32": fp32,
         "adjoint": adj,
         "batch": bat,
     }

@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(BLOCK)
             }
             double eps2 = 0.0;
             if (LIM == 2) {
-                const double V = m.vol[i];
+                const double V = fabs(m.vol[i]);  // cell size h^dim = |V|
                 const double K3 = venkat_K * venkat_K * venkat_K;
                 eps2 = (DIM == 3) ? K3 * V : K3 * (V * sqrt(V));
             }

@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(320)
     S phi = S(1.0);
     double eps2 = 0.0;
     if (LIM == 2) {
-        const double V = m.vol[i];
+        const double V = fabs(m.vol[i]);  // cell size h^dim = |V|
         const double K3 = venkat_K * venkat_K * venkat_K;
         eps2 = (DIM == 3) ? K3 * V : K3 * (V * sqrt(V));
     }
@@ -797,7 +797,7 @@ __global__ void __launch_bounds__(BLOCK)
     if (LIM != 0) {
         double eps2 = 0.0;
         if (LIM == 2) {
-            const double V = m.vol[i];
+            const double V = fabs(m.vol[i]);  // cell size h^dim = |V|
             const double K3 = venkat_K * venkat_K * venkat_K;
             eps2 = (DIM == 3) ? K3 * V : K3 * (V * sqrt(V));
         }
@@ -874,6 +874,144 @@ __global__ void __launch_bounds__(BLOCK)
                 __builtin_nontemporal_store(W(val), FL + ((size_t)j * 5 + v) * N + i);
             else
                 rec[v] = W(val);
+        }
+    }
+}
+
+// Cell kernel of the FP32 reconstruction mode (layout 2, float face values):
+// the steps of k_cell_fs -- LSQ gradient, limiter, MUSCL value on each face
+// -- in float arithmetic.  Differences of the FP64 inputs (centroid offsets,
+// value differences) are formed in FP64 and rounded once; the extremes are
+// kept as differences from the cell value, so the limiter's d1 is one
+// rounding away from the FP64 one.  stage != 0 keeps the face offsets of
+// the limiter pass in LDS as floats for the reconstruction pass.
+template <int DIM, int LIM, bool OFFS>
+__global__ void __launch_bounds__(BLOCK)
+    k_cell_fs32(DevMesh m, VarSet vs, const double* __restrict__ prim,
+                const double* __restrict__ grad, double eps, double venkat_K, int nlogical,
+                float* __restrict__ FL, const double* __restrict__ offs, int stage) {
+    extern __shared__ float soff32[];
+    const int lb = xcd_block(nlogical);
+    if (lb >= nlogical) return;
+    const int i = lb * BLOCK + threadIdx.x;
+    const int N = m.N, F = m.F;
+    if (i >= N) return;
+    constexpr int NG = DIM == 3 ? 5 : 4;
+    double ui[5];
+    double xi, yi, zi;
+    load_cell<false>(m, vs, nullptr, prim, i, ui, xi, yi, zi);
+    const int nf = m.nfaces[i];
+    const float r11 = (float)m.lsq[i], r12 = (float)m.lsq[N + i],
+                r22 = (float)m.lsq[2 * (size_t)N + i];
+    float r13 = 0.0f, r23 = 0.0f, r33 = 0.0f;
+    if (DIM == 3) {
+        r13 = (float)m.lsq[3 * (size_t)N + i];
+        r23 = (float)m.lsq[4 * (size_t)N + i];
+        r33 = (float)m.lsq[5 * (size_t)N + i];
+    }
+    const float beta = (r12 * r23 - r13 * r23) / (r11 * r22);  // GR/gradient.h:89 (sic)
+    const float r11s = r11 * r11;
+    const float i22 = 1.0f / (r22 * r22);
+    const float q12 = r12 / r11;
+    const float i33 = 1.0f / (r33 * r33);
+    const float q23 = r23 / r22;
+    float gx[5], gy[5], gz[5], dmin[5], dmax[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) gx[v] = gy[v] = gz[v] = dmin[v] = dmax[v] = 0.0f;
+    for (int j = 0; j < nf; j++) {
+        const int n = m.ell_nbr[(size_t)j * N + i];
+        double un[5], xn, yn, zn;
+        load_cell<false>(m, vs, nullptr, prim, n, un, xn, yn, zn);
+        const float dx = (float)(xn - xi), dy = (float)(yn - yi), dz = (float)(zn - zi);
+        const float alpha_1 = dx / r11s;
+        const float alpha_2 = i22 * (dy - q12 * dx);
+        float alpha_3 = 0.0f;
+        if (DIM == 3) alpha_3 = i33 * (dz - q23 * dy + beta * dx);
+        const float w_1 = alpha_1 - q12 * alpha_2 + beta * alpha_3;
+        const float w_2 = alpha_2 - q23 * alpha_3;
+        const float w_3 = alpha_3;
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            const float diff = (float)(un[v] - ui[v]);
+            if (v < NG) {
+                gx[v] += w_1 * diff;
+                gy[v] += w_2 * diff;
+                gz[v] += w_3 * diff;
+            }
+            dmin[v] = diff < dmin[v] ? diff : dmin[v];
+            dmax[v] = dmax[v] < diff ? diff : dmax[v];
+        }
+    }
+#pragma unroll
+    for (int v = NG; v < 5; v++) {  // 2D vz: the gradient stored in its slot
+        const double* g = grad + (size_t)vs.slot[v] * 3 * N;
+        gx[v] = (float)g[i];
+        gy[v] = (float)g[N + i];
+        gz[v] = (float)g[2 * (size_t)N + i];
+    }
+    auto offset = [&](int j, float& dx, float& dy, float& dz) {
+        if (OFFS) {
+            dx = (float)offs[(size_t)(j * 3 + 0) * N + i];
+            dy = (float)offs[(size_t)(j * 3 + 1) * N + i];
+            dz = (float)offs[(size_t)(j * 3 + 2) * N + i];
+        } else {
+            const int f = m.ell_face[(size_t)j * N + i];
+            dx = (float)(m.geom[9 * (size_t)F + f] - xi);
+            dy = (float)(m.geom[10 * (size_t)F + f] - yi);
+            dz = (float)(m.geom[11 * (size_t)F + f] - zi);
+        }
+    };
+    float phi[5] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f};
+    if (LIM != 0) {
+        float eps2 = 0.0f;
+        if (LIM == 2) {
+            const double V = fabs(m.vol[i]);  // cell size h^dim = |V|
+            const double K3 = venkat_K * venkat_K * venkat_K;
+            eps2 = (float)((DIM == 3) ? K3 * V : K3 * (V * sqrt(V)));
+        }
+        const float epsf = (float)eps;
+        for (int j = 0; j < nf; j++) {
+            float dx, dy, dz;
+            offset(j, dx, dy, dz);
+            if (stage) {
+                soff32[(j * 3 + 0) * BLOCK + threadIdx.x] = dx;
+                soff32[(j * 3 + 1) * BLOCK + threadIdx.x] = dy;
+                soff32[(j * 3 + 2) * BLOCK + threadIdx.x] = dz;
+            }
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                float d2 = gx[v] * dx + gy[v] * dy + gz[v] * dz;
+                if (LIM == 1) {  // Barth-Jespersen, FV/limiter.cpp:289-303
+                    const int sg = (d2 > 0.0f) - (d2 < 0.0f);
+                    d2 = sg * (fabsf(d2) + epsf);
+                    const float q = (sg > 0 ? dmax[v] : dmin[v]) / d2;
+                    if (sg != 0) phi[v] = q < phi[v] ? q : phi[v];
+                } else {  // Venkatakrishnan
+                    if (d2 == 0.0f) continue;
+                    const float d1 = d2 > 0.0f ? dmax[v] : dmin[v];
+                    const float d1s = d1 * d1;
+                    const float d2s = d2 * d2;
+                    const float num = d1s + eps2 + 2.0f * (d1 * d2);
+                    const float den = d1s + 2.0f * d2s + d1 * d2 + eps2;
+                    const float q = num / den;
+                    phi[v] = q < phi[v] ? q : phi[v];
+                }
+            }
+        }
+    }
+    for (int j = 0; j < nf; j++) {
+        float dx, dy, dz;
+        if (LIM != 0 && stage) {
+            dx = soff32[(j * 3 + 0) * BLOCK + threadIdx.x];
+            dy = soff32[(j * 3 + 1) * BLOCK + threadIdx.x];
+            dz = soff32[(j * 3 + 2) * BLOCK + threadIdx.x];
+        } else {
+            offset(j, dx, dy, dz);
+        }
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            const float val = (float)ui[v] + phi[v] * (gx[v] * dx + gy[v] * dy + gz[v] * dz);
+            __builtin_nontemporal_store(val, FL + ((size_t)j * 5 + v) * N + i);
         }
     }
 }
@@ -2143,9 +2281,15 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
                       int nlog, const double* prim, hipStream_t st) {
     const size_t shm = cell_stage_bytes(D, LIM, r->opt_stage);
     const int stage = shm ? 1 : 0;
+    if (r->prec == IBIS_PREC_FP32_RECON) {
+        float* fl = reinterpret_cast<float*>(r->FL);
+        k_cell_fs32<DIM, LIM, OFFS><<<xcd_grid(nlog), BLOCK, shm / 2, st>>>(
+            D, vs, prim, r->grad, d.lim_eps, d.venkat_K, nlog, fl, r->offs, stage);
+        return;
+    }
     if (r->prec == IBIS_PREC_FP32_FLUX) {
         float* fl = reinterpret_cast<float*>(r->FL);
-        k_cell_fs<DIM, LIM, 2, OFFS, float><<<xcd_grid(nlog), BLOCK, shm, st>>>(
+        k_cell_fs<DIM, LIM, 2, OFFS, float, false><<<xcd_grid(nlog), BLOCK, shm, st>>>(
             D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, fl, fl, r->offs, prim,
             stage);
         return;
@@ -2444,7 +2588,7 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
     VarSet vs;
     residual_params(r, &fp, &vs);
     const bool fused = !d.viscous && !r->has_post;
-    if (r->prec == IBIS_PREC_FP32_FLUX && !(fused && recon))
+    if (r->prec != IBIS_PREC_FP64 && !(fused && recon))
         return ibis::fail(IBIS_ERR_INVALID,
                           "FP32 flux mode needs the inviscid second-order residual without "
                           "flux-override boundaries");
@@ -2463,7 +2607,7 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
                 const unsigned g = nblocks(D.nb);
                 const int* kd = r->kinds;
                 const double* pr = r->pars;
-                if (r->prec == IBIS_PREC_FP32_FLUX) {
+                if (r->prec != IBIS_PREC_FP64) {
                     float* fl = reinterpret_cast<float*>(r->FL);
                     if (r->has_pre)
                         k_bc_ghost_fs<2, float><<<g, BLOCK, 0, st>>>(D, kd, pr, gp, vs, prim, fl, fl);
@@ -2486,7 +2630,8 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
                         k_ghost_fs<2><<<g, BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
                 }
             }
-            const bool packed = r->opt_pack || r->opt_layout != 2 || r->prec == IBIS_PREC_FP32_FLUX;
+            const bool packed =
+                r->prec == IBIS_PREC_FP64 && (r->opt_pack || r->opt_layout != 2);
             if (packed) k_pack<<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, vs, prim, r->pack);
             pf.end(0);
             const int nlog = (D.N + BLOCK - 1) / BLOCK;
@@ -2497,7 +2642,7 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
                 launch_cell_fs<2>(D, vs, d, r, nlog, prim, st);
             pf.end(1);
             pf.beg();
-            const int lay = r->prec == IBIS_PREC_FP32_FLUX ? 3 : r->opt_layout;
+            const int lay = r->prec != IBIS_PREC_FP64 ? 3 : r->opt_layout;
             const int fk = (lay >= 2) ? r->opt_face_kernel : 0;
             if (D.dim == 3)
                 launch_face_fs<3>(D, fp, d.flux, lay, fk, r->ent_idx, r->ent_geom, r->FL, r->FR,
@@ -2640,7 +2785,7 @@ ibis_status ibis_cons_to_prim_jvp(ibis_residual* r, const double* cons, const do
 
 ibis_status ibis_residual_set_precision(ibis_residual* r, int mode) {
     if (!r) return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_set_precision: null residual");
-    if (mode != IBIS_PREC_FP64 && mode != IBIS_PREC_FP32_FLUX)
+    if (mode != IBIS_PREC_FP64 && mode != IBIS_PREC_FP32_FLUX && mode != IBIS_PREC_FP32_RECON)
         return ibis::fail(IBIS_ERR_INVALID, "Unknown precision mode");
     r->prec = mode;
     return IBIS_OK;

@@ -311,9 +311,10 @@ class FiniteVolume:
         return g.value, lm.value, fx.value, bool(pa.value)
 
     def set_precision(self, mode: str):
-        """"fp64" (default) or "fp32_flux": float face values, thermo closure
-        and flux calculator with FP64 accumulation."""
-        modes = {"fp64": 0, "fp32_flux": 1}
+        """"fp64" (default); "fp32_flux": float face values, thermo closure
+        and flux calculator with FP64 accumulation; "fp32_recon": also the
+        gradients, limiter and reconstruction in float."""
+        modes = {"fp64": 0, "fp32_flux": 1, "fp32_recon": 2}
         if mode not in modes:
             raise IbisError(f"unknown precision mode {mode!r}; expected one of {sorted(modes)}")
         check(lib().ibis_residual_set_precision(self._h, modes[mode]), "ibis_residual_set_precision")

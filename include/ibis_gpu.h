@@ -136,7 +136,7 @@ enum {
     IBIS_BC_SUBSONIC_OUTFLOW = 7,/* par: p */
     IBIS_BC_CONSTANT_FLUX = 8    /* post-convective flux action; par as flow_state_copy */
 };
-enum { IBIS_PREC_FP64 = 0, IBIS_PREC_FP32_FLUX = 1 };
+enum { IBIS_PREC_FP64 = 0, IBIS_PREC_FP32_FLUX = 1, IBIS_PREC_FP32_RECON = 2 };
 
 typedef struct {
     int32_t flux, order, thermo, limiter, viscous, ntags;
@@ -161,9 +161,12 @@ void ibis_residual_destroy(ibis_residual* r);
  * values as float and evaluates the thermo closure, frame rotation and flux
  * calculator in float; gradients, limiters, the face-to-cell sum, the
  * division by the volume, estimate_dt and every solver reduction stay FP64.
- * Supported on the inviscid second-order residual without flux-override
- * boundaries (ibis_compute_dudt fails otherwise).  No counterpart in the
- * reference (its Ibis::real is double). */
+ * IBIS_PREC_FP32_RECON also computes the LSQ gradients, the limiter and the
+ * reconstructed values in float (differences of the FP64 inputs are formed
+ * in FP64 and rounded once); accumulation and reductions stay FP64.
+ * Both are supported on the inviscid second-order residual without
+ * flux-override boundaries (ibis_compute_dudt fails otherwise).  No
+ * counterpart in the reference (its Ibis::real is double). */
 ibis_status ibis_residual_set_precision(ibis_residual* r, int mode);
 
 /* dim, n_cells, n_total, n_faces, n_ghost of the residual's mesh. */

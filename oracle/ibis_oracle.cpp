@@ -828,11 +828,13 @@ struct FV {
                 }
             } else {
                 // Venkatakrishnan (1995) limiter.  Not in the reference; the
-                // formulation is this project's: eps^2 = K^3 * h^3 with
-                // h^dim = V, i.e. (K*K*K)*V in 3D and (K*K*K)*(V*sqrt(V)) in
-                // 2D, per-face phi_f = (d1^2 + eps^2 + 2 d1 d2) /
-                // (d1^2 + 2 d2^2 + d1 d2 + eps^2), phi = min(1, min_f phi_f).
-                double V = G.vol[i];
+                // formulation is this project's: eps^2 = K^3 * h^3 with the
+                // cell size h^dim = |V| (the reference's volume of a cell with
+                // non-planar faces can be negative), i.e. (K*K*K)*|V| in 3D and
+                // (K*K*K)*(|V|*sqrt(|V|)) in 2D, per-face phi_f =
+                // (d1^2 + eps^2 + 2 d1 d2) / (d1^2 + 2 d2^2 + d1 d2 + eps^2),
+                // phi = min(1, min_f phi_f).
+                double V = std::fabs(G.vol[i]);
                 double K3 = K * K * K;
                 double eps2 = (dim == 3) ? K3 * V : K3 * (V * std::sqrt(V));
                 for (int64_t j = G.cf_off[i]; j < G.cf_off[i + 1]; j++) {

@@ -29,14 +29,14 @@ def drift(d32, d64, n):
     return out
 
 
-def _pair(grid_io, cfg, prim_ref):
+def _pair(grid_io, cfg, prim_ref, mode="fp32_flux"):
     import torch
     gb, fv = gpu_objects(grid_io, cfg)
     prim = to_dev(gb.to_grid_order(prim_ref))
     d64 = torch.zeros((gb.dim + 2, gb.Nt), dtype=torch.float64, device="cuda")
     d32 = torch.zeros_like(d64)
     fv.compute_dudt(prim, gb, d64)
-    fv.set_precision("fp32_flux")
+    fv.set_precision(mode)
     fv.compute_dudt(prim, gb, d32)
     # the slot-major face kernel gives the same float results
     fv.set_option("face_kernel", 128)
@@ -57,24 +57,26 @@ def _pair(grid_io, cfg, prim_ref):
     return gb, d32.cpu().numpy(), d64.cpu().numpy()
 
 
+@pytest.mark.parametrize("mode", ["fp32_flux", "fp32_recon"])
 @pytest.mark.parametrize("flux", ["hanel", "ausmdv"])
-def test_hex_fp32_flux_drift(flux):
+def test_hex_fp32_drift(flux, mode):
     g_io, cfg = cases.hex_case(24, flux=flux, limiter="venkatakrishnan")
     prim = cases.hex_prim(g_io, 24 ** 3 + 6 * 24 ** 2)
-    gb, d32, d64 = _pair(g_io, cfg, prim)
+    gb, d32, d64 = _pair(g_io, cfg, prim, mode)
     dr = drift(d32, d64, gb.N)
-    print(f"hex24 {flux} drift (max_rel, rms_rel): {dr}", flush=True)
+    print(f"hex24 {flux} {mode} drift (max_rel, rms_rel): {dr}", flush=True)
     for name, (mx, rms) in dr.items():
         assert 1e-12 < mx < 1e-3, (name, mx)
         assert 1e-12 < rms < 1e-3, (name, rms)
 
 
-def test_wedge_fp32_flux_drift():
+@pytest.mark.parametrize("mode", ["fp32_flux", "fp32_recon"])
+def test_wedge_fp32_drift(mode):
     g_io, cfg, fs = cases.wedge_case(96, 48)
     prim = cases.perturbed_prim(g_io, fs, 96 * 48 + 2 * (96 + 48), seed=61)
-    gb, d32, d64 = _pair(g_io, cfg, prim)
+    gb, d32, d64 = _pair(g_io, cfg, prim, mode)
     dr = drift(d32, d64, gb.N)
-    print(f"wedge drift (max_rel, rms_rel): {dr}", flush=True)
+    print(f"wedge {mode} drift (max_rel, rms_rel): {dr}", flush=True)
     for name, (mx, rms) in dr.items():
         assert mx < 1e-3, (name, mx)
 

@@ -110,3 +110,33 @@ def test_lsq_gradient_of_linear_field(g2):
     grad = g2.gradient(vals)
     assert np.allclose(grad[0], 1.0, rtol=1e-12, atol=0)
     assert np.allclose(grad[1], 0.5, rtol=1e-12, atol=0)
+
+
+def test_hex_volume_formula_is_origin_dependent_for_non_planar_faces():
+    """The reference's cell volume, 1/3 sum_f (x_f . n_f) A_f about the origin
+    with vertex-average face centres, n = t1 x t2 and A = two triangle areas
+    (GR/geom.h:64-85, GR/interface.h:180-323), is exact for planar faces but
+    changes with a translation of the grid when faces are not planar, and on
+    the jittered 160^3 box it is negative for a large share of the cells.
+    The builder reproduces this; the limiter's cell size uses |V|."""
+    import copy
+    from ibis_amd import cases
+    from ibis_amd.finite_volume import GridBlock
+
+    def volumes(g_io, cfg, shift):
+        g = copy.deepcopy(g_io)
+        g.xyz = g_io.xyz + shift
+        gb = GridBlock(g, cfg, reorder=False, device=False)
+        return gb.export()["volume"][:gb.N]
+
+    # planar faces: every cell 1/n^3 wherever the box sits
+    g_io, cfg = cases.hex_case(6, permute=False)
+    flat = copy.deepcopy(g_io)
+    flat.xyz = np.round(g_io.xyz * 6) / 6
+    for shift in (0.0, 10.0):
+        assert np.allclose(volumes(flat, cfg, shift), 1 / 216, rtol=1e-12, atol=0)
+    # jittered (non-planar) faces: the volumes depend on the translation
+    g_io, cfg = cases.hex_case(24, permute=False)
+    v0, v1 = volumes(g_io, cfg, 0.0), volumes(g_io, cfg, 10.0)
+    assert (v0 < 0).any() and (v1 < 0).mean() > (v0 < 0).mean()
+    assert np.abs(v1 - v0).max() > 1 / 24 ** 3
