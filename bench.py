@@ -240,6 +240,8 @@ def main():
                     help="fix the face_kernel option instead of autotuning")
     ap.add_argument("--stage", type=int, default=None, choices=[0, 1],
                     help="fix the stage_offsets option instead of autotuning")
+    ap.add_argument("--stage-cells", type=int, default=None, choices=[0, 1],
+                    help="fix the stage_cells option instead of autotuning")
     ap.add_argument("--cell-prefetch", type=int, default=None, choices=[0, 1],
                     help="fix the cell_prefetch option instead of autotuning")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/primal leg")
@@ -302,7 +304,8 @@ def main():
                                ("precomputed_offsets", args.offsets),
                                ("pack_cells", args.pack), ("face_kernel", args.face_kernel),
                                ("stage_offsets", args.stage),
-                               ("cell_prefetch", args.cell_prefetch)) if v is not None}
+                               ("cell_prefetch", args.cell_prefetch),
+                               ("stage_cells", args.stage_cells)) if v is not None}
     for k, vals in OPTION_SPACE:
         fv.set_option(k, fixed.get(k, vals[0]))
     for _ in range(10):

@@ -2,7 +2,8 @@
 
 Every option of ibis_residual_set_option listed here changes only how the
 work is laid out (face-value layout, precomputed offsets, packing, face
-kernel and its block size, LDS staging, prefetching), never the arithmetic:
+kernel and its block size, LDS staging of offsets or of the cell stencil,
+prefetching), never the arithmetic:
 all choices give the same bits.  The tuner times a caller-supplied workload
 under each candidate and keeps the fastest, by coordinate descent.
 """
@@ -13,9 +14,10 @@ from typing import Callable, Dict, Mapping, Tuple
 # each option with its candidate values; the first is the starting point
 OPTION_SPACE = (("face_state_layout", (2, 1, 0)), ("precomputed_offsets", (1, 0)),
                 ("pack_cells", (0, 1)), ("face_kernel", (1, 0, 64, 128, 256)),
-                ("stage_offsets", (1, 0)), ("cell_prefetch", (0, 1)))
+                ("stage_offsets", (1, 0)), ("cell_prefetch", (0, 1)), ("stage_cells", (1, 0)))
 SHORT = {"face_state_layout": "layout", "precomputed_offsets": "offsets", "pack_cells": "pack",
-         "face_kernel": "fk", "stage_offsets": "stage", "cell_prefetch": "pf"}
+         "face_kernel": "fk", "stage_offsets": "stage", "cell_prefetch": "pf",
+         "stage_cells": "sc"}
 
 
 def variant_key(opt: Mapping[str, int]) -> str:

@@ -95,6 +95,12 @@ struct DevMesh {
     const int* tile_face_off;         // ntiles + 1
     const int* tile_faces;            // face ids, ascending within a tile
     const unsigned short* cell_lds;   // maxf x N: position of a cell's face in its tile list
+    // stencil halo of each tile (neighbours outside it) and, per (cell,
+    // slot), the neighbour's index in the tile's LDS stencil
+    int maxhalo;
+    const int* halo_off;              // ntiles + 1
+    const int* halo_ids;
+    const unsigned short* nbr_lds;    // maxf x N
     // face-state records: slot of face f in the face list of its left/right cell
     const unsigned char* slotL;
     const unsigned char* slotR;
@@ -686,7 +692,7 @@ __device__ __forceinline__ void load_cell(const DevMesh& m, const VarSet& vs,
 }
 
 template <int DIM, int LIM, int LAYOUT, bool OFFS, class W = double, bool PACKED = true,
-          bool PF = false>
+          bool PF = false, bool SN = false>
 __global__ void __launch_bounds__(BLOCK)
     k_cell_fs(DevMesh m, VarSet vs, const double* __restrict__ pack, double* __restrict__ grad,
               double* __restrict__ lim, double eps, double venkat_K, int nlogical,
@@ -694,17 +700,46 @@ __global__ void __launch_bounds__(BLOCK)
               const double* __restrict__ offs, const double* __restrict__ prim, int stage) {
     // stage != 0: the limiter pass keeps each face offset in LDS
     // (maxf x 3 doubles per thread) for the reconstruction pass, so the
-    // offsets are read from memory once
+    // offsets are read from memory once.
+    // SN: the block (one tile of TILE cells) first copies its cells' and its
+    // stencil halo's values and centroids into LDS (8 planes of
+    // TILE + maxhalo doubles) with one load per cell, and the neighbour loop
+    // reads them from there (no offset staging then).
     extern __shared__ double soff[];
     const int lb = xcd_block(nlogical);
     if (lb >= nlogical) return;
     const int i = lb * BLOCK + threadIdx.x;
     const int N = m.N, F = m.F;
+    const int SS = TILE + m.maxhalo;  // plane length of the LDS stencil
+    if (SN) {
+        const int c0 = lb * TILE;
+        const int h0 = m.halo_off[lb], nh = m.halo_off[lb + 1] - h0;
+        for (int k = threadIdx.x; k < TILE + nh; k += BLOCK) {
+            const int c = k < TILE ? c0 + k : m.halo_ids[h0 + k - TILE];
+            if (k < TILE && c >= N) continue;
+            double u[5], x, y, z;
+            load_cell<PACKED>(m, vs, pack, prim, c, u, x, y, z);
+#pragma unroll
+            for (int v = 0; v < 5; v++) soff[v * SS + k] = u[v];
+            soff[5 * SS + k] = x;
+            soff[6 * SS + k] = y;
+            soff[7 * SS + k] = z;
+        }
+        __syncthreads();
+    }
     if (i >= N) return;
     constexpr int NG = DIM == 3 ? 5 : 4;  // variables whose gradient is computed here
     double ui[5];
     double xi, yi, zi;
-    load_cell<PACKED>(m, vs, pack, prim, i, ui, xi, yi, zi);
+    if (SN) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) ui[v] = soff[v * SS + threadIdx.x];
+        xi = soff[5 * SS + threadIdx.x];
+        yi = soff[6 * SS + threadIdx.x];
+        zi = soff[7 * SS + threadIdx.x];
+    } else {
+        load_cell<PACKED>(m, vs, pack, prim, i, ui, xi, yi, zi);
+    }
     const int nf = m.nfaces[i];
     const double r11 = m.lsq[i], r12 = m.lsq[N + i], r22 = m.lsq[2 * (size_t)N + i];
     double r13 = 0.0, r23 = 0.0, r33 = 0.0;
@@ -749,7 +784,15 @@ __global__ void __launch_bounds__(BLOCK)
             umax[v] = ibis::kmax(umax[v], un[v]);
         }
     };
-    if (PF) {
+    if (SN) {
+        for (int j = 0; j < nf; j++) {
+            const int k = m.nbr_lds[(size_t)j * N + i];
+            double un[5];
+#pragma unroll
+            for (int v = 0; v < 5; v++) un[v] = soff[v * SS + k];
+            accumulate(un, soff[5 * SS + k], soff[6 * SS + k], soff[7 * SS + k]);
+        }
+    } else if (PF) {
         // every neighbour index first, then each neighbour's values loaded
         // while the previous neighbour is accumulated (same order of sums)
         int nbr[MAXF];
@@ -813,7 +856,7 @@ __global__ void __launch_bounds__(BLOCK)
                 dy = m.geom[10 * (size_t)F + f] - yi;
                 dz = m.geom[11 * (size_t)F + f] - zi;
             }
-            if (stage) {
+            if (!SN && stage) {
                 soff[(j * 3 + 0) * BLOCK + threadIdx.x] = dx;
                 soff[(j * 3 + 1) * BLOCK + threadIdx.x] = dy;
                 soff[(j * 3 + 2) * BLOCK + threadIdx.x] = dz;
@@ -852,7 +895,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int j = 0; j < nf; j++) {
         const int f = m.ell_face[(size_t)j * N + i];
         double dx, dy, dz;
-        if (LIM != 0 && stage) {
+        if (LIM != 0 && !SN && stage) {
             dx = soff[(j * 3 + 0) * BLOCK + threadIdx.x];
             dy = soff[(j * 3 + 1) * BLOCK + threadIdx.x];
             dz = soff[(j * 3 + 2) * BLOCK + threadIdx.x];
@@ -1925,6 +1968,42 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
             toff[t + 1] = (int)tfaces.size();
         }
     }
+    // stencil halo of each tile: the neighbours (valid or ghost) of its
+    // cells that lie outside it, ascending; for every (cell, face slot) the
+    // neighbour's position in the tile's LDS stencil (own cells first, then
+    // the halo)
+    std::vector<int> hoff(ntiles + 1, 0), hids;
+    std::vector<unsigned short> nlds((size_t)maxf * N, 0);
+    int maxhalo = 0;
+    {
+        std::vector<int> h;
+        for (int t = 0; t < ntiles; t++) {
+            h.clear();
+            const int64_t c0 = (int64_t)t * TILE, c1 = std::min<int64_t>(N, c0 + TILE);
+            for (int64_t c = c0; c < c1; c++)
+                for (int64_t j = d->cell_face_off[c]; j < d->cell_face_off[c + 1]; j++) {
+                    const int64_t n = d->cell_neighbours[j];
+                    if (n < c0 || n >= c1) h.push_back((int)n);
+                }
+            std::sort(h.begin(), h.end());
+            h.erase(std::unique(h.begin(), h.end()), h.end());
+            maxhalo = std::max<int>(maxhalo, (int)h.size());
+            if (TILE + (int64_t)h.size() > 65535)
+                return ibis::fail(IBIS_ERR_INVALID, "ibis_mesh_create: tile stencil too large");
+            for (int64_t c = c0; c < c1; c++) {
+                const int64_t o = d->cell_face_off[c];
+                for (int64_t j = o; j < d->cell_face_off[c + 1]; j++) {
+                    const int64_t n = d->cell_neighbours[j];
+                    const int64_t k = (n >= c0 && n < c1)
+                                          ? n - c0
+                                          : TILE + (std::lower_bound(h.begin(), h.end(), (int)n) - h.begin());
+                    nlds[(size_t)(j - o) * N + c] = (unsigned short)k;
+                }
+            }
+            hids.insert(hids.end(), h.begin(), h.end());
+            hoff[t + 1] = (int)hids.size();
+        }
+    }
     std::vector<unsigned char> slotL(F, 0), slotR(F, 0);
     for (int64_t c = 0; c < N; c++)
         for (int64_t j = d->cell_face_off[c]; j < d->cell_face_off[c + 1]; j++) {
@@ -1974,6 +2053,9 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
     UP(toff, tile_face_off);
     UP(tfaces, tile_faces);
     UP(clds, cell_lds);
+    UP(hoff, halo_off);
+    UP(hids, halo_ids);
+    UP(nlds, nbr_lds);
     UP(slotL, slotL);
     UP(slotR, slotR);
 #undef UP
@@ -1981,6 +2063,7 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
     D.ntiles = ntiles;
     D.maxtf = maxtf;
     D.ntf = toff[ntiles];
+    D.maxhalo = maxhalo;
     *out = m;
     return IBIS_OK;
 }
@@ -2011,6 +2094,7 @@ struct ibis_residual {
     int opt_pack = 1;        // option pack_cells
     int opt_stage = 1;       // option stage_offsets
     int opt_cell_prefetch = 0;  // option cell_prefetch (layout 2, pack_cells 0)
+    int opt_stage_cells = 0;    // option stage_cells (layout 2, pack_cells 0)
     int opt_face_kernel = 0; // option face_kernel: 0 tile face lists, 1 the same with
                              // per-entry face data, 64/128/256 slot-major
     int* ent_idx = nullptr;      // per tile-list entry face data (face_kernel 1)
@@ -2295,6 +2379,14 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
         return;
     }
     if (!r->opt_pack && LAYOUT == 2) {
+        const size_t shs = (size_t)8 * (TILE + D.maxhalo) * sizeof(double);
+        if (r->opt_stage_cells && shs <= 65536) {
+            k_cell_fs<DIM, LIM, 2, OFFS, double, false, false, true>
+                <<<xcd_grid(nlog), BLOCK, shs, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps,
+                                                     d.venkat_K, nlog, 0, r->FL, r->FR, r->offs,
+                                                     prim, 0);
+            return;
+        }
         if (r->opt_cell_prefetch)
             k_cell_fs<DIM, LIM, 2, OFFS, double, false, true><<<xcd_grid(nlog), BLOCK, shm, st>>>(
                 D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR,
@@ -2856,6 +2948,10 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
     }
     if (k == "cell_prefetch") {
         r->opt_cell_prefetch = value ? 1 : 0;
+        return IBIS_OK;
+    }
+    if (k == "stage_cells") {
+        r->opt_stage_cells = value ? 1 : 0;
         return IBIS_OK;
     }
     return ibis::fail(IBIS_ERR_INVALID, ("unknown option " + k).c_str());

@@ -283,6 +283,11 @@ ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** lim
  *                            limiter pass in LDS for the reconstruction
  *                            pass (limited schemes, at most 10 faces/cell)
  *                         0: both passes read them from memory
+ *   "stage_cells"         1: each block of the cell kernel copies its 256
+ *                            cells' and their outside neighbours' values
+ *                            and centroids into LDS once, and the stencil
+ *                            loop reads them there (layout 2, pack_cells 0;
+ *                            replaces stage_offsets)
  *   "cell_prefetch"       1: the cell kernel loads all neighbour indices
  *                            first and each neighbour's values while the
  *                            previous one is summed (layout 2, pack_cells 0)

@@ -107,15 +107,16 @@ def test_hex_jittered_permuted(flux, limiter):
     _check(g_io, cfg, prim, exact=True)
 
 
-@pytest.mark.parametrize("layout,pack,fk,stage,pf", [(0, 1, 0, 1, 0), (1, 1, 0, 0, 0),
-                                                      (2, 1, 0, 1, 0), (2, 0, 0, 1, 0),
-                                                      (2, 0, 0, 0, 1), (2, 1, 64, 0, 0),
-                                                      (2, 0, 1, 1, 1), (2, 1, 1, 0, 0),
-                                                      (2, 0, 128, 1, 1), (2, 1, 256, 1, 0)])
+@pytest.mark.parametrize("layout,pack,fk,stage,pf,sc", [(0, 1, 0, 1, 0, 0), (1, 1, 0, 0, 0, 0),
+                                                         (2, 1, 0, 1, 0, 0), (2, 0, 0, 1, 0, 0),
+                                                         (2, 0, 0, 0, 1, 0), (2, 1, 64, 0, 0, 0),
+                                                         (2, 0, 1, 1, 1, 0), (2, 1, 1, 0, 0, 0),
+                                                         (2, 0, 128, 1, 1, 0), (2, 1, 256, 1, 0, 0),
+                                                         (2, 0, 1, 1, 0, 1), (2, 0, 0, 0, 1, 1)])
 @pytest.mark.parametrize("offsets", [0, 1])
-def test_implementation_options_give_identical_bits(layout, pack, fk, stage, pf, offsets):
+def test_implementation_options_give_identical_bits(layout, pack, fk, stage, pf, sc, offsets):
     opts = {"face_state_layout": layout, "precomputed_offsets": offsets, "pack_cells": pack,
-            "face_kernel": fk, "stage_offsets": stage, "cell_prefetch": pf}
+            "face_kernel": fk, "stage_offsets": stage, "cell_prefetch": pf, "stage_cells": sc}
     g_io, cfg = cases.hex_case(12, flux="hanel", limiter="venkatakrishnan")
     _check(g_io, cfg, cases.hex_prim(g_io, 12 ** 3 + 6 * 12 ** 2), exact=True, options=opts)
     g_io, cfg, fs = cases.wedge_case(40, 20)
