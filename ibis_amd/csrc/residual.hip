@@ -37,6 +37,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -95,6 +96,8 @@ struct DevMesh {
     const int* tile_face_off;         // ntiles + 1
     const int* tile_faces;            // face ids, ascending within a tile
     const unsigned short* cell_lds;   // maxf x N: position of a cell's face in its tile list
+    const int* ent_faces;             // the tile lists in entry order (face_kernel 1)
+    const unsigned short* ent_lds;    // maxf x N: position in the entry order
     // stencil halo of each tile (neighbours outside it) and, per (cell,
     // slot), the neighbour's index in the tile's LDS stencil
     int maxhalo;
@@ -1260,7 +1263,7 @@ __global__ void __launch_bounds__(BLOCK)
     const int e = blockIdx.x * BLOCK + threadIdx.x;
     const int NE = m.ntf, F = m.F;
     if (e >= NE) return;
-    const int f = m.tile_faces[e];
+    const int f = m.ent_faces[e];
     eidx[e] = (int)soa_index(m, m.L[f], m.slotL[f], 0);
     eidx[(size_t)NE + e] = (int)soa_index(m, m.R[f], m.slotR[f], 0);
 #pragma unroll
@@ -1328,7 +1331,7 @@ __global__ void __launch_bounds__(TILE)
     for (int q = 0; q < NC; q++) d[q] = 0.0;
     const int nf = m.nfaces[c];
     for (int j = 0; j < nf; j++) {
-        const int k = m.cell_lds[(size_t)j * N + c];
+        const int k = m.ent_lds[(size_t)j * N + c];
         const int s = m.ell_sign[(size_t)j * N + c];
 #pragma unroll
         for (int q = 0; q < NC; q++) {
@@ -2013,6 +2016,34 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
             else
                 slotR[f] = (unsigned char)(j - d->cell_face_off[c]);
         }
+    // entry order of the tile lists for the per-entry face kernel: within a
+    // tile, faces by (left slot, right slot, left cell), so that neighbouring
+    // lanes read neighbouring cells of one face-value plane; ent_lds is each
+    // (cell, slot)'s position in that order
+    std::vector<int> efaces(tfaces.size());
+    std::vector<unsigned short> elds((size_t)maxf * N, 0);
+    {
+        std::vector<int> idx, inv;
+        for (int t = 0; t < ntiles; t++) {
+            const int a = toff[t], n = toff[t + 1] - a;
+            idx.resize(n);
+            inv.resize(n);
+            std::iota(idx.begin(), idx.end(), 0);
+            auto key = [&](int k) {
+                const int f = tfaces[a + k];
+                return std::make_tuple(slotL[f], slotR[f], d->face_left[f], f);
+            };
+            std::sort(idx.begin(), idx.end(), [&](int x, int y) { return key(x) < key(y); });
+            for (int k = 0; k < n; k++) {
+                efaces[a + k] = tfaces[a + idx[k]];
+                inv[idx[k]] = k;
+            }
+            const int64_t c0 = (int64_t)t * TILE, c1 = std::min<int64_t>(N, c0 + TILE);
+            for (int64_t c = c0; c < c1; c++)
+                for (int64_t j = 0; j < d->cell_face_off[c + 1] - d->cell_face_off[c]; j++)
+                    elds[(size_t)j * N + c] = (unsigned short)inv[clds[(size_t)j * N + c]];
+        }
+    }
     int n_from_t = 1;
     for (int64_t f = 0; f < F && n_from_t; f++) {
         const double* gg = d->face_geom;
@@ -2053,6 +2084,8 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
     UP(toff, tile_face_off);
     UP(tfaces, tile_faces);
     UP(clds, cell_lds);
+    UP(efaces, ent_faces);
+    UP(elds, ent_lds);
     UP(hoff, halo_off);
     UP(hids, halo_ids);
     UP(nlds, nbr_lds);
