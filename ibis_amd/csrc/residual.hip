@@ -2297,6 +2297,20 @@ ibis_status ibis_residual_create_tables(const ibis_mesh* m, const ibis_fv_desc* 
         ibis_residual_destroy(r);
         return ibis::fail(IBIS_ERR_HIP, "ibis_residual_create: pack initialisation failed");
     }
+    // the fused second-order inviscid path starts from the options that the
+    // autotuner picks on the 3D and 2D workloads (every choice gives the
+    // same bits; ibis_residual_set_option changes them)
+    if (d.order == 2 && !d.viscous && !r->has_post) {
+        for (const auto& kv : {std::make_pair("face_state_layout", 2), std::make_pair("pack_cells", 0),
+                               std::make_pair("precomputed_offsets", 1),
+                               std::make_pair("face_kernel", 1), std::make_pair("stage_cells", 1)}) {
+            const ibis_status st = ibis_residual_set_option(r, kv.first, kv.second);
+            if (st != IBIS_OK) {
+                ibis_residual_destroy(r);
+                return st;
+            }
+        }
+    }
     *out = r;
     return IBIS_OK;
 }

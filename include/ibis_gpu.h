@@ -265,7 +265,10 @@ ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** lim
                                   double** flux, int* flux_has_area);
 
 /* Implementation choices of the 2nd-order inviscid path; every choice gives
- * the same bits.  Keys:
+ * the same bits.  A residual of that path is created with face_state_layout
+ * 2, pack_cells 0, precomputed_offsets 1, face_kernel 1 and stage_cells 1
+ * (the autotuner's choice on the 3D and 2D workloads); others start at 0 /
+ * layout 0.  Keys:
  *   "face_state_layout"   0: face states indexed by face (left/right arrays)
  *                         1: face states stored per (cell, face slot)
  *                         2: per (face slot, variable) planes over cells
