@@ -20,7 +20,7 @@ import math
 
 import numpy as np
 
-from .gridio import HEX, LINE, QUAD, GridIO
+from .gridio import HEX, LINE, PYRAMID, QUAD, TRI, GridIO
 
 MASK64 = (1 << 64) - 1
 
@@ -208,6 +208,57 @@ def hex_box(n: int = 160, jitter: float = 0.1, jitter_seed: int = 3, perm_seed: 
     if permute:
         grid = grid.permute_cells(seeded_permutation(perm_seed, n ** 3))
     return grid
+
+
+def split_quads(grid: GridIO, every: int = 2) -> GridIO:
+    """2D: every ``every``-th quad (storage order) split into the triangles
+    (0, 1, 2) and (0, 2, 3) along its diagonal.  The mesh stays conforming
+    and the boundary edges do not change: a mixed triangle / quad mesh whose
+    cells have 3 or 4 faces."""
+    ctype, cvtx = [], []
+    for c in range(grid.num_cells):
+        v = grid.cvtx[c]
+        if grid.ctype[c] == QUAD and c % every == 0:
+            for tri in ((v[0], v[1], v[2]), (v[0], v[2], v[3])):
+                ctype.append(TRI)
+                cvtx.append(list(tri) + [-1] * 5)
+        else:
+            ctype.append(int(grid.ctype[c]))
+            cvtx.append(list(v))
+    return GridIO(grid.dim, grid.xyz.copy(), np.array(ctype, np.int32),
+                  np.array(cvtx, np.int64), dict(grid.markers))
+
+
+# each hex face (VTK local vertex order, as hex_box writes the cells) listed
+# inward, i.e. counter-clockwise seen from the hex centre: the base of the
+# pyramid (base, centre) over that face
+_HEX_FACES_INWARD = ((1, 2, 3, 0), (7, 6, 5, 4), (4, 5, 1, 0), (5, 6, 2, 1), (6, 7, 3, 2),
+                     (7, 4, 0, 3))
+
+
+def split_hexes(grid: GridIO, every: int = 3) -> GridIO:
+    """3D: every ``every``-th hex (storage order) split into six pyramids
+    with their apex at the mean of its vertices (a new vertex).  Each
+    pyramid's base is one of the hex's faces, so the mesh stays conforming
+    with the neighbouring hexes and the boundary faces do not change: a mixed
+    hex / pyramid mesh whose cells have 5 or 6 faces, quad and triangle."""
+    xyz = [grid.xyz]
+    nv = grid.xyz.shape[0]
+    ctype, cvtx = [], []
+    for c in range(grid.num_cells):
+        v = grid.cvtx[c]
+        if grid.ctype[c] == HEX and c % every == 0:
+            xyz.append(grid.xyz[v[:8]].mean(axis=0)[None, :])
+            apex = nv
+            nv += 1
+            for f in _HEX_FACES_INWARD:
+                ctype.append(PYRAMID)
+                cvtx.append([v[k] for k in f] + [apex, -1, -1, -1])
+        else:
+            ctype.append(int(grid.ctype[c]))
+            cvtx.append(list(v))
+    return GridIO(grid.dim, np.concatenate(xyz), np.array(ctype, np.int32),
+                  np.array(cvtx, np.int64), dict(grid.markers))
 
 
 def cell_centres(grid: GridIO) -> np.ndarray:

@@ -78,6 +78,14 @@ def test_dot_product_hex():
     _dot(g_io, cfg, prim, seed=84, limiter="active")
 
 
+def test_dot_product_mixed_hex_pyramid():
+    from ibis_amd import gridgen
+    g_io, cfg = cases.hex_case(8, flux="hanel", limiter="venkatakrishnan")
+    gh = gridgen.split_hexes(g_io, 3)
+    prim = cases.hex_prim(gh, gh.num_cells + 6 * 64)
+    _dot(gh, cfg, prim, seed=85, limiter="active")
+
+
 def test_wall_force_gradient_central_differences():
     """Config-4 check on a 64 x 32 flat plate: d J / d prim . v against
     (J(p + h v) - J(p - h v)) / 2h, h = 1e-7, 32 directions (seed 6)."""

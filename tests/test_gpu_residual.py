@@ -124,6 +124,27 @@ def test_implementation_options_give_identical_bits(layout, pack, fk, stage, pf,
     _check(g_io, cfg, prim, exact=True, options=opts)
 
 
+MIXED_OPTIONS = [{}, {"face_state_layout": 0, "face_kernel": 0, "stage_cells": 0,
+                      "precomputed_offsets": 0, "pack_cells": 1},
+                 {"stage_cells": 0, "cell_prefetch": 1, "face_kernel": 0},
+                 {"face_kernel": 128, "stage_cells": 0}]
+
+
+@pytest.mark.parametrize("opts", MIXED_OPTIONS, ids=["default", "layout0", "prefetch", "slot128"])
+def test_mixed_elements_ragged_stencils(opts):
+    """Cells with different face counts: a 2D triangle / quad mesh (3 and 4
+    faces) and a 3D hex / pyramid mesh (5 and 6 faces, triangle and quad
+    faces), bit for bit against the oracle."""
+    from ibis_amd import gridgen
+    g_io, cfg, fs = cases.wedge_case(24, 12)
+    gm = gridgen.split_quads(g_io, 2)
+    prim = cases.perturbed_prim(gm, fs, gm.num_cells + 2 * (24 + 12), seed=31)
+    _check(gm, cfg, prim, exact=True, options=opts)
+    g_io, cfg = cases.hex_case(8, flux="hanel", limiter="venkatakrishnan")
+    gh = gridgen.split_hexes(g_io, 3)
+    _check(gh, cfg, cases.hex_prim(gh, gh.num_cells + 6 * 64), exact=True, options=opts)
+
+
 def test_plate_viscous():
     g_io, cfg, fs = cases.plate_case(64, 32)
     prim = cases.perturbed_prim(g_io, fs, 64 * 32 + 2 * (64 + 32), seed=2)
