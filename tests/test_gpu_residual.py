@@ -201,6 +201,53 @@ def test_rk_steps_match_oracle():
     assert np.array_equal(c_gpu[:, :n], c_cpu[:, :n])
 
 
+@pytest.mark.timeout(900)
+def test_wedge_config1_2000_steps():
+    """The first workload as specified: 192 x 96 wedge, uniform Mach-3 free
+    stream, ssp-rk3 at CFL 0.5, 2000 steps.  The plain path (a host dt per
+    step) equals the oracle's state after steps 1, 10, 100 and 2000 and its
+    dt at every step; the device-resident path (a batch of one, each step
+    one HIP graph replay) equals it after 2000 steps."""
+    import torch
+    from ibis_amd.batch import Batch
+    from ibis_amd.config import ssp_rk3
+    from ibis_amd.finite_volume import RungeKutta
+    from ibis_amd.native import check, lib, ptr
+    g_io, cfg, fs = cases.wedge_case(192, 96)
+    n = g_io.num_cells
+    prim = cases.uniform_prim(fs, n, n + 2 * (192 + 96))
+    og, ofv = oracle_pair(g_io, cfg)
+    cons = ofv.prim_to_cons(prim)
+    gb, fv = gpu_objects(g_io, cfg)
+    d_prim, d_cons = to_dev(gb.to_grid_order(prim)), to_dev(gb.to_grid_order(cons))
+    rk = RungeKutta(fv, "ssp-rk3", cfl=0.5, max_time=1e30)
+    a, b = ssp_rk3()
+    dt_prev, t = np.finfo(np.float64).max, 0.0
+    p_cpu, c_cpu = prim.copy(), cons.copy()
+    for step in range(1, 2001):
+        dt_cpu = ofv.rk_step(a, b, 0.5, c_cpu, p_cpu, dt_prev, t, 1e30)
+        dt_prev = dt_cpu
+        t += dt_cpu
+        assert rk.take_step(d_cons, d_prim) == dt_cpu, step
+        if step in (1, 10, 100, 2000):
+            torch.cuda.synchronize()
+            c_gpu = gb.to_reference_order(d_cons.cpu().numpy())
+            assert np.array_equal(c_gpu[:, :n], c_cpu[:, :n]), step
+            print(f"config 1: step {step}, t = {t:.6e}, state identical to the oracle", flush=True)
+    bt = Batch([(g_io, cfg)])
+    b0 = bt.blocks[0]
+    bp = to_dev(bt.gather([b0.to_grid_order(prim)]))
+    bc = torch.zeros((bt.ncons, bt.Nt), dtype=torch.float64, device="cuda")
+    check(lib().ibis_prim_to_cons(bt.residual_handle(), ptr(bp), ptr(bc), None), "prim_to_cons")
+    bt.rk_init(bc, bp, cfl=0.5, max_time=1e30)
+    bt.rk_steps(2000, use_graph=True)
+    c_bat = b0.to_reference_order(bt.scatter(bc.cpu().numpy(), 0))
+    assert np.array_equal(c_bat[:, :n], c_cpu[:, :n])
+    tb, _ = bt.times()
+    assert tb[0] == t
+    bt.close()
+
+
 @pytest.mark.timeout(600)
 def test_renumbering_invariance_hex160():
     """Config-3 size (160^3 jittered, cell-permuted hexes): the residual on
