@@ -136,6 +136,50 @@ def adjoint_leg(steps):
     return out
 
 
+def rk_leg(nsteps=500):
+    """Config 1 (192 x 96 wedge, uniform Mach-3 free stream, ssp-rk3 at CFL
+    0.5): time per step of the plain driver (RungeKutta.take_step: the dt
+    goes to the host every step) and of the device-resident driver (a batch
+    of one: dt on the device, each step one HIP graph replay), same states."""
+    import torch
+    from ibis_amd import cases
+    from ibis_amd.batch import Batch
+    from ibis_amd.finite_volume import FiniteVolume, GridBlock, RungeKutta
+    from ibis_amd.native import check, lib, ptr
+    g_io, cfg, fs = cases.wedge_case(192, 96)
+    gb = GridBlock(g_io, cfg)
+    fv = FiniteVolume(gb, cfg)
+    prim = cases.uniform_prim(fs, gb.N, gb.Nt)
+    dp = torch.from_numpy(prim).to("cuda")
+    dc = torch.zeros((gb.dim + 2, gb.Nt), dtype=torch.float64, device="cuda")
+    fv.prim_to_cons(dp, dc)
+    rk = RungeKutta(fv, "ssp-rk3", cfl=0.5, max_time=1e30)
+    rk.take_step(dc, dp)
+    torch.cuda.synchronize()
+    w = time.perf_counter()
+    for _ in range(nsteps):
+        rk.take_step(dc, dp)
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - w) / nsteps
+    bt = Batch([(g_io, cfg)])
+    bp = torch.from_numpy(bt.gather([prim])).to("cuda")
+    bc = torch.zeros((bt.ncons, bt.Nt), dtype=torch.float64, device="cuda")
+    check(lib().ibis_prim_to_cons(bt.residual_handle(), ptr(bp), ptr(bc), None), "prim_to_cons")
+    bt.rk_init(bc, bp, cfl=0.5, max_time=1e30)
+    bt.rk_steps(1, use_graph=True)
+    w = time.perf_counter()
+    bt.rk_steps(nsteps, use_graph=True)
+    t_graph = (time.perf_counter() - w) / nsteps
+    same = bool(np.array_equal(bt.scatter(bc.cpu().numpy(), 0)[:, :gb.N], dc.cpu().numpy()[:, :gb.N]))
+    out = {"workload": "wedge 192x96 (config 1), ssp-rk3, CFL 0.5", "steps": nsteps,
+           "plain_us_per_step": round(t_plain * 1e6, 1), "graph_us_per_step": round(t_graph * 1e6, 1),
+           "speedup": round(t_plain / t_graph, 3), "identical_fp64": same}
+    print(f"[bench] rk driver: {out}", file=sys.stderr, flush=True)
+    fv.close()
+    bt.close()
+    return out
+
+
 def batch_leg(nsteps, n_inst=64):
     """64 seeded wedge/ramp instances (10k-400k cells, seed 7), ssp-rk3 at
     CFL 0.5: one at a time on the plain path vs all together in one HIP
@@ -386,6 +430,7 @@ def main():
 
     adj = None if args.no_adjoint else adjoint_leg(max(10, args.steps // 2))
     bat = None if args.no_batch else batch_leg(args.batch_steps)
+    rkd = None if args.no_batch else rk_leg()
 
     N, F, Nt = gb.N, gb.F, gb.Nt
     ncons = gb.dim + 2
@@ -441,6 +486,7 @@ def main():
         "fp32": fp32,
         "adjoint": adj,
         "batch": bat,
+        "rk_driver": rkd,
     }
     print(json.dumps(out), flush=True)
 
