@@ -867,26 +867,24 @@ __global__ void __launch_bounds__(BLOCK)
 #pragma unroll
             for (int v = 0; v < 5; v++) {
                 double d2 = gx[v] * dx + gy[v] * dy + gz[v] * dz;
+                // written without branches: the quotient of a face that the
+                // reference skips (d2 zero or NaN) is formed and discarded,
+                // so every lane runs the same instructions
                 if (LIM == 1) {  // Barth-Jespersen, FV/limiter.cpp:289-303
                     const int sg = (d2 > 0) - (d2 < 0);
                     d2 = sg * (fabs(d2) + eps);
-                    if (sg > 0)
-                        phi[v] = ibis::kmin(phi[v], (umax[v] - ui[v]) / d2);
-                    else if (sg < 0)
-                        phi[v] = ibis::kmin(phi[v], (umin[v] - ui[v]) / d2);
+                    const double q = (sg > 0 ? umax[v] - ui[v] : umin[v] - ui[v]) / d2;
+                    const double cand = ibis::kmin(phi[v], q);
+                    phi[v] = sg != 0 ? cand : phi[v];
                 } else {  // Venkatakrishnan
-                    double d1;
-                    if (d2 > 0.0)
-                        d1 = umax[v] - ui[v];
-                    else if (d2 < 0.0)
-                        d1 = umin[v] - ui[v];
-                    else
-                        continue;
+                    const bool pos = d2 > 0.0, neg = d2 < 0.0;
+                    const double d1 = pos ? umax[v] - ui[v] : umin[v] - ui[v];
                     const double d1s = d1 * d1;
                     const double d2s = d2 * d2;
                     const double num = d1s + eps2 + 2.0 * (d1 * d2);
                     const double den = d1s + 2.0 * d2s + d1 * d2 + eps2;
-                    phi[v] = ibis::kmin(phi[v], num / den);
+                    const double cand = ibis::kmin(phi[v], num / den);
+                    phi[v] = (pos || neg) ? cand : phi[v];
                 }
             }
         }
