@@ -58,6 +58,11 @@ struct AdjBufs {
     double* vg;            // 4 x 3 x N
     double* bcb;           // 7 x nb: what each ghost sends to its valid cell
     double* pb;            // 7 x Nt output
+    // geometry tables, built once: the LSQ weights of each cell's stencil
+    // for the neighbour in slot j, w[(j*3 + k)*N + c], and the face offsets
+    // x_face - x_cell, offs[(j*3 + k)*N + c]
+    const double* lsqw;
+    const double* offs;
 };
 
 // LSQ weights of cell c for a neighbour at centroid offset (dx, dy, dz)
@@ -82,6 +87,22 @@ __device__ __forceinline__ void lsq_weights(const DevMesh& m, int c, double dx, 
     w[0] = a1 - q12 * a2 + beta * a3;
     w[1] = a2 - q23 * a3;
     w[2] = a3;
+}
+
+// the weight table: lsq_weights of cell i for the neighbour in each of its
+// slots, with the centroid differences written as the kernels write them
+template <int DIM>
+__global__ void __launch_bounds__(BLOCK) k_lsq_table(DevMesh m, double* __restrict__ w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int N = m.N, Nt = m.Nt;
+    if (i >= N) return;
+    for (int j = 0; j < m.nfaces[i]; j++) {
+        const int nb = m.ell_nbr[(size_t)j * N + i];
+        double wj[3];
+        lsq_weights<DIM>(m, i, m.cen[nb] - m.cen[i], m.cen[Nt + nb] - m.cen[Nt + i],
+                         m.cen[2 * (size_t)Nt + nb] - m.cen[2 * (size_t)Nt + i], wj);
+        for (int k = 0; k < 3; k++) w[(size_t)(j * 3 + k) * N + i] = wj[k];
+    }
 }
 
 // cotangent of the face flux (global frame, conserved rows) from dU/dt and
@@ -341,7 +362,6 @@ __global__ void __launch_bounds__(BLOCK)
     const int N = m.N, Nt = m.Nt, F = m.F;
     if (i >= N) return;
     const int nf = m.nfaces[i];
-    const double xi = m.cen[i], yi = m.cen[Nt + i], zi = m.cen[2 * (size_t)Nt + i];
     const int ngrad = DIM == 3 ? 5 : 4;
     for (int v = 0; v < 5; v++) {
         const int row = vs.row[v], slot = vs.slot[v];
@@ -355,8 +375,8 @@ __global__ void __launch_bounds__(BLOCK)
         for (int j = 0; j < nf; j++) {
             const int f = m.ell_face[(size_t)j * N + i];
             const double qb = (m.ell_sign[(size_t)j * N + i] > 0 ? B.QLb : B.QRb)[(size_t)v * F + f];
-            const double d[3] = {m.geom[9 * (size_t)F + f] - xi, m.geom[10 * (size_t)F + f] - yi,
-                                 m.geom[11 * (size_t)F + f] - zi};
+            const double d[3] = {B.offs[(size_t)(j * 3 + 0) * N + i], B.offs[(size_t)(j * 3 + 1) * N + i],
+                                 B.offs[(size_t)(j * 3 + 2) * N + i]};
             self += qb;
             phib += qb * (g[0] * d[0] + g[1] * d[1] + g[2] * d[2]);
             for (int k = 0; k < 3; k++) gb[k] += qb * phi * d[k];
@@ -390,10 +410,9 @@ __global__ void __launch_bounds__(BLOCK)
             double d1s = 0.0, d2s = 0.0, dstar[3] = {0.0, 0.0, 0.0};
             int sgs = 0;
             for (int j = 0; j < nf; j++) {
-                const int f = m.ell_face[(size_t)j * N + i];
-                const double d[3] = {m.geom[9 * (size_t)F + f] - xi,
-                                     m.geom[10 * (size_t)F + f] - yi,
-                                     m.geom[11 * (size_t)F + f] - zi};
+                const double d[3] = {B.offs[(size_t)(j * 3 + 0) * N + i],
+                                     B.offs[(size_t)(j * 3 + 1) * N + i],
+                                     B.offs[(size_t)(j * 3 + 2) * N + i]};
                 const double d2 = g[0] * d[0] + g[1] * d[1] + g[2] * d[2];
                 const int sg = (d2 > 0.0) - (d2 < 0.0);
                 if (sg == 0) continue;
@@ -442,11 +461,8 @@ __global__ void __launch_bounds__(BLOCK)
         // own LSQ stencil: g = sum_j w_j (u_j - u_i)
         if (v < ngrad) {
             for (int j = 0; j < nf; j++) {
-                const int nb = m.ell_nbr[(size_t)j * N + i];
-                double w[3];
-                lsq_weights<DIM>(m, i, m.cen[nb] - xi, m.cen[Nt + nb] - yi,
-                                 m.cen[2 * (size_t)Nt + nb] - zi, w);
-                self -= w[0] * gb[0] + w[1] * gb[1] + w[2] * gb[2];
+                const double* w = B.lsqw + (size_t)j * 3 * N + i;
+                self -= w[0] * gb[0] + w[N] * gb[1] + w[2 * (size_t)N] * gb[2];
             }
         } else {
             gb[0] = gb[1] = gb[2] = 0.0;  // 2D vz: no gradient is computed
@@ -467,11 +483,8 @@ __global__ void __launch_bounds__(BLOCK)
                     gb[d] += (left ? B.GLb : B.GRb)[(size_t)(k * 3 + d) * F + f];
             }
             for (int j = 0; j < nf; j++) {
-                const int nb = m.ell_nbr[(size_t)j * N + i];
-                double w[3];
-                lsq_weights<DIM>(m, i, m.cen[nb] - xi, m.cen[Nt + nb] - yi,
-                                 m.cen[2 * (size_t)Nt + nb] - zi, w);
-                self -= w[0] * gb[0] + w[1] * gb[1] + w[2] * gb[2];
+                const double* w = B.lsqw + (size_t)j * 3 * N + i;
+                self -= w[0] * gb[0] + w[N] * gb[1] + w[2 * (size_t)N] * gb[2];
             }
             B.vself[(size_t)k * N + i] = self;
             for (int d = 0; d < 3; d++) B.vg[((size_t)k * 3 + d) * N + i] = gb[d];
@@ -539,13 +552,12 @@ __global__ void __launch_bounds__(BLOCK)
     k_vjp_ghost(DevMesh m, VarSet vs, const int* __restrict__ kinds, int ntab, AdjBufs B) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= m.nb) return;
-    const int N = m.N, Nt = m.Nt, F = m.F;
+    const int N = m.N, F = m.F;
     const int f = m.b_face[b], g = m.b_ghost[b], va = m.b_valid[b], t = m.b_tag[b];
     const int ngrad = DIM == 3 ? 5 : 4;
     const int j = slot_of(m, f, va);
-    double w[3];
-    lsq_weights<DIM>(m, va, m.cen[g] - m.cen[va], m.cen[Nt + g] - m.cen[Nt + va],
-                     m.cen[2 * (size_t)Nt + g] - m.cen[2 * (size_t)Nt + va], w);
+    const double w[3] = {B.lsqw[(size_t)(j * 3 + 0) * N + va], B.lsqw[(size_t)(j * 3 + 1) * N + va],
+                         B.lsqw[(size_t)(j * 3 + 2) * N + va]};
     double gb[NPRIM], vb[NPRIM];
     for (int k = 0; k < NPRIM; k++) gb[k] = vb[k] = 0.0;
     if (VISC) {  // viscous path reads the ghost after the viscous actions
@@ -600,9 +612,8 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_gather(DevMesh m, VarSet vs, AdjB
         }
         // cell nb's stencil holds this cell at slot s
         const int s = slot_of(m, f, nb);
-        double w[3];
-        lsq_weights<DIM>(m, nb, m.cen[i] - m.cen[nb], m.cen[Nt + i] - m.cen[Nt + nb],
-                         m.cen[2 * (size_t)Nt + i] - m.cen[2 * (size_t)Nt + nb], w);
+        const double w[3] = {B.lsqw[(size_t)(s * 3 + 0) * N + nb], B.lsqw[(size_t)(s * 3 + 1) * N + nb],
+                             B.lsqw[(size_t)(s * 3 + 2) * N + nb]};
         for (int v = 0; v < 5; v++) {
             double x = 0.0;
             if (v < ngrad)

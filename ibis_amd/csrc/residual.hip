@@ -2157,6 +2157,7 @@ struct ibis_residual {
     double *adj_prim_c = nullptr, *adj_vgrad = nullptr, *adj_face = nullptr, *adj_cell = nullptr;
     int* adj_extj = nullptr;
     double *adj_bcb = nullptr, *adj_zero = nullptr, *adj_dudt = nullptr, *adj_force = nullptr;
+    double* adj_lsqw = nullptr;  // LSQ weight table of the reverse kernels
     int freeze_lim = 0;  // option freeze_limiter (tangent residual only)
     // tangent (JVP) state and scratch, allocated on first use
     ibis::Dual* dual_prim = nullptr;
@@ -3027,6 +3028,19 @@ ibis_status ensure_adjoint(ibis_residual* r) {
     HIP_TRY(al((void**)&r->adj_zero, sizeof(double) * nc * Nt));
     HIP_TRY(al((void**)&r->adj_dudt, sizeof(double) * nc * Nt));
     HIP_TRY(al((void**)&r->adj_force, sizeof(double) * 3 * nb));
+    // geometry tables of the reverse kernels: LSQ weights per (cell, slot)
+    // and the face offsets (the same expressions as the forward kernels)
+    HIP_TRY(al((void**)&r->adj_lsqw, sizeof(double) * 3 * (size_t)D.maxf * N));
+    if (D.dim == 3)
+        k_lsq_table<3><<<nblocks(D.N), BLOCK>>>(D, r->adj_lsqw);
+    else
+        k_lsq_table<2><<<nblocks(D.N), BLOCK>>>(D, r->adj_lsqw);
+    if (!r->offs) {
+        HIP_TRY(hipMalloc(&r->offs, sizeof(double) * 3 * (size_t)D.maxf * D.N));
+        k_offsets<<<nblocks(D.N), BLOCK>>>(D, r->offs);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
     return IBIS_OK;
 }
 
@@ -3163,6 +3177,8 @@ ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* 
     B.cextj = r->adj_extj;
     B.bcb = r->adj_bcb;
     B.pb = prim_bar;
+    B.lsqw = r->adj_lsqw;
+    B.offs = r->offs;
     const bool active = limiter_mode == IBIS_LIMITER_ACTIVE;
     if (D.dim == 3)
         launch_vjp<3>(r, vs, fp, B, active, st);
