@@ -2492,47 +2492,49 @@ void launch_face_slot_ts(const DevMesh& D, const FaceParams& fp, int ts, const R
         launch_face_slot<DIM, FLUX, 256, R>(D, fp, FL, dudt, st);
 }
 
-// layout 3 = layout 2 holding float values; face_kernel 0 = tile face
-// lists, otherwise the slot-major kernel with that many cells per block
+// f32: float face values in layout 2 (reduced-precision modes);
+// face_kernel 0 = tile face lists, 1 = the same with per-entry face data,
+// otherwise the slot-major kernel with that many cells per block
 template <int DIM, int FLUX>
-void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, int face_kernel,
+void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, bool f32, int face_kernel,
                       const int* eidx, const double* egeom, const double* FL, const double* FR,
                       double* dudt, hipStream_t st) {
     const unsigned g = xcd_grid(D.ntiles);
     const size_t shm = (size_t)(DIM + 2) * D.maxtf * sizeof(double);
-    if (face_kernel == 1 && layout >= 2) {
+    if (f32) {
         const float* fl = reinterpret_cast<const float*>(FL);
-        if (layout == 3 && D.n_from_t)
+        if (face_kernel == 1 && D.n_from_t)
             k_face_entry_fs<DIM, FLUX, false, float><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, fl, dudt);
-        else if (layout == 3)
+        else if (face_kernel == 1)
             k_face_entry_fs<DIM, FLUX, true, float><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, fl, dudt);
-        else if (D.n_from_t)
-            k_face_entry_fs<DIM, FLUX, false><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, FL, dudt);
-        else
-            k_face_entry_fs<DIM, FLUX, true><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, FL, dudt);
-        return;
-    }
-    if (layout == 3) {
-        const float* fl = reinterpret_cast<const float*>(FL);
-        if (face_kernel)
+        else if (face_kernel)
             launch_face_slot_ts<DIM, FLUX, float>(D, fp, face_kernel, fl, dudt, st);
         else
             k_face_tile_fs<DIM, FLUX, 2, float><<<g, TILE, shm, st>>>(D, fp, fl, fl, dudt);
+        return;
+    }
+    if (layout == 2 && face_kernel == 1) {
+        if (D.n_from_t)
+            k_face_entry_fs<DIM, FLUX, false><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, FL, dudt);
+        else
+            k_face_entry_fs<DIM, FLUX, true><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, FL, dudt);
     } else if (layout == 2 && face_kernel) {
         launch_face_slot_ts<DIM, FLUX, double>(D, fp, face_kernel, FL, dudt, st);
-    } else if (layout == 0)
+    } else if (layout == 0) {
         k_face_tile_fs<DIM, FLUX, 0><<<g, TILE, shm, st>>>(D, fp, FL, FR, dudt);
-    else if (layout == 1)
+    } else if (layout == 1) {
         k_face_tile_fs<DIM, FLUX, 1><<<g, TILE, shm, st>>>(D, fp, FL, FR, dudt);
-    else
+    } else {
         k_face_tile_fs<DIM, FLUX, 2><<<g, TILE, shm, st>>>(D, fp, FL, FR, dudt);
+    }
 }
 
 template <int DIM>
-void launch_face_fs(const DevMesh& D, const FaceParams& fp, int flux, int layout, int face_kernel,
-                    const int* eidx, const double* egeom, const double* FL, const double* FR,
-                    double* dudt, hipStream_t st) {
-#define IBIS_FF(K) launch_face_fs_f<DIM, K>(D, fp, layout, face_kernel, eidx, egeom, FL, FR, dudt, st)
+void launch_face_fs(const DevMesh& D, const FaceParams& fp, int flux, int layout, bool f32,
+                    int face_kernel, const int* eidx, const double* egeom, const double* FL,
+                    const double* FR, double* dudt, hipStream_t st) {
+#define IBIS_FF(K) \
+    launch_face_fs_f<DIM, K>(D, fp, layout, f32, face_kernel, eidx, egeom, FL, FR, dudt, st)
     switch (flux) {
         case IBIS_FLUX_HANEL: IBIS_FF(IBIS_FLUX_HANEL); break;
         case IBIS_FLUX_AUSMDV: IBIS_FF(IBIS_FLUX_AUSMDV); break;
@@ -2780,14 +2782,15 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
                 launch_cell_fs<2>(D, vs, d, r, nlog, prim, st);
             pf.end(1);
             pf.beg();
-            const int lay = r->prec != IBIS_PREC_FP64 ? 3 : r->opt_layout;
-            const int fk = (lay >= 2) ? r->opt_face_kernel : 0;
+            const bool f32 = r->prec != IBIS_PREC_FP64;
+            const int lay = f32 ? 2 : r->opt_layout;
+            const int fk = lay == 2 ? r->opt_face_kernel : 0;
             if (D.dim == 3)
-                launch_face_fs<3>(D, fp, d.flux, lay, fk, r->ent_idx, r->ent_geom, r->FL, r->FR,
-                                  dudt, st);
+                launch_face_fs<3>(D, fp, d.flux, lay, f32, fk, r->ent_idx, r->ent_geom, r->FL,
+                                  r->FR, dudt, st);
             else
-                launch_face_fs<2>(D, fp, d.flux, lay, fk, r->ent_idx, r->ent_geom, r->FL, r->FR,
-                                  dudt, st);
+                launch_face_fs<2>(D, fp, d.flux, lay, f32, fk, r->ent_idx, r->ent_geom, r->FL,
+                                  r->FR, dudt, st);
             pf.end(2);
         } else {
             // first order: face flux and surface integral as one tiled kernel
