@@ -134,7 +134,7 @@ def test_autograd_functions_match_vjp():
     pb = torch.zeros_like(prim0)
     fv.compute_dudt_vjp(prim0.clone(), pb, force_tags=fv.tag_mask("wall"))
     assert torch.equal(x.grad, pb)
-    assert float(J) == float(fv.wall_force(prim0.clone(), fv.tag_mask("wall"))[0])
+    assert float(J.detach()) == float(fv.wall_force(prim0.clone(), fv.tag_mask("wall"))[0])
     y = prim0.clone().requires_grad_(True)
     d = residual(y, fv)
     w = torch.randn(d.shape, dtype=torch.float64, device="cuda", generator=torch.Generator(
