@@ -104,6 +104,18 @@ def test_hex_jvp():
     _jvp_case(g_io, cfg, prim, seed=37, exact=True)
 
 
+def test_mixed_elements_jvp():
+    """Tangent residual on ragged stencils: triangle / quad and hex / pyramid."""
+    from ibis_amd import gridgen
+    g_io, cfg, fs = cases.wedge_case(24, 12)
+    gm = gridgen.split_quads(g_io, 2)
+    prim = cases.perturbed_prim(gm, fs, gm.num_cells + 2 * (24 + 12), seed=33)
+    _jvp_case(gm, cfg, prim, seed=38, exact=True)
+    g_io, cfg = cases.hex_case(8, flux="hanel", limiter="venkatakrishnan")
+    gh = gridgen.split_hexes(g_io, 3)
+    _jvp_case(gh, cfg, cases.hex_prim(gh, gh.num_cells + 6 * 64), seed=39, exact=True)
+
+
 def test_plate_viscous_jvp():
     g_io, cfg, fs = cases.plate_case(64, 32)
     prim = cases.perturbed_prim(g_io, fs, 64 * 32 + 2 * (64 + 32), seed=2)

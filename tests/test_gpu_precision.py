@@ -81,6 +81,18 @@ def test_wedge_fp32_drift(mode):
         assert mx < 1e-3, (name, mx)
 
 
+@pytest.mark.parametrize("mode", ["fp32_flux", "fp32_recon"])
+def test_mixed_elements_fp32_drift(mode):
+    from ibis_amd import gridgen
+    g_io, cfg = cases.hex_case(8, flux="hanel", limiter="venkatakrishnan")
+    gh = gridgen.split_hexes(g_io, 3)
+    gb, d32, d64 = _pair(gh, cfg, cases.hex_prim(gh, gh.num_cells + 6 * 64), mode)
+    dr = drift(d32, d64, gb.N)
+    print(f"hex/pyramid {mode} drift (max_rel, rms_rel): {dr}", flush=True)
+    for name, (mx, rms) in dr.items():
+        assert mx < 1e-3, (name, mx)
+
+
 def test_fp32_flux_rejected_where_unsupported():
     import torch
     from ibis_amd.native import IbisError
