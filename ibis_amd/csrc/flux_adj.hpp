@@ -45,25 +45,13 @@ __host__ __device__ inline void flux_ausmdv_vjp(const double* L, const double* R
     const double duL = 0.5 * (uL + fabs(uL));
     const bool subL = fabs(ML) <= 1.0;
     const double YL = (uL + am) * (uL + am) / (4.0 * am);
-    double pLplus, uLplus;
-    if (subL) {
-        pLplus = pL * (ML + 1.0) * (ML + 1.0) * (2.0 - ML) * 0.25;
-        uLplus = alphaL * (YL - duL) + duL;
-    } else {
-        pLplus = pL * duL / uL;
-        uLplus = duL;
-    }
+    // the split pressures themselves are not needed: their cotangents
+    // are formed from the derivative expressions below
+    const double uLplus = subL ? alphaL * (YL - duL) + duL : duL;
     const double duR = 0.5 * (uR - fabs(uR));
     const bool subR = fabs(MR) <= 1.0;
     const double XR = -(uR - am) * (uR - am) / (4.0 * am);
-    double pRminus, uRminus;
-    if (subR) {
-        pRminus = pR * (MR - 1.0) * (MR - 1.0) * (2.0 + MR) * 0.25;
-        uRminus = alphaR * (XR - duR) + duR;
-    } else {
-        pRminus = pR * duR / uR;
-        uRminus = duR;
-    }
+    const double uRminus = subR ? alphaR * (XR - duR) + duR : duR;
     const double ru_half = uLplus * rL + uRminus * rR;
     const double dp = pL - pR;
     const bool pmin_is_L = pL < pR;  // Dual min: second argument on ties
@@ -241,22 +229,18 @@ __host__ __device__ inline void flux_hanel_vjp(const double* L, const double* Rs
     const double HL = Cv * TL + pLrL + 0.5 * (uL * uL + vL * vL + wL * wL);
     const double HR = Cv * TR + pRrR + 0.5 * (uR * uR + vR * vR + wR * wR);
     const bool subL = fabs(uL) <= aL, subR = fabs(uR) <= aR;
-    double uLplus, pLplus, DL = 0.0, uRminus, pRminus, ER = 0.0;
+    double uLplus, DL = 0.0, uRminus, ER = 0.0;
     if (subL) {
         uLplus = 1.0 / (4.0 * aL) * (uL + aL) * (uL + aL);
         DL = 1.0 / aL * (2.0 - uL / aL);
-        pLplus = pL * uLplus * DL;
     } else {
         uLplus = 0.5 * (uL + fabs(uL));
-        pLplus = pL * uLplus * (1.0 / uL);
     }
     if (subR) {
         uRminus = -1.0 / (4.0 * aR) * (uR - aR) * (uR - aR);
         ER = 1.0 / aR * (-2.0 - uR / aR);
-        pRminus = pR * uRminus * ER;
     } else {
         uRminus = 0.5 * (uR - fabs(uR));
-        pRminus = pR * uRminus * (1.0 / uR);
     }
     const double PL = uLplus * rL, QR = uRminus * rR;
     const double o0 = ob[0], o1 = ob[1], o2 = ob[2], o3 = THREE_D ? ob[3] : 0.0, o4 = ob[4];

@@ -714,19 +714,47 @@ __global__ void __launch_bounds__(BLOCK)
     const int i = lb * BLOCK + threadIdx.x;
     const int N = m.N, F = m.F;
     const int SS = TILE + m.maxhalo;  // plane length of the LDS stencil
+    // per-cell inputs that do not depend on the staged stencil, loaded
+    // before the staging loop and its barrier so that their latency
+    // overlaps them (a thread past the last cell reads cell 0's and
+    // discards them)
+    const int il = i < N ? i : 0;
+    const int nf = m.nfaces[il];
+    const double r11 = m.lsq[il], r12 = m.lsq[N + il], r22 = m.lsq[2 * (size_t)N + il];
+    double r13 = 0.0, r23 = 0.0, r33 = 0.0;
+    if (DIM == 3) {
+        r13 = m.lsq[3 * (size_t)N + il];
+        r23 = m.lsq[4 * (size_t)N + il];
+        r33 = m.lsq[5 * (size_t)N + il];
+    }
+    int kst[MAXF];  // the neighbours' positions in the LDS stencil
     if (SN) {
-        const int c0 = lb * TILE;
+#pragma unroll
+        for (int j = 0; j < MAXF; j++) kst[j] = j < m.maxf ? m.nbr_lds[(size_t)j * N + il] : 0;
+    }
+    if (SN) {
+        static_assert(TILE == BLOCK, "one stencil entry per thread and pass");
         const int h0 = m.halo_off[lb], nh = m.halo_off[lb + 1] - h0;
-        for (int k = threadIdx.x; k < TILE + nh; k += BLOCK) {
-            const int c = k < TILE ? c0 + k : m.halo_ids[h0 + k - TILE];
-            if (k < TILE && c >= N) continue;
-            double u[5], x, y, z;
-            load_cell<PACKED>(m, vs, pack, prim, c, u, x, y, z);
+        auto put = [&](int k, const double* u, double x, double y, double z) {
 #pragma unroll
             for (int v = 0; v < 5; v++) soff[v * SS + k] = u[v];
             soff[5 * SS + k] = x;
             soff[6 * SS + k] = y;
             soff[7 * SS + k] = z;
+        };
+        // the thread's own cell and its first halo cell are gathered
+        // together, then written; further halo cells one per pass
+        const int kh = threadIdx.x;
+        const int ch = kh < nh ? m.halo_ids[h0 + kh] : 0;
+        double u0[5], x0, y0, z0, u1[5], x1, y1, z1;
+        if (i < N) load_cell<PACKED>(m, vs, pack, prim, i, u0, x0, y0, z0);
+        if (kh < nh) load_cell<PACKED>(m, vs, pack, prim, ch, u1, x1, y1, z1);
+        if (i < N) put(threadIdx.x, u0, x0, y0, z0);
+        if (kh < nh) put(TILE + kh, u1, x1, y1, z1);
+        for (int k = kh + BLOCK; k < nh; k += BLOCK) {
+            double u[5], x, y, z;
+            load_cell<PACKED>(m, vs, pack, prim, m.halo_ids[h0 + k], u, x, y, z);
+            put(TILE + k, u, x, y, z);
         }
         __syncthreads();
     }
@@ -742,14 +770,6 @@ __global__ void __launch_bounds__(BLOCK)
         zi = soff[7 * SS + threadIdx.x];
     } else {
         load_cell<PACKED>(m, vs, pack, prim, i, ui, xi, yi, zi);
-    }
-    const int nf = m.nfaces[i];
-    const double r11 = m.lsq[i], r12 = m.lsq[N + i], r22 = m.lsq[2 * (size_t)N + i];
-    double r13 = 0.0, r23 = 0.0, r33 = 0.0;
-    if (DIM == 3) {
-        r13 = m.lsq[3 * (size_t)N + i];
-        r23 = m.lsq[4 * (size_t)N + i];
-        r33 = m.lsq[5 * (size_t)N + i];
     }
     const double beta = (r12 * r23 - r13 * r23) / (r11 * r22);  // GR/gradient.h:89 (sic)
     const double r11s = r11 * r11;
@@ -788,12 +808,15 @@ __global__ void __launch_bounds__(BLOCK)
         }
     };
     if (SN) {
-        for (int j = 0; j < nf; j++) {
-            const int k = m.nbr_lds[(size_t)j * N + i];
-            double un[5];
 #pragma unroll
-            for (int v = 0; v < 5; v++) un[v] = soff[v * SS + k];
-            accumulate(un, soff[5 * SS + k], soff[6 * SS + k], soff[7 * SS + k]);
+        for (int j = 0; j < MAXF; j++) {
+            if (j < nf) {
+                const int k = kst[j];
+                double un[5];
+#pragma unroll
+                for (int v = 0; v < 5; v++) un[v] = soff[v * SS + k];
+                accumulate(un, soff[5 * SS + k], soff[6 * SS + k], soff[7 * SS + k]);
+            }
         }
     } else if (PF) {
         // every neighbour index first, then each neighbour's values loaded
@@ -839,6 +862,19 @@ __global__ void __launch_bounds__(BLOCK)
             gz[v] = g[2 * (size_t)N + i];
         }
     }
+    // x_face - x_cell of the cell's face in slot j
+    auto face_offset = [&](int j, double& dx, double& dy, double& dz) {
+        if (OFFS) {
+            dx = offs[(size_t)(j * 3 + 0) * N + i];
+            dy = offs[(size_t)(j * 3 + 1) * N + i];
+            dz = offs[(size_t)(j * 3 + 2) * N + i];
+        } else {
+            const int f = m.ell_face[(size_t)j * N + i];
+            dx = m.geom[9 * (size_t)F + f] - xi;
+            dy = m.geom[10 * (size_t)F + f] - yi;
+            dz = m.geom[11 * (size_t)F + f] - zi;
+        }
+    };
     double phi[5] = {1.0, 1.0, 1.0, 1.0, 1.0};
     if (LIM != 0) {
         double eps2 = 0.0;
@@ -847,18 +883,14 @@ __global__ void __launch_bounds__(BLOCK)
             const double K3 = venkat_K * venkat_K * venkat_K;
             eps2 = (DIM == 3) ? K3 * V : K3 * (V * sqrt(V));
         }
-        for (int j = 0; j < nf; j++) {
-            double dx, dy, dz;
-            if (OFFS) {
-                dx = offs[(size_t)(j * 3 + 0) * N + i];
-                dy = offs[(size_t)(j * 3 + 1) * N + i];
-                dz = offs[(size_t)(j * 3 + 2) * N + i];
-            } else {
-                const int f = m.ell_face[(size_t)j * N + i];
-                dx = m.geom[9 * (size_t)F + f] - xi;
-                dy = m.geom[10 * (size_t)F + f] - yi;
-                dz = m.geom[11 * (size_t)F + f] - zi;
-            }
+        // the next face's offset is loaded while this face is limited
+        double ndx, ndy, ndz;
+        face_offset(0, ndx, ndy, ndz);
+#pragma unroll
+        for (int j = 0; j < MAXF; j++) {
+            if (j >= nf) break;
+            const double dx = ndx, dy = ndy, dz = ndz;
+            if (j + 1 < MAXF) face_offset(j + 1 < nf ? j + 1 : j, ndx, ndy, ndz);
             if (!SN && stage) {
                 soff[(j * 3 + 0) * BLOCK + threadIdx.x] = dx;
                 soff[(j * 3 + 1) * BLOCK + threadIdx.x] = dy;
@@ -893,21 +925,24 @@ __global__ void __launch_bounds__(BLOCK)
             for (int v = 0; v < 5; v++) lim[(size_t)vs.slot[v] * N + i] = phi[v];
         }
     }
-    for (int j = 0; j < nf; j++) {
+    // the reconstruction's offsets: from LDS when the limiter staged them,
+    // else all loaded before the face loop (a slot past the cell's last face
+    // reads its last face's and does not use them)
+    constexpr bool FROM_LDS = LIM != 0 && !SN;
+    double ox[MAXF], oy[MAXF], oz[MAXF];
+    if (!(FROM_LDS && stage)) {
+#pragma unroll
+        for (int j = 0; j < MAXF; j++) face_offset(j < nf ? j : nf - 1, ox[j], oy[j], oz[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < MAXF; j++) {
+        if (j >= nf) break;
         const int f = m.ell_face[(size_t)j * N + i];
-        double dx, dy, dz;
-        if (LIM != 0 && !SN && stage) {
+        double dx = ox[j], dy = oy[j], dz = oz[j];
+        if (FROM_LDS && stage) {
             dx = soff[(j * 3 + 0) * BLOCK + threadIdx.x];
             dy = soff[(j * 3 + 1) * BLOCK + threadIdx.x];
             dz = soff[(j * 3 + 2) * BLOCK + threadIdx.x];
-        } else if (OFFS) {
-            dx = offs[(size_t)(j * 3 + 0) * N + i];
-            dy = offs[(size_t)(j * 3 + 1) * N + i];
-            dz = offs[(size_t)(j * 3 + 2) * N + i];
-        } else {
-            dx = m.geom[9 * (size_t)F + f] - xi;
-            dy = m.geom[10 * (size_t)F + f] - yi;
-            dz = m.geom[11 * (size_t)F + f] - zi;
         }
         W* rec = LAYOUT == 0 ? (m.ell_sign[(size_t)j * N + i] > 0 ? FL : FR) + (size_t)f * 5
                              : FL + ((size_t)i * m.maxf + j) * 5;

@@ -64,7 +64,11 @@ def gmres(matvec, rhs, x0, max_iters, tol, precondition=None):
             H[i, j] = np.dot(w, V[i])
             w = w + V[i] * (-H[i, j])
         H[j + 1, j] = np.sqrt(np.dot(w, w))
-        V[j + 1] = w * (1.0 / H[j + 1, j])
+        # a breakdown (w == 0) makes this basis vector non-finite, as in
+        # LA/gmres.cpp:119-182; the rotation then zeroes g[j + 1], so the
+        # residual test below ends the loop before the vector is used
+        with np.errstate(divide="ignore", invalid="ignore"):
+            V[j + 1] = w * (1.0 / H[j + 1, j])
         _rotate(H, g, cs, sn, j)
         residual = abs(g[j + 1]) / beta
         k = j + 1
