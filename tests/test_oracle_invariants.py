@@ -120,6 +120,21 @@ def _two_state_fluxes(flux, left, right, normal=(1.0, 0.0)):
     return og, inter, f
 
 
+@pytest.mark.parametrize("flux", ["hanel", "ausmdv", "ldfss2", "rusanov"])
+@pytest.mark.parametrize("mach", [0.3, 0.9, 2.5])
+def test_consistency_equal_states_give_euler_flux(flux, mach):
+    """F(U, U) is the exact Euler flux for every calculator (the consistency
+    of a two-point flux), subsonic and supersonic, with a tangential velocity
+    so that the rotation to the face frame and back is exercised."""
+    a = GAS.speed_of_sound(280.0)
+    st = (1.1, 1.1 * GAS.R * 280.0, mach * a, -0.4 * a)
+    og, inter, f = _two_state_fluxes(flux, st, st)
+    n = og.norm()[:2, f]
+    rho, p, vx, vy = st
+    exact = euler_flux(rho, p, p / (rho * GAS.R), np.array([vx, vy]), n)
+    assert np.allclose(inter["flux"][:, f], exact, rtol=1e-12, atol=1e-9 * np.abs(exact).max())
+
+
 @pytest.mark.parametrize("flux", ["hanel", "ausmdv"])
 def test_supersonic_upwinding_gives_left_flux(flux):
     """M_L, M_R > 1 along the normal: AUSMDV and Hanel return F(U_upwind)
