@@ -1286,32 +1286,37 @@ __global__ void __launch_bounds__(TILE)
 
 // Per-entry copies of the tile lists' face data (face_kernel 1): for list
 // entry e (face f = tile_faces[e]) the start of each side's face values in
-// the layout-2 array (values at start + q * stride, stride N for a valid
-// cell's record and G for a ghost's, told apart by start >= maxf*5*N), and
-// t1, t2, area (+ the normal when it is not cross(t1, t2)) as planes of
-// length ntf.  A face's inputs are then contiguous loads at e plus the
-// gathers of the two sides' values.
+// the layout-2 array as one int2 (values at start + q * stride, stride N for
+// a valid cell's record and G for a ghost's, told apart by
+// start >= maxf*5*N), and the geometry as 16-byte pairs over the entries:
+// planes (t1x, t1y), (t1z, t2x), (t2y, t2z) [, (nx, ny) when the normal is
+// not cross(t1, t2)], then the area [and nz] as plain planes.  A face's
+// inputs are then 1 + 4 (+1 +1) contiguous loads at e plus the gathers of
+// the two sides' values.
 __global__ void __launch_bounds__(BLOCK)
-    k_entries(DevMesh m, int with_n, int* __restrict__ eidx, double* __restrict__ egeom) {
+    k_entries(DevMesh m, int with_n, int2* __restrict__ eidx, double* __restrict__ egeom) {
     const int e = blockIdx.x * BLOCK + threadIdx.x;
     const int NE = m.ntf, F = m.F;
     if (e >= NE) return;
     const int f = m.ent_faces[e];
-    eidx[e] = (int)soa_index(m, m.L[f], m.slotL[f], 0);
-    eidx[(size_t)NE + e] = (int)soa_index(m, m.R[f], m.slotR[f], 0);
-#pragma unroll
-    for (int k = 0; k < 6; k++) egeom[(size_t)k * NE + e] = m.geom[(3 + k) * (size_t)F + f];
-    egeom[(size_t)6 * NE + e] = m.geom[12 * (size_t)F + f];
-    if (with_n)
-#pragma unroll
-        for (int k = 0; k < 3; k++) egeom[(size_t)(7 + k) * NE + e] = m.geom[(size_t)k * F + f];
+    eidx[e] = make_int2((int)soa_index(m, m.L[f], m.slotL[f], 0),
+                        (int)soa_index(m, m.R[f], m.slotR[f], 0));
+    const double* gf = m.geom;
+    double2* g2 = reinterpret_cast<double2*>(egeom);
+    g2[e] = make_double2(gf[3 * (size_t)F + f], gf[4 * (size_t)F + f]);
+    g2[(size_t)NE + e] = make_double2(gf[5 * (size_t)F + f], gf[6 * (size_t)F + f]);
+    g2[2 * (size_t)NE + e] = make_double2(gf[7 * (size_t)F + f], gf[8 * (size_t)F + f]);
+    const int np = with_n ? 4 : 3;  // pair planes
+    if (with_n) g2[3 * (size_t)NE + e] = make_double2(gf[f], gf[(size_t)F + f]);
+    egeom[(size_t)2 * np * NE + e] = gf[12 * (size_t)F + f];
+    if (with_n) egeom[(size_t)(2 * np + 1) * NE + e] = gf[2 * (size_t)F + f];
 }
 
 // k_face_tile_fs over the per-entry face data (layout 2 values); NRM: the
 // normals are stored (they are not cross(t1, t2))
 template <int DIM, int FLUX, bool NRM, class R = double>
 __global__ void __launch_bounds__(TILE)
-    k_face_entry_fs(DevMesh m, FaceParams fp, const int* __restrict__ eidx,
+    k_face_entry_fs(DevMesh m, FaceParams fp, const int2* __restrict__ eidx,
                     const double* __restrict__ egeom, const R* __restrict__ FL,
                     double* __restrict__ dudt) {
     extern __shared__ double sflux[];  // NC x maxtf
@@ -1323,17 +1328,33 @@ __global__ void __launch_bounds__(TILE)
     const int nfl = m.tile_face_off[t + 1] - f0;
     const int gbase = m.maxf * 5 * N;
     constexpr int NG = NRM ? 10 : 7;
+    const double2* eg2 = reinterpret_cast<const double2*>(egeom);
+    constexpr int NP = NRM ? 4 : 3;  // pair planes of the entry geometry
     auto load = [&](int k, R (&v)[2][5], double (&g)[NG]) {
         const int e = f0 + k;
+        const int2 bb = eidx[e];
 #pragma unroll
         for (int side = 0; side < 2; side++) {
-            const int b = eidx[(size_t)side * NE + e];
+            const int b = side ? bb.y : bb.x;
             const size_t stride = b >= gbase ? (size_t)m.G : (size_t)N;
 #pragma unroll
             for (int q = 0; q < 5; q++) v[side][q] = FL[(size_t)b + q * stride];
         }
-#pragma unroll
-        for (int q = 0; q < NG; q++) g[q] = egeom[(size_t)q * NE + e];
+        // g: t1 (0-2), t2 (3-5), area (6) [, n (7-9)]
+        const double2 p0 = eg2[e], p1 = eg2[(size_t)NE + e], p2 = eg2[2 * (size_t)NE + e];
+        g[0] = p0.x;
+        g[1] = p0.y;
+        g[2] = p1.x;
+        g[3] = p1.y;
+        g[4] = p2.x;
+        g[5] = p2.y;
+        g[6] = egeom[(size_t)2 * NP * NE + e];
+        if (NRM) {
+            const double2 p3 = eg2[3 * (size_t)NE + e];
+            g[7] = p3.x;
+            g[8] = p3.y;
+            g[NG - 1] = egeom[(size_t)(2 * NP + 1) * NE + e];
+        }
     };
     int k = threadIdx.x;
     R vn[2][5];
@@ -2163,7 +2184,7 @@ struct ibis_residual {
     int opt_stage_cells = 0;    // option stage_cells (layout 2, pack_cells 0)
     int opt_face_kernel = 0; // option face_kernel: 0 tile face lists, 1 the same with
                              // per-entry face data, 64/128/256 slot-major
-    int* ent_idx = nullptr;      // per tile-list entry face data (face_kernel 1)
+    int2* ent_idx = nullptr;     // per tile-list entry face data (face_kernel 1)
     double* ent_geom = nullptr;
     double* face_rho = nullptr;
     double* face_T = nullptr;
@@ -2532,7 +2553,7 @@ void launch_face_slot_ts(const DevMesh& D, const FaceParams& fp, int ts, const R
 // otherwise the slot-major kernel with that many cells per block
 template <int DIM, int FLUX>
 void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, bool f32, int face_kernel,
-                      const int* eidx, const double* egeom, const double* FL, const double* FR,
+                      const int2* eidx, const double* egeom, const double* FL, const double* FR,
                       double* dudt, hipStream_t st) {
     const unsigned g = xcd_grid(D.ntiles);
     const size_t shm = (size_t)(DIM + 2) * D.maxtf * sizeof(double);
@@ -2566,7 +2587,7 @@ void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, bool f
 
 template <int DIM>
 void launch_face_fs(const DevMesh& D, const FaceParams& fp, int flux, int layout, bool f32,
-                    int face_kernel, const int* eidx, const double* egeom, const double* FL,
+                    int face_kernel, const int2* eidx, const double* egeom, const double* FL,
                     const double* FR, double* dudt, hipStream_t st) {
 #define IBIS_FF(K) \
     launch_face_fs_f<DIM, K>(D, fp, layout, f32, face_kernel, eidx, egeom, FL, FR, dudt, st)
@@ -3009,7 +3030,7 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
                 return ibis::fail(IBIS_ERR_INVALID, "face_kernel 1: mesh too large for 32-bit value indices");
             const size_t ne = std::max(D.ntf, 1);
             const int with_n = D.n_from_t ? 0 : 1;
-            HIP_TRY(hipMalloc(&r->ent_idx, sizeof(int) * 2 * ne));
+            HIP_TRY(hipMalloc(&r->ent_idx, sizeof(int2) * ne));
             HIP_TRY(hipMalloc(&r->ent_geom, sizeof(double) * (with_n ? 10 : 7) * ne));
             if (D.ntf > 0) k_entries<<<nblocks(D.ntf), BLOCK>>>(D, with_n, r->ent_idx, r->ent_geom);
             HIP_TRY(hipGetLastError());

@@ -145,6 +145,66 @@ def test_mixed_elements_ragged_stencils(opts):
     _check(gh, cfg, cases.hex_prim(gh, gh.num_cells + 6 * 64), exact=True, options=opts)
 
 
+@pytest.mark.parametrize("case", ["wedge", "hex"])
+def test_stored_normals_entry_kernel_matches_tile_kernel(case):
+    """A caller's mesh whose normals are not exactly cross(t1, t2) keeps its
+    stored normals.  The per-entry face kernel then loads them with the
+    entry geometry; the tile-list kernel reads them from the face geometry.
+    The two kernels must agree bit for bit, and the stored normals must be
+    used (the residual differs from the unperturbed mesh's)."""
+    import ctypes as C
+    from types import SimpleNamespace
+
+    import torch
+
+    from ibis_amd.batch import _concat_grids
+    from ibis_amd.finite_volume import FiniteVolume, GridBlock
+    from ibis_amd.native import MeshDescStruct, check, lib, ptr
+
+    if case == "wedge":
+        g_io, cfg, fs = cases.wedge_case(32, 16)
+        prim_ref = cases.perturbed_prim(g_io, fs, 32 * 16 + 2 * (32 + 16), seed=41)
+    else:
+        g_io, cfg = cases.hex_case(10, flux="hanel", limiter="venkatakrishnan")
+        prim_ref = cases.hex_prim(g_io, 10 ** 3 + 6 * 10 ** 2)
+    gb = GridBlock(g_io, cfg)
+    arr, N, G, F, _, _ = _concat_grids([gb])
+    geom = arr["face_geom"]
+    geom[0, ::3] = np.nextafter(geom[0, ::3], np.inf)  # n_x of every third face, one ulp
+    d = MeshDescStruct()
+    d.dim, d.ntags = gb.dim, len(arr["tag_off"]) - 1
+    d.n_cells, d.n_ghost, d.n_faces = N, G, F
+    for k, v in arr.items():
+        setattr(d, k, ptr(v))
+    m = C.c_void_p()
+    check(lib().ibis_mesh_create(C.byref(d), C.byref(m)), "ibis_mesh_create")
+    fv = ref_fv = None
+    try:
+        mesh = SimpleNamespace(_mesh=m, dim=gb.dim, Nt=gb.Nt)
+        fv = FiniteVolume(mesh, cfg)
+        prim0 = to_dev(gb.to_grid_order(prim_ref))
+        out = {}
+        for fk in (1, 0):
+            fv.set_option("face_state_layout", 2)
+            fv.set_option("face_kernel", fk)
+            prim = prim0.clone()
+            dudt = torch.zeros((gb.dim + 2, gb.Nt), dtype=torch.float64, device="cuda")
+            fv.compute_dudt(prim, None, dudt)
+            torch.cuda.synchronize()
+            out[fk] = dudt.cpu().numpy()[:, :N]
+        print(f"stored normals {case}: N={N} F={F}", flush=True)
+        assert np.array_equal(out[1], out[0])
+        ref_fv = FiniteVolume(gb, cfg)
+        d_ref, _ = gpu_dudt(gb, ref_fv, prim_ref)
+        assert not np.array_equal(out[1], gb.to_grid_order(d_ref)[:, :N])
+    finally:
+        for h in (fv, ref_fv):
+            if h is not None:
+                h.close()
+        lib().ibis_mesh_destroy(m)
+        gb.close()
+
+
 def test_plate_viscous():
     g_io, cfg, fs = cases.plate_case(64, 32)
     prim = cases.perturbed_prim(g_io, fs, 64 * 32 + 2 * (64 + 32), seed=2)
