@@ -694,6 +694,41 @@ __device__ __forceinline__ void load_cell(const DevMesh& m, const VarSet& vs,
     }
 }
 
+// Stage block lb's cell stencil in LDS: its TILE cells' and its halo's
+// values and centroids as 8 planes of SS = TILE + maxhalo doubles (own
+// cells first, then the halo in the tile's halo order).  Each thread
+// gathers its own cell and its first halo cell together, then writes both;
+// further halo cells one per pass.  Ends with the block barrier.
+template <bool PACKED>
+__device__ __forceinline__ void stage_stencil(const DevMesh& m, const VarSet& vs,
+                                              const double* __restrict__ pack,
+                                              const double* __restrict__ prim, int lb, int i,
+                                              double* soff) {
+    static_assert(TILE == BLOCK, "one stencil entry per thread and pass");
+    const int N = m.N, SS = TILE + m.maxhalo;
+    const int h0 = m.halo_off[lb], nh = m.halo_off[lb + 1] - h0;
+    auto put = [&](int k, const double* u, double x, double y, double z) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) soff[v * SS + k] = u[v];
+        soff[5 * SS + k] = x;
+        soff[6 * SS + k] = y;
+        soff[7 * SS + k] = z;
+    };
+    const int kh = threadIdx.x;
+    const int ch = kh < nh ? m.halo_ids[h0 + kh] : 0;
+    double u0[5], x0, y0, z0, u1[5], x1, y1, z1;
+    if (i < N) load_cell<PACKED>(m, vs, pack, prim, i, u0, x0, y0, z0);
+    if (kh < nh) load_cell<PACKED>(m, vs, pack, prim, ch, u1, x1, y1, z1);
+    if (i < N) put(threadIdx.x, u0, x0, y0, z0);
+    if (kh < nh) put(TILE + kh, u1, x1, y1, z1);
+    for (int k = kh + BLOCK; k < nh; k += BLOCK) {
+        double u[5], x, y, z;
+        load_cell<PACKED>(m, vs, pack, prim, m.halo_ids[h0 + k], u, x, y, z);
+        put(TILE + k, u, x, y, z);
+    }
+    __syncthreads();
+}
+
 template <int DIM, int LIM, int LAYOUT, bool OFFS, class W = double, bool PACKED = true,
           bool PF = false, bool SN = false>
 __global__ void __launch_bounds__(BLOCK)
@@ -732,32 +767,7 @@ __global__ void __launch_bounds__(BLOCK)
 #pragma unroll
         for (int j = 0; j < MAXF; j++) kst[j] = j < m.maxf ? m.nbr_lds[(size_t)j * N + il] : 0;
     }
-    if (SN) {
-        static_assert(TILE == BLOCK, "one stencil entry per thread and pass");
-        const int h0 = m.halo_off[lb], nh = m.halo_off[lb + 1] - h0;
-        auto put = [&](int k, const double* u, double x, double y, double z) {
-#pragma unroll
-            for (int v = 0; v < 5; v++) soff[v * SS + k] = u[v];
-            soff[5 * SS + k] = x;
-            soff[6 * SS + k] = y;
-            soff[7 * SS + k] = z;
-        };
-        // the thread's own cell and its first halo cell are gathered
-        // together, then written; further halo cells one per pass
-        const int kh = threadIdx.x;
-        const int ch = kh < nh ? m.halo_ids[h0 + kh] : 0;
-        double u0[5], x0, y0, z0, u1[5], x1, y1, z1;
-        if (i < N) load_cell<PACKED>(m, vs, pack, prim, i, u0, x0, y0, z0);
-        if (kh < nh) load_cell<PACKED>(m, vs, pack, prim, ch, u1, x1, y1, z1);
-        if (i < N) put(threadIdx.x, u0, x0, y0, z0);
-        if (kh < nh) put(TILE + kh, u1, x1, y1, z1);
-        for (int k = kh + BLOCK; k < nh; k += BLOCK) {
-            double u[5], x, y, z;
-            load_cell<PACKED>(m, vs, pack, prim, m.halo_ids[h0 + k], u, x, y, z);
-            put(TILE + k, u, x, y, z);
-        }
-        __syncthreads();
-    }
+    if (SN) stage_stencil<PACKED>(m, vs, pack, prim, lb, i, soff);
     if (i >= N) return;
     constexpr int NG = DIM == 3 ? 5 : 4;  // variables whose gradient is computed here
     double ui[5];
@@ -964,30 +974,53 @@ __global__ void __launch_bounds__(BLOCK)
 // kept as differences from the cell value, so the limiter's d1 is one
 // rounding away from the FP64 one.  stage != 0 keeps the face offsets of
 // the limiter pass in LDS as floats for the reconstruction pass.
-template <int DIM, int LIM, bool OFFS>
+template <int DIM, int LIM, bool OFFS, bool SN = false>
 __global__ void __launch_bounds__(BLOCK)
     k_cell_fs32(DevMesh m, VarSet vs, const double* __restrict__ prim,
                 const double* __restrict__ grad, double eps, double venkat_K, int nlogical,
                 float* __restrict__ FL, const double* __restrict__ offs, int stage) {
-    extern __shared__ float soff32[];
+    // SN: the cell stencil is staged in LDS as doubles, as in k_cell_fs;
+    // otherwise stage != 0 keeps the limiter pass's face offsets in LDS as
+    // floats for the reconstruction pass
+    extern __shared__ double sh32[];
+    float* soff32 = reinterpret_cast<float*>(sh32);
     const int lb = xcd_block(nlogical);
     if (lb >= nlogical) return;
     const int i = lb * BLOCK + threadIdx.x;
     const int N = m.N, F = m.F;
+    const int SS = TILE + m.maxhalo;
+    // per-cell inputs that do not depend on the stencil, loaded before the
+    // staging barrier (a thread past the last cell reads cell 0's)
+    const int il = i < N ? i : 0;
+    const int nf = m.nfaces[il];
+    const double l11 = m.lsq[il], l12 = m.lsq[N + il], l22 = m.lsq[2 * (size_t)N + il];
+    double l13 = 0.0, l23 = 0.0, l33 = 0.0;
+    if (DIM == 3) {
+        l13 = m.lsq[3 * (size_t)N + il];
+        l23 = m.lsq[4 * (size_t)N + il];
+        l33 = m.lsq[5 * (size_t)N + il];
+    }
+    int kst[MAXF];  // the neighbours' positions in the LDS stencil
+    if (SN) {
+#pragma unroll
+        for (int j = 0; j < MAXF; j++) kst[j] = j < m.maxf ? m.nbr_lds[(size_t)j * N + il] : 0;
+        stage_stencil<false>(m, vs, nullptr, prim, lb, i, sh32);
+    }
     if (i >= N) return;
     constexpr int NG = DIM == 3 ? 5 : 4;
     double ui[5];
     double xi, yi, zi;
-    load_cell<false>(m, vs, nullptr, prim, i, ui, xi, yi, zi);
-    const int nf = m.nfaces[i];
-    const float r11 = (float)m.lsq[i], r12 = (float)m.lsq[N + i],
-                r22 = (float)m.lsq[2 * (size_t)N + i];
-    float r13 = 0.0f, r23 = 0.0f, r33 = 0.0f;
-    if (DIM == 3) {
-        r13 = (float)m.lsq[3 * (size_t)N + i];
-        r23 = (float)m.lsq[4 * (size_t)N + i];
-        r33 = (float)m.lsq[5 * (size_t)N + i];
+    if (SN) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) ui[v] = sh32[v * SS + threadIdx.x];
+        xi = sh32[5 * SS + threadIdx.x];
+        yi = sh32[6 * SS + threadIdx.x];
+        zi = sh32[7 * SS + threadIdx.x];
+    } else {
+        load_cell<false>(m, vs, nullptr, prim, i, ui, xi, yi, zi);
     }
+    const float r11 = (float)l11, r12 = (float)l12, r22 = (float)l22;
+    const float r13 = (float)l13, r23 = (float)l23, r33 = (float)l33;
     const float beta = (r12 * r23 - r13 * r23) / (r11 * r22);  // GR/gradient.h:89 (sic)
     const float r11s = r11 * r11;
     const float i22 = 1.0f / (r22 * r22);
@@ -997,10 +1030,8 @@ __global__ void __launch_bounds__(BLOCK)
     float gx[5], gy[5], gz[5], dmin[5], dmax[5];
 #pragma unroll
     for (int v = 0; v < 5; v++) gx[v] = gy[v] = gz[v] = dmin[v] = dmax[v] = 0.0f;
-    for (int j = 0; j < nf; j++) {
-        const int n = m.ell_nbr[(size_t)j * N + i];
-        double un[5], xn, yn, zn;
-        load_cell<false>(m, vs, nullptr, prim, n, un, xn, yn, zn);
+    // one neighbour's contribution to the gradient and the extremes
+    auto accumulate = [&](const double* un, double xn, double yn, double zn) {
         const float dx = (float)(xn - xi), dy = (float)(yn - yi), dz = (float)(zn - zi);
         const float alpha_1 = dx / r11s;
         const float alpha_2 = i22 * (dy - q12 * dx);
@@ -1019,6 +1050,25 @@ __global__ void __launch_bounds__(BLOCK)
             }
             dmin[v] = diff < dmin[v] ? diff : dmin[v];
             dmax[v] = dmax[v] < diff ? diff : dmax[v];
+        }
+    };
+    if (SN) {
+#pragma unroll
+        for (int j = 0; j < MAXF; j++) {
+            if (j < nf) {
+                const int k = kst[j];
+                double un[5];
+#pragma unroll
+                for (int v = 0; v < 5; v++) un[v] = sh32[v * SS + k];
+                accumulate(un, sh32[5 * SS + k], sh32[6 * SS + k], sh32[7 * SS + k]);
+            }
+        }
+    } else {
+        for (int j = 0; j < nf; j++) {
+            const int n = m.ell_nbr[(size_t)j * N + i];
+            double un[5], xn, yn, zn;
+            load_cell<false>(m, vs, nullptr, prim, n, un, xn, yn, zn);
+            accumulate(un, xn, yn, zn);
         }
     }
 #pragma unroll
@@ -1040,6 +1090,7 @@ __global__ void __launch_bounds__(BLOCK)
             dz = (float)(m.geom[11 * (size_t)F + f] - zi);
         }
     };
+    const bool staged = LIM != 0 && !SN && stage;
     float phi[5] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f};
     if (LIM != 0) {
         float eps2 = 0.0f;
@@ -1049,10 +1100,15 @@ __global__ void __launch_bounds__(BLOCK)
             eps2 = (float)((DIM == 3) ? K3 * V : K3 * (V * sqrt(V)));
         }
         const float epsf = (float)eps;
-        for (int j = 0; j < nf; j++) {
-            float dx, dy, dz;
-            offset(j, dx, dy, dz);
-            if (stage) {
+        // the next face's offset is loaded while this face is limited
+        float ndx, ndy, ndz;
+        offset(0, ndx, ndy, ndz);
+#pragma unroll
+        for (int j = 0; j < MAXF; j++) {
+            if (j >= nf) break;
+            const float dx = ndx, dy = ndy, dz = ndz;
+            if (j + 1 < MAXF) offset(j + 1 < nf ? j + 1 : j, ndx, ndy, ndz);
+            if (staged) {
                 soff32[(j * 3 + 0) * BLOCK + threadIdx.x] = dx;
                 soff32[(j * 3 + 1) * BLOCK + threadIdx.x] = dy;
                 soff32[(j * 3 + 2) * BLOCK + threadIdx.x] = dz;
@@ -1078,14 +1134,22 @@ __global__ void __launch_bounds__(BLOCK)
             }
         }
     }
-    for (int j = 0; j < nf; j++) {
-        float dx, dy, dz;
-        if (LIM != 0 && stage) {
+    // the reconstruction's offsets: from LDS when staged, else all loaded
+    // before the face loop (a slot past the cell's last face reads its last
+    // face's and does not use them)
+    float ox[MAXF], oy[MAXF], oz[MAXF];
+    if (!staged) {
+#pragma unroll
+        for (int j = 0; j < MAXF; j++) offset(j < nf ? j : nf - 1, ox[j], oy[j], oz[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < MAXF; j++) {
+        if (j >= nf) break;
+        float dx = ox[j], dy = oy[j], dz = oz[j];
+        if (staged) {
             dx = soff32[(j * 3 + 0) * BLOCK + threadIdx.x];
             dy = soff32[(j * 3 + 1) * BLOCK + threadIdx.x];
             dz = soff32[(j * 3 + 2) * BLOCK + threadIdx.x];
-        } else {
-            offset(j, dx, dy, dz);
         }
 #pragma unroll
         for (int v = 0; v < 5; v++) {
@@ -2467,22 +2531,33 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
                       int nlog, const double* prim, hipStream_t st) {
     const size_t shm = cell_stage_bytes(D, LIM, r->opt_stage);
     const int stage = shm ? 1 : 0;
+    // the LDS cell stencil (option stage_cells), in all precisions
+    const size_t shs = (size_t)8 * (TILE + D.maxhalo) * sizeof(double);
+    const bool sn = r->opt_stage_cells && shs <= 65536;
     if (r->prec == IBIS_PREC_FP32_RECON) {
         float* fl = reinterpret_cast<float*>(r->FL);
-        k_cell_fs32<DIM, LIM, OFFS><<<xcd_grid(nlog), BLOCK, shm / 2, st>>>(
-            D, vs, prim, r->grad, d.lim_eps, d.venkat_K, nlog, fl, r->offs, stage);
+        if (sn)
+            k_cell_fs32<DIM, LIM, OFFS, true><<<xcd_grid(nlog), BLOCK, shs, st>>>(
+                D, vs, prim, r->grad, d.lim_eps, d.venkat_K, nlog, fl, r->offs, 0);
+        else
+            k_cell_fs32<DIM, LIM, OFFS><<<xcd_grid(nlog), BLOCK, shm / 2, st>>>(
+                D, vs, prim, r->grad, d.lim_eps, d.venkat_K, nlog, fl, r->offs, stage);
         return;
     }
     if (r->prec == IBIS_PREC_FP32_FLUX) {
         float* fl = reinterpret_cast<float*>(r->FL);
-        k_cell_fs<DIM, LIM, 2, OFFS, float, false><<<xcd_grid(nlog), BLOCK, shm, st>>>(
-            D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, fl, fl, r->offs, prim,
-            stage);
+        if (sn)
+            k_cell_fs<DIM, LIM, 2, OFFS, float, false, false, true><<<xcd_grid(nlog), BLOCK, shs, st>>>(
+                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, fl, fl, r->offs,
+                prim, 0);
+        else
+            k_cell_fs<DIM, LIM, 2, OFFS, float, false><<<xcd_grid(nlog), BLOCK, shm, st>>>(
+                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, fl, fl, r->offs,
+                prim, stage);
         return;
     }
     if (!r->opt_pack && LAYOUT == 2) {
-        const size_t shs = (size_t)8 * (TILE + D.maxhalo) * sizeof(double);
-        if (r->opt_stage_cells && shs <= 65536) {
+        if (sn) {
             k_cell_fs<DIM, LIM, 2, OFFS, double, false, false, true>
                 <<<xcd_grid(nlog), BLOCK, shs, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps,
                                                      d.venkat_K, nlog, 0, r->FL, r->FR, r->offs,
