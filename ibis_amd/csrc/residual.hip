@@ -742,7 +742,8 @@ __global__ void __launch_bounds__(BLOCK)
     // SN: the block (one tile of TILE cells) first copies its cells' and its
     // stencil halo's values and centroids into LDS (8 planes of
     // TILE + maxhalo doubles) with one load per cell, and the neighbour loop
-    // reads them from there (no offset staging then).
+    // reads them from there.  With stage != 0 as well, the offsets take the
+    // same LDS once the whole block has finished the neighbour loop.
     extern __shared__ double soff[];
     const int lb = xcd_block(nlogical);
     if (lb >= nlogical) return;
@@ -768,7 +769,11 @@ __global__ void __launch_bounds__(BLOCK)
         for (int j = 0; j < MAXF; j++) kst[j] = j < m.maxf ? m.nbr_lds[(size_t)j * N + il] : 0;
     }
     if (SN) stage_stencil<PACKED>(m, vs, pack, prim, lb, i, soff);
-    if (i >= N) return;
+    // with the offsets staged over the stencil, a thread past the last cell
+    // stays to the second barrier (it works on LDS slots that are not read
+    // back and stores nothing)
+    const bool reuse = SN && LIM != 0 && stage;
+    if (i >= N && !reuse) return;
     constexpr int NG = DIM == 3 ? 5 : 4;  // variables whose gradient is computed here
     double ui[5];
     double xi, yi, zi;
@@ -857,6 +862,10 @@ __global__ void __launch_bounds__(BLOCK)
             accumulate(un, xn, yn, zn);
         }
     }
+    if (reuse) {
+        __syncthreads();  // the stencil is read: its LDS now takes the offsets
+        if (i >= N) return;
+    }
 #pragma unroll
     for (int v = 0; v < 5; v++) {
         double* g = grad + (size_t)vs.slot[v] * 3 * N;
@@ -901,7 +910,7 @@ __global__ void __launch_bounds__(BLOCK)
             if (j >= nf) break;
             const double dx = ndx, dy = ndy, dz = ndz;
             if (j + 1 < MAXF) face_offset(j + 1 < nf ? j + 1 : j, ndx, ndy, ndz);
-            if (!SN && stage) {
+            if (stage) {
                 soff[(j * 3 + 0) * BLOCK + threadIdx.x] = dx;
                 soff[(j * 3 + 1) * BLOCK + threadIdx.x] = dy;
                 soff[(j * 3 + 2) * BLOCK + threadIdx.x] = dz;
@@ -938,7 +947,7 @@ __global__ void __launch_bounds__(BLOCK)
     // the reconstruction's offsets: from LDS when the limiter staged them,
     // else all loaded before the face loop (a slot past the cell's last face
     // reads its last face's and does not use them)
-    constexpr bool FROM_LDS = LIM != 0 && !SN;
+    constexpr bool FROM_LDS = LIM != 0;
     double ox[MAXF], oy[MAXF], oz[MAXF];
     if (!(FROM_LDS && stage)) {
 #pragma unroll
@@ -2531,13 +2540,15 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
                       int nlog, const double* prim, hipStream_t st) {
     const size_t shm = cell_stage_bytes(D, LIM, r->opt_stage);
     const int stage = shm ? 1 : 0;
-    // the LDS cell stencil (option stage_cells), in all precisions
-    const size_t shs = (size_t)8 * (TILE + D.maxhalo) * sizeof(double);
-    const bool sn = r->opt_stage_cells && shs <= 65536;
+    // the LDS cell stencil (option stage_cells), in all precisions; the
+    // FP64 and FP32-flux kernels reuse it for the offsets (stage_offsets)
+    const size_t shs0 = (size_t)8 * (TILE + D.maxhalo) * sizeof(double);
+    const bool sn = r->opt_stage_cells && shs0 <= 65536;
+    const size_t shs = std::max(shs0, shm);
     if (r->prec == IBIS_PREC_FP32_RECON) {
         float* fl = reinterpret_cast<float*>(r->FL);
         if (sn)
-            k_cell_fs32<DIM, LIM, OFFS, true><<<xcd_grid(nlog), BLOCK, shs, st>>>(
+            k_cell_fs32<DIM, LIM, OFFS, true><<<xcd_grid(nlog), BLOCK, shs0, st>>>(
                 D, vs, prim, r->grad, d.lim_eps, d.venkat_K, nlog, fl, r->offs, 0);
         else
             k_cell_fs32<DIM, LIM, OFFS><<<xcd_grid(nlog), BLOCK, shm / 2, st>>>(
@@ -2549,7 +2560,7 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
         if (sn)
             k_cell_fs<DIM, LIM, 2, OFFS, float, false, false, true><<<xcd_grid(nlog), BLOCK, shs, st>>>(
                 D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, fl, fl, r->offs,
-                prim, 0);
+                prim, stage);
         else
             k_cell_fs<DIM, LIM, 2, OFFS, float, false><<<xcd_grid(nlog), BLOCK, shm, st>>>(
                 D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, fl, fl, r->offs,
@@ -2561,7 +2572,7 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
             k_cell_fs<DIM, LIM, 2, OFFS, double, false, false, true>
                 <<<xcd_grid(nlog), BLOCK, shs, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps,
                                                      d.venkat_K, nlog, 0, r->FL, r->FR, r->offs,
-                                                     prim, 0);
+                                                     prim, stage);
             return;
         }
         if (r->opt_cell_prefetch)
