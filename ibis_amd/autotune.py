@@ -27,12 +27,15 @@ def variant_key(opt: Mapping[str, int]) -> str:
 def coordinate_descent(set_option: Callable[[str, int], None], measure: Callable[[], float],
                        fixed: Mapping[str, int] = None,
                        log: Callable[[str, float], None] = None,
-                       sweeps: int = 2) -> Tuple[Dict[str, int], float, Dict[str, float]]:
+                       sweeps: int = 2,
+                       margin: float = 0.005) -> Tuple[Dict[str, int], float, Dict[str, float]]:
     """Sweeps over OPTION_SPACE, each trying every value of one option with
     the others at their best so far.  ``measure`` times the workload under
-    the options set (lower is better); ``fixed`` pins options.  Leaves the
-    best options set and returns them, their time and every measured
-    variant (keyed by variant_key)."""
+    the options set (lower is better); ``fixed`` pins options.  A candidate
+    replaces the best only when it is faster by more than ``margin``
+    (relative), so timing noise does not move the choice away from the
+    earlier values of an option.  Leaves the best options set and returns
+    them, their time and every measured variant (keyed by variant_key)."""
     fixed = dict(fixed or {})
     cur = {k: fixed.get(k, vals[0]) for k, vals in OPTION_SPACE}
     seen: Dict[str, float] = {}
@@ -55,30 +58,34 @@ def coordinate_descent(set_option: Callable[[str, int], None], measure: Callable
             for v in vals:
                 cand = dict(cur, **{k: v})
                 t = score(cand)
-                if t < best:
+                if t < best * (1.0 - margin):
                     best, cur = t, cand
     for k, v in cur.items():
         set_option(k, v)
     return cur, best, seen
 
 
-def tune_residual(fv, prim, gb, dudt, reps: int = 5, fixed: Mapping[str, int] = None,
-                  log: Callable[[str, float], None] = None):
-    """Tune a FiniteVolume on one state: each candidate runs 2 untimed and
-    ``reps`` timed residuals (HIP events on the current stream; µs per
-    residual)."""
+def tune_residual(fv, prim, gb, dudt, reps: int = 10, batches: int = 3,
+                  fixed: Mapping[str, int] = None, log: Callable[[str, float], None] = None):
+    """Tune a FiniteVolume on one state: each candidate runs 2 untimed
+    residuals, then ``batches`` batches of ``reps`` timed ones (HIP events
+    on the current stream); its time is the fastest batch's µs per
+    residual."""
     import torch
     stream = torch.cuda.current_stream()
 
     def measure():
         for _ in range(2):
             fv.compute_dudt(prim, gb, dudt)
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        for _ in range(reps):
-            fv.compute_dudt(prim, gb, dudt)
-        b.record(stream)
-        torch.cuda.synchronize()
-        return a.elapsed_time(b) * 1e3 / reps
+        out = []
+        for _ in range(batches):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            for _ in range(reps):
+                fv.compute_dudt(prim, gb, dudt)
+            b.record(stream)
+            torch.cuda.synchronize()
+            out.append(a.elapsed_time(b) * 1e3 / reps)
+        return min(out)
 
     return coordinate_descent(fv.set_option, measure, fixed, log)

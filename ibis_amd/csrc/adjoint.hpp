@@ -358,11 +358,26 @@ __global__ void __launch_bounds__(BLOCK)
 template <int DIM, int LIM, bool ACTIVE, bool VISC>
 __global__ void __launch_bounds__(BLOCK)
     k_vjp_cell(DevMesh m, VarSet vs, double eps, double venkat_K, AdjBufs B) {
+    // each thread's face offsets (maxf x 3 doubles), face ids (one's
+    // complement when the cell is the face's right cell) and neighbours
+    // (maxf ints each) are copied to LDS once, in planes of BLOCK, and read
+    // from there by every variable's loops; a thread reads only its own slots
+    extern __shared__ double vsh[];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int N = m.N, Nt = m.Nt, F = m.F;
+    const int N = m.N, Nt = m.Nt, F = m.F, MF = m.maxf;
     if (i >= N) return;
     const int nf = m.nfaces[i];
     const int ngrad = DIM == 3 ? 5 : 4;
+    double* sd = vsh;  // (j * 3 + k) * BLOCK + tid
+    int* sf = reinterpret_cast<int*>(vsh + (size_t)MF * 3 * BLOCK);  // j * BLOCK + tid
+    int* sn = sf + (size_t)MF * BLOCK;
+    const int tid = threadIdx.x;
+    for (int j = 0; j < nf; j++) {
+        const int f = m.ell_face[(size_t)j * N + i];
+        sf[j * BLOCK + tid] = m.ell_sign[(size_t)j * N + i] > 0 ? f : ~f;
+        if (ACTIVE) sn[j * BLOCK + tid] = m.ell_nbr[(size_t)j * N + i];
+        for (int k = 0; k < 3; k++) sd[(j * 3 + k) * BLOCK + tid] = B.offs[(size_t)(j * 3 + k) * N + i];
+    }
     for (int v = 0; v < 5; v++) {
         const int row = vs.row[v], slot = vs.slot[v];
         const double* u = B.prim_c + (size_t)row * Nt;
@@ -373,10 +388,10 @@ __global__ void __launch_bounds__(BLOCK)
         double self = 0.0, phib = 0.0, gb[3] = {0.0, 0.0, 0.0};
         // MUSCL: q = u + phi * (g . d) on each face of the cell
         for (int j = 0; j < nf; j++) {
-            const int f = m.ell_face[(size_t)j * N + i];
-            const double qb = (m.ell_sign[(size_t)j * N + i] > 0 ? B.QLb : B.QRb)[(size_t)v * F + f];
-            const double d[3] = {B.offs[(size_t)(j * 3 + 0) * N + i], B.offs[(size_t)(j * 3 + 1) * N + i],
-                                 B.offs[(size_t)(j * 3 + 2) * N + i]};
+            const int fe = sf[j * BLOCK + tid];
+            const double qb = (fe >= 0 ? B.QLb : B.QRb)[(size_t)v * F + (fe >= 0 ? fe : ~fe)];
+            const double d[3] = {sd[(j * 3 + 0) * BLOCK + tid], sd[(j * 3 + 1) * BLOCK + tid],
+                                 sd[(j * 3 + 2) * BLOCK + tid]};
             self += qb;
             phib += qb * (g[0] * d[0] + g[1] * d[1] + g[2] * d[2]);
             for (int k = 0; k < 3; k++) gb[k] += qb * phi * d[k];
@@ -389,7 +404,7 @@ __global__ void __launch_bounds__(BLOCK)
             double umax = ui, umin = ui;
             int jmax = -1, jmin = -1;
             for (int j = 0; j < nf; j++) {
-                const double un = u[m.ell_nbr[(size_t)j * N + i]];
+                const double un = u[sn[j * BLOCK + tid]];
                 if (!(umax > un)) {
                     umax = un;
                     jmax = j;
@@ -410,9 +425,8 @@ __global__ void __launch_bounds__(BLOCK)
             double d1s = 0.0, d2s = 0.0, dstar[3] = {0.0, 0.0, 0.0};
             int sgs = 0;
             for (int j = 0; j < nf; j++) {
-                const double d[3] = {B.offs[(size_t)(j * 3 + 0) * N + i],
-                                     B.offs[(size_t)(j * 3 + 1) * N + i],
-                                     B.offs[(size_t)(j * 3 + 2) * N + i]};
+                const double d[3] = {sd[(j * 3 + 0) * BLOCK + tid], sd[(j * 3 + 1) * BLOCK + tid],
+                                     sd[(j * 3 + 2) * BLOCK + tid]};
                 const double d2 = g[0] * d[0] + g[1] * d[1] + g[2] * d[2];
                 const int sg = (d2 > 0.0) - (d2 < 0.0);
                 if (sg == 0) continue;
@@ -476,8 +490,9 @@ __global__ void __launch_bounds__(BLOCK)
         for (int k = 0; k < 4; k++) {
             double self = 0.0, gb[3] = {0.0, 0.0, 0.0};
             for (int j = 0; j < nf; j++) {
-                const int f = m.ell_face[(size_t)j * N + i];
-                const bool left = m.ell_sign[(size_t)j * N + i] > 0;
+                const int fe = sf[j * BLOCK + tid];
+                const bool left = fe >= 0;
+                const int f = left ? fe : ~fe;
                 self += (left ? B.VLb : B.VRb)[(size_t)k * F + f];
                 for (int d = 0; d < 3; d++)
                     gb[d] += (left ? B.GLb : B.GRb)[(size_t)(k * 3 + d) * F + f];

@@ -3222,8 +3222,10 @@ void launch_vjp(ibis_residual* r, const VarSet& vs, const FaceParams& fp, const 
     }
     ViscParams vp{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr};
     if (visc) k_vjp_visc_face<DIM><<<nf, BLOCK, 0, st>>>(D, vp, B);
+    // LDS of k_vjp_cell: maxf x 3 doubles and 2 x maxf ints per thread
+    const size_t vshm = (size_t)D.maxf * BLOCK * (3 * sizeof(double) + 2 * sizeof(int));
 #define CELL(LIM, ACT, VIS) \
-    k_vjp_cell<DIM, LIM, ACT, VIS><<<nc, BLOCK, 0, st>>>(D, vs, d.lim_eps, d.venkat_K, B)
+    k_vjp_cell<DIM, LIM, ACT, VIS><<<nc, BLOCK, vshm, st>>>(D, vs, d.lim_eps, d.venkat_K, B)
     const int lim = d.limiter;
     if (visc) {
         if (lim == 0) CELL(0, false, true);

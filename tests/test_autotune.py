@@ -34,3 +34,22 @@ def test_fixed_options_are_not_changed():
                                    fixed={"face_kernel": 64})
     assert set(calls) == {64}
     assert opt["face_kernel"] == 64 and opt["face_state_layout"] == 0
+
+
+def test_near_ties_keep_the_earlier_value():
+    # stage_offsets 0 measures 0.3% faster than 1 (within the margin) and
+    # face_kernel 0 2% faster than 1 (beyond it)
+    state = {}
+
+    def measure():
+        t = 1000.0
+        if state["stage_offsets"] == 0:
+            t -= 3.0
+        if state["face_kernel"] == 0:
+            t -= 20.0
+        return t
+
+    opt, t, _ = coordinate_descent(lambda k, v: state.__setitem__(k, v), measure)
+    assert opt["stage_offsets"] == 1
+    assert opt["face_kernel"] == 0
+    assert t == 980.0
