@@ -1359,9 +1359,11 @@ __global__ void __launch_bounds__(TILE)
 
 // Per-entry copies of the tile lists' face data (face_kernel 1): for list
 // entry e (face f = tile_faces[e]) the start of each side's face values in
-// the layout-2 array as one int2 (values at start + q * stride, stride N for
-// a valid cell's record and G for a ghost's, told apart by
-// start >= maxf*5*N), and the geometry as 16-byte pairs over the entries:
+// the layout-2 array as one int2, the lower-numbered cell's first and the
+// other's second, one's-complemented when the lower cell is the face's right
+// cell (values at start + q * stride, stride N for a valid cell's record and
+// G for a ghost's, told apart by start >= maxf*5*N), and the geometry as
+// 16-byte pairs over the entries:
 // planes (t1x, t1y), (t1z, t2x), (t2y, t2z) [, (nx, ny) when the normal is
 // not cross(t1, t2)], then the area [and nz] as plain planes.  A face's
 // inputs are then 1 + 4 (+1 +1) contiguous loads at e plus the gathers of
@@ -1372,8 +1374,9 @@ __global__ void __launch_bounds__(BLOCK)
     const int NE = m.ntf, F = m.F;
     if (e >= NE) return;
     const int f = m.ent_faces[e];
-    eidx[e] = make_int2((int)soa_index(m, m.L[f], m.slotL[f], 0),
-                        (int)soa_index(m, m.R[f], m.slotR[f], 0));
+    const int bl = (int)soa_index(m, m.L[f], m.slotL[f], 0);
+    const int br = (int)soa_index(m, m.R[f], m.slotR[f], 0);
+    eidx[e] = m.L[f] < m.R[f] ? make_int2(bl, br) : make_int2(br, ~bl);
     const double* gf = m.geom;
     double2* g2 = reinterpret_cast<double2*>(egeom);
     g2[e] = make_double2(gf[3 * (size_t)F + f], gf[4 * (size_t)F + f]);
@@ -1406,12 +1409,19 @@ __global__ void __launch_bounds__(TILE)
     auto load = [&](int k, R (&v)[2][5], double (&g)[NG]) {
         const int e = f0 + k;
         const int2 bb = eidx[e];
+        const bool swap = bb.y < 0;  // the first start is the right cell's
+        R u[2][5];
 #pragma unroll
         for (int side = 0; side < 2; side++) {
-            const int b = side ? bb.y : bb.x;
+            const int b = side ? (swap ? ~bb.y : bb.y) : bb.x;
             const size_t stride = b >= gbase ? (size_t)m.G : (size_t)N;
 #pragma unroll
-            for (int q = 0; q < 5; q++) v[side][q] = FL[(size_t)b + q * stride];
+            for (int q = 0; q < 5; q++) u[side][q] = FL[(size_t)b + q * stride];
+        }
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            v[0][q] = swap ? u[1][q] : u[0][q];
+            v[1][q] = swap ? u[0][q] : u[1][q];
         }
         // g: t1 (0-2), t2 (3-5), area (6) [, n (7-9)]
         const double2 p0 = eg2[e], p1 = eg2[(size_t)NE + e], p2 = eg2[2 * (size_t)NE + e];
@@ -2144,9 +2154,10 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
                 slotR[f] = (unsigned char)(j - d->cell_face_off[c]);
         }
     // entry order of the tile lists for the per-entry face kernel: within a
-    // tile, faces by (left slot, right slot, left cell), so that neighbouring
-    // lanes read neighbouring cells of one face-value plane; ent_lds is each
-    // (cell, slot)'s position in that order
+    // tile, faces by (slot of the lower-numbered cell, slot of the other,
+    // lower cell), so that neighbouring lanes read neighbouring cells of one
+    // face-value plane on both sides whichever side is the face's left;
+    // ent_lds is each (cell, slot)'s position in that order
     std::vector<int> efaces(tfaces.size());
     std::vector<unsigned short> elds((size_t)maxf * N, 0);
     {
@@ -2158,7 +2169,10 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
             std::iota(idx.begin(), idx.end(), 0);
             auto key = [&](int k) {
                 const int f = tfaces[a + k];
-                return std::make_tuple(slotL[f], slotR[f], d->face_left[f], f);
+                const bool lo_is_left = d->face_left[f] < d->face_right[f];
+                return std::make_tuple(lo_is_left ? slotL[f] : slotR[f],
+                                       lo_is_left ? slotR[f] : slotL[f],
+                                       std::min(d->face_left[f], d->face_right[f]), f);
             };
             std::sort(idx.begin(), idx.end(), [&](int x, int y) { return key(x) < key(y); });
             for (int k = 0; k < n; k++) {
