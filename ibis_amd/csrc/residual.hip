@@ -988,9 +988,10 @@ __global__ void __launch_bounds__(BLOCK)
     k_cell_fs32(DevMesh m, VarSet vs, const double* __restrict__ prim,
                 const double* __restrict__ grad, double eps, double venkat_K, int nlogical,
                 float* __restrict__ FL, const double* __restrict__ offs, int stage) {
-    // SN: the cell stencil is staged in LDS as doubles, as in k_cell_fs;
-    // otherwise stage != 0 keeps the limiter pass's face offsets in LDS as
-    // floats for the reconstruction pass
+    // SN: the cell stencil is staged in LDS as doubles, as in k_cell_fs.
+    // stage != 0 keeps the limiter pass's face offsets in LDS as floats for
+    // the reconstruction pass; with SN they take the stencil's LDS once the
+    // whole block has finished the neighbour loop
     extern __shared__ double sh32[];
     float* soff32 = reinterpret_cast<float*>(sh32);
     const int lb = xcd_block(nlogical);
@@ -1015,7 +1016,11 @@ __global__ void __launch_bounds__(BLOCK)
         for (int j = 0; j < MAXF; j++) kst[j] = j < m.maxf ? m.nbr_lds[(size_t)j * N + il] : 0;
         stage_stencil<false>(m, vs, nullptr, prim, lb, i, sh32);
     }
-    if (i >= N) return;
+    // with the offsets staged over the stencil, a thread past the last cell
+    // stays to the second barrier (it works on LDS slots that are not read
+    // back and stores nothing)
+    const bool reuse = SN && LIM != 0 && stage;
+    if (i >= N && !reuse) return;
     constexpr int NG = DIM == 3 ? 5 : 4;
     double ui[5];
     double xi, yi, zi;
@@ -1080,6 +1085,10 @@ __global__ void __launch_bounds__(BLOCK)
             accumulate(un, xn, yn, zn);
         }
     }
+    if (reuse) {
+        __syncthreads();  // the stencil is read: its LDS now takes the offsets
+        if (i >= N) return;
+    }
 #pragma unroll
     for (int v = NG; v < 5; v++) {  // 2D vz: the gradient stored in its slot
         const double* g = grad + (size_t)vs.slot[v] * 3 * N;
@@ -1099,7 +1108,7 @@ __global__ void __launch_bounds__(BLOCK)
             dz = (float)(m.geom[11 * (size_t)F + f] - zi);
         }
     };
-    const bool staged = LIM != 0 && !SN && stage;
+    const bool staged = LIM != 0 && stage;
     float phi[5] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f};
     if (LIM != 0) {
         float eps2 = 0.0f;
@@ -2562,8 +2571,8 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
     if (r->prec == IBIS_PREC_FP32_RECON) {
         float* fl = reinterpret_cast<float*>(r->FL);
         if (sn)
-            k_cell_fs32<DIM, LIM, OFFS, true><<<xcd_grid(nlog), BLOCK, shs0, st>>>(
-                D, vs, prim, r->grad, d.lim_eps, d.venkat_K, nlog, fl, r->offs, 0);
+            k_cell_fs32<DIM, LIM, OFFS, true><<<xcd_grid(nlog), BLOCK, std::max(shs0, shm / 2), st>>>(
+                D, vs, prim, r->grad, d.lim_eps, d.venkat_K, nlog, fl, r->offs, stage);
         else
             k_cell_fs32<DIM, LIM, OFFS><<<xcd_grid(nlog), BLOCK, shm / 2, st>>>(
                 D, vs, prim, r->grad, d.lim_eps, d.venkat_K, nlog, fl, r->offs, stage);
