@@ -472,7 +472,7 @@ def main():
                    "variant": best[1],
                    "autotune_us": variants},
         "roofline": {"bound": "hbm", "kernel": "residual (all kernels of one compute_dudt: "
-                                               "boundary = k_bc + k_ghost_fs, cell_stencil = "
+                                               "boundary = k_bc_ghost_fs, cell_stencil = "
                                                "k_cell_fs, face_flux = k_face_*_fs)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),

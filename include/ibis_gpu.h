@@ -290,7 +290,8 @@ ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** lim
  *                            cells' and their outside neighbours' values
  *                            and centroids into LDS once, and the stencil
  *                            loop reads them there (layout 2, pack_cells 0;
- *                            replaces stage_offsets)
+ *                            with stage_offsets 1 the offsets then reuse
+ *                            that LDS after the stencil loop)
  *   "cell_prefetch"       1: the cell kernel loads all neighbour indices
  *                            first and each neighbour's values while the
  *                            previous one is summed (layout 2, pack_cells 0)
@@ -301,8 +302,8 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
 
 /* Per-kernel timing with HIP events recorded around every launch of
  * ibis_compute_dudt on its stream.  Kernel slots:
- *   0 boundary actions before reconstruction (k_bc; on the fused path also
- *     k_pack and k_ghost_fs)
+ *   0 boundary actions before reconstruction (k_bc; on the fused path
+ *     k_bc_ghost_fs, or k_bc, k_pack and k_ghost_fs)
  *   1 cell stencil: k_cell_grad_var, or k_cell_fs on the fused path
  *     (gradients, limiter and face values)
  *   2 face flux: k_face_flux, or k_face_*_fs on the fused path (flux and
