@@ -108,6 +108,7 @@ struct DevMesh {
     const unsigned char* slotL;
     const unsigned char* slotR;
     int n_from_t;  // face normals equal cross(t1, t2) bit for bit (grid-built meshes)
+    int planar2d;  // 2D, and every face has t1z = +0 and t2 = (+0, +0, +1) bit for bit
 };
 
 constexpr int TILE = 256;
@@ -1376,7 +1377,8 @@ __global__ void __launch_bounds__(TILE)
 // planes (t1x, t1y), (t1z, t2x), (t2y, t2z) [, (nx, ny) when the normal is
 // not cross(t1, t2)], then the area [and nz] as plain planes.  A face's
 // inputs are then 1 + 4 (+1 +1) contiguous loads at e plus the gathers of
-// the two sides' values.
+// the two sides' values.  On a planar 2D mesh (with_n 2) only (t1x, t1y)
+// and the area are stored: the other tangent components are 0, 0, 0, 1.
 __global__ void __launch_bounds__(BLOCK)
     k_entries(DevMesh m, int with_n, int2* __restrict__ eidx, double* __restrict__ egeom) {
     const int e = blockIdx.x * BLOCK + threadIdx.x;
@@ -1389,6 +1391,10 @@ __global__ void __launch_bounds__(BLOCK)
     const double* gf = m.geom;
     double2* g2 = reinterpret_cast<double2*>(egeom);
     g2[e] = make_double2(gf[3 * (size_t)F + f], gf[4 * (size_t)F + f]);
+    if (with_n == 2) {
+        egeom[(size_t)2 * NE + e] = gf[12 * (size_t)F + f];
+        return;
+    }
     g2[(size_t)NE + e] = make_double2(gf[5 * (size_t)F + f], gf[6 * (size_t)F + f]);
     g2[2 * (size_t)NE + e] = make_double2(gf[7 * (size_t)F + f], gf[8 * (size_t)F + f]);
     const int np = with_n ? 4 : 3;  // pair planes
@@ -1399,7 +1405,7 @@ __global__ void __launch_bounds__(BLOCK)
 
 // k_face_tile_fs over the per-entry face data (layout 2 values); NRM: the
 // normals are stored (they are not cross(t1, t2))
-template <int DIM, int FLUX, bool NRM, class R = double>
+template <int DIM, int FLUX, bool NRM, class R = double, bool P2 = false>
 __global__ void __launch_bounds__(TILE)
     k_face_entry_fs(DevMesh m, FaceParams fp, const int2* __restrict__ eidx,
                     const double* __restrict__ egeom, const R* __restrict__ FL,
@@ -1433,6 +1439,17 @@ __global__ void __launch_bounds__(TILE)
             v[1][q] = swap ? u[0][q] : u[1][q];
         }
         // g: t1 (0-2), t2 (3-5), area (6) [, n (7-9)]
+        if (P2) {  // planar 2D: t1z = 0, t2 = (0, 0, 1), the same bits as stored
+            const double2 p0 = eg2[e];
+            g[0] = p0.x;
+            g[1] = p0.y;
+            g[2] = 0.0;
+            g[3] = 0.0;
+            g[4] = 0.0;
+            g[5] = 1.0;
+            g[6] = egeom[(size_t)2 * NE + e];
+            return;
+        }
         const double2 p0 = eg2[e], p1 = eg2[(size_t)NE + e], p2 = eg2[2 * (size_t)NE + e];
         g[0] = p0.x;
         g[1] = p0.y;
@@ -2206,6 +2223,14 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
             std::memcmp(&nz, gg + 2 * F + f, 8))
             n_from_t = 0;
     }
+    int planar2d = d->dim == 2 ? 1 : 0;
+    for (int64_t f = 0; f < F && planar2d; f++) {
+        const double zero = 0.0, one = 1.0;
+        const double* gg = d->face_geom;
+        if (std::memcmp(gg + 5 * F + f, &zero, 8) || std::memcmp(gg + 6 * F + f, &zero, 8) ||
+            std::memcmp(gg + 7 * F + f, &zero, 8) || std::memcmp(gg + 8 * F + f, &one, 8))
+            planar2d = 0;
+    }
     std::vector<double> geom(d->face_geom, d->face_geom + 13 * F);
     std::vector<double> cen(d->centroid, d->centroid + 3 * Nt);
     std::vector<double> vol(d->volume, d->volume + Nt);
@@ -2243,6 +2268,7 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
     UP(slotR, slotR);
 #undef UP
     D.n_from_t = n_from_t;
+    D.planar2d = planar2d;
     D.ntiles = ntiles;
     D.maxtf = maxtf;
     D.ntf = toff[ntiles];
@@ -2668,7 +2694,9 @@ void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, bool f
     const size_t shm = (size_t)(DIM + 2) * D.maxtf * sizeof(double);
     if (f32) {
         const float* fl = reinterpret_cast<const float*>(FL);
-        if (face_kernel == 1 && D.n_from_t)
+        if (face_kernel == 1 && D.n_from_t && DIM == 2 && D.planar2d)
+            k_face_entry_fs<DIM, FLUX, false, float, true><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, fl, dudt);
+        else if (face_kernel == 1 && D.n_from_t)
             k_face_entry_fs<DIM, FLUX, false, float><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, fl, dudt);
         else if (face_kernel == 1)
             k_face_entry_fs<DIM, FLUX, true, float><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, fl, dudt);
@@ -2679,7 +2707,9 @@ void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, bool f
         return;
     }
     if (layout == 2 && face_kernel == 1) {
-        if (D.n_from_t)
+        if (D.n_from_t && DIM == 2 && D.planar2d)
+            k_face_entry_fs<DIM, FLUX, false, double, true><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, FL, dudt);
+        else if (D.n_from_t)
             k_face_entry_fs<DIM, FLUX, false><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, FL, dudt);
         else
             k_face_entry_fs<DIM, FLUX, true><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, FL, dudt);
@@ -3138,9 +3168,10 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
             if ((int64_t)D.maxf * 5 * D.N + 5 * (int64_t)D.G >= (int64_t)INT32_MAX)
                 return ibis::fail(IBIS_ERR_INVALID, "face_kernel 1: mesh too large for 32-bit value indices");
             const size_t ne = std::max(D.ntf, 1);
-            const int with_n = D.n_from_t ? 0 : 1;
+            // entry geometry: 1 = with normals, 2 = planar 2D, 0 = t1, t2, area
+            const int with_n = !D.n_from_t ? 1 : (D.dim == 2 && D.planar2d ? 2 : 0);
             HIP_TRY(hipMalloc(&r->ent_idx, sizeof(int2) * ne));
-            HIP_TRY(hipMalloc(&r->ent_geom, sizeof(double) * (with_n ? 10 : 7) * ne));
+            HIP_TRY(hipMalloc(&r->ent_geom, sizeof(double) * (with_n == 1 ? 10 : 7) * ne));
             if (D.ntf > 0) k_entries<<<nblocks(D.ntf), BLOCK>>>(D, with_n, r->ent_idx, r->ent_geom);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipDeviceSynchronize());
