@@ -1403,9 +1403,28 @@ __global__ void __launch_bounds__(BLOCK)
     if (with_n) egeom[(size_t)(2 * np + 1) * NE + e] = gf[2 * (size_t)F + f];
 }
 
+// Float copies of the entry tangents for the reduced-precision modes, whose
+// face kernel rounds t1 and t2 to float before use: planes of float4
+// (t1x, t1y, t1z, t2x) and float2 (t2y, t2z) over the entries, then the area
+// as doubles (32 bytes per entry instead of 56).
+__global__ void __launch_bounds__(BLOCK) k_entries32(DevMesh m, char* __restrict__ g) {
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    const int NE = m.ntf, F = m.F;
+    if (e >= NE) return;
+    const int f = m.ent_faces[e];
+    const double* gf = m.geom;
+    float4* a = reinterpret_cast<float4*>(g);
+    float2* b = reinterpret_cast<float2*>(g + (size_t)16 * NE);
+    double* c = reinterpret_cast<double*>(g + (size_t)24 * NE);
+    a[e] = make_float4((float)gf[3 * (size_t)F + f], (float)gf[4 * (size_t)F + f],
+                       (float)gf[5 * (size_t)F + f], (float)gf[6 * (size_t)F + f]);
+    b[e] = make_float2((float)gf[7 * (size_t)F + f], (float)gf[8 * (size_t)F + f]);
+    c[e] = gf[12 * (size_t)F + f];
+}
+
 // k_face_tile_fs over the per-entry face data (layout 2 values); NRM: the
 // normals are stored (they are not cross(t1, t2))
-template <int DIM, int FLUX, bool NRM, class R = double, bool P2 = false>
+template <int DIM, int FLUX, bool NRM, class R = double, bool P2 = false, bool G32 = false>
 __global__ void __launch_bounds__(TILE)
     k_face_entry_fs(DevMesh m, FaceParams fp, const int2* __restrict__ eidx,
                     const double* __restrict__ egeom, const R* __restrict__ FL,
@@ -1439,6 +1458,19 @@ __global__ void __launch_bounds__(TILE)
             v[1][q] = swap ? u[0][q] : u[1][q];
         }
         // g: t1 (0-2), t2 (3-5), area (6) [, n (7-9)]
+        if (G32) {  // the float copies of k_entries32 (egeom is their base)
+            const char* gb = reinterpret_cast<const char*>(egeom);
+            const float4 a = reinterpret_cast<const float4*>(gb)[e];
+            const float2 b = reinterpret_cast<const float2*>(gb + (size_t)16 * NE)[e];
+            g[0] = a.x;
+            g[1] = a.y;
+            g[2] = a.z;
+            g[3] = a.w;
+            g[4] = b.x;
+            g[5] = b.y;
+            g[6] = reinterpret_cast<const double*>(gb + (size_t)24 * NE)[e];
+            return;
+        }
         if (P2) {  // planar 2D: t1z = 0, t2 = (0, 0, 1), the same bits as stored
             const double2 p0 = eg2[e];
             g[0] = p0.x;
@@ -2307,6 +2339,7 @@ struct ibis_residual {
     int opt_face_kernel = 0; // option face_kernel: 0 tile face lists, 1 the same with
                              // per-entry face data, 64/128/256 slot-major
     int2* ent_idx = nullptr;     // per tile-list entry face data (face_kernel 1)
+    void* ent_geom32 = nullptr;  // float entry tangents of the reduced-precision modes
     double* ent_geom = nullptr;
     double* face_rho = nullptr;
     double* face_T = nullptr;
@@ -2347,7 +2380,7 @@ struct ibis_residual {
 
 void ibis_residual_destroy(ibis_residual* r) {
     if (!r) return;
-    for (void* p : {(void*)r->grad, (void*)r->lim, (void*)r->flux, (void*)r->pack, (void*)r->FL, (void*)r->FR, (void*)r->offs, (void*)r->ent_idx, (void*)r->ent_geom, (void*)r->face_rho,
+    for (void* p : {(void*)r->grad, (void*)r->lim, (void*)r->flux, (void*)r->pack, (void*)r->FL, (void*)r->FR, (void*)r->offs, (void*)r->ent_idx, (void*)r->ent_geom, r->ent_geom32, (void*)r->face_rho,
                     (void*)r->face_T, (void*)r->kinds, (void*)r->pars, (void*)r->partial,
                     (void*)r->dt_dev, (void*)r->count_dev, (void*)r->k_tmp, (void*)r->flow_tmp,
                     (void*)r->dual_prim, (void*)r->dual_dudt, (void*)r->dual_grad,
@@ -2688,14 +2721,17 @@ void launch_face_slot_ts(const DevMesh& D, const FaceParams& fp, int ts, const R
 // otherwise the slot-major kernel with that many cells per block
 template <int DIM, int FLUX>
 void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, bool f32, int face_kernel,
-                      const int2* eidx, const double* egeom, const double* FL, const double* FR,
-                      double* dudt, hipStream_t st) {
+                      const int2* eidx, const double* egeom, const void* eg32, const double* FL,
+                      const double* FR, double* dudt, hipStream_t st) {
     const unsigned g = xcd_grid(D.ntiles);
     const size_t shm = (size_t)(DIM + 2) * D.maxtf * sizeof(double);
     if (f32) {
         const float* fl = reinterpret_cast<const float*>(FL);
         if (face_kernel == 1 && D.n_from_t && DIM == 2 && D.planar2d)
             k_face_entry_fs<DIM, FLUX, false, float, true><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, fl, dudt);
+        else if (face_kernel == 1 && D.n_from_t && eg32)
+            k_face_entry_fs<DIM, FLUX, false, float, false, true><<<g, TILE, shm, st>>>(
+                D, fp, eidx, reinterpret_cast<const double*>(eg32), fl, dudt);
         else if (face_kernel == 1 && D.n_from_t)
             k_face_entry_fs<DIM, FLUX, false, float><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, fl, dudt);
         else if (face_kernel == 1)
@@ -2726,10 +2762,11 @@ void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, bool f
 
 template <int DIM>
 void launch_face_fs(const DevMesh& D, const FaceParams& fp, int flux, int layout, bool f32,
-                    int face_kernel, const int2* eidx, const double* egeom, const double* FL,
+                    int face_kernel, const int2* eidx, const double* egeom, const void* eg32,
+                    const double* FL,
                     const double* FR, double* dudt, hipStream_t st) {
 #define IBIS_FF(K) \
-    launch_face_fs_f<DIM, K>(D, fp, layout, f32, face_kernel, eidx, egeom, FL, FR, dudt, st)
+    launch_face_fs_f<DIM, K>(D, fp, layout, f32, face_kernel, eidx, egeom, eg32, FL, FR, dudt, st)
     switch (flux) {
         case IBIS_FLUX_HANEL: IBIS_FF(IBIS_FLUX_HANEL); break;
         case IBIS_FLUX_AUSMDV: IBIS_FF(IBIS_FLUX_AUSMDV); break;
@@ -2981,11 +3018,11 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
             const int lay = f32 ? 2 : r->opt_layout;
             const int fk = lay == 2 ? r->opt_face_kernel : 0;
             if (D.dim == 3)
-                launch_face_fs<3>(D, fp, d.flux, lay, f32, fk, r->ent_idx, r->ent_geom, r->FL,
-                                  r->FR, dudt, st);
+                launch_face_fs<3>(D, fp, d.flux, lay, f32, fk, r->ent_idx, r->ent_geom,
+                                  r->ent_geom32, r->FL, r->FR, dudt, st);
             else
-                launch_face_fs<2>(D, fp, d.flux, lay, f32, fk, r->ent_idx, r->ent_geom, r->FL,
-                                  r->FR, dudt, st);
+                launch_face_fs<2>(D, fp, d.flux, lay, f32, fk, r->ent_idx, r->ent_geom,
+                                  r->ent_geom32, r->FL, r->FR, dudt, st);
             pf.end(2);
         } else {
             // first order: face flux and surface integral as one tiled kernel
@@ -3123,6 +3160,13 @@ ibis_status ibis_residual_set_precision(ibis_residual* r, int mode) {
     if (!r) return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_set_precision: null residual");
     if (mode != IBIS_PREC_FP64 && mode != IBIS_PREC_FP32_FLUX && mode != IBIS_PREC_FP32_RECON)
         return ibis::fail(IBIS_ERR_INVALID, "Unknown precision mode");
+    const DevMesh& D = r->m->d;
+    if (mode != IBIS_PREC_FP64 && r->ent_idx && !r->ent_geom32 && D.ntf > 0 && D.n_from_t) {
+        HIP_TRY(hipMalloc(&r->ent_geom32, (size_t)32 * D.ntf));
+        k_entries32<<<nblocks(D.ntf), BLOCK>>>(D, reinterpret_cast<char*>(r->ent_geom32));
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipDeviceSynchronize());
+    }
     r->prec = mode;
     return IBIS_OK;
 }
