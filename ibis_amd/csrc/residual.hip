@@ -988,7 +988,8 @@ template <int DIM, int LIM, bool OFFS, bool SN = false>
 __global__ void __launch_bounds__(BLOCK)
     k_cell_fs32(DevMesh m, VarSet vs, const double* __restrict__ prim,
                 const double* __restrict__ grad, double eps, double venkat_K, int nlogical,
-                float* __restrict__ FL, const double* __restrict__ offs, int stage) {
+                float* __restrict__ FL, const double* __restrict__ offs, int stage,
+                const float* __restrict__ offs32) {
     // SN: the cell stencil is staged in LDS as doubles, as in k_cell_fs.
     // stage != 0 keeps the limiter pass's face offsets in LDS as floats for
     // the reconstruction pass; with SN they take the stencil's LDS once the
@@ -1097,8 +1098,14 @@ __global__ void __launch_bounds__(BLOCK)
         gy[v] = (float)g[N + i];
         gz[v] = (float)g[2 * (size_t)N + i];
     }
+    // offs32: the offsets already rounded to float (the same values), when
+    // the mode made them
     auto offset = [&](int j, float& dx, float& dy, float& dz) {
-        if (OFFS) {
+        if (OFFS && offs32) {
+            dx = offs32[(size_t)(j * 3 + 0) * N + i];
+            dy = offs32[(size_t)(j * 3 + 1) * N + i];
+            dz = offs32[(size_t)(j * 3 + 2) * N + i];
+        } else if (OFFS) {
             dx = (float)offs[(size_t)(j * 3 + 0) * N + i];
             dy = (float)offs[(size_t)(j * 3 + 1) * N + i];
             dz = (float)offs[(size_t)(j * 3 + 2) * N + i];
@@ -1401,6 +1408,13 @@ __global__ void __launch_bounds__(BLOCK)
     if (with_n) g2[3 * (size_t)NE + e] = make_double2(gf[f], gf[(size_t)F + f]);
     egeom[(size_t)2 * np * NE + e] = gf[12 * (size_t)F + f];
     if (with_n) egeom[(size_t)(2 * np + 1) * NE + e] = gf[2 * (size_t)F + f];
+}
+
+// a[k] rounded to float (the FP32 reconstruction mode's face offsets)
+__global__ void __launch_bounds__(BLOCK) k_to_float(size_t n, const double* __restrict__ a,
+                                                   float* __restrict__ b) {
+    const size_t k = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (k < n) b[k] = (float)a[k];
 }
 
 // Float copies of the entry tangents for the reduced-precision modes, whose
@@ -2340,6 +2354,7 @@ struct ibis_residual {
                              // per-entry face data, 64/128/256 slot-major
     int2* ent_idx = nullptr;     // per tile-list entry face data (face_kernel 1)
     void* ent_geom32 = nullptr;  // float entry tangents of the reduced-precision modes
+    float* offs32 = nullptr;     // float face offsets of the FP32 reconstruction mode
     double* ent_geom = nullptr;
     double* face_rho = nullptr;
     double* face_T = nullptr;
@@ -2380,7 +2395,7 @@ struct ibis_residual {
 
 void ibis_residual_destroy(ibis_residual* r) {
     if (!r) return;
-    for (void* p : {(void*)r->grad, (void*)r->lim, (void*)r->flux, (void*)r->pack, (void*)r->FL, (void*)r->FR, (void*)r->offs, (void*)r->ent_idx, (void*)r->ent_geom, r->ent_geom32, (void*)r->face_rho,
+    for (void* p : {(void*)r->grad, (void*)r->lim, (void*)r->flux, (void*)r->pack, (void*)r->FL, (void*)r->FR, (void*)r->offs, (void*)r->ent_idx, (void*)r->ent_geom, r->ent_geom32, (void*)r->offs32, (void*)r->face_rho,
                     (void*)r->face_T, (void*)r->kinds, (void*)r->pars, (void*)r->partial,
                     (void*)r->dt_dev, (void*)r->count_dev, (void*)r->k_tmp, (void*)r->flow_tmp,
                     (void*)r->dual_prim, (void*)r->dual_dudt, (void*)r->dual_grad,
@@ -2631,10 +2646,10 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
         float* fl = reinterpret_cast<float*>(r->FL);
         if (sn)
             k_cell_fs32<DIM, LIM, OFFS, true><<<xcd_grid(nlog), BLOCK, std::max(shs0, shm / 2), st>>>(
-                D, vs, prim, r->grad, d.lim_eps, d.venkat_K, nlog, fl, r->offs, stage);
+                D, vs, prim, r->grad, d.lim_eps, d.venkat_K, nlog, fl, r->offs, stage, r->offs32);
         else
             k_cell_fs32<DIM, LIM, OFFS><<<xcd_grid(nlog), BLOCK, shm / 2, st>>>(
-                D, vs, prim, r->grad, d.lim_eps, d.venkat_K, nlog, fl, r->offs, stage);
+                D, vs, prim, r->grad, d.lim_eps, d.venkat_K, nlog, fl, r->offs, stage, r->offs32);
         return;
     }
     if (r->prec == IBIS_PREC_FP32_FLUX) {
@@ -3164,6 +3179,13 @@ ibis_status ibis_residual_set_precision(ibis_residual* r, int mode) {
     if (mode != IBIS_PREC_FP64 && r->ent_idx && !r->ent_geom32 && D.ntf > 0 && D.n_from_t) {
         HIP_TRY(hipMalloc(&r->ent_geom32, (size_t)32 * D.ntf));
         k_entries32<<<nblocks(D.ntf), BLOCK>>>(D, reinterpret_cast<char*>(r->ent_geom32));
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipDeviceSynchronize());
+    }
+    if (mode == IBIS_PREC_FP32_RECON && r->offs && !r->offs32) {
+        const size_t n = (size_t)3 * D.maxf * D.N;
+        HIP_TRY(hipMalloc(&r->offs32, sizeof(float) * n));
+        k_to_float<<<(unsigned)((n + BLOCK - 1) / BLOCK), BLOCK>>>(n, r->offs, r->offs32);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipDeviceSynchronize());
     }
