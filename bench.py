@@ -71,7 +71,21 @@ def build_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(g_io, cfg, prim, n_res, threads):
+def cpu_model():
+    """CPU model name from /proc/cpuinfo (what lscpu reports as Model name)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(g_io, cfg, prim, n_res, threads, warmup=3):
+    """The oracle's residual on this host: median of n_res calls after
+    `warmup` calls on `threads` OpenMP threads, then one call on 1 thread.
+    Returns (Mcells/s at `threads`, median s, Mcells/s at 1 thread, s)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import OracleFV, OracleGrid, set_threads
     set_threads(threads)
@@ -79,12 +93,20 @@ def cpu_baseline(g_io, cfg, prim, n_res, threads):
     og = OracleGrid(g_io, enc["tags"], enc["ghost"])
     fv = OracleFV(og, cfg)
     p = prim.copy()
-    fv.compute_dudt(p)  # warm-up
-    t0 = time.perf_counter()
-    for _ in range(n_res):
+    for _ in range(warmup):
         fv.compute_dudt(p)
-    dt = (time.perf_counter() - t0) / n_res
-    return og.N / dt / 1e6, dt
+    times = []
+    for _ in range(n_res):
+        t0 = time.perf_counter()
+        fv.compute_dudt(p)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
+    set_threads(1)
+    t0 = time.perf_counter()
+    fv.compute_dudt(p)
+    dt1 = time.perf_counter() - t0
+    set_threads(threads)
+    return og.N / dt / 1e6, dt, og.N / dt1 / 1e6, dt1
 
 
 def time_calls(fn, n, stream):
@@ -272,7 +294,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="hex160", choices=["hex160", "plate", "wedge"])
-    ap.add_argument("--cpu-residuals", type=int, default=3)
+    ap.add_argument("--cpu-residuals", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--layout", type=int, default=None, choices=[0, 1, 2],
                     help="fix the face-state layout instead of autotuning")
@@ -442,13 +464,22 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        # the host's CPU share: OMP_NUM_THREADS where the launcher sets it,
+        # else every core the process may run on
+        threads = (int(os.environ.get("OMP_NUM_THREADS", "0"))
+                   or len(os.sched_getaffinity(0)) or 1)
         print(f"[bench] CPU oracle on {threads} threads ...", file=sys.stderr, flush=True)
-        v, dt = cpu_baseline(g_io, cfg, prim_ref, args.cpu_residuals, threads)
+        v, dt, v1, dt1 = cpu_baseline(g_io, cfg, prim_ref, args.cpu_residuals, threads)
         cpu = {"value": round(v, 3), "unit": "Mcells/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_residuals} residuals of the same {args.workload} grid and "
-                         f"state (after 1 warm-up), {dt * 1e3:.0f} ms each, oracle/ibis_oracle.cpp "
-                         f"built -O3 with OpenMP, reference cell order"}
+               "single_thread_value": round(v1, 3),
+               "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+               "sample": f"median of {args.cpu_residuals} residuals after 3 warm-ups on "
+                         f"{threads} threads ({dt * 1e3:.0f} ms each) and one residual on 1 "
+                         f"thread ({dt1 * 1e3:.0f} ms), same {args.workload} grid and state, "
+                         f"oracle/ibis_oracle.cpp (the reference's residual restated loop by "
+                         f"loop; the reference itself cannot be built here) at -O3 with "
+                         f"OpenMP where the reference has Kokkos parallel_for, reference "
+                         f"cell order"}
 
     out = {
         "metric": "residual Mcells/s (FP64, 2nd-order residual, % HBM peak in roofline)",

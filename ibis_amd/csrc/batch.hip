@@ -169,6 +169,7 @@ struct ibis_batch {
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    uint64_t graph_generation = 0;  // the residual's option generation at capture
 };
 
 void ibis_batch_destroy(ibis_batch* b) {
@@ -381,7 +382,16 @@ ibis_status ibis_batch_rk_steps(ibis_batch* b, int nsteps, int use_graph) {
     if (!use_graph) {
         for (int n = 0; n < nsteps; n++) IBIS_TRY(batch_step(b));
     } else {
+        // an option or precision change of the residual since the capture
+        // makes the captured kernels stale: capture the step again
+        if (b->exec && b->graph_generation != ibis_residual_generation(b->r)) {
+            (void)hipGraphExecDestroy(b->exec);
+            (void)hipGraphDestroy(b->graph);
+            b->exec = nullptr;
+            b->graph = nullptr;
+        }
         if (!b->exec) {
+            b->graph_generation = ibis_residual_generation(b->r);
             HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             ibis_status s = batch_step(b);
             hipGraph_t g = nullptr;

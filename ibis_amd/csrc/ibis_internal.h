@@ -36,6 +36,14 @@ int ibis_mesh_ntags(const ibis_mesh* m);
 ibis_status ibis_residual_cell_dt(ibis_residual* r, const double* prim, double* dt_cells,
                                   ibis_stream_t stream);
 
+// 1 when the value starts of the per-entry face kernel (face_kernel 1), at
+// most maxf*5*n_cells + 5*n_ghost, fit in a 32-bit index; residuals of
+// larger meshes keep the tile-list face kernel.  Exported for host tests.
+extern "C" int ibis_entry_indices_fit(int64_t maxf, int64_t n_cells, int64_t n_ghost);
+// Counter bumped by every ibis_residual_set_option / set_precision call, so
+// that a captured HIP graph of the residual can tell it is stale.
+uint64_t ibis_residual_generation(const ibis_residual* r);
+
 namespace ibis {
 // Record a failure message for ibis_last_error() and return the status.
 ibis_status fail(ibis_status s, const char* msg);

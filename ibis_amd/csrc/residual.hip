@@ -351,7 +351,7 @@ __global__ void __launch_bounds__(320)
         const double dy = m.geom[10 * (size_t)F + f] - yi;
         const double dz = m.geom[11 * (size_t)F + f] - zi;
         S d2 = gx * dx + gy * dy + gz * dz;
-        if (LIM == 1) {  // Barth-Jespersen, FV/limiter.cpp:289-303
+        if (LIM == 1) {  // Barth-Jespersen, FV/limiter.cpp:24-62
             const int sg = (d2 > S(0.0)) - (d2 < S(0.0));
             d2 = (double)sg * (fabs(d2) + eps);
             if (sg > 0)
@@ -922,7 +922,7 @@ __global__ void __launch_bounds__(BLOCK)
                 // written without branches: the quotient of a face that the
                 // reference skips (d2 zero or NaN) is formed and discarded,
                 // so every lane runs the same instructions
-                if (LIM == 1) {  // Barth-Jespersen, FV/limiter.cpp:289-303
+                if (LIM == 1) {  // Barth-Jespersen, FV/limiter.cpp:24-62
                     const int sg = (d2 > 0) - (d2 < 0);
                     d2 = sg * (fabs(d2) + eps);
                     const double q = (sg > 0 ? umax[v] - ui[v] : umin[v] - ui[v]) / d2;
@@ -1142,7 +1142,7 @@ __global__ void __launch_bounds__(BLOCK)
 #pragma unroll
             for (int v = 0; v < 5; v++) {
                 float d2 = gx[v] * dx + gy[v] * dy + gz[v] * dz;
-                if (LIM == 1) {  // Barth-Jespersen, FV/limiter.cpp:289-303
+                if (LIM == 1) {  // Barth-Jespersen, FV/limiter.cpp:24-62
                     const int sg = (d2 > 0.0f) - (d2 < 0.0f);
                     d2 = sg * (fabsf(d2) + epsf);
                     const float q = (sg > 0 ? dmax[v] : dmin[v]) / d2;
@@ -2022,6 +2022,10 @@ ibis_status fail(ibis_status s, const char* msg) {
 
 extern "C" const char* ibis_last_error(void) { return g_last_error.c_str(); }
 
+int ibis_entry_indices_fit(int64_t maxf, int64_t n_cells, int64_t n_ghost) {
+    return maxf * 5 * n_cells + 5 * n_ghost < (int64_t)INT32_MAX ? 1 : 0;
+}
+
 struct ibis_mesh {
     DevMesh d;
     int ntags = 0;
@@ -2100,6 +2104,20 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
                 return ibis::fail(IBIS_ERR_INVALID, "a ghost cell on two boundary faces");
             b_of_ghost[b_ghost[k] - N] = (int)k;
         }
+    // every ghost a face or a cell list reaches must lie on a tagged
+    // boundary face: the boundary and adjoint kernels index their tables
+    // with the ghost's boundary face
+    for (int64_t f = 0; f < F; f++)
+        for (int64_t c : {d->face_left[f], d->face_right[f]})
+            if (c >= N && b_of_ghost[c - N] == -1)
+                return ibis::fail(IBIS_ERR_INVALID,
+                                  "ibis_mesh_create: a face reaches a ghost cell that is on no "
+                                  "tagged boundary face");
+    for (int64_t j = 0; j < nnz; j++)
+        if (d->cell_neighbours[j] >= N && b_of_ghost[d->cell_neighbours[j] - N] == -1)
+            return ibis::fail(IBIS_ERR_INVALID,
+                              "ibis_mesh_create: a cell's neighbour is a ghost cell that is on no "
+                              "tagged boundary face");
     // device order of the boundary faces: by ghost id, so that neighbouring
     // threads of the boundary kernels write neighbouring ghost rows and read
     // nearby valid cells (the locality renumbering places each ghost after
@@ -2345,11 +2363,15 @@ struct ibis_residual {
     int prec = IBIS_PREC_FP64;  // ibis_residual_set_precision
     int ntab = 0;               // tag x action entries of each boundary table
     bool has_pre = false, has_visc = false, has_post = false;
+    bool has_subsonic = false;  // a subsonic inflow/outflow action in the pre or viscous tables
+    uint64_t generation = 0;    // bumped by every option or precision change
     int opt_offsets = 0;     // option precomputed_offsets
     int opt_pack = 1;        // option pack_cells
     int opt_stage = 1;       // option stage_offsets
     int opt_cell_prefetch = 0;  // option cell_prefetch (layout 2, pack_cells 0)
     int opt_stage_cells = 0;    // option stage_cells (layout 2, pack_cells 0)
+    int opt_keep_grads = 0;     // option keep_gradients: the fused path writes the
+                                // gradient and limiter slots too
     int opt_face_kernel = 0; // option face_kernel: 0 tile face lists, 1 the same with
                              // per-entry face data, 64/128/256 slot-major
     int2* ent_idx = nullptr;     // per tile-list entry face data (face_kernel 1)
@@ -2392,6 +2414,8 @@ struct ibis_residual {
     ibis::Dual* dual_lim = nullptr;
     ibis::Dual* dual_flux = nullptr;
 };
+
+uint64_t ibis_residual_generation(const ibis_residual* r) { return r ? r->generation : 0; }
 
 void ibis_residual_destroy(ibis_residual* r) {
     if (!r) return;
@@ -2506,6 +2530,9 @@ ibis_status ibis_residual_create_tables(const ibis_mesh* m, const ibis_fv_desc* 
         r->has_visc = r->has_visc || kinds3[ntab + q] != IBIS_BC_NONE;
         r->has_post = r->has_post || kinds3[2 * ntab + q] == IBIS_BC_CONSTANT_FLUX;
     }
+    for (int q = 0; q < 2 * ntab; q++)
+        r->has_subsonic = r->has_subsonic || kinds3[q] == IBIS_BC_SUBSONIC_INFLOW ||
+                          kinds3[q] == IBIS_BC_SUBSONIC_OUTFLOW;
     r->has_pre = r->has_pre && D.nb > 0;
     r->has_visc = r->has_visc && D.nb > 0;
     r->has_post = r->has_post && D.nb > 0;
@@ -2526,9 +2553,12 @@ ibis_status ibis_residual_create_tables(const ibis_mesh* m, const ibis_fv_desc* 
     // autotuner picks on the 3D and 2D workloads (every choice gives the
     // same bits; ibis_residual_set_option changes them)
     if (d.order == 2 && !d.viscous && !r->has_post) {
+        // face_kernel 1 indexes the face values with 32-bit entries; a mesh
+        // too large for them keeps the tile-list kernel (face_kernel 0)
+        const int fk = ibis_entry_indices_fit(D.maxf, D.N, D.G) ? 1 : 0;
         for (const auto& kv : {std::make_pair("face_state_layout", 2), std::make_pair("pack_cells", 0),
                                std::make_pair("precomputed_offsets", 1),
-                               std::make_pair("face_kernel", 1), std::make_pair("stage_cells", 1)}) {
+                               std::make_pair("face_kernel", fk), std::make_pair("stage_cells", 1)}) {
             const ibis_status st = ibis_residual_set_option(r, kv.first, kv.second);
             if (st != IBIS_OK) {
                 ibis_residual_destroy(r);
@@ -2656,11 +2686,11 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
         float* fl = reinterpret_cast<float*>(r->FL);
         if (sn)
             k_cell_fs<DIM, LIM, 2, OFFS, float, false, false, true><<<xcd_grid(nlog), BLOCK, shs, st>>>(
-                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, fl, fl, r->offs,
+                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, fl, fl, r->offs,
                 prim, stage);
         else
             k_cell_fs<DIM, LIM, 2, OFFS, float, false><<<xcd_grid(nlog), BLOCK, shm, st>>>(
-                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, fl, fl, r->offs,
+                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, fl, fl, r->offs,
                 prim, stage);
         return;
     }
@@ -2668,22 +2698,22 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
         if (sn) {
             k_cell_fs<DIM, LIM, 2, OFFS, double, false, false, true>
                 <<<xcd_grid(nlog), BLOCK, shs, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps,
-                                                     d.venkat_K, nlog, 0, r->FL, r->FR, r->offs,
+                                                     d.venkat_K, nlog, r->opt_keep_grads, r->FL, r->FR, r->offs,
                                                      prim, stage);
             return;
         }
         if (r->opt_cell_prefetch)
             k_cell_fs<DIM, LIM, 2, OFFS, double, false, true><<<xcd_grid(nlog), BLOCK, shm, st>>>(
-                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR,
+                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, r->FL, r->FR,
                 r->offs, prim, stage);
         else
             k_cell_fs<DIM, LIM, 2, OFFS, double, false><<<xcd_grid(nlog), BLOCK, shm, st>>>(
-                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR,
+                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, r->FL, r->FR,
                 r->offs, prim, stage);
         return;
     }
     k_cell_fs<DIM, LIM, LAYOUT, OFFS><<<xcd_grid(nlog), BLOCK, shm, st>>>(
-        D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, 0, r->FL, r->FR, r->offs,
+        D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, r->FL, r->FR, r->offs,
         prim, stage);
 }
 
@@ -3175,6 +3205,7 @@ ibis_status ibis_residual_set_precision(ibis_residual* r, int mode) {
     if (!r) return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_set_precision: null residual");
     if (mode != IBIS_PREC_FP64 && mode != IBIS_PREC_FP32_FLUX && mode != IBIS_PREC_FP32_RECON)
         return ibis::fail(IBIS_ERR_INVALID, "Unknown precision mode");
+    r->generation++;
     const DevMesh& D = r->m->d;
     if (mode != IBIS_PREC_FP64 && r->ent_idx && !r->ent_geom32 && D.ntf > 0 && D.n_from_t) {
         HIP_TRY(hipMalloc(&r->ent_geom32, (size_t)32 * D.ntf));
@@ -3207,6 +3238,7 @@ ibis_status ibis_residual_sizes(const ibis_residual* r, int64_t* sizes) {
 ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t value) {
     if (!r || !key) return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_set_option: null argument");
     const std::string k(key);
+    r->generation++;
     if (k == "face_state_layout") {
         if (value < 0 || value > 2)
             return ibis::fail(IBIS_ERR_INVALID, "face_state_layout must be 0, 1 or 2");
@@ -3231,7 +3263,7 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
         if (value > 1 && (size_t)(D.dim + 2) * D.maxf * value * sizeof(double) > 65536)
             return ibis::fail(IBIS_ERR_INVALID, "face_kernel: block does not fit in LDS");
         if (value == 1 && !r->ent_idx) {
-            if ((int64_t)D.maxf * 5 * D.N + 5 * (int64_t)D.G >= (int64_t)INT32_MAX)
+            if (!ibis_entry_indices_fit(D.maxf, D.N, D.G))
                 return ibis::fail(IBIS_ERR_INVALID, "face_kernel 1: mesh too large for 32-bit value indices");
             const size_t ne = std::max(D.ntf, 1);
             // entry geometry: 1 = with normals, 2 = planar 2D, 0 = t1, t2, area
@@ -3263,6 +3295,10 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
     }
     if (k == "stage_cells") {
         r->opt_stage_cells = value ? 1 : 0;
+        return IBIS_OK;
+    }
+    if (k == "keep_gradients") {
+        r->opt_keep_grads = value ? 1 : 0;
         return IBIS_OK;
     }
     return ibis::fail(IBIS_ERR_INVALID, ("unknown option " + k).c_str());
@@ -3319,12 +3355,8 @@ ibis_status adjoint_supported(const ibis_residual* r) {
         return ibis::fail(IBIS_ERR_INVALID, "the adjoint does not support constant-flux boundaries");
     if (r->prec != IBIS_PREC_FP64)
         return ibis::fail(IBIS_ERR_INVALID, "the adjoint is FP64 only");
-    std::vector<int> kinds(2 * r->ntab);
-    if (hipMemcpy(kinds.data(), r->kinds, sizeof(int) * 2 * r->ntab, hipMemcpyDeviceToHost) != hipSuccess)
-        return ibis::fail(IBIS_ERR_HIP, "adjoint: reading the boundary tables failed");
-    for (int k : kinds)
-        if (k == IBIS_BC_SUBSONIC_INFLOW || k == IBIS_BC_SUBSONIC_OUTFLOW)
-            return ibis::fail(IBIS_ERR_INVALID, "the adjoint does not support subsonic boundaries");
+    if (r->has_subsonic)
+        return ibis::fail(IBIS_ERR_INVALID, "the adjoint does not support subsonic boundaries");
     return IBIS_OK;
 }
 
@@ -3659,6 +3691,67 @@ ibis_status ibis_rk_step(ibis_residual* r, int s, const double* a, const double*
     if ((rc = ibis_cons_to_prim(r, cons, prim, stream)) != IBIS_OK) return rc;
     *dt_prev = dt;
     if (dt_taken) *dt_taken = dt;
+    return IBIS_OK;
+}
+
+// FiniteVolume::compute_convective_gradient / compute_viscous_gradient
+// (FV/finite_volume.cpp:273-288): the boundary actions write the ghost rows,
+// then the LSQ gradients of the reconstruction variables (convective:
+// FV/convective_flux.cpp:181-209) or of T, vx, vy, vz (viscous:
+// FV/viscous_flux.cpp:122-130) overwrite their slots of the gradient
+// storage; the other slots keep their values.
+ibis_status ibis_compute_convective_gradient(ibis_residual* r, double* prim,
+                                             ibis_stream_t stream) {
+    if (!r || !prim)
+        return ibis::fail(IBIS_ERR_INVALID, "ibis_compute_convective_gradient: null argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const DevMesh& D = r->m->d;
+    const ibis_fv_desc& d = r->desc;
+    const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
+    if (r->has_pre) k_bc<double><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, prim);
+    FaceParams fp;
+    VarSet vs;
+    residual_params(r, &fp, &vs);
+    vs.n = vs.ngrad = D.dim == 3 ? 5 : 4;  // vz only in 3D (convective_flux.cpp:206-208)
+    if (D.dim == 3)
+        launch_cell_grad<double, 3>(D, vs, 0, prim, r->grad, r->lim, 0.0, 0.0, st);
+    else
+        launch_cell_grad<double, 2>(D, vs, 0, prim, r->grad, r->lim, 0.0, 0.0, st);
+    HIP_TRY(hipGetLastError());
+    return IBIS_OK;
+}
+
+ibis_status ibis_compute_viscous_gradient(ibis_residual* r, double* prim, ibis_stream_t stream) {
+    if (!r || !prim)
+        return ibis::fail(IBIS_ERR_INVALID, "ibis_compute_viscous_gradient: null argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const DevMesh& D = r->m->d;
+    const ibis_fv_desc& d = r->desc;
+    const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
+    if (r->has_pre) k_bc<double><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, prim);
+    if (r->has_visc)
+        k_bc<double><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds + r->ntab,
+                                                      r->pars + (size_t)r->ntab * IBIS_NPAR, gp,
+                                                      prim);
+    VarSet vv;
+    vv.n = vv.ngrad = 4;
+    const int rows[5] = {P_T, P_VX, P_VY, P_VZ, P_VZ};
+    const int slots[5] = {G_T, G_VX, G_VY, G_VZ, G_VZ};
+    for (int q = 0; q < 5; q++) {
+        vv.row[q] = rows[q];
+        vv.slot[q] = slots[q];
+    }
+    if (D.dim == 3)
+        launch_cell_grad<double, 3>(D, vv, 0, prim, r->grad, r->lim, 0.0, 0.0, st);
+    else
+        launch_cell_grad<double, 2>(D, vv, 0, prim, r->grad, r->lim, 0.0, 0.0, st);
+    HIP_TRY(hipGetLastError());
+    return IBIS_OK;
+}
+
+ibis_status ibis_cell_gradients(const ibis_residual* r, const double** grads) {
+    if (!r || !grads) return ibis::fail(IBIS_ERR_INVALID, "ibis_cell_gradients: null argument");
+    *grads = r->grad;
     return IBIS_OK;
 }
 

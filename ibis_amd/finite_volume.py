@@ -40,6 +40,17 @@ def _stream_handle(stream=None):
     return C.c_void_p(s.cuda_stream)
 
 
+def _device_view(addr: int, shape):
+    """A float64 CUDA tensor over library-owned device memory (no copy)."""
+    torch = _torch()
+    n = int(np.prod(shape))
+
+    class _Ptr:
+        __cuda_array_interface__ = {"shape": (n,), "typestr": "<f8", "data": (addr, False),
+                                    "version": 3, "strides": None}
+    return torch.as_tensor(_Ptr(), device="cuda").view(*shape)
+
+
 class GridBlock:
     """Host grid built by libibis_gpu (reference numbering, then optionally
     renumbered for locality) plus its device copy."""
@@ -300,6 +311,34 @@ class FiniteVolume:
     def prim_to_cons(self, prim, cons, stream=None):
         check(lib().ibis_prim_to_cons(self._h, ptr(prim), ptr(cons), _stream_handle(stream)),
               "ibis_prim_to_cons")
+
+    def compute_convective_gradient(self, flow_state, grid=None, gas_model=None,
+                                    trans_prop=None, stream=None):
+        """FiniteVolume<T>::compute_convective_gradient (finite_volume.cpp:283-288):
+        boundary actions on ``flow_state``, then the gradients of the
+        reconstruction variables into the gradient storage."""
+        prim = flow_state.data if isinstance(flow_state, FlowStates) else flow_state
+        self._check_state(prim, NPRIM, self.grid.Nt, "flow_state")
+        check(lib().ibis_compute_convective_gradient(self._h, ptr(prim), _stream_handle(stream)),
+              "ibis_compute_convective_gradient")
+
+    def compute_viscous_gradient(self, flow_state, grid=None, gas_model=None, trans_prop=None,
+                                 stream=None):
+        """FiniteVolume<T>::compute_viscous_gradient (finite_volume.cpp:273-280):
+        boundary actions, then the gradients of T, vx, vy, vz."""
+        prim = flow_state.data if isinstance(flow_state, FlowStates) else flow_state
+        self._check_state(prim, NPRIM, self.grid.Nt, "flow_state")
+        check(lib().ibis_compute_viscous_gradient(self._h, ptr(prim), _stream_handle(stream)),
+              "ibis_compute_viscous_gradient")
+
+    def cell_gradients(self):
+        """FiniteVolume<T>::cell_gradients() (finite_volume.h:102): the
+        gradient storage as a (7, 3, N) float64 CUDA tensor view, slots p,
+        rho, T, u, vx, vy, vz (no copy; valid while this object lives)."""
+        torch = _torch()
+        g = C.c_void_p()
+        check(lib().ibis_cell_gradients(self._h, C.byref(g)), "ibis_cell_gradients")
+        return _device_view(g.value, (7, 3, self.grid.N))
 
     def scratch(self):
         """Device pointers (ints) of gradients, limiters and face flux, and

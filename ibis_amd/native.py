@@ -116,6 +116,10 @@ _SIGS = {
                                C.POINTER(C.c_double), _P]),
     "ibis_residual_scratch": (C.c_int, [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                         C.POINTER(C.c_void_p), C.POINTER(C.c_int)]),
+    "ibis_compute_convective_gradient": (C.c_int, [_P, _P, _P]),
+    "ibis_compute_viscous_gradient": (C.c_int, [_P, _P, _P]),
+    "ibis_cell_gradients": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
+    "ibis_entry_indices_fit": (C.c_int, [C.c_int64, C.c_int64, C.c_int64]),
     "ibis_residual_profile": (C.c_int, [_P, C.c_int]),
     "ibis_residual_set_option": (C.c_int, [_P, C.c_char_p, C.c_int64]),
     "ibis_residual_kernel_times": (C.c_int, [_P, _P, _P, C.c_int]),

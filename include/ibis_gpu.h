@@ -257,10 +257,42 @@ ibis_status ibis_rk_step(ibis_residual* r, int nstages, const double* a, const d
                          double max_time, double plot_remaining, double* dt_taken,
                          ibis_stream_t stream);
 
-/* Device pointers of the library's intermediates (for inspection and
- * tests): gradients 7 x 3 x n_cells (slots p, rho, T, u, vx, vy, vz),
- * limiters 7 x n_cells, face flux (dim+2) x n_faces.  The face flux is
- * multiplied by the face area when the configuration is inviscid. */
+/* FiniteVolume<T>::compute_convective_gradient and
+ * compute_viscous_gradient (finite_volume.h:104-112, finite_volume.cpp:
+ * 273-288), the post-processing gradients read by IO/accessor.cpp:115-250.
+ * Both run the pre-reconstruction boundary actions on prim (ghost rows are
+ * written; the viscous one also the pre-viscous-gradient actions), then
+ * overwrite slots of the residual's gradient storage with the LSQ
+ * gradients of the state:
+ *   convective: the thermo interpolator's pair (rho_p: p, rho; rho_u: rho,
+ *     u; rho_T: rho, T; p_T: p, T), vx, vy and, in 3D only, vz
+ *     (convective_flux.cpp:181-209);
+ *   viscous: T, vx, vy, vz (viscous_flux.cpp:122-130).
+ * Other slots keep their values.  Asynchronous. */
+ibis_status ibis_compute_convective_gradient(ibis_residual* r, double* prim,
+                                             ibis_stream_t stream);
+ibis_status ibis_compute_viscous_gradient(ibis_residual* r, double* prim, ibis_stream_t stream);
+
+/* FiniteVolume<T>::cell_gradients() (finite_volume.h:102): device pointer
+ * of the gradient storage, 7 x 3 x n_cells FP64, slot-major (slots p, rho,
+ * T, u, vx, vy, vz; components x, y, z of each slot over the cells). */
+ibis_status ibis_cell_gradients(const ibis_residual* r, const double** grads);
+
+/* Device pointers of the library's intermediates: gradients (as
+ * ibis_cell_gradients), limiters 7 x n_cells, face flux (dim+2) x n_faces.
+ * What they hold after a call:
+ *   - ibis_compute_convective_gradient / ibis_compute_viscous_gradient:
+ *     the gradient slots those write (above);
+ *   - ibis_compute_dudt on the reference-shaped path (viscous,
+ *     constant-flux boundaries, order 1 or allow_reconstruction = 0):
+ *     gradients, limiters and face flux as the reference leaves cell_grad_,
+ *     its limiters and flux_ (viscous gradients overwrite vx, vy, vz);
+ *   - ibis_compute_dudt on the fused second-order inviscid path: gradients
+ *     and limiters only with the option "keep_gradients" = 1 (FP64 and
+ *     FP32-flux modes); otherwise, and for the face flux always, the
+ *     buffers keep what an earlier call left.
+ * The face flux is multiplied by the face area when the configuration is
+ * inviscid. */
 ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** limiters,
                                   double** flux, int* flux_has_area);
 
@@ -297,6 +329,10 @@ ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** lim
  *                            previous one is summed (layout 2, pack_cells 0)
  *   "freeze_limiter"      1: the tangent (JVP) treats the limiter values
  *                            as constants; 0 (default) differentiates them
+ *   "keep_gradients"      1: the fused second-order path also writes the
+ *                            gradient and limiter storage (the reference's
+ *                            cell_grad_ and limiters after compute_dudt);
+ *                            0 (default) skips those stores
  * Setting precomputed_offsets allocates and synchronises once. */
 ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t value);
 

@@ -1349,6 +1349,20 @@ struct FV {
     }
 
     // --- FiniteVolume::compute_dudt, FV/finite_volume.cpp:57-96 ------------------
+    // FiniteVolume::compute_convective_gradient, FV/finite_volume.cpp:283-288:
+    // pre-reconstruction actions, then the convective gradients
+    void compute_convective_gradient(T* fs) {
+        apply_bc_list(cfg.pre_kind, cfg.pre_par, fs);
+        convective_gradients(fs);
+    }
+    // FiniteVolume::compute_viscous_gradient, FV/finite_volume.cpp:273-280:
+    // pre-reconstruction and pre-viscous-gradient actions, then T, vx, vy, vz
+    void compute_viscous_gradient(T* fs) {
+        apply_bc_list(cfg.pre_kind, cfg.pre_par, fs);
+        apply_bc_list(cfg.visc_kind, cfg.visc_par, fs);
+        viscous_gradients(fs);
+    }
+
     int compute_dudt(T* fs, T* dudt, bool allow_reconstruction) {
         apply_bc_list(cfg.pre_kind, cfg.pre_par, fs);
         convective_flux(fs, allow_reconstruction);
@@ -1592,6 +1606,14 @@ void or_cons_to_prim_jvp(void* p, const double* cons, const double* cons_dot, do
 
 void or_prim_to_cons(void* p, const double* prim, double* cons) {
     static_cast<FVHandle*>(p)->fv->prim_to_cons(prim, cons);
+}
+
+void or_compute_convective_gradient(void* p, double* prim) {
+    static_cast<FVHandle*>(p)->fv->compute_convective_gradient(prim);
+}
+
+void or_compute_viscous_gradient(void* p, double* prim) {
+    static_cast<FVHandle*>(p)->fv->compute_viscous_gradient(prim);
 }
 
 void or_fv_intermediates(void* p, double* grads, double* lim, double* flux, double* left,

@@ -103,6 +103,11 @@ void or_cons_to_prim_jvp(void* fv, const double* cons, const double* cons_dot,
 void or_prim_to_cons(void* fv, const double* prim, double* cons);
 /* grads: 7 vars (p, rho, T, u, vx, vy, vz) x 3 x N; lim: 7 x N; flux: ncons x F;
    left/right: 7 x F.  Any pointer may be NULL. */
+/* FiniteVolume::compute_convective_gradient / compute_viscous_gradient
+ * (FV/finite_volume.cpp:273-288): the boundary actions, then the gradients
+ * into the gradient slots (read back with or_fv_intermediates). */
+void or_compute_convective_gradient(void* fv, double* prim);
+void or_compute_viscous_gradient(void* fv, double* prim);
 void or_fv_intermediates(void* fv, double* grads, double* lim, double* flux,
                          double* left, double* right);
 /* One explicit RK step (SO/runge_kutta.cpp:131-203).  cons/prim are the

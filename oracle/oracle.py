@@ -77,6 +77,8 @@ def lib():
         L.or_cons_to_prim_jvp.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _f64p]
         L.or_prim_to_cons.argtypes = [C.c_void_p, _f64p, _f64p]
         L.or_fv_intermediates.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+        L.or_compute_convective_gradient.argtypes = [C.c_void_p, _f64p]
+        L.or_compute_viscous_gradient.argtypes = [C.c_void_p, _f64p]
         L.or_rk_step.restype = C.c_double
         L.or_rk_step.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, C.c_double, _f64p, _f64p,
                                  C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double]
@@ -246,6 +248,15 @@ class OracleFV:
             cons = self.new_cons()
         lib().or_prim_to_cons(self.h, prim, cons)
         return cons
+
+    def compute_convective_gradient(self, prim):
+        """Boundary actions on prim (ghost rows written) and the convective
+        gradients into the gradient slots (FV/finite_volume.cpp:283-288)."""
+        lib().or_compute_convective_gradient(self.h, prim)
+
+    def compute_viscous_gradient(self, prim):
+        """FV/finite_volume.cpp:273-280."""
+        lib().or_compute_viscous_gradient(self.h, prim)
 
     def intermediates(self):
         g = self.grid

@@ -1,0 +1,199 @@
+"""The workloads at their stated sizes (run on the GPU box).
+
+  * third workload: the 160^3 jittered, cell-permuted hex box (4,096,000
+    cells), Hanel / Venkatakrishnan / rho_u -- the GPU residual equals the
+    oracle's bit for bit on the state the benchmark times;
+  * fourth workload: the discrete adjoint on the 1024 x 512 viscous flat
+    plate -- dot-product test to 1e-10 (limiter active and frozen) and the
+    wall-force gradient against central differences (h = 1e-7, 32 seeded
+    directions, seed 6, both force directions);
+  * fifth workload: 64 wedge/ramp instances of 10k-400k cells (seed 7) --
+    the batched, graph-replayed steps equal one-at-a-time stepping bit for
+    bit, and the smallest instance stepped alone equals the oracle's steps;
+  * adjoint configurations the VJP does not cover are refused.
+"""
+import numpy as np
+import pytest
+
+from helpers import gpu_dudt, gpu_objects, oracle_pair, to_dev
+from ibis_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.timeout(900)
+def test_hex160_residual_matches_oracle():
+    """160^3 box as the benchmark builds it (hex_prim seed 5): the GPU
+    residual, the ghost rows it writes and dt equal the oracle's."""
+    g_io, cfg = cases.hex_case(160)
+    nt = 160 ** 3 + 6 * 160 ** 2
+    prim = cases.hex_prim(g_io, nt)
+    print("hex160: oracle grid and residual", flush=True)
+    og, ofv = oracle_pair(g_io, cfg)
+    p_cpu = prim.copy()
+    d_cpu = ofv.compute_dudt(p_cpu)
+    dt_cpu = ofv.estimate_dt(p_cpu)
+    del ofv, og
+    print("hex160: GPU grid and residual", flush=True)
+    gb, fv = gpu_objects(g_io, cfg)
+    d_gpu, p_gpu = gpu_dudt(gb, fv, prim)
+    n = 160 ** 3
+    assert np.isfinite(d_cpu[:, :n]).all()
+    same = np.array_equal(d_gpu[:, :n], d_cpu[:, :n])
+    if not same:
+        bad = np.argwhere(d_gpu[:, :n] != d_cpu[:, :n])
+        print(f"hex160: {len(bad)} entries differ, first {bad[:5].tolist()}", flush=True)
+    assert same
+    assert np.array_equal(p_gpu[:, n:], p_cpu[:, n:])
+    assert fv.estimate_dt(to_dev(gb.to_grid_order(p_cpu))) == dt_cpu
+    print("hex160: residual, ghost rows and dt identical to the oracle", flush=True)
+    fv.close()
+    gb.close()
+
+
+def _plate_full():
+    g_io, cfg, fs = cases.plate_case(1024, 512)
+    prim_ref = cases.perturbed_prim(g_io, fs, 1024 * 512 + 2 * (1024 + 512), seed=2)
+    return g_io, cfg, prim_ref
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("mode", ["active", "frozen"])
+def test_plate_adjoint_dot_product_full_size(mode):
+    """<w, J v> = <J^T w, v> to 1e-10 on the 1024 x 512 plate, J v from the
+    Dual tangent residual (equal to the oracle's FiniteVolume<Dual>)."""
+    from test_gpu_adjoint import _dot
+    g_io, cfg, prim = _plate_full()
+    gb, fv = _dot(g_io, cfg, prim, seed=86, limiter=mode)
+    fv.close()
+    gb.close()
+
+
+@pytest.mark.timeout(900)
+def test_plate_wall_force_gradient_full_size():
+    """d J / d prim . v against (J(p + h v) - J(p - h v)) / 2h on the 1024 x
+    512 plate, h = 1e-7 relative to each row's scale, 32 seeded directions
+    (seed 6), force along x and along y.
+
+    The functional is the wall force, so only the wall cells and their
+    stencils matter; the limiter is active.  A direction whose +-h states
+    put a wall-adjacent limiter on different faces has a kink between them,
+    where central differences are not a derivative: the bar is a median
+    relative error below 1e-6 and every direction below 1e-4."""
+    import torch
+    g_io, cfg, prim_ref = _plate_full()
+    gb, fv = gpu_objects(g_io, cfg)
+    N, Nt = gb.N, gb.Nt
+    prim0 = gb.to_grid_order(prim_ref)
+    tags = fv.tag_mask("wall")
+    scale = np.abs(prim0[:, :N]).max(axis=1)[:, None]
+    for direction in ((1.0, 0.0, 0.0), (0.0, 1.0, 0.0)):
+        pb = torch.zeros((7, Nt), dtype=torch.float64, device="cuda")
+        fv.compute_dudt_vjp(to_dev(prim0), pb, force_tags=tags, force_dir=direction)
+        grad = pb.cpu().numpy()
+        rng = np.random.default_rng(6)
+        errs = []
+        h = 1e-7
+        for k in range(32):
+            v = rng.standard_normal((7, Nt)) * scale
+            v[:, N:] = 0.0
+            fp = fv.wall_force(to_dev(prim0 + h * v), tags) @ np.array(direction)
+            fm = fv.wall_force(to_dev(prim0 - h * v), tags) @ np.array(direction)
+            fd = (fp - fm) / (2 * h)
+            ad = float((grad[:, :N] * v[:, :N]).sum())
+            errs.append(abs(fd - ad) / max(abs(ad), 1e-300))
+        errs = np.array(errs)
+        print(f"plate 1024x512 force dir {direction}: FD rel err median {np.median(errs):.2e} "
+              f"max {errs.max():.2e}", flush=True)
+        assert np.median(errs) < 1e-6
+        assert errs.max() < 1e-4
+    fv.close()
+    gb.close()
+
+
+@pytest.mark.timeout(900)
+def test_batch64_config5_matches_one_at_a_time_and_oracle():
+    """64 seeded instances (10k-400k cells, 13.0M in all): three ssp-rk3
+    steps of the batch, each step one HIP graph replay, equal three steps
+    of every instance on its own FiniteVolume + RungeKutta, bit for bit
+    (state, dt and time).  The smallest instance stepped by the oracle
+    gives the same state and dt as well."""
+    import torch
+    from ibis_amd.batch import Batch
+    from ibis_amd.config import ssp_rk3
+    from ibis_amd.finite_volume import FiniteVolume, RungeKutta
+    from ibis_amd.native import check, lib, ptr
+    nsteps = 3
+    specs = cases.batch_instances(n=64, seed=7)
+    inst = [cases.batch_case(s) for s in specs]
+    print("batch64: building the 64 grids", flush=True)
+    bt = Batch([(g, c) for g, c, _ in inst], device_blocks=True)
+    prims = [cases.uniform_prim(f, b.N, b.Nt) for (g, c, f), b in zip(inst, bt.blocks)]
+    d_prim = to_dev(bt.gather(prims))
+    d_cons = torch.zeros((bt.ncons, bt.Nt), dtype=torch.float64, device="cuda")
+    check(lib().ibis_prim_to_cons(bt.residual_handle(), ptr(d_prim), ptr(d_cons), None),
+          "ibis_prim_to_cons")
+    torch.cuda.synchronize()
+    bt.rk_init(d_cons, d_prim, cfl=0.5, max_time=1e30)
+    print(f"batch64: {bt.n} instances, {bt.N} cells, {nsteps} graph-replayed steps", flush=True)
+    bt.rk_steps(nsteps, use_graph=True)
+    torch.cuda.synchronize()
+    t_b, dt_b = bt.times()
+    cons_b = d_cons.cpu().numpy()
+    for i, ((g, c, f), b, p) in enumerate(zip(inst, bt.blocks, prims)):
+        fv = FiniteVolume(b, c)
+        dp = to_dev(p)
+        dc = torch.zeros((b.dim + 2, b.Nt), dtype=torch.float64, device="cuda")
+        fv.prim_to_cons(dp, dc)
+        rk = RungeKutta(fv, "ssp-rk3", cfl=0.5, max_time=1e30)
+        dts = [rk.take_step(dc, dp) for _ in range(nsteps)]
+        torch.cuda.synchronize()
+        alone = dc.cpu().numpy()
+        assert np.array_equal(bt.scatter(cons_b, i)[:, :b.N], alone[:, :b.N]), i
+        assert dt_b[i] == dts[-1], i
+        assert t_b[i] == rk.t, i
+        fv.close()
+    print("batch64: every instance identical to stepping it alone", flush=True)
+    k = int(np.argmin([b.N for b in bt.blocks]))
+    g_io, cfg, fs = inst[k]
+    b = bt.blocks[k]
+    og, ofv = oracle_pair(g_io, cfg)
+    prim = cases.uniform_prim(fs, og.N, og.Nt)
+    c_cpu, p_cpu = ofv.prim_to_cons(prim), prim.copy()
+    a, bw = ssp_rk3()
+    dt_prev, t = np.finfo(np.float64).max, 0.0
+    for _ in range(nsteps):
+        dt_prev = ofv.rk_step(a, bw, 0.5, c_cpu, p_cpu, dt_prev, t, 1e30)
+        t += dt_prev
+    got = b.to_reference_order(bt.scatter(cons_b, k))
+    assert np.array_equal(got[:, :og.N], c_cpu[:, :og.N])
+    assert dt_b[k] == dt_prev and t_b[k] == t
+    print(f"batch64: instance {k} ({og.N} cells) identical to the oracle after {nsteps} steps",
+          flush=True)
+    bt.close()
+
+
+@pytest.mark.parametrize("case", ["order1", "ldfss2", "subsonic"])
+def test_vjp_refuses_configurations_it_does_not_cover(case):
+    """First-order reconstruction, LDFSS2 and subsonic boundaries have no
+    reverse kernels: ibis_compute_dudt_vjp reports them and writes nothing."""
+    import torch
+    from ibis_amd import config as C
+    from ibis_amd.native import IbisError
+    kw = {"order1": dict(order=1), "ldfss2": dict(flux="ldfss2"), "subsonic": {}}[case]
+    g_io, cfg, fs = cases.wedge_case(16, 8, mach=0.5 if case == "subsonic" else 3.0, **kw)
+    if case == "subsonic":
+        cfg.boundaries = {"inflow": C.subsonic_inflow(fs),
+                          "outflow": C.subsonic_outflow(fs.p * 0.98), "wall": C.slip_wall()}
+    gb, fv = gpu_objects(g_io, cfg)
+    prim = to_dev(gb.to_grid_order(cases.perturbed_prim(g_io, fs, gb.Nt, seed=3)))
+    pb = torch.full((7, gb.Nt), 7.0, dtype=torch.float64, device="cuda")
+    match = {"order1": "second-order", "ldfss2": "AUSMDV, Hanel and Rusanov",
+             "subsonic": "subsonic boundaries"}[case]
+    with pytest.raises(IbisError, match=match):
+        fv.compute_dudt_vjp(prim, pb, dudt_bar=torch.ones((4, gb.Nt), dtype=torch.float64,
+                                                          device="cuda"))
+    torch.cuda.synchronize()
+    assert bool((pb == 7.0).all())
+    fv.close()
+    gb.close()

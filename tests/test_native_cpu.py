@@ -91,3 +91,52 @@ def test_config_json_round_trip():
     back = config.FVConfig.from_json(cfg.to_json())
     assert back.to_json() == cfg.to_json()
     assert back.tags() == ["inflow", "outflow", "wall"]
+
+
+def _mesh_desc(gb, ex, tag_off=None):
+    """An ibis_mesh_desc over a grid export (arrays kept alive by the caller)."""
+    keep = dict(ex)
+    keep["left"] = np.ascontiguousarray(ex["face_lr"][0])
+    keep["right"] = np.ascontiguousarray(ex["face_lr"][1])
+    if tag_off is not None:
+        keep["tag_off"] = tag_off
+    d = native.MeshDescStruct(gb.dim, gb.ntags, gb.N, gb.G, gb.F, native.ptr(keep["left"]),
+                              native.ptr(keep["right"]), native.ptr(keep["face_geom"]),
+                              native.ptr(keep["cell_off"]), native.ptr(keep["cell_faces"]),
+                              native.ptr(keep["outsigns"]), native.ptr(keep["neighbours"]),
+                              native.ptr(keep["centroid"]), native.ptr(keep["volume"]),
+                              native.ptr(keep["lsq"]), native.ptr(keep["tag_off"]),
+                              native.ptr(keep["bfaces"]))
+    return d, keep
+
+
+def test_mesh_create_rejects_a_ghost_on_no_tagged_face():
+    """A descriptor whose last tag lists none of its boundary faces leaves
+    those faces' ghosts on no tagged face: ibis_mesh_create refuses it on
+    the host, before any device call (the boundary and adjoint kernels
+    index their tables with the ghost's boundary face)."""
+    g_io, cfg = cases.wedge_case(8, 4)[:2]
+    gb = GridBlock(g_io, cfg, reorder=True, device=False)
+    ex = gb.export()
+    bad = ex["tag_off"].copy()
+    bad[-1] = bad[-2]  # the last tag keeps no boundary faces
+    d, keep = _mesh_desc(gb, ex, tag_off=bad)
+    h = C.c_void_p()
+    rc = native.lib().ibis_mesh_create(C.byref(d), C.byref(h))
+    assert rc == 1  # IBIS_ERR_INVALID
+    assert b"on no tagged boundary face" in native.lib().ibis_last_error()
+    assert not h.value
+
+
+def test_entry_kernel_index_range():
+    """face_kernel 1 addresses the face values with 32-bit starts, up to
+    maxf*5*N + 5*G; larger meshes are created with the tile-list kernel."""
+    fit = native.lib().ibis_entry_indices_fit
+    fit.restype = C.c_int
+    fit.argtypes = [C.c_int64, C.c_int64, C.c_int64]
+    assert fit(6, 4_096_000, 153_600) == 1  # the 160^3 box
+    n = (2 ** 31 - 1) // 30  # largest hex mesh without ghosts that fits
+    assert fit(6, n, 0) == 1
+    assert fit(6, n + 1, 0) == 0
+    assert fit(6, 72_000_000, 1_000_000) == 0  # ~72M hexes fit in HBM but not in int32 starts
+    assert fit(4, 100_000_000, 0) == 1 and fit(4, 110_000_000, 0) == 0  # 2D quads

@@ -156,7 +156,7 @@ def test_supersonic_upwinding_gives_left_flux(flux):
 def test_rusanov_closed_form():
     """Rusanov: 1/2 (F_L + F_R) - 1/2 S (W_R - W_L), S = max|u +- a|, on a
     subsonic pair.  The reference dissipates (rho, rho u, rho v, rho H), i.e.
-    rho*H rather than the total energy in the last row (rusanov.cpp:142-143);
+    rho*H rather than the total energy in the last row (rusanov.cpp:50-51);
     a textbook Rusanov (rho*E) differs from it by 1/2 S (p_R - p_L)."""
     left = (1.3, 1.2e5, 80.0, 20.0)
     right = (1.0, 0.9e5, -40.0, 10.0)
