@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libibis_gpu.so")
-SOURCES = ["grid.cpp", "residual.hip", "jfnk.hip", "batch.hip"]
+SOURCES = ["status.cpp", "grid.cpp", "residual.hip", "jfnk.hip", "batch.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

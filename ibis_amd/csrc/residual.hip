@@ -48,8 +48,6 @@
 
 namespace {
 
-thread_local std::string g_last_error;
-
 #define HIP_TRY(expr)                                                                   \
     do {                                                                                \
         hipError_t e_ = (expr);                                                         \
@@ -2013,15 +2011,6 @@ inline unsigned xcd_grid(long nlogical) {
 // ===========================================================================
 // host side
 // ===========================================================================
-namespace ibis {
-ibis_status fail(ibis_status s, const char* msg) {
-    g_last_error = msg ? msg : "";
-    return s;
-}
-}  // namespace ibis
-
-extern "C" const char* ibis_last_error(void) { return g_last_error.c_str(); }
-
 int ibis_entry_indices_fit(int64_t maxf, int64_t n_cells, int64_t n_ghost) {
     return maxf * 5 * n_cells + 5 * n_ghost < (int64_t)INT32_MAX ? 1 : 0;
 }

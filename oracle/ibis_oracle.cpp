@@ -608,6 +608,7 @@ struct FV {
     std::vector<T> grad;   // NGRAD x 3 x N
     std::vector<T> lim;    // NGRAD x N
     std::vector<T> face_fs;  // 7 x F (viscous face states)
+    std::vector<T> face_grad;  // 4 x 3 x F: face gradients of T, vx, vy, vz (viscous)
 
     FV(Grid* grid, const or_fv_config* c) : g(grid), cfg(*c) {
         N = g->N;
@@ -620,6 +621,7 @@ struct FV {
         grad.assign((size_t)NGRAD * 3 * N, T(0.0));
         lim.assign((size_t)NGRAD * N, T(0.0));
         face_fs.assign((size_t)NPRIM * F, T(0.0));
+        face_grad.assign((size_t)12 * F, T(0.0));
     }
     T* gcomp(int var, int comp) { return &grad[((size_t)var * 3 + comp) * N]; }
 
@@ -1287,6 +1289,12 @@ struct FV {
                 hasselbacher(G_VZ, fs[P_VZ * Nt + l], fs[P_VZ * Nt + r], l, r, ehat, n, len_e, edn, gvz);
                 hasselbacher(G_T, fs[P_T * Nt + l], fs[P_T * Nt + r], l, r, ehat, n, len_e, edn, gT);
             }
+            for (int k = 0; k < 3; k++) {
+                face_grad[(size_t)(0 * 3 + k) * F + f] = gT[k];
+                face_grad[(size_t)(1 * 3 + k) * F + f] = gvx[k];
+                face_grad[(size_t)(2 * 3 + k) * F + f] = gvy[k];
+                face_grad[(size_t)(3 * 3 + k) * F + f] = gvz[k];
+            }
             // average_flow_states_pT (GAS/flow_state.h:89-91, GAS/gas_state.h:108-114)
             T p = 0.5 * (fs[P_P * Nt + l] + fs[P_P * Nt + r]);
             T temp = 0.5 * (fs[P_T * Nt + l] + fs[P_T * Nt + r]);
@@ -1624,6 +1632,67 @@ void or_fv_intermediates(void* p, double* grads, double* lim, double* flux, doub
     if (flux) std::memcpy(flux, fv.flux.data(), fv.flux.size() * 8);
     if (left) std::memcpy(left, fv.left.data(), fv.left.size() * 8);
     if (right) std::memcpy(right, fv.right.data(), fv.right.size() * 8);
+}
+
+void or_fv_face_gradients(void* p, double* out) {
+    FV<double>& fv = *static_cast<FVHandle*>(p)->fv;
+    std::memcpy(out, fv.face_grad.data(), fv.face_grad.size() * 8);
+}
+
+// One flux calculator on one face, states already in the face frame
+// (rho, p, T, e, u_n, u_t1, u_t2); out: mass, n-, t1-, t2-momentum, energy.
+// The calculators read only the gas constants, the LDFSS delta and dim.
+void or_flux_face(const or_fv_config* cfg, int dim, const double* L, const double* R,
+                  double* out) {
+    Grid g;
+    g.dim = dim;
+    FV<double> fv(&g, cfg);
+    double l[NPRIM], r[NPRIM];
+    for (int v = 0; v < NPRIM; v++) {
+        l[v] = L[v];
+        r[v] = R[v];
+    }
+    switch (cfg->flux) {
+        case OR_FLUX_HANEL: fv.flux_hanel(l, r, out); break;
+        case OR_FLUX_AUSMDV: fv.flux_ausmdv(l, r, out); break;
+        case OR_FLUX_LDFSS2: fv.flux_ldfss2(l, r, out); break;
+        default: fv.flux_rusanov(l, r, out); break;
+    }
+}
+
+// Dual<double> arithmetic (UT/dual.h), for checking against known answers.
+// a, b: (real, dual) pairs; out: (real, dual).  op: 0 a+b, 1 a-b, 2 a*b,
+// 3 a/b, 4 sqrt(a), 5 abs(a), 6 pow(a, b), 7 min(a, b), 8 max(a, b),
+// 9 floor(a), 10 copysign(a, b), 11 a+b.r, 12 a-b.r, 13 b.r-a, 14 a*b.r,
+// 15 b.r*a, 16 a/b.r, 17 b.r/a, 18 a+=b, 19 a-=b
+void or_dual_eval(int op, const double* a, const double* b, double* out) {
+    Dual x(a[0], a[1]), y(b[0], b[1]), z;
+    double yr = b[0];
+    switch (op) {
+        case 0: z = x + y; break;
+        case 1: z = x - y; break;
+        case 2: z = x * y; break;
+        case 3: z = x / y; break;
+        case 4: z = ksqrt(x); break;
+        case 5: z = kabs(x); break;
+        case 6: z = kpow(x, y); break;
+        case 7: z = kmin(x, y); break;
+        case 8: z = kmax(x, y); break;
+        case 9: z = kfloor(x); break;
+        case 10: z = kcopysign(x, y); break;
+        case 11: z = x + yr; break;
+        case 12: z = x - yr; break;
+        case 13: z = yr - x; break;
+        case 14: z = x * yr; break;
+        case 15: z = yr * x; break;
+        case 16: z = x / yr; break;
+        case 17: z = yr / x; break;
+        case 18: z = x; z += y; break;
+        case 19: z = x; z -= y; break;
+        default: z = Dual(std::nan(""), std::nan("")); break;
+    }
+    out[0] = z.r;
+    out[1] = z.d;
 }
 
 // RungeKutta::take_step + estimate_dt, SO/runge_kutta.cpp:108-203 (static grid)

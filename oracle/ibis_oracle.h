@@ -110,6 +110,14 @@ void or_compute_convective_gradient(void* fv, double* prim);
 void or_compute_viscous_gradient(void* fv, double* prim);
 void or_fv_intermediates(void* fv, double* grads, double* lim, double* flux,
                          double* left, double* right);
+/* face gradients of the last viscous residual: 4 vars (T, vx, vy, vz) x 3 x F */
+void or_fv_face_gradients(void* fv, double* out);
+/* one flux calculator on one face; L, R: rho, p, T, e, u_n, u_t1, u_t2 in
+   the face frame; out: mass, n-, t1-, t2-momentum, energy */
+void or_flux_face(const or_fv_config* cfg, int dim, const double* L, const double* R,
+                  double* out);
+/* Dual<double> operations (see the op codes in ibis_oracle.cpp) */
+void or_dual_eval(int op, const double* a, const double* b, double* out);
 /* One explicit RK step (SO/runge_kutta.cpp:131-203).  cons/prim are the
    solution; dt_prev in/out.  Returns the dt taken. */
 double or_rk_step(void* fv, int nstages, const double* a /* nstages x nstages, row i col j */,

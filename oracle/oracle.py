@@ -79,6 +79,9 @@ def lib():
         L.or_fv_intermediates.argtypes = [C.c_void_p] + [C.c_void_p] * 5
         L.or_compute_convective_gradient.argtypes = [C.c_void_p, _f64p]
         L.or_compute_viscous_gradient.argtypes = [C.c_void_p, _f64p]
+        L.or_fv_face_gradients.argtypes = [C.c_void_p, _f64p]
+        L.or_flux_face.argtypes = [C.POINTER(FVConfigStruct), C.c_int, _f64p, _f64p, _f64p]
+        L.or_dual_eval.argtypes = [C.c_int, _f64p, _f64p, _f64p]
         L.or_rk_step.restype = C.c_double
         L.or_rk_step.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, C.c_double, _f64p, _f64p,
                                  C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double]
@@ -174,6 +177,32 @@ class OracleGrid:
             self.h = None
 
 
+def flux_face(cfg, dim, left, right):
+    """The configured flux calculator on one face.  left/right: rho, p, T,
+    e, u_n, u_t1, u_t2 in the face frame.  Returns (mass, n-, t1-, t2-momentum,
+    energy)."""
+    s = fv_struct(cfg)
+    out = np.zeros(5)
+    lib().or_flux_face(C.byref(s), int(dim), np.ascontiguousarray(left, dtype=np.float64),
+                       np.ascontiguousarray(right, dtype=np.float64), out)
+    return out
+
+
+DUAL_OPS = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4, "abs": 5, "pow": 6, "min": 7,
+            "max": 8, "floor": 9, "copysign": 10, "add_real": 11, "sub_real": 12,
+            "real_sub": 13, "mul_real": 14, "real_mul": 15, "div_real": 16, "real_div": 17,
+            "iadd": 18, "isub": 19}
+
+
+def dual_eval(op, a, b=(0.0, 0.0)):
+    """The oracle's Dual<double> operation `op` on (real, dual) pairs a, b
+    (for the *_real forms only b's real part is used)."""
+    out = np.zeros(2)
+    lib().or_dual_eval(DUAL_OPS[op], np.array(a, dtype=np.float64),
+                       np.array(b, dtype=np.float64), out)
+    return out[0], out[1]
+
+
 def fv_struct(cfg) -> FVConfigStruct:
     from ibis_amd.config import FLUX, LIMITER, THERMO  # input encoding only
     enc = cfg.encode()
@@ -265,6 +294,13 @@ class OracleFV:
                "right": np.zeros((7, g.F))}
         lib().or_fv_intermediates(self.h, *[_ptr(out[k]) for k in
                                             ("grads", "lim", "flux", "left", "right")])
+        return out
+
+    def face_gradients(self):
+        """Face gradients of T, vx, vy, vz (4 x 3 x F) of the last viscous
+        residual (copy_gradients_to_face / hasselbacher_average)."""
+        out = np.zeros((4, 3, self.grid.F))
+        lib().or_fv_face_gradients(self.h, out)
         return out
 
     def rk_step(self, a, b, cfl, cons, prim, dt_prev, t, max_time, plot_remaining=-1.0):
