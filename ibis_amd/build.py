@@ -29,16 +29,43 @@ def hipcc() -> str:
     return "hipcc"
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(LIBDIR, exist_ok=True)
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(HERE, "..", "include"),
-           *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp]
+HOST = os.path.join(HERE, "host")
+HOST_LIB = os.path.join(LIBDIR, "libibis_host.so")
+FV_RUN = os.path.join(LIBDIR, "ibis_fv_run")
+# the C++ host layer (FiniteVolume / RungeKutta over the C ABI) and its
+# command-line driver: host code only, linked against libibis_gpu.so
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
+
+
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = LIB + ".tmp"
+    _run([hipcc(), *HIPCC_FLAGS, "-I", os.path.join(HERE, "..", "include"),
+          *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp], verbose)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+def build_host(verbose: bool = False) -> str:
+    """libibis_host.so and ibis_fv_run, linked against libibis_gpu.so (found
+    next to them at run time)."""
+    tmp = HOST_LIB + ".tmp"
+    _run([hipcc(), *HOST_FLAGS, "-shared", os.path.join(HOST, "json.cpp"),
+          os.path.join(HOST, "ibis_fv.cpp"), "-L", LIBDIR, "-libis_gpu",
+          "-Wl,-rpath,$ORIGIN", "-o", tmp], verbose)
+    os.replace(tmp, HOST_LIB)
+    tmp = FV_RUN + ".tmp"
+    _run([hipcc(), *HOST_FLAGS, os.path.join(HOST, "fv_run.cpp"), "-L", LIBDIR, "-libis_host",
+          "-libis_gpu", "-Wl,-rpath,$ORIGIN", "-o", tmp], verbose)
+    os.replace(tmp, FV_RUN)
+    return HOST_LIB
 
 
 if __name__ == "__main__":

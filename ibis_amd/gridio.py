@@ -127,7 +127,9 @@ def write_su2(grid: GridIO, path: str) -> None:
             fh.write(f"{int(grid.ctype[k])} {ids} {k}\n")
         fh.write(f"NPOIN= {grid.xyz.shape[0]}\n")
         for k in range(grid.xyz.shape[0]):
-            c = grid.xyz[k]
+            # repr of a Python float: the shortest string that reads back
+            # to the same double
+            c = [float(v) for v in grid.xyz[k]]
             if grid.dim == 3:
                 fh.write(f"{c[0]!r} {c[1]!r} {c[2]!r} {k}\n")
             else:
