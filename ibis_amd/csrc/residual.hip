@@ -35,6 +35,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <tuple>
@@ -3632,6 +3633,45 @@ ibis_status ibis_apply_time_derivative(ibis_residual* r, const double* u0, doubl
     return IBIS_OK;
 }
 
+// One RK step on caller-supplied stage buffers: k[0..s) and k_tmp of
+// (dim+2) x n_total doubles, flow_tmp of 7 x n_total.
+static ibis_status rk_step_on(ibis_residual* r, int s, const double* a, const double* b,
+                              double cfl, double* cons, double* prim, double* dt_prev, double t,
+                              double max_time, double plot_remaining, double* dt_taken,
+                              double* const* k, double* k_tmp, double* flow_tmp,
+                              ibis_stream_t stream) {
+    ibis_status rc;
+    // SO/runge_kutta.cpp:143-147
+    if ((rc = ibis_compute_dudt(r, prim, k[0], 1, stream)) != IBIS_OK) return rc;
+    double stable_dt;
+    if ((rc = ibis_estimate_dt(r, prim, &stable_dt, stream)) != IBIS_OK) return rc;
+    // RungeKutta::estimate_dt, SO/runge_kutta.cpp:108-119 (Kokkos::min ordering)
+    auto kmin = [](double x, double y) { return (y < x) ? y : x; };
+    double dt = kmin(cfl * stable_dt, 1.5 * (*dt_prev));
+    dt = kmin(dt, max_time - t);
+    if (plot_remaining > 0.0) dt = kmin(dt, plot_remaining);
+    for (int i = 1; i < s; i++) {  // SO/runge_kutta.cpp:151-182
+        if ((rc = ibis_apply_time_derivative(r, cons, k_tmp, k[0], a[i * s + 0] * dt, stream)) !=
+            IBIS_OK)
+            return rc;
+        for (int j = 1; j < i; j++) {
+            if (a[i * s + j] < 1e-14) continue;
+            if ((rc = ibis_apply_time_derivative(r, k_tmp, k_tmp, k[j], a[i * s + j] * dt,
+                                                 stream)) != IBIS_OK)
+                return rc;
+        }
+        if ((rc = ibis_cons_to_prim(r, k_tmp, flow_tmp, stream)) != IBIS_OK) return rc;
+        if ((rc = ibis_compute_dudt(r, flow_tmp, k[i], 1, stream)) != IBIS_OK) return rc;
+    }
+    for (int i = 0; i < s; i++)  // SO/runge_kutta.cpp:185-191
+        if ((rc = ibis_apply_time_derivative(r, cons, cons, k[i], b[i] * dt, stream)) != IBIS_OK)
+            return rc;
+    if ((rc = ibis_cons_to_prim(r, cons, prim, stream)) != IBIS_OK) return rc;
+    *dt_prev = dt;
+    if (dt_taken) *dt_taken = dt;
+    return IBIS_OK;
+}
+
 ibis_status ibis_rk_step(ibis_residual* r, int s, const double* a, const double* b, double cfl,
                          double* cons, double* prim, double* dt_prev, double t, double max_time,
                          double plot_remaining, double* dt_taken, ibis_stream_t stream) {
@@ -3651,36 +3691,86 @@ ibis_status ibis_rk_step(ibis_residual* r, int s, const double* a, const double*
         HIP_TRY(hipMalloc(&r->flow_tmp, (size_t)NPRIM * D.Nt * sizeof(double)));
         HIP_TRY(hipMemset(r->flow_tmp, 0, (size_t)NPRIM * D.Nt * sizeof(double)));
     }
-    ibis_status rc;
-    // SO/runge_kutta.cpp:143-147
-    if ((rc = ibis_compute_dudt(r, prim, r->k[0], 1, stream)) != IBIS_OK) return rc;
-    double stable_dt;
-    if ((rc = ibis_estimate_dt(r, prim, &stable_dt, stream)) != IBIS_OK) return rc;
-    // RungeKutta::estimate_dt, SO/runge_kutta.cpp:108-119 (Kokkos::min ordering)
-    auto kmin = [](double x, double y) { return (y < x) ? y : x; };
-    double dt = kmin(cfl * stable_dt, 1.5 * (*dt_prev));
-    dt = kmin(dt, max_time - t);
-    if (plot_remaining > 0.0) dt = kmin(dt, plot_remaining);
-    for (int i = 1; i < s; i++) {  // SO/runge_kutta.cpp:151-182
-        if ((rc = ibis_apply_time_derivative(r, cons, r->k_tmp, r->k[0], a[i * s + 0] * dt,
-                                             stream)) != IBIS_OK)
-            return rc;
-        for (int j = 1; j < i; j++) {
-            if (a[i * s + j] < 1e-14) continue;
-            if ((rc = ibis_apply_time_derivative(r, r->k_tmp, r->k_tmp, r->k[j], a[i * s + j] * dt,
-                                                 stream)) != IBIS_OK)
-                return rc;
-        }
-        if ((rc = ibis_cons_to_prim(r, r->k_tmp, r->flow_tmp, stream)) != IBIS_OK) return rc;
-        if ((rc = ibis_compute_dudt(r, r->flow_tmp, r->k[i], 1, stream)) != IBIS_OK) return rc;
+    return rk_step_on(r, s, a, b, cfl, cons, prim, dt_prev, t, max_time, plot_remaining,
+                      dt_taken, r->k.data(), r->k_tmp, r->flow_tmp, stream);
+}
+
+// ---------------------------------------------------------------------------
+// ibis_rk_step_noalloc: the RK step with its stage buffers in `scratch`, a
+// fixed device array of the library, instead of buffers allocated on the
+// first call.  Each GPU holds its own copy of the array (its address is
+// looked up once per device with hipGetSymbolAddress); every call on a GPU
+// shares it and finds the previous call's contents in it.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[IBIS_RK_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices];  // per device: the device address of `scratch`
+
+// ghost rows [n_valid, n_total) of `nrows` rows of length n_total
+__global__ void k_zero_ghost_rows(int nrows, long n_valid, long n_total, double* a) {
+    const long ng = n_total - n_valid;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)nrows * ng) return;
+    a[(i / ng) * n_total + n_valid + i % ng] = 0.0;
+}
+}  // namespace
+
+static ibis_status scratch_address(double** out) {
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return ibis::fail(IBIS_ERR_INVALID, "ibis_rk_step_noalloc: device id out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = static_cast<double*>(p);
     }
-    for (int i = 0; i < s; i++)  // SO/runge_kutta.cpp:185-191
-        if ((rc = ibis_apply_time_derivative(r, cons, cons, r->k[i], b[i] * dt, stream)) != IBIS_OK)
-            return rc;
-    if ((rc = ibis_cons_to_prim(r, cons, prim, stream)) != IBIS_OK) return rc;
-    *dt_prev = dt;
-    if (dt_taken) *dt_taken = dt;
+    *out = g_scratch_dev[dev];
     return IBIS_OK;
+}
+
+ibis_status ibis_rk_step_noalloc(ibis_residual* r, int s, const double* a, const double* b,
+                                 double cfl, double* cons, double* prim, double* dt_prev,
+                                 double t, double max_time, double plot_remaining,
+                                 double* dt_taken, ibis_stream_t stream) {
+    if (!r || !a || !b || !cons || !prim || !dt_prev || s < 1 || s > 8)
+        return ibis::fail(IBIS_ERR_INVALID, "ibis_rk_step_noalloc: bad argument");
+    const DevMesh& D = r->m->d;
+    const size_t nc = (size_t)(D.dim + 2);
+    const size_t ncv = nc * D.Nt;
+    const size_t need = ((size_t)s + 1) * ncv + (size_t)NPRIM * D.Nt;  // doubles
+    if (need * sizeof(double) > (size_t)IBIS_RK_NOALLOC_SCRATCH_BYTES) {
+        char msg[200];
+        snprintf(msg, sizeof(msg),
+                 "ibis_rk_step_noalloc: %d stages on %lld cells need %zu bytes of stage "
+                 "buffers, above the fixed %zu-byte scratch",
+                 s, (long long)D.Nt, need * sizeof(double), (size_t)IBIS_RK_NOALLOC_SCRATCH_BYTES);
+        return ibis::fail(IBIS_ERR_INVALID, msg);
+    }
+    double* base = nullptr;
+    ibis_status rc = scratch_address(&base);
+    if (rc != IBIS_OK) return rc;
+    double* k[8];
+    for (int i = 0; i < s; i++) k[i] = base + (size_t)i * ncv;
+    double* k_tmp = base + (size_t)s * ncv;
+    double* flow_tmp = base + ((size_t)s + 1) * ncv;
+    // the residual writes only the valid rows of each k_j; its ghost rows
+    // must read 0, as in the zero-initialised buffers of ibis_rk_step (the
+    // scratch holds whatever the previous call left)
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const long ng = (long)D.Nt - (long)D.N;
+    if (ng > 0) {
+        const long n = (long)s * (long)nc * ng;
+        k_zero_ghost_rows<<<(unsigned)((n + BLOCK - 1) / BLOCK), BLOCK, 0, st>>>(
+            (int)(s * nc), D.N, D.Nt, base);
+        HIP_TRY(hipGetLastError());
+    }
+    return rk_step_on(r, s, a, b, cfl, cons, prim, dt_prev, t, max_time, plot_remaining,
+                      dt_taken, k, k_tmp, flow_tmp, stream);
 }
 
 // FiniteVolume::compute_convective_gradient / compute_viscous_gradient

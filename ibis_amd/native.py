@@ -114,6 +114,9 @@ _SIGS = {
     "ibis_rk_step": (C.c_int, [_P, C.c_int, _P, _P, C.c_double, _P, _P,
                                C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double,
                                C.POINTER(C.c_double), _P]),
+    "ibis_rk_step_noalloc": (C.c_int, [_P, C.c_int, _P, _P, C.c_double, _P, _P,
+                               C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double,
+                               C.POINTER(C.c_double), _P]),
     "ibis_residual_scratch": (C.c_int, [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                         C.POINTER(C.c_void_p), C.POINTER(C.c_int)]),
     "ibis_compute_convective_gradient": (C.c_int, [_P, _P, _P]),

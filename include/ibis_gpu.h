@@ -257,6 +257,22 @@ ibis_status ibis_rk_step(ibis_residual* r, int nstages, const double* a, const d
                          double max_time, double plot_remaining, double* dt_taken,
                          ibis_stream_t stream);
 
+/* ibis_rk_step without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight time-stepping or serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and the same results, bit for bit, as
+ * ibis_rk_step; the stage buffers ((nstages + 1) x (dim+2) x n_total +
+ * 7 x n_total doubles) live in a fixed library array of
+ * IBIS_RK_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends, and
+ * larger meshes are refused with IBIS_ERR_INVALID.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (two streams, two
+ * threads) race and must not be issued. */
+#define IBIS_RK_NOALLOC_SCRATCH_BYTES (64u << 20)
+ibis_status ibis_rk_step_noalloc(ibis_residual* r, int nstages, const double* a,
+                                 const double* b, double cfl, double* cons, double* prim,
+                                 double* dt_prev, double t, double max_time,
+                                 double plot_remaining, double* dt_taken, ibis_stream_t stream);
+
 /* FiniteVolume<T>::compute_convective_gradient and
  * compute_viscous_gradient (finite_volume.h:104-112, finite_volume.cpp:
  * 273-288), the post-processing gradients read by IO/accessor.cpp:115-250.
