@@ -4,6 +4,10 @@ Forward passes run the MI355X residual; backward passes run its hand-written
 discrete adjoint (ibis_compute_dudt_vjp), which recomputes reconstructed
 states instead of storing them.  The limiter is differentiated ("active") or
 held fixed ("frozen").
+
+``fv`` is either ibis_amd.finite_volume.FiniteVolume (ctypes on the C ABI)
+or ibis_amd.pyapi.PyFiniteVolume (the pybind11 module python_api); both
+give the same bits.
 """
 from __future__ import annotations
 

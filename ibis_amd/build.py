@@ -65,7 +65,27 @@ def build_host(verbose: bool = False) -> str:
     _run([hipcc(), *HOST_FLAGS, os.path.join(HOST, "fv_run.cpp"), "-L", LIBDIR, "-libis_host",
           "-libis_gpu", "-Wl,-rpath,$ORIGIN", "-o", tmp], verbose)
     os.replace(tmp, FV_RUN)
+    build_python_api(verbose)
     return HOST_LIB
+
+
+def python_api_path() -> str:
+    import sysconfig
+    return os.path.join(LIBDIR, "python_api" + sysconfig.get_config_var("EXT_SUFFIX"))
+
+
+def build_python_api(verbose: bool = False) -> str:
+    """The pybind11 module python_api (ibis_amd/host/python_api.cpp), linked
+    against libibis_host.so and libibis_gpu.so next to it."""
+    import sysconfig
+    import pybind11
+    out = python_api_path()
+    tmp = out + ".tmp"
+    _run([hipcc(), *HOST_FLAGS, "-shared", "-fvisibility=hidden", "-I", pybind11.get_include(),
+          "-I", sysconfig.get_paths()["include"], os.path.join(HOST, "python_api.cpp"),
+          "-L", LIBDIR, "-libis_host", "-libis_gpu", "-Wl,-rpath,$ORIGIN", "-o", tmp], verbose)
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -87,6 +87,8 @@ public:
     size_t num_total_cells() const { return (size_t)(N_ + G_); }
     size_t num_interfaces() const { return (size_t)F_; }
     const std::vector<std::string>& boundary_tags() const { return tags_; }
+    // grid index of each reference cell (valid cells, then ghosts)
+    const std::vector<int64_t>& cell_new_of_old() const { return cell_new_of_old_; }
 
     // rows x n_total arrays between the reference numbering and the grid's
     std::vector<double> to_grid_order(const std::vector<double>& ref, size_t rows) const;
