@@ -224,6 +224,62 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
+// First-order convective face flux backwards (copy_reconstruct,
+// FV/convective_flux.cpp:212-240): the face states are the two cells'
+// whole prim rows, rotated to the face frame, so the flux's cotangents go
+// straight to rho, p, T, e and (rotated back) vx, vy, vz of each side:
+// QLb / QRb hold 7 rows here, in prim row order.
+template <int DIM, int FLUX>
+__global__ void __launch_bounds__(BLOCK) k_vjp_conv_face1(DevMesh m, FaceParams fp, AdjBufs B) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int F = m.F, Nt = m.Nt;
+    if (f >= F) return;
+    constexpr int NC = DIM + 2;
+    constexpr bool T3 = DIM == 3;
+    double fb[NC];
+    face_flux_bar<DIM>(m, B, f, fb);
+    const double nx = m.geom[f], ny = m.geom[(size_t)F + f], nz = m.geom[2 * (size_t)F + f];
+    const double t1x = m.geom[3 * (size_t)F + f], t1y = m.geom[4 * (size_t)F + f],
+                 t1z = m.geom[5 * (size_t)F + f];
+    const double t2x = m.geom[6 * (size_t)F + f], t2y = m.geom[7 * (size_t)F + f],
+                 t2z = m.geom[8 * (size_t)F + f];
+    double st[2][NPRIM];
+    for (int side = 0; side < 2; side++) {
+        const int c = side ? m.R[f] : m.L[f];
+        for (int k = 0; k < NPRIM; k++) st[side][k] = B.prim_c[(size_t)k * Nt + c];
+        const double a = st[side][4], b = st[side][5], cc = st[side][6];
+        st[side][4] = a * nx + b * ny + cc * nz;
+        st[side][5] = a * t1x + b * t1y + cc * t1z;
+        st[side][6] = a * t2x + b * t2y + cc * t2z;
+    }
+    double ob[5];
+    ob[0] = fb[0];
+    ob[1] = nx * fb[1] + ny * fb[2] + (T3 ? nz * fb[3] : 0.0);
+    ob[2] = t1x * fb[1] + t1y * fb[2] + (T3 ? t1z * fb[3] : 0.0);
+    ob[3] = T3 ? t2x * fb[1] + t2y * fb[2] + t2z * fb[3] : 0.0;
+    ob[4] = fb[NC - 1];
+    double sb[2][NPRIM];
+    for (int side = 0; side < 2; side++)
+        for (int k = 0; k < NPRIM; k++) sb[side][k] = 0.0;
+    if (FLUX == IBIS_FLUX_AUSMDV)
+        ibis::flux_ausmdv_vjp<T3>(st[0], st[1], fp.R, fp.Cv, fp.gamma, ob, sb[0], sb[1]);
+    else if (FLUX == IBIS_FLUX_HANEL)
+        ibis::flux_hanel_vjp<T3>(st[0], st[1], fp.R, fp.Cv, fp.gamma, ob, sb[0], sb[1]);
+    else
+        ibis::flux_rusanov_vjp<T3>(st[0], st[1], fp.R, fp.Cv, fp.gamma, ob, sb[0], sb[1]);
+    for (int side = 0; side < 2; side++) {
+        const double s4 = sb[side][4], s5 = sb[side][5], s6 = sb[side][6];
+        double* out = side ? B.QRb : B.QLb;
+        out[(size_t)P_RHO * F + f] = sb[side][P_RHO];
+        out[(size_t)P_P * F + f] = sb[side][P_P];
+        out[(size_t)P_T * F + f] = sb[side][P_T];
+        out[(size_t)P_E * F + f] = sb[side][P_E];
+        out[(size_t)P_VX * F + f] = nx * s4 + t1x * s5 + t2x * s6;
+        out[(size_t)P_VY * F + f] = ny * s4 + t1y * s5 + t2y * s6;
+        out[(size_t)P_VZ * F + f] = nz * s4 + t1z * s5 + t2z * s6;
+    }
+}
+
 // GAS/transport_properties.h:27-42: d mu / d T of Sutherland's law
 __device__ __forceinline__ double sutherland_dT(double T, double mu, const ViscParams& vp) {
     return mu * (1.5 / T - 1.0 / (T + vp.Ts));
@@ -355,7 +411,9 @@ __global__ void __launch_bounds__(BLOCK)
 }
 
 // Per valid cell: reconstruction, limiter and own-stencil terms
-template <int DIM, int LIM, bool ACTIVE, bool VISC>
+// ORD1: first-order residual -- the cell's own cotangent is the sum of its
+// faces' (7 prim rows); there is no gradient or limiter to go through.
+template <int DIM, int LIM, bool ACTIVE, bool VISC, bool ORD1 = false>
 __global__ void __launch_bounds__(BLOCK)
     k_vjp_cell(DevMesh m, VarSet vs, double eps, double venkat_K, AdjBufs B) {
     // each thread's face offsets (maxf x 3 doubles), face ids (one's
@@ -378,7 +436,17 @@ __global__ void __launch_bounds__(BLOCK)
         if (ACTIVE) sn[j * BLOCK + tid] = m.ell_nbr[(size_t)j * N + i];
         for (int k = 0; k < 3; k++) sd[(j * 3 + k) * BLOCK + tid] = B.offs[(size_t)(j * 3 + k) * N + i];
     }
-    for (int v = 0; v < 5; v++) {
+    if (ORD1) {
+        for (int k = 0; k < NPRIM; k++) {
+            double self = 0.0;
+            for (int j = 0; j < nf; j++) {
+                const int fe = sf[j * BLOCK + tid];
+                self += (fe >= 0 ? B.QLb : B.QRb)[(size_t)k * F + (fe >= 0 ? fe : ~fe)];
+            }
+            B.cself[(size_t)k * N + i] = self;
+        }
+    }
+    for (int v = 0; v < (ORD1 ? 0 : 5); v++) {
         const int row = vs.row[v], slot = vs.slot[v];
         const double* u = B.prim_c + (size_t)row * Nt;
         const double ui = u[i];
@@ -513,11 +581,83 @@ __device__ __forceinline__ int slot_of(const DevMesh& m, int f, int c) {
 }
 
 // reverse of one boundary action on the ghost bar gb (7 prim rows) and
-// the valid cell's bar vb (boundary.cpp:15-277)
-__device__ __forceinline__ void bc_action_vjp(const DevMesh& m, int kind, int f, double* gb,
-                                              double* vb) {
+// the valid cell's bar vb (boundary.cpp:15-399); par: the action's
+// parameters, vst: the valid cell's prim values (forward)
+__device__ __forceinline__ void bc_action_vjp(const DevMesh& m, int kind, int f,
+                                              const double* par, const double* vst,
+                                              const GasParams& gp, double* gb, double* vb) {
     const int F = m.F;
     switch (kind) {
+        case IBIS_BC_SUBSONIC_INFLOW:    // boundary.cpp:286-342
+        case IBIS_BC_SUBSONIC_OUTFLOW: {  // boundary.cpp:348-399
+            const double n[3] = {m.geom[f], m.geom[(size_t)F + f], m.geom[2 * (size_t)F + f]};
+            const double gR = gp.gamma * gp.R;
+            const double a = sqrt(gR * vst[P_T]);
+            const double rr = vst[P_RHO];
+            const double D = rr * a;
+            const bool in = kind == IBIS_BC_SUBSONIC_INFLOW;
+            // forward: face pressure and density, ghost pressure/density/T
+            double pf, rf, vc = 0.0, X;
+            if (in) {
+                for (int k = 0; k < 3; k++) vc += n[k] * (par[4 + k] - vst[P_VX + k]);
+                pf = 0.5 * (par[1] + vst[P_P] - rr * a * vc);
+                rf = par[2] + (pf - par[1]) / a / a;
+                X = (par[1] - pf) / D;  // v_face = v_in - n X
+            } else {
+                pf = par[0];
+                rf = rr + (pf - vst[P_P]) / a / a;
+                X = (vst[P_P] - pf) / D;  // v_face = v_valid + n X
+            }
+            const double pg = 2.0 * pf - vst[P_P];
+            const double rg = 2.0 * rf - rr;
+            const double Tg = pg / (rg * gp.R);
+            // backward: e = Cv T, T = p / (rho R), ghost = 2 face - valid
+            const double Tgb = gb[P_T] + gp.Cv * gb[P_E];
+            const double pgb = gb[P_P] + Tgb / (rg * gp.R);
+            const double rgb = gb[P_RHO] - Tgb * Tg / rg;
+            double vfb[3];
+            for (int k = 0; k < 3; k++) {
+                vfb[k] = 2.0 * gb[P_VX + k];
+                vb[P_VX + k] -= gb[P_VX + k];
+            }
+            const double rfb = 2.0 * rgb;
+            vb[P_RHO] -= rgb;
+            vb[P_P] -= pgb;
+            double pfb = in ? 2.0 * pgb : 0.0;  // the outflow's face pressure is given
+            const double nv = n[0] * vfb[0] + n[1] * vfb[1] + n[2] * vfb[2];
+            double Xb, prb = 0.0, rrb = 0.0, ab = 0.0;
+            if (in) {
+                Xb = -nv;
+                pfb += -Xb / D;
+            } else {
+                for (int k = 0; k < 3; k++) vb[P_VX + k] += vfb[k];
+                Xb = nv;
+                prb += Xb / D;
+            }
+            const double Db = -Xb * X / D;
+            rrb += Db * a;
+            ab += Db * rr;
+            const double dpa = in ? pf - par[1] : pf - vst[P_P];  // rho_face's pressure term
+            if (in) {
+                pfb += rfb / (a * a);
+            } else {
+                vb[P_RHO] += rfb;
+                prb += -rfb / (a * a);
+            }
+            ab += rfb * (-2.0) * dpa / (a * a * a);
+            if (in) {  // p_face = (p_in + p - rho a vc) / 2
+                prb += 0.5 * pfb;
+                rrb += -0.5 * pfb * a * vc;
+                ab += -0.5 * pfb * rr * vc;
+                const double vcb = -0.5 * pfb * rr * a;
+                for (int k = 0; k < 3; k++) vb[P_VX + k] += -n[k] * vcb;
+            }
+            vb[P_P] += prb;
+            vb[P_RHO] += rrb;
+            vb[P_T] += ab * gR / (2.0 * a);
+            for (int k = 0; k < NPRIM; k++) gb[k] = 0.0;
+            break;
+        }
         case IBIS_BC_FLOW_STATE_COPY:
             for (int k = 0; k < NPRIM; k++) gb[k] = 0.0;
             break;
@@ -562,13 +702,16 @@ __device__ __forceinline__ void bc_action_vjp(const DevMesh& m, int kind, int f,
 }
 
 // Per boundary face: the ghost's bars, the boundary actions backwards
-template <int DIM, bool VISC>
+template <int DIM, bool VISC, bool ORD1 = false>
 __global__ void __launch_bounds__(BLOCK)
-    k_vjp_ghost(DevMesh m, VarSet vs, const int* __restrict__ kinds, int ntab, AdjBufs B) {
+    k_vjp_ghost(DevMesh m, VarSet vs, const int* __restrict__ kinds,
+                const double* __restrict__ pars, int ntab, GasParams gp, AdjBufs B) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= m.nb) return;
-    const int N = m.N, F = m.F;
+    const int N = m.N, F = m.F, Nt = m.Nt;
     const int f = m.b_face[b], g = m.b_ghost[b], va = m.b_valid[b], t = m.b_tag[b];
+    double vst[NPRIM];  // the valid cell's state, which the actions read
+    for (int k = 0; k < NPRIM; k++) vst[k] = B.prim_c[(size_t)k * Nt + va];
     const int ngrad = DIM == 3 ? 5 : 4;
     const int j = slot_of(m, f, va);
     const double w[3] = {B.lsqw[(size_t)(j * 3 + 0) * N + va], B.lsqw[(size_t)(j * 3 + 1) * N + va],
@@ -585,9 +728,13 @@ __global__ void __launch_bounds__(BLOCK)
             gb[rows[k]] += x;
         }
         for (int a = IBIS_MAX_ACT - 1; a >= 0; a--)
-            bc_action_vjp(m, kinds[ntab + t * IBIS_MAX_ACT + a], f, gb, vb);
+            bc_action_vjp(m, kinds[ntab + t * IBIS_MAX_ACT + a], f,
+                          pars + ((size_t)ntab + t * IBIS_MAX_ACT + a) * IBIS_NPAR, vst, gp, gb, vb);
     }
-    {  // convective path reads the ghost after the pre-reconstruction actions
+    if (ORD1) {  // first order: the ghost's whole row is the face state
+        const bool gl = m.L[f] == g;
+        for (int k = 0; k < NPRIM; k++) gb[k] += (gl ? B.QLb : B.QRb)[(size_t)k * F + f];
+    } else {  // convective path reads the ghost after the pre-reconstruction actions
         const bool gl = m.L[f] == g;
         for (int v = 0; v < 5; v++) {
             double x = (gl ? B.QLb : B.QRb)[(size_t)v * F + f];  // ghost side: plain copy
@@ -599,12 +746,14 @@ __global__ void __launch_bounds__(BLOCK)
             gb[vs.row[v]] += x;
         }
     }
-    for (int a = IBIS_MAX_ACT - 1; a >= 0; a--) bc_action_vjp(m, kinds[t * IBIS_MAX_ACT + a], f, gb, vb);
+    for (int a = IBIS_MAX_ACT - 1; a >= 0; a--)
+        bc_action_vjp(m, kinds[t * IBIS_MAX_ACT + a], f, pars + ((size_t)t * IBIS_MAX_ACT + a) * IBIS_NPAR,
+                      vst, gp, gb, vb);
     for (int k = 0; k < NPRIM; k++) B.bcb[(size_t)k * m.nb + b] = vb[k];
 }
 
 // Per valid cell: own terms + neighbours' stencils + ghosts' actions
-template <int DIM, bool VISC>
+template <int DIM, bool VISC, bool ORD1 = false>
 __global__ void __launch_bounds__(BLOCK) k_vjp_gather(DevMesh m, VarSet vs, AdjBufs B) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int N = m.N, Nt = m.Nt;
@@ -612,7 +761,10 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_gather(DevMesh m, VarSet vs, AdjB
     const int ngrad = DIM == 3 ? 5 : 4;
     double out[NPRIM];
     for (int k = 0; k < NPRIM; k++) out[k] = 0.0;
-    for (int v = 0; v < 5; v++) out[vs.row[v]] += B.cself[(size_t)v * N + i];
+    if (ORD1)
+        for (int k = 0; k < NPRIM; k++) out[k] += B.cself[(size_t)k * N + i];
+    else
+        for (int v = 0; v < 5; v++) out[vs.row[v]] += B.cself[(size_t)v * N + i];
     const int vrows[4] = {P_T, P_VX, P_VY, P_VZ};
     if (VISC)
         for (int k = 0; k < 4; k++) out[vrows[k]] += B.vself[(size_t)k * N + i];
@@ -629,7 +781,7 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_gather(DevMesh m, VarSet vs, AdjB
         const int s = slot_of(m, f, nb);
         const double w[3] = {B.lsqw[(size_t)(s * 3 + 0) * N + nb], B.lsqw[(size_t)(s * 3 + 1) * N + nb],
                              B.lsqw[(size_t)(s * 3 + 2) * N + nb]};
-        for (int v = 0; v < 5; v++) {
+        for (int v = 0; v < (ORD1 ? 0 : 5); v++) {
             double x = 0.0;
             if (v < ngrad)
                 x = w[0] * B.cg[((size_t)v * 3 + 0) * N + nb] + w[1] * B.cg[((size_t)v * 3 + 1) * N + nb] +

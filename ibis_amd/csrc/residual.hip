@@ -3312,8 +3312,10 @@ ibis_status ensure_adjoint(ibis_residual* r) {
     };
     HIP_TRY(al((void**)&r->adj_prim_c, sizeof(double) * NPRIM * Nt));
     HIP_TRY(al((void**)&r->adj_vgrad, sizeof(double) * NGRAD * 3 * N));
-    HIP_TRY(al((void**)&r->adj_face, sizeof(double) * (5 + 5 + 4 + 4 + 12 + 12) * F));
-    HIP_TRY(al((void**)&r->adj_cell, sizeof(double) * (5 + 15 + 5 + 4 + 12) * N));
+    // face and cell cotangents: 7 rows each for the reconstructed values
+    // (5 used by the second-order path, 7 prim rows by the first-order one)
+    HIP_TRY(al((void**)&r->adj_face, sizeof(double) * (7 + 7 + 4 + 4 + 12 + 12) * F));
+    HIP_TRY(al((void**)&r->adj_cell, sizeof(double) * (7 + 15 + 5 + 4 + 12) * N));
     HIP_TRY(al((void**)&r->adj_extj, sizeof(int) * 5 * N));
     HIP_TRY(al((void**)&r->adj_bcb, sizeof(double) * NPRIM * nb));
     HIP_TRY(al((void**)&r->adj_zero, sizeof(double) * nc * Nt));
@@ -3337,16 +3339,17 @@ ibis_status ensure_adjoint(ibis_residual* r) {
 
 ibis_status adjoint_supported(const ibis_residual* r) {
     const ibis_fv_desc& d = r->desc;
-    if (d.order != 2)
-        return ibis::fail(IBIS_ERR_INVALID, "the adjoint needs the second-order residual");
+    if (d.order != 1 && d.order != 2)
+        return ibis::fail(IBIS_ERR_INVALID, "the adjoint needs the first- or second-order residual");
+    // LDFSS2: the reference's own tangent of it (FiniteVolume<Dual>) is NaN
+    // on subsonic faces (the Dual pow of a negative base, dual.h:239-245),
+    // so there is no reference derivative for a reverse mode to match
     if (d.flux == IBIS_FLUX_LDFSS2)
         return ibis::fail(IBIS_ERR_INVALID, "the adjoint supports the AUSMDV, Hanel and Rusanov fluxes");
     if (r->has_post)
         return ibis::fail(IBIS_ERR_INVALID, "the adjoint does not support constant-flux boundaries");
     if (r->prec != IBIS_PREC_FP64)
         return ibis::fail(IBIS_ERR_INVALID, "the adjoint is FP64 only");
-    if (r->has_subsonic)
-        return ibis::fail(IBIS_ERR_INVALID, "the adjoint does not support subsonic boundaries");
     return IBIS_OK;
 }
 
@@ -3356,11 +3359,21 @@ void launch_vjp(ibis_residual* r, const VarSet& vs, const FaceParams& fp, const 
     const DevMesh& D = r->m->d;
     const ibis_fv_desc& d = r->desc;
     const bool visc = d.viscous != 0;
+    const bool ord1 = d.order == 1;
     const unsigned nf = nblocks(D.F), nc = nblocks(D.N);
-    switch (d.flux) {
-        case IBIS_FLUX_AUSMDV: k_vjp_conv_face<DIM, IBIS_FLUX_AUSMDV><<<nf, BLOCK, 0, st>>>(D, fp, vs, B); break;
-        case IBIS_FLUX_HANEL: k_vjp_conv_face<DIM, IBIS_FLUX_HANEL><<<nf, BLOCK, 0, st>>>(D, fp, vs, B); break;
-        default: k_vjp_conv_face<DIM, IBIS_FLUX_RUSANOV><<<nf, BLOCK, 0, st>>>(D, fp, vs, B); break;
+    const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
+    if (ord1) {
+        switch (d.flux) {
+            case IBIS_FLUX_AUSMDV: k_vjp_conv_face1<DIM, IBIS_FLUX_AUSMDV><<<nf, BLOCK, 0, st>>>(D, fp, B); break;
+            case IBIS_FLUX_HANEL: k_vjp_conv_face1<DIM, IBIS_FLUX_HANEL><<<nf, BLOCK, 0, st>>>(D, fp, B); break;
+            default: k_vjp_conv_face1<DIM, IBIS_FLUX_RUSANOV><<<nf, BLOCK, 0, st>>>(D, fp, B); break;
+        }
+    } else {
+        switch (d.flux) {
+            case IBIS_FLUX_AUSMDV: k_vjp_conv_face<DIM, IBIS_FLUX_AUSMDV><<<nf, BLOCK, 0, st>>>(D, fp, vs, B); break;
+            case IBIS_FLUX_HANEL: k_vjp_conv_face<DIM, IBIS_FLUX_HANEL><<<nf, BLOCK, 0, st>>>(D, fp, vs, B); break;
+            default: k_vjp_conv_face<DIM, IBIS_FLUX_RUSANOV><<<nf, BLOCK, 0, st>>>(D, fp, vs, B); break;
+        }
     }
     ViscParams vp{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr};
     if (visc) k_vjp_visc_face<DIM><<<nf, BLOCK, 0, st>>>(D, vp, B);
@@ -3369,7 +3382,12 @@ void launch_vjp(ibis_residual* r, const VarSet& vs, const FaceParams& fp, const 
 #define CELL(LIM, ACT, VIS) \
     k_vjp_cell<DIM, LIM, ACT, VIS><<<nc, BLOCK, vshm, st>>>(D, vs, d.lim_eps, d.venkat_K, B)
     const int lim = d.limiter;
-    if (visc) {
+    if (ord1) {
+        if (visc)
+            k_vjp_cell<DIM, 0, false, true, true><<<nc, BLOCK, vshm, st>>>(D, vs, d.lim_eps, d.venkat_K, B);
+        else
+            k_vjp_cell<DIM, 0, false, false, true><<<nc, BLOCK, vshm, st>>>(D, vs, d.lim_eps, d.venkat_K, B);
+    } else if (visc) {
         if (lim == 0) CELL(0, false, true);
         else if (lim == 1) { if (active) CELL(1, true, true); else CELL(1, false, true); }
         else { if (active) CELL(2, true, true); else CELL(2, false, true); }
@@ -3380,13 +3398,23 @@ void launch_vjp(ibis_residual* r, const VarSet& vs, const FaceParams& fp, const 
     }
 #undef CELL
     if (D.nb > 0) {
-        if (visc)
-            k_vjp_ghost<DIM, true><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, r->kinds, r->ntab, B);
+        const unsigned ng = nblocks(D.nb);
+        const int nt = r->ntab;
+        if (visc && ord1)
+            k_vjp_ghost<DIM, true, true><<<ng, BLOCK, 0, st>>>(D, vs, r->kinds, r->pars, nt, gp, B);
+        else if (visc)
+            k_vjp_ghost<DIM, true><<<ng, BLOCK, 0, st>>>(D, vs, r->kinds, r->pars, nt, gp, B);
+        else if (ord1)
+            k_vjp_ghost<DIM, false, true><<<ng, BLOCK, 0, st>>>(D, vs, r->kinds, r->pars, nt, gp, B);
         else
-            k_vjp_ghost<DIM, false><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, r->kinds, r->ntab, B);
+            k_vjp_ghost<DIM, false><<<ng, BLOCK, 0, st>>>(D, vs, r->kinds, r->pars, nt, gp, B);
     }
-    if (visc)
+    if (visc && ord1)
+        k_vjp_gather<DIM, true, true><<<nc, BLOCK, 0, st>>>(D, vs, B);
+    else if (visc)
         k_vjp_gather<DIM, true><<<nc, BLOCK, 0, st>>>(D, vs, B);
+    else if (ord1)
+        k_vjp_gather<DIM, false, true><<<nc, BLOCK, 0, st>>>(D, vs, B);
     else
         k_vjp_gather<DIM, false><<<nc, BLOCK, 0, st>>>(D, vs, B);
     if (D.G > 0) k_zero_rows<<<nblocks(D.G), BLOCK, 0, st>>>(NPRIM, D.N, D.Nt, B.pb);
@@ -3417,12 +3445,14 @@ ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* 
     if (r->has_pre) k_bc<double><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, prim);
     HIP_TRY(hipMemcpyAsync(r->adj_prim_c, prim, sizeof(double) * NPRIM * Nt,
                            hipMemcpyDeviceToDevice, st));
-    if (D.dim == 3)
-        launch_cell_grad<double, 3>(D, vs, d.limiter, r->adj_prim_c, r->grad, r->lim, d.lim_eps,
-                                    d.venkat_K, st);
-    else
-        launch_cell_grad<double, 2>(D, vs, d.limiter, r->adj_prim_c, r->grad, r->lim, d.lim_eps,
-                                    d.venkat_K, st);
+    if (d.order == 2) {
+        if (D.dim == 3)
+            launch_cell_grad<double, 3>(D, vs, d.limiter, r->adj_prim_c, r->grad, r->lim,
+                                        d.lim_eps, d.venkat_K, st);
+        else
+            launch_cell_grad<double, 2>(D, vs, d.limiter, r->adj_prim_c, r->grad, r->lim,
+                                        d.lim_eps, d.venkat_K, st);
+    }
     if (d.viscous) {
         if (r->has_visc)
             k_bc<double><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds + r->ntab,
@@ -3452,17 +3482,17 @@ ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* 
     for (int k = 0; k < 3; k++) B.force_dir[k] = force_tags ? force_dir[k] : 0.0;
     double* fa = r->adj_face;
     B.QLb = fa;
-    B.QRb = fa + 5 * F;
-    B.VLb = fa + 10 * F;
-    B.VRb = fa + 14 * F;
-    B.GLb = fa + 18 * F;
-    B.GRb = fa + 30 * F;
+    B.QRb = fa + 7 * F;
+    B.VLb = fa + 14 * F;
+    B.VRb = fa + 18 * F;
+    B.GLb = fa + 22 * F;
+    B.GRb = fa + 34 * F;
     double* ca = r->adj_cell;
     B.cself = ca;
-    B.cg = ca + 5 * N;
-    B.cext = ca + 20 * N;
-    B.vself = ca + 25 * N;
-    B.vg = ca + 29 * N;
+    B.cg = ca + 7 * N;
+    B.cext = ca + 22 * N;
+    B.vself = ca + 27 * N;
+    B.vg = ca + 31 * N;
     B.cextj = r->adj_extj;
     B.bcb = r->adj_bcb;
     B.pb = prim_bar;

@@ -201,9 +201,10 @@ ibis_status ibis_compute_dudt_jvp(ibis_residual* r, double* prim, double* prim_d
  * be 0.  limiter_mode: IBIS_LIMITER_ACTIVE differentiates through the
  * limiter (the branch the Dual residual takes), IBIS_LIMITER_FROZEN holds it
  * constant (match it with the option "freeze_limiter" of the tangent
- * residual).  Second-order residual, AUSMDV / Hanel / Rusanov, FP64, no
- * subsonic or constant-flux boundaries.  Reconstructed states are
- * recomputed, not stored.  Asynchronous. */
+ * residual).  First- or second-order residual, AUSMDV / Hanel / Rusanov (the
+ * reference's own tangent through LDFSS2 is NaN on subsonic faces), FP64,
+ * every boundary action but the constant-flux override.  Reconstructed
+ * states are recomputed, not stored.  Asynchronous. */
 enum { IBIS_LIMITER_FROZEN = 0, IBIS_LIMITER_ACTIVE = 1 };
 ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* dudt_bar,
                                   uint64_t force_tags, const double* force_dir, double* prim_bar,

@@ -86,6 +86,32 @@ def test_dot_product_mixed_hex_pyramid():
     _dot(gh, cfg, prim, seed=85, limiter="active")
 
 
+@pytest.mark.parametrize("case", ["wedge_ausmdv", "wedge_hanel", "plate_viscous"])
+def test_dot_product_first_order(case):
+    """First-order residual (copy_reconstruct, convective_flux.cpp:212-240):
+    the face states are the cells' whole prim rows."""
+    if case == "plate_viscous":
+        g_io, cfg, fs = cases.plate_case(48, 24)
+        cfg.reconstruction_order = 1
+        prim = cases.perturbed_prim(g_io, fs, 48 * 24 + 2 * (48 + 24), seed=86)
+    else:
+        g_io, cfg, fs = cases.wedge_case(48, 24, flux=case.split("_")[1], order=1)
+        prim = cases.perturbed_prim(g_io, fs, 48 * 24 + 2 * (48 + 24), seed=87)
+    _dot(g_io, cfg, prim, seed=88, limiter="frozen")
+
+
+@pytest.mark.parametrize("order,mode", [(2, "active"), (2, "frozen"), (1, "frozen")])
+def test_dot_product_subsonic_boundaries(order, mode):
+    """Subsonic inflow and outflow ghosts (boundary.cpp:286-399) backwards,
+    with a slip wall, on a Mach 0.5 wedge."""
+    from ibis_amd import config as C
+    g_io, cfg, fs = cases.wedge_case(48, 24, mach=0.5, order=order)
+    cfg.boundaries = {"inflow": C.subsonic_inflow(fs), "outflow": C.subsonic_outflow(fs.p * 0.98),
+                      "wall": C.slip_wall()}
+    prim = cases.perturbed_prim(g_io, fs, 48 * 24 + 2 * (48 + 24), seed=89)
+    _dot(g_io, cfg, prim, seed=90, limiter=mode)
+
+
 def test_wall_force_gradient_central_differences():
     """Config-4 check on a 64 x 32 flat plate: d J / d prim . v against
     (J(p + h v) - J(p - h v)) / 2h, h = 1e-7, 32 directions (seed 6)."""

@@ -10,7 +10,8 @@
   * fifth workload: 64 wedge/ramp instances of 10k-400k cells (seed 7) --
     the batched, graph-replayed steps equal one-at-a-time stepping bit for
     bit, and the smallest instance stepped alone equals the oracle's steps;
-  * adjoint configurations the VJP does not cover are refused.
+  * the adjoint configurations the VJP does not cover (LDFSS2, the
+    constant-flux override) are refused.
 """
 import numpy as np
 import pytest
@@ -201,23 +202,23 @@ def test_batch64_config5_matches_one_at_a_time_and_oracle():
     bt.close()
 
 
-@pytest.mark.parametrize("case", ["order1", "ldfss2", "subsonic"])
+@pytest.mark.parametrize("case", ["ldfss2", "constant_flux"])
 def test_vjp_refuses_configurations_it_does_not_cover(case):
-    """First-order reconstruction, LDFSS2 and subsonic boundaries have no
-    reverse kernels: ibis_compute_dudt_vjp reports them and writes nothing."""
+    """LDFSS2 (whose reference tangent is NaN on subsonic faces) and the
+    constant-flux override have no reverse kernels: ibis_compute_dudt_vjp
+    reports them and writes nothing."""
     import torch
     from ibis_amd import config as C
     from ibis_amd.native import IbisError
-    kw = {"order1": dict(order=1), "ldfss2": dict(flux="ldfss2"), "subsonic": {}}[case]
-    g_io, cfg, fs = cases.wedge_case(16, 8, mach=0.5 if case == "subsonic" else 3.0, **kw)
-    if case == "subsonic":
-        cfg.boundaries = {"inflow": C.subsonic_inflow(fs),
-                          "outflow": C.subsonic_outflow(fs.p * 0.98), "wall": C.slip_wall()}
+    kw = {"ldfss2": dict(flux="ldfss2"), "constant_flux": {}}[case]
+    g_io, cfg, fs = cases.wedge_case(16, 8, **kw)
+    if case == "constant_flux":
+        cfg.boundaries = {"inflow": C.constant_flux(fs), "outflow": C.supersonic_outflow(),
+                          "wall": C.slip_wall()}
     gb, fv = gpu_objects(g_io, cfg)
     prim = to_dev(gb.to_grid_order(cases.perturbed_prim(g_io, fs, gb.Nt, seed=3)))
     pb = torch.full((7, gb.Nt), 7.0, dtype=torch.float64, device="cuda")
-    match = {"order1": "second-order", "ldfss2": "AUSMDV, Hanel and Rusanov",
-             "subsonic": "subsonic boundaries"}[case]
+    match = {"ldfss2": "AUSMDV, Hanel and Rusanov", "constant_flux": "constant-flux"}[case]
     with pytest.raises(IbisError, match=match):
         fv.compute_dudt_vjp(prim, pb, dudt_bar=torch.ones((4, gb.Nt), dtype=torch.float64,
                                                           device="cuda"))
