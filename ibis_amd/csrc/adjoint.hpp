@@ -38,6 +38,7 @@ struct AdjBufs {
     const double* cgrad;   // NGRAD x 3 x N convective gradients
     const double* clim;    // NGRAD x N limiters
     const double* vgrad;   // NGRAD x 3 x N viscous gradients
+    const int* post_kinds; // post-convective action table (ntags x IBIS_MAX_ACT)
     // seeds
     const double* db;      // (dim+2) x Nt
     unsigned long long force_tags;
@@ -138,6 +139,19 @@ __device__ __forceinline__ void face_flux_bar(const DevMesh& m, const AdjBufs& B
     }
 }
 
+// 1 when face f is a boundary face whose tag overrides its convective flux
+// with a constant (ConstantFlux, boundary.cpp:405-443): that flux does not
+// depend on the state, so no cotangent flows through it
+__device__ __forceinline__ bool constant_flux_face(const DevMesh& m, const AdjBufs& B, int f) {
+    const int l = m.L[f], r = m.R[f], N = m.N;
+    if (l < N && r < N) return false;
+    const int g = l >= N ? l : r;
+    const int t = m.b_tag[m.b_of_ghost[g - N]];
+    for (int a = 0; a < IBIS_MAX_ACT; a++)
+        if (B.post_kinds[t * IBIS_MAX_ACT + a] == IBIS_BC_CONSTANT_FLUX) return true;
+    return false;
+}
+
 // Convective face flux backwards: bars of the reconstructed values
 template <int DIM, int FLUX>
 __global__ void __launch_bounds__(BLOCK)
@@ -149,6 +163,8 @@ __global__ void __launch_bounds__(BLOCK)
     constexpr bool T3 = DIM == 3;
     double fb[NC];
     face_flux_bar<DIM>(m, B, f, fb);
+    if (constant_flux_face(m, B, f))
+        for (int q = 0; q < NC; q++) fb[q] = 0.0;
     // forward: reconstruction of both sides (FV/convective_flux.cpp:243-420)
     double q[2][5], st[2][NPRIM];
     for (int side = 0; side < 2; side++) {
@@ -238,6 +254,8 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_conv_face1(DevMesh m, FaceParams 
     constexpr bool T3 = DIM == 3;
     double fb[NC];
     face_flux_bar<DIM>(m, B, f, fb);
+    if (constant_flux_face(m, B, f))
+        for (int q = 0; q < NC; q++) fb[q] = 0.0;
     const double nx = m.geom[f], ny = m.geom[(size_t)F + f], nz = m.geom[2 * (size_t)F + f];
     const double t1x = m.geom[3 * (size_t)F + f], t1y = m.geom[4 * (size_t)F + f],
                  t1z = m.geom[5 * (size_t)F + f];

@@ -3346,8 +3346,6 @@ ibis_status adjoint_supported(const ibis_residual* r) {
     // so there is no reference derivative for a reverse mode to match
     if (d.flux == IBIS_FLUX_LDFSS2)
         return ibis::fail(IBIS_ERR_INVALID, "the adjoint supports the AUSMDV, Hanel and Rusanov fluxes");
-    if (r->has_post)
-        return ibis::fail(IBIS_ERR_INVALID, "the adjoint does not support constant-flux boundaries");
     if (r->prec != IBIS_PREC_FP64)
         return ibis::fail(IBIS_ERR_INVALID, "the adjoint is FP64 only");
     return IBIS_OK;
@@ -3477,6 +3475,7 @@ ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* 
     B.cgrad = r->grad;
     B.clim = r->lim;
     B.vgrad = r->adj_vgrad;
+    B.post_kinds = r->kinds + 2 * r->ntab;
     B.db = dudt_bar ? dudt_bar : r->adj_zero;
     B.force_tags = force_tags;
     for (int k = 0; k < 3; k++) B.force_dir[k] = force_tags ? force_dir[k] : 0.0;

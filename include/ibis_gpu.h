@@ -203,8 +203,8 @@ ibis_status ibis_compute_dudt_jvp(ibis_residual* r, double* prim, double* prim_d
  * constant (match it with the option "freeze_limiter" of the tangent
  * residual).  First- or second-order residual, AUSMDV / Hanel / Rusanov (the
  * reference's own tangent through LDFSS2 is NaN on subsonic faces), FP64,
- * every boundary action but the constant-flux override.  Reconstructed
- * states are recomputed, not stored.  Asynchronous. */
+ * every boundary action (a constant-flux face passes no cotangent to the
+ * state).  Reconstructed states are recomputed, not stored.  Asynchronous. */
 enum { IBIS_LIMITER_FROZEN = 0, IBIS_LIMITER_ACTIVE = 1 };
 ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* dudt_bar,
                                   uint64_t force_tags, const double* force_dir, double* prim_bar,

@@ -112,6 +112,19 @@ def test_dot_product_subsonic_boundaries(order, mode):
     _dot(g_io, cfg, prim, seed=90, limiter=mode)
 
 
+@pytest.mark.parametrize("viscous", [False, True])
+def test_dot_product_constant_flux_boundary(viscous):
+    """A constant-flux inflow (boundary.cpp:405-443): its faces' convective
+    flux is a constant; the viscous flux on them still depends on the state."""
+    from ibis_amd import config as C
+    g_io, cfg, fs = cases.wedge_case(48, 24, mach=2.0)
+    cfg.boundaries = {"inflow": C.constant_flux(fs), "outflow": C.supersonic_outflow(),
+                      "wall": C.fixed_temperature_no_slip_wall(300.0) if viscous else C.slip_wall()}
+    cfg.viscous = viscous
+    prim = cases.perturbed_prim(g_io, fs, 48 * 24 + 2 * (48 + 24), seed=91)
+    _dot(g_io, cfg, prim, seed=92, limiter="active")
+
+
 def test_wall_force_gradient_central_differences():
     """Config-4 check on a 64 x 32 flat plate: d J / d prim . v against
     (J(p + h v) - J(p - h v)) / 2h, h = 1e-7, 32 directions (seed 6)."""

@@ -10,8 +10,7 @@
   * fifth workload: 64 wedge/ramp instances of 10k-400k cells (seed 7) --
     the batched, graph-replayed steps equal one-at-a-time stepping bit for
     bit, and the smallest instance stepped alone equals the oracle's steps;
-  * the adjoint configurations the VJP does not cover (LDFSS2, the
-    constant-flux override) are refused.
+  * the VJP refuses LDFSS2, the one configuration it does not cover.
 """
 import numpy as np
 import pytest
@@ -202,24 +201,17 @@ def test_batch64_config5_matches_one_at_a_time_and_oracle():
     bt.close()
 
 
-@pytest.mark.parametrize("case", ["ldfss2", "constant_flux"])
-def test_vjp_refuses_configurations_it_does_not_cover(case):
-    """LDFSS2 (whose reference tangent is NaN on subsonic faces) and the
-    constant-flux override have no reverse kernels: ibis_compute_dudt_vjp
-    reports them and writes nothing."""
+def test_vjp_refuses_ldfss2():
+    """LDFSS2, whose reference tangent (FiniteVolume<Dual>) is NaN on
+    subsonic faces, has no reverse kernel: ibis_compute_dudt_vjp reports it
+    and writes nothing."""
     import torch
-    from ibis_amd import config as C
     from ibis_amd.native import IbisError
-    kw = {"ldfss2": dict(flux="ldfss2"), "constant_flux": {}}[case]
-    g_io, cfg, fs = cases.wedge_case(16, 8, **kw)
-    if case == "constant_flux":
-        cfg.boundaries = {"inflow": C.constant_flux(fs), "outflow": C.supersonic_outflow(),
-                          "wall": C.slip_wall()}
+    g_io, cfg, fs = cases.wedge_case(16, 8, flux="ldfss2")
     gb, fv = gpu_objects(g_io, cfg)
     prim = to_dev(gb.to_grid_order(cases.perturbed_prim(g_io, fs, gb.Nt, seed=3)))
     pb = torch.full((7, gb.Nt), 7.0, dtype=torch.float64, device="cuda")
-    match = {"ldfss2": "AUSMDV, Hanel and Rusanov", "constant_flux": "constant-flux"}[case]
-    with pytest.raises(IbisError, match=match):
+    with pytest.raises(IbisError, match="AUSMDV, Hanel and Rusanov"):
         fv.compute_dudt_vjp(prim, pb, dudt_bar=torch.ones((4, gb.Nt), dtype=torch.float64,
                                                           device="cuda"))
     torch.cuda.synchronize()
