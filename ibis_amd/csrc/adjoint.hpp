@@ -506,7 +506,10 @@ __global__ void __launch_bounds__(BLOCK)
                 const double K3 = venkat_K * venkat_K * venkat_K;
                 eps2 = (DIM == 3) ? K3 * V : K3 * (V * sqrt(V));
             }
-            double cur = 1.0;
+            // Venkatakrishnan: the running minimum as the fraction
+            // bn / bd, compared by cross-multiplication as the forward
+            // residual does (ibis::venkat_take on Dual: new on ties)
+            double cur = 1.0, bn = 1.0, bd = 1.0;
             int jstar = -1;
             double d1s = 0.0, d2s = 0.0, dstar[3] = {0.0, 0.0, 0.0};
             int sgs = 0;
@@ -517,14 +520,20 @@ __global__ void __launch_bounds__(BLOCK)
                 const int sg = (d2 > 0.0) - (d2 < 0.0);
                 if (sg == 0) continue;
                 const double d1 = sg > 0 ? umax - ui : umin - ui;
-                double val;
-                if (LIM == 1)
+                bool take;
+                double val = 0.0, num = 0.0, den = 1.0;
+                if (LIM == 1) {
                     val = d1 / (sg * (fabs(d2) + eps));
-                else
-                    val = (d1 * d1 + eps2 + 2.0 * (d1 * d2)) /
-                          (d1 * d1 + 2.0 * (d2 * d2) + d1 * d2 + eps2);
-                if (!(cur < val)) {  // Dual min: the new value on ties
+                    take = !(cur < val);  // Dual min: the new value on ties
+                } else {
+                    num = d1 * d1 + eps2 + 2.0 * (d1 * d2);
+                    den = d1 * d1 + 2.0 * (d2 * d2) + d1 * d2 + eps2;
+                    take = !(bn * den < num * bd);
+                }
+                if (take) {
                     cur = val;
+                    bn = num;
+                    bd = den;
                     jstar = j;
                     d1s = d1;
                     d2s = d2;

@@ -94,6 +94,19 @@ __host__ __device__ __forceinline__ Dual kmax(double a, Dual b) { return a > b.r
 __host__ __device__ __forceinline__ Dual kmin(Dual a, double b) { return a.r < b ? a : Dual(b); }
 __host__ __device__ __forceinline__ Dual kmax(Dual a, double b) { return a.r > b ? a : Dual(b); }
 
+// Venkatakrishnan limiter: the running minimum of the face ratios
+// num_f / den_f (den_f > 0) is kept as the fraction (bn, bd), and a face's
+// ratio replaces it when num * bd < bn * den, the comparison of the ratios
+// without dividing.  As with kmin above, a Dual candidate also replaces it on
+// a tie.  One division, bn / bd, ends the cell's loop.
+__host__ __device__ __forceinline__ bool venkat_take(double num, double den, double bn, double bd) {
+    return num * bd < bn * den;
+}
+__host__ __device__ __forceinline__ bool venkat_take(const Dual& num, const Dual& den, const Dual& bn,
+                                                     const Dual& bd) {
+    return !(bn.r * den.r < num.r * bd.r);
+}
+
 __host__ __device__ __forceinline__ double real_part(double x) { return x; }
 __host__ __device__ __forceinline__ double real_part(const Dual& x) { return x.r; }
 

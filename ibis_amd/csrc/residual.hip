@@ -338,6 +338,7 @@ __global__ void __launch_bounds__(320)
     }
     if (LIM == 0) return;
     S phi = S(1.0);
+    S bn = S(1.0), bd = S(1.0);  // Venkatakrishnan: the minimum as a fraction
     double eps2 = 0.0;
     if (LIM == 2) {
         const double V = fabs(m.vol[i]);  // cell size h^dim = |V|
@@ -369,9 +370,13 @@ __global__ void __launch_bounds__(320)
             const S d2s = d2 * d2;
             const S num = d1s + eps2 + 2.0 * (d1 * d2);
             const S den = d1s + 2.0 * d2s + d1 * d2 + eps2;
-            phi = ibis::kmin(phi, num / den);
+            if (ibis::venkat_take(num, den, bn, bd)) {
+                bn = num;
+                bd = den;
+            }
         }
     }
+    if (LIM == 2) phi = bn / bd;
     lim[(size_t)slot * N + i] = phi;
 }
 
@@ -897,6 +902,9 @@ __global__ void __launch_bounds__(BLOCK)
     double phi[5] = {1.0, 1.0, 1.0, 1.0, 1.0};
     if (LIM != 0) {
         double eps2 = 0.0;
+        // Venkatakrishnan: each variable's minimum ratio as the fraction
+        // bn / bd, divided out after the face loop
+        double bn[5] = {1.0, 1.0, 1.0, 1.0, 1.0}, bd[5] = {1.0, 1.0, 1.0, 1.0, 1.0};
         if (LIM == 2) {
             const double V = fabs(m.vol[i]);  // cell size h^dim = |V|
             const double K3 = venkat_K * venkat_K * venkat_K;
@@ -934,10 +942,15 @@ __global__ void __launch_bounds__(BLOCK)
                     const double d2s = d2 * d2;
                     const double num = d1s + eps2 + 2.0 * (d1 * d2);
                     const double den = d1s + 2.0 * d2s + d1 * d2 + eps2;
-                    const double cand = ibis::kmin(phi[v], num / den);
-                    phi[v] = (pos || neg) ? cand : phi[v];
+                    const bool take = (pos || neg) && ibis::venkat_take(num, den, bn[v], bd[v]);
+                    bn[v] = take ? num : bn[v];
+                    bd[v] = take ? den : bd[v];
                 }
             }
+        }
+        if (LIM == 2) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) phi[v] = bn[v] / bd[v];
         }
         if (write_slots) {
 #pragma unroll
@@ -1119,6 +1132,8 @@ __global__ void __launch_bounds__(BLOCK)
     float phi[5] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f};
     if (LIM != 0) {
         float eps2 = 0.0f;
+        // Venkatakrishnan: the minimum ratio as a fraction, as in k_cell_fs
+        float bn[5] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f}, bd[5] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f};
         if (LIM == 2) {
             const double V = fabs(m.vol[i]);  // cell size h^dim = |V|
             const double K3 = venkat_K * venkat_K * venkat_K;
@@ -1153,10 +1168,15 @@ __global__ void __launch_bounds__(BLOCK)
                     const float d2s = d2 * d2;
                     const float num = d1s + eps2 + 2.0f * (d1 * d2);
                     const float den = d1s + 2.0f * d2s + d1 * d2 + eps2;
-                    const float q = num / den;
-                    phi[v] = q < phi[v] ? q : phi[v];
+                    const bool take = num * bd[v] < bn[v] * den;
+                    bn[v] = take ? num : bn[v];
+                    bd[v] = take ? den : bd[v];
                 }
             }
+        }
+        if (LIM == 2) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) phi[v] = bn[v] / bd[v];
         }
     }
     // the reconstruction's offsets: from LDS when staged, else all loaded

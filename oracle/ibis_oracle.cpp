@@ -93,6 +93,13 @@ inline Dual kmin(double a, const Dual& b) { return a < b.r ? Dual(a) : b; }
 inline Dual kmax(double a, const Dual& b) { return a > b.r ? Dual(a) : b; }
 inline Dual kmin(const Dual& a, double b) { return a.r < b ? a : Dual(b); }
 inline Dual kmax(const Dual& a, double b) { return a.r > b ? a : Dual(b); }
+// Venkatakrishnan running minimum as a fraction (bn, bd): a face ratio
+// num / den (den > 0) replaces it when num * bd < bn * den; a Dual candidate
+// also on a tie, as kmin does.
+inline bool venkat_take(double num, double den, double bn, double bd) { return num * bd < bn * den; }
+inline bool venkat_take(const Dual& num, const Dual& den, const Dual& bn, const Dual& bd) {
+    return !(bn.r * den.r < num.r * bd.r);
+}
 inline double kabs(double x) { return std::fabs(x); }
 inline Dual kabs(const Dual& x) { return (x.r < 0.0) ? -x : x; }
 inline double ksqrt(double x) { return std::sqrt(x); }
@@ -835,10 +842,13 @@ struct FV {
                 // non-planar faces can be negative), i.e. (K*K*K)*|V| in 3D and
                 // (K*K*K)*(|V|*sqrt(|V|)) in 2D, per-face phi_f =
                 // (d1^2 + eps^2 + 2 d1 d2) / (d1^2 + 2 d2^2 + d1 d2 + eps^2),
-                // phi = min(1, min_f phi_f).
+                // phi = min(1, min_f phi_f), the minimum taken over the
+                // fractions by cross-multiplication (venkat_take) and divided
+                // out once.
                 double V = std::fabs(G.vol[i]);
                 double K3 = K * K * K;
                 double eps2 = (dim == 3) ? K3 * V : K3 * (V * std::sqrt(V));
+                T bn = 1.0, bd = 1.0;
                 for (int64_t j = G.cf_off[i]; j < G.cf_off[i + 1]; j++) {
                     int64_t f = G.cf[j];
                     double dx = G.cx[f] - x;
@@ -856,8 +866,12 @@ struct FV {
                     T d2s = d2 * d2;
                     T num = d1s + eps2 + 2.0 * (d1 * d2);
                     T den = d1s + 2.0 * d2s + d1 * d2 + eps2;
-                    phi = kmin(phi, num / den);
+                    if (venkat_take(num, den, bn, bd)) {
+                        bn = num;
+                        bd = den;
+                    }
                 }
+                phi = bn / bd;
             }
             out[i] = phi;
         }
