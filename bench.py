@@ -266,18 +266,27 @@ def batch_leg(nsteps, n_inst=64):
     del seq_default
     t_seq, seq = sequential(best_opt)
 
-    bt.rk_init(d_cons, d_prim, cfl=0.5, max_time=1e30)
-    bt.rk_steps(1, use_graph=True)
-    w0 = time.perf_counter()
-    bt.rk_steps(nsteps, use_graph=True)
-    t_bat = time.perf_counter() - w0
-    cons = d_cons.cpu().numpy()
+    # the batched run, with the stage updates in the residual's epilogue
+    # (fuse_stages 1, the default) and, for reference, as separate passes
+    def batched(fuse):
+        c2, p2 = d_cons.clone(), d_prim.clone()
+        bt.set_option("fuse_stages", fuse)
+        bt.rk_init(c2, p2, cfl=0.5, max_time=1e30)
+        bt.rk_steps(1, use_graph=True)
+        w0 = time.perf_counter()
+        bt.rk_steps(nsteps, use_graph=True)
+        return time.perf_counter() - w0, c2.cpu().numpy()
+
+    t_sep, cons_sep = batched(0)
+    t_bat, cons = batched(1)
     identical = all(
         np.array_equal(bt.scatter(cons, i)[:, :b.N], dc.cpu().numpy()[:, :b.N])
         for i, (b, (fv, rk, dc, dp)) in enumerate(zip(bt.blocks, seq)))
+    identical = identical and cons.tobytes() == cons_sep.tobytes()
     out = {"instances": bt.n, "cells": int(bt.N), "steps": nsteps,
            "sequential_s": round(t_seq, 4), "batched_s": round(t_bat, 4),
            "speedup": round(t_seq / t_bat, 3), "identical_fp64": bool(identical),
+           "batched_separate_stage_passes_s": round(t_sep, 4),
            "sequential_default_options_s": round(t_seq_default, 4),
            "speedup_vs_default_options": round(t_seq_default / t_bat, 3),
            "autotune_step_s": {k: round(v, 5) for k, v in tune.items()}, "options": best_opt}
@@ -286,6 +295,39 @@ def batch_leg(nsteps, n_inst=64):
         fv.close()
     bt.close()
     return out
+
+
+def batch_profile_leg(nsteps, n_inst=64):
+    """Only the batched steps of the 64-instance set (default options), with
+    the stage updates fused into the residual's epilogue and then as separate
+    passes, for a kernel profile of one batched step; prints one JSON line."""
+    import torch
+    from ibis_amd import cases
+    from ibis_amd.batch import Batch
+    from ibis_amd.native import check, lib, ptr
+    specs = cases.batch_instances(n=n_inst, seed=7)
+    inst = [cases.batch_case(s) for s in specs]
+    bt = Batch([(g, c) for g, c, _ in inst], device_blocks=True)
+    prims = [cases.uniform_prim(f, b.N, b.Nt) for (g, c, f), b in zip(inst, bt.blocks)]
+    d_prim = torch.from_numpy(bt.gather(prims)).to("cuda")
+    d_cons = torch.zeros((bt.ncons, bt.Nt), dtype=torch.float64, device="cuda")
+    check(lib().ibis_prim_to_cons(bt.residual_handle(), ptr(d_prim), ptr(d_cons), None),
+          "ibis_prim_to_cons")
+    torch.cuda.synchronize()
+    out = {"instances": bt.n, "cells": int(bt.N), "steps": nsteps}
+    for fuse in (1, 0):
+        c2, p2 = d_cons.clone(), d_prim.clone()
+        bt.set_option("fuse_stages", fuse)
+        bt.rk_init(c2, p2, cfl=0.5, max_time=1e30)
+        bt.rk_steps(1, use_graph=True)
+        w0 = time.perf_counter()
+        bt.rk_steps(nsteps, use_graph=True)
+        ms = (time.perf_counter() - w0) * 1e3 / nsteps
+        out["fused_ms_per_step" if fuse else "separate_ms_per_step"] = round(ms, 3)
+        print(f"[bench] batch fuse_stages={fuse}: {ms:.3f} ms per step", file=sys.stderr,
+              flush=True)
+    bt.close()
+    print(json.dumps({"batch_profile": out}))
 
 
 def main():
@@ -313,6 +355,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/primal leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-stepping leg")
     ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--batch-profile", action="store_true",
+                    help="only time the batched steps (fused and separate stage updates), "
+                         "for a kernel profile of the batch")
     ap.add_argument("--no-fp32", action="store_true",
                     help="skip the FP32-flux / FP64-accumulate comparison leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -334,6 +379,9 @@ def main():
         print(f"bench.py measures one GPU: the residual of one block runs on one MI355X "
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
         sys.exit(2)
+    if args.batch_profile:
+        batch_profile_leg(args.batch_steps)
+        return
 
     import torch
     from ibis_amd.finite_volume import FiniteVolume, GridBlock

@@ -134,6 +134,11 @@ class Batch:
               "ibis_batch_rk_steps")
 
     def set_option(self, key: str, value: int):
+        """A residual option, or the batch option "fuse_stages"."""
+        if key == "fuse_stages":
+            check(lib().ibis_batch_set_option(self._h, key.encode(), int(value)),
+                  "ibis_batch_set_option")
+            return
         check(lib().ibis_residual_set_option(self.residual_handle(), key.encode(), int(value)),
               "ibis_residual_set_option")
 

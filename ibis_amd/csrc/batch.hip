@@ -116,13 +116,14 @@ struct StageArgs {
 // runge_kutta.cpp:151-191), then converted to the primitive rows by the
 // residual's cons_to_prim expressions without storing the intermediate
 // conserved state; cons_out (may alias u) receives it when not null.
+// Cells [i0, i1) of the Nt rows.
 template <int DIM>
 __global__ void __launch_bounds__(BB)
-    k_stage_prim(int Nt, const int32_t* __restrict__ inst_of_cell, const double* __restrict__ dt,
-                 const double* u, StageArgs sa, double* cons_out, double R, double Cv,
-                 double* __restrict__ prim) {
-    const int i = blockIdx.x * BB + threadIdx.x;
-    if (i >= Nt) return;
+    k_stage_prim(int Nt, int i0, int i1, const int32_t* __restrict__ inst_of_cell,
+                 const double* __restrict__ dt, const double* u, StageArgs sa, double* cons_out,
+                 double R, double Cv, double* __restrict__ prim) {
+    const int i = i0 + blockIdx.x * BB + threadIdx.x;
+    if (i >= i1) return;
     constexpr int NC = DIM + 2;
     const double d = dt[inst_of_cell[i]];
     double q[NC], out[ibis::NPRIM];
@@ -170,6 +171,7 @@ struct ibis_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint64_t graph_generation = 0;  // the residual's option generation at capture
+    int fuse_stages = 1;  // stage updates in the residual's epilogue where its path allows
 };
 
 void ibis_batch_destroy(ibis_batch* b) {
@@ -325,40 +327,128 @@ ibis_status ibis_batch_rk_init(ibis_batch* b, int nstages, const double* a, cons
     return IBIS_OK;
 }
 
-// stage values u + sum_j k_j (c_j dt) of every cell -> primitive rows
-static ibis_status stage_prim(ibis_batch* b, const StageArgs& sa, double* cons_out, double* prim) {
+// stage values u + sum_j k_j (c_j dt) of cells [i0, i1) -> primitive rows
+static ibis_status stage_prim(ibis_batch* b, const StageArgs& sa, double* cons_out, double* prim,
+                              int i0, int i1) {
+    if (i1 <= i0) return IBIS_OK;
+    const unsigned g = nblk(i1 - i0);
     if (b->dim == 3)
-        k_stage_prim<3><<<nblk(b->Nt), BB, 0, b->stream>>>(b->Nt, b->inst_of_cell, b->dt, b->cons,
-                                                            sa, cons_out, b->R, b->Cv, prim);
+        k_stage_prim<3><<<g, BB, 0, b->stream>>>(b->Nt, i0, i1, b->inst_of_cell, b->dt, b->cons,
+                                                  sa, cons_out, b->R, b->Cv, prim);
     else
-        k_stage_prim<2><<<nblk(b->Nt), BB, 0, b->stream>>>(b->Nt, b->inst_of_cell, b->dt, b->cons,
-                                                            sa, cons_out, b->R, b->Cv, prim);
+        k_stage_prim<2><<<g, BB, 0, b->stream>>>(b->Nt, i0, i1, b->inst_of_cell, b->dt, b->cons,
+                                                  sa, cons_out, b->R, b->Cv, prim);
+    HIP_TRY(hipGetLastError());
+    return IBIS_OK;
+}
+
+static ibis_status stage_prim(ibis_batch* b, const StageArgs& sa, double* cons_out, double* prim) {
+    return stage_prim(b, sa, cons_out, prim, 0, b->Nt);
+}
+
+// the time step of every instance from the state in b->prim
+static void batch_dt(ibis_batch* b, hipStream_t st) {
+    if (b->nchunks) k_chunk_min<<<b->nchunks, BB, 0, st>>>(b->chunk_lo, b->dt_cells, b->partial);
+    k_seg_min<<<b->n_inst, 64, 0, st>>>(b->chunk_off, b->partial, b->stable);
+    k_dt_rules<<<nblk(b->n_inst), BB, 0, st>>>(b->n_inst, b->stable, b->cfl, b->dt_prev, b->t,
+                                              b->max_time, b->dt);
+}
+
+// the stored stages of stage i's value (runge_kutta.cpp:151-191): k_0
+// always, k_j (0 < j < i) unless a_ij < 1e-14
+static StageArgs stage_terms(const ibis_batch* b, int i) {
+    const int s = b->s;
+    StageArgs sa{};
+    sa.k[0] = b->k[0];
+    sa.c[0] = b->a[i * s + 0];
+    sa.nk = 1;
+    for (int j = 1; j < i; j++) {
+        if (b->a[i * s + j] < 1e-14) continue;  // runge_kutta.cpp:166
+        sa.k[sa.nk] = b->k[j];
+        sa.c[sa.nk] = b->a[i * s + j];
+        sa.nk++;
+    }
+    return sa;
+}
+
+// One step with the stage updates in the residual's face kernel: the
+// time step first (it reads only the valid cells' state, which the
+// residual does not change), then each residual's epilogue evaluates the
+// next stage's value (or the step's final one) from the stored stages and
+// its own dU/dt in registers, with the same expressions in the same order
+// as the separate stage pass; a pass over the ghost rows keeps them as the
+// separate pass leaves them.  The last stage's dU/dt is not stored.
+static ibis_status batch_step_fused(ibis_batch* b) {
+    hipStream_t st = b->stream;
+    ibis_stream_t sh = reinterpret_cast<ibis_stream_t>(st);
+    const int s = b->s;
+    IBIS_TRY(ibis_residual_cell_dt(b->r, b->prim, b->dt_cells, sh));
+    batch_dt(b, st);
+    for (int i = 0; i < s; i++) {
+        ibis_stage_epilogue e;
+        e.u = b->cons;
+        e.inst_of_cell = b->inst_of_cell;
+        e.dt = b->dt;
+        e.R = b->R;
+        e.Cv = b->Cv;
+        StageArgs sa{};
+        double* dudt = b->k[i];
+        if (i + 1 < s) {
+            // stage i+1: the stored k_0 .. k_{i-1}, then this residual's k_i
+            const StageArgs all = stage_terms(b, i + 1);
+            const bool reg = all.nk > 0 && all.k[all.nk - 1] == b->k[i];
+            for (int j = 0; j < all.nk - (reg ? 1 : 0); j++) {
+                e.k[e.nk] = all.k[j];
+                e.c[e.nk] = all.c[j];
+                e.nk++;
+            }
+            e.use_reg = reg ? 1 : 0;
+            e.c_reg = reg ? all.c[all.nk - 1] : 0.0;
+            e.prim_out = b->flow_tmp;
+            sa = all;
+        } else {
+            // the step's value: k_0 .. k_{s-2} stored, k_{s-1} in registers
+            for (int j = 0; j + 1 < s; j++) {
+                e.k[e.nk] = b->k[j];
+                e.c[e.nk] = b->bw[j];
+                e.nk++;
+            }
+            e.use_reg = 1;
+            e.c_reg = b->bw[s - 1];
+            e.cons_out = b->cons;
+            e.prim_out = b->prim;
+            dudt = nullptr;
+            for (int j = 0; j < s; j++) {
+                sa.k[j] = b->k[j];
+                sa.c[j] = b->bw[j];
+            }
+            sa.nk = s;
+        }
+        double* src = i == 0 ? b->prim : b->flow_tmp;
+        IBIS_TRY(ibis_compute_dudt_stage(b->r, src, dudt, &e, sh));
+        // ghost rows (their dU/dt rows stay zero): as the separate pass
+        if (i + 1 < s)
+            IBIS_TRY(stage_prim(b, sa, nullptr, b->flow_tmp, b->N, b->Nt));
+        else
+            IBIS_TRY(stage_prim(b, sa, b->cons, b->prim, b->N, b->Nt));
+    }
+    k_advance<<<nblk(b->n_inst), BB, 0, st>>>(b->n_inst, b->dt, b->dt_prev, b->t);
     HIP_TRY(hipGetLastError());
     return IBIS_OK;
 }
 
 // one step of every instance on b->stream
 static ibis_status batch_step(ibis_batch* b) {
+    if (b->fuse_stages && ibis_residual_stage_fusable(b->r) && b->s <= IBIS_EPI_MAXK)
+        return batch_step_fused(b);
     hipStream_t st = b->stream;
     ibis_stream_t sh = reinterpret_cast<ibis_stream_t>(st);
     const int s = b->s;
     IBIS_TRY(ibis_compute_dudt(b->r, b->prim, b->k[0], 1, sh));
     IBIS_TRY(ibis_residual_cell_dt(b->r, b->prim, b->dt_cells, sh));
-    if (b->nchunks) k_chunk_min<<<b->nchunks, BB, 0, st>>>(b->chunk_lo, b->dt_cells, b->partial);
-    k_seg_min<<<b->n_inst, 64, 0, st>>>(b->chunk_off, b->partial, b->stable);
-    k_dt_rules<<<nblk(b->n_inst), BB, 0, st>>>(b->n_inst, b->stable, b->cfl, b->dt_prev, b->t,
-                                              b->max_time, b->dt);
+    batch_dt(b, st);
     for (int i = 1; i < s; i++) {
-        StageArgs sa{};
-        sa.k[0] = b->k[0];
-        sa.c[0] = b->a[i * s + 0];
-        sa.nk = 1;
-        for (int j = 1; j < i; j++) {
-            if (b->a[i * s + j] < 1e-14) continue;  // runge_kutta.cpp:166
-            sa.k[sa.nk] = b->k[j];
-            sa.c[sa.nk] = b->a[i * s + j];
-            sa.nk++;
-        }
+        const StageArgs sa = stage_terms(b, i);
         IBIS_TRY(stage_prim(b, sa, nullptr, b->flow_tmp));
         IBIS_TRY(ibis_compute_dudt(b->r, b->flow_tmp, b->k[i], 1, sh));
     }
@@ -407,6 +497,19 @@ ibis_status ibis_batch_rk_steps(ibis_batch* b, int nsteps, int use_graph) {
         for (int n = 0; n < nsteps; n++) HIP_TRY(hipGraphLaunch(b->exec, b->stream));
     }
     HIP_TRY(hipStreamSynchronize(b->stream));
+    return IBIS_OK;
+}
+
+ibis_status ibis_batch_set_option(ibis_batch* b, const char* key, int64_t value) {
+    if (!b || !key) return ibis::fail(IBIS_ERR_INVALID, "ibis_batch_set_option: null argument");
+    if (std::string(key) != "fuse_stages")
+        return ibis::fail(IBIS_ERR_INVALID, ("unknown batch option " + std::string(key)).c_str());
+    b->fuse_stages = value ? 1 : 0;
+    // a captured step holds the other form
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
     return IBIS_OK;
 }
 

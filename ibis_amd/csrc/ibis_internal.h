@@ -36,6 +36,34 @@ int ibis_mesh_ntags(const ibis_mesh* m);
 ibis_status ibis_residual_cell_dt(ibis_residual* r, const double* prim, double* dt_cells,
                                   ibis_stream_t stream);
 
+// Runge-Kutta stage epilogue of the residual (batched stepping): with a
+// valid cell's dU/dt in registers, the face kernel writes it to `dudt`
+// (unless null), evaluates the stage value
+//   u + k_0 (c_0 dt) + ... + k_{nk-1} (c_{nk-1} dt) [+ dU/dt (c_reg dt)]
+// of each conserved row left to right (dt = dt[inst_of_cell[cell]]),
+// writes it to cons_out (unless null) and its primitive rows to prim_out.
+// Ghost rows are not touched.
+constexpr int IBIS_EPI_MAXK = 8;
+struct ibis_stage_epilogue {
+    const double* u = nullptr;
+    const double* k[IBIS_EPI_MAXK] = {};
+    double c[IBIS_EPI_MAXK] = {};
+    int nk = 0;
+    int use_reg = 0;  // the residual being computed enters as the last term
+    double c_reg = 0.0;
+    const int32_t* inst_of_cell = nullptr;
+    const double* dt = nullptr;
+    double* cons_out = nullptr;
+    double* prim_out = nullptr;
+    double R = 0.0, Cv = 0.0;
+};
+// 1 when ibis_compute_dudt_stage fuses the epilogue into this residual's
+// kernels (the inviscid second-order path with the per-entry face kernel
+// in FP64); else the caller runs the stage update separately.
+int ibis_residual_stage_fusable(const ibis_residual* r);
+ibis_status ibis_compute_dudt_stage(ibis_residual* r, double* prim, double* dudt,
+                                    const ibis_stage_epilogue* epi, ibis_stream_t stream);
+
 // 1 when the value starts of the per-entry face kernel (face_kernel 1), at
 // most maxf*5*n_cells + 5*n_ghost, fit in a 32-bit index; residuals of
 // larger meshes keep the tile-list face kernel.  Exported for host tests.

@@ -1456,12 +1456,15 @@ __global__ void __launch_bounds__(BLOCK) k_entries32(DevMesh m, char* __restrict
 }
 
 // k_face_tile_fs over the per-entry face data (layout 2 values); NRM: the
-// normals are stored (they are not cross(t1, t2))
-template <int DIM, int FLUX, bool NRM, class R = double, bool P2 = false, bool G32 = false>
+// normals are stored (they are not cross(t1, t2)).  EPI: the Runge-Kutta
+// stage epilogue (ibis_stage_epilogue) follows each cell's dU/dt; dudt may
+// then be null.
+template <int DIM, int FLUX, bool NRM, class R = double, bool P2 = false, bool G32 = false,
+          bool EPI = false>
 __global__ void __launch_bounds__(TILE)
     k_face_entry_fs(DevMesh m, FaceParams fp, const int2* __restrict__ eidx,
                     const double* __restrict__ egeom, const R* __restrict__ FL,
-                    double* __restrict__ dudt) {
+                    double* __restrict__ dudt, ibis_stage_epilogue epi) {
     extern __shared__ double sflux[];  // NC x maxtf
     const int t = xcd_block(m.ntiles);
     if (t >= m.ntiles) return;
@@ -1568,8 +1571,33 @@ __global__ void __launch_bounds__(TILE)
         }
     }
     const double V = m.vol[c];
+    if (!EPI) {
 #pragma unroll
-    for (int q = 0; q < NC; q++) dudt[(size_t)q * Nt + c] = d[q] / V;
+        for (int q = 0; q < NC; q++) dudt[(size_t)q * Nt + c] = d[q] / V;
+        return;
+    }
+    // stage epilogue: the same expressions, in the same order, as a
+    // separate stage-update pass over the stored dU/dt
+    double kr[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+        kr[q] = d[q] / V;
+        if (dudt) dudt[(size_t)q * Nt + c] = kr[q];
+    }
+    const double dtc = epi.dt[epi.inst_of_cell[c]];
+    double qv[NC], out[NPRIM];
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+        const size_t e = (size_t)q * Nt + c;
+        double v = epi.u[e];
+        for (int j = 0; j < epi.nk; j++) v = v + epi.k[j][e] * (epi.c[j] * dtc);
+        if (epi.use_reg) v = v + kr[q] * (epi.c_reg * dtc);
+        qv[q] = v;
+        if (epi.cons_out) epi.cons_out[e] = v;
+    }
+    ibis::cons_to_prim_cell<double, DIM>(qv, epi.R, epi.Cv, out);
+#pragma unroll
+    for (int q = 0; q < NPRIM; q++) epi.prim_out[(size_t)q * Nt + c] = out[q];
 }
 
 // Fused face flux + surface integral, slot-major variant for face values in
@@ -2783,14 +2811,17 @@ void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, bool f
     if (f32) {
         const float* fl = reinterpret_cast<const float*>(FL);
         if (face_kernel == 1 && D.n_from_t && DIM == 2 && D.planar2d)
-            k_face_entry_fs<DIM, FLUX, false, float, true><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, fl, dudt);
+            k_face_entry_fs<DIM, FLUX, false, float, true><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, fl, dudt,
+                                                                                 ibis_stage_epilogue{});
         else if (face_kernel == 1 && D.n_from_t && eg32)
             k_face_entry_fs<DIM, FLUX, false, float, false, true><<<g, TILE, shm, st>>>(
-                D, fp, eidx, reinterpret_cast<const double*>(eg32), fl, dudt);
+                D, fp, eidx, reinterpret_cast<const double*>(eg32), fl, dudt, ibis_stage_epilogue{});
         else if (face_kernel == 1 && D.n_from_t)
-            k_face_entry_fs<DIM, FLUX, false, float><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, fl, dudt);
+            k_face_entry_fs<DIM, FLUX, false, float><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, fl, dudt,
+                                                                          ibis_stage_epilogue{});
         else if (face_kernel == 1)
-            k_face_entry_fs<DIM, FLUX, true, float><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, fl, dudt);
+            k_face_entry_fs<DIM, FLUX, true, float><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, fl, dudt,
+                                                                         ibis_stage_epilogue{});
         else if (face_kernel)
             launch_face_slot_ts<DIM, FLUX, float>(D, fp, face_kernel, fl, dudt, st);
         else
@@ -2798,12 +2829,14 @@ void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, bool f
         return;
     }
     if (layout == 2 && face_kernel == 1) {
+        const ibis_stage_epilogue none{};
         if (D.n_from_t && DIM == 2 && D.planar2d)
-            k_face_entry_fs<DIM, FLUX, false, double, true><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, FL, dudt);
+            k_face_entry_fs<DIM, FLUX, false, double, true><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, FL, dudt,
+                                                                                  none);
         else if (D.n_from_t)
-            k_face_entry_fs<DIM, FLUX, false><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, FL, dudt);
+            k_face_entry_fs<DIM, FLUX, false><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, FL, dudt, none);
         else
-            k_face_entry_fs<DIM, FLUX, true><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, FL, dudt);
+            k_face_entry_fs<DIM, FLUX, true><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, FL, dudt, none);
     } else if (layout == 2 && face_kernel) {
         launch_face_slot_ts<DIM, FLUX, double>(D, fp, face_kernel, FL, dudt, st);
     } else if (layout == 0) {
@@ -2812,6 +2845,42 @@ void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, bool f
         k_face_tile_fs<DIM, FLUX, 1><<<g, TILE, shm, st>>>(D, fp, FL, FR, dudt);
     } else {
         k_face_tile_fs<DIM, FLUX, 2><<<g, TILE, shm, st>>>(D, fp, FL, FR, dudt);
+    }
+}
+
+// the per-entry face kernel with the Runge-Kutta stage epilogue (FP64,
+// normals equal to cross(t1, t2))
+template <int DIM, int FLUX>
+void launch_face_epi_f(const DevMesh& D, const FaceParams& fp, const int2* eidx,
+                       const double* egeom, const double* FL, double* dudt,
+                       const ibis_stage_epilogue& epi, hipStream_t st) {
+    const unsigned g = xcd_grid(D.ntiles);
+    const size_t shm = (size_t)(DIM + 2) * D.maxtf * sizeof(double);
+    if (DIM == 2 && D.planar2d)
+        k_face_entry_fs<DIM, FLUX, false, double, true, false, true><<<g, TILE, shm, st>>>(
+            D, fp, eidx, egeom, FL, dudt, epi);
+    else
+        k_face_entry_fs<DIM, FLUX, false, double, false, false, true><<<g, TILE, shm, st>>>(
+            D, fp, eidx, egeom, FL, dudt, epi);
+}
+
+template <int DIM>
+void launch_face_epi(const DevMesh& D, const FaceParams& fp, int flux, const int2* eidx,
+                     const double* egeom, const double* FL, double* dudt,
+                     const ibis_stage_epilogue& epi, hipStream_t st) {
+    switch (flux) {
+        case IBIS_FLUX_HANEL:
+            launch_face_epi_f<DIM, IBIS_FLUX_HANEL>(D, fp, eidx, egeom, FL, dudt, epi, st);
+            break;
+        case IBIS_FLUX_AUSMDV:
+            launch_face_epi_f<DIM, IBIS_FLUX_AUSMDV>(D, fp, eidx, egeom, FL, dudt, epi, st);
+            break;
+        case IBIS_FLUX_LDFSS2:
+            launch_face_epi_f<DIM, IBIS_FLUX_LDFSS2>(D, fp, eidx, egeom, FL, dudt, epi, st);
+            break;
+        default:
+            launch_face_epi_f<DIM, IBIS_FLUX_RUSANOV>(D, fp, eidx, egeom, FL, dudt, epi, st);
+            break;
     }
 }
 
@@ -3001,11 +3070,11 @@ void run_general(ibis_residual* r, const GenBufs<S>& B, S* prim, S* dudt, bool r
     pf.end(7);
 }
 
-}  // namespace
-
-ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
-                              int allow_reconstruction, ibis_stream_t stream) {
-    if (!r || !prim || !dudt) return ibis::fail(IBIS_ERR_INVALID, "ibis_compute_dudt: null argument");
+// the residual; with epi (only when stage_fusable) the face kernel runs
+// the Runge-Kutta stage epilogue and dudt may be null
+ibis_status compute_dudt_impl(ibis_residual* r, double* prim, double* dudt,
+                              int allow_reconstruction, ibis_stream_t stream,
+                              const ibis_stage_epilogue* epi) {
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const DevMesh& D = r->m->d;
     const ibis_fv_desc& d = r->desc;
@@ -3072,7 +3141,11 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
             const bool f32 = r->prec != IBIS_PREC_FP64;
             const int lay = f32 ? 2 : r->opt_layout;
             const int fk = lay == 2 ? r->opt_face_kernel : 0;
-            if (D.dim == 3)
+            if (epi && D.dim == 3)
+                launch_face_epi<3>(D, fp, d.flux, r->ent_idx, r->ent_geom, r->FL, dudt, *epi, st);
+            else if (epi)
+                launch_face_epi<2>(D, fp, d.flux, r->ent_idx, r->ent_geom, r->FL, dudt, *epi, st);
+            else if (D.dim == 3)
                 launch_face_fs<3>(D, fp, d.flux, lay, f32, fk, r->ent_idx, r->ent_geom,
                                   r->ent_geom32, r->FL, r->FR, dudt, st);
             else
@@ -3095,6 +3168,37 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
     run_general<double>(r, B, prim, dudt, recon, st, pf);
     HIP_TRY(hipGetLastError());
     return IBIS_OK;
+}
+
+}  // namespace
+
+ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
+                              int allow_reconstruction, ibis_stream_t stream) {
+    if (!r || !prim || !dudt) return ibis::fail(IBIS_ERR_INVALID, "ibis_compute_dudt: null argument");
+    return compute_dudt_impl(r, prim, dudt, allow_reconstruction, stream, nullptr);
+}
+
+int ibis_residual_stage_fusable(const ibis_residual* r) {
+    if (!r) return 0;
+    const DevMesh& D = r->m->d;
+    const ibis_fv_desc& d = r->desc;
+    return (r->prec == IBIS_PREC_FP64 && !d.viscous && !r->has_post && d.order == 2 &&
+            r->opt_layout == 2 && r->opt_face_kernel == 1 && r->ent_idx && D.n_from_t)
+               ? 1
+               : 0;
+}
+
+ibis_status ibis_compute_dudt_stage(ibis_residual* r, double* prim, double* dudt,
+                                    const ibis_stage_epilogue* epi, ibis_stream_t stream) {
+    if (!r || !prim || !epi || !epi->u || !epi->inst_of_cell || !epi->dt || !epi->prim_out ||
+        epi->nk < 0 || epi->nk > IBIS_EPI_MAXK)
+        return ibis::fail(IBIS_ERR_INVALID, "ibis_compute_dudt_stage: bad argument");
+    for (int j = 0; j < epi->nk; j++)
+        if (!epi->k[j]) return ibis::fail(IBIS_ERR_INVALID, "ibis_compute_dudt_stage: null stage");
+    if (!ibis_residual_stage_fusable(r))
+        return ibis::fail(IBIS_ERR_INVALID,
+                          "ibis_compute_dudt_stage: the residual's path has no stage epilogue");
+    return compute_dudt_impl(r, prim, dudt, 1, stream, epi);
 }
 
 // tangent state and scratch, allocated (and zeroed) on first use

@@ -151,6 +151,7 @@ _SIGS = {
                                      _P, _P]),
     "ibis_batch_rk_steps": (C.c_int, [_P, C.c_int, C.c_int]),
     "ibis_batch_times": (C.c_int, [_P, _P, _P]),
+    "ibis_batch_set_option": (C.c_int, [_P, C.c_char_p, C.c_int64]),
     "ibis_last_error": (C.c_char_p, []),
 }
 

@@ -524,6 +524,13 @@ ibis_status ibis_batch_rk_init(ibis_batch* b, int nstages, const double* a, cons
 ibis_status ibis_batch_rk_steps(ibis_batch* b, int nsteps, int use_graph);
 /* Per-instance time and last dt (n_inst each; either may be NULL). */
 ibis_status ibis_batch_times(ibis_batch* b, double* t, double* dt_prev);
+/* Batch options (any value gives the same bits):
+ *   "fuse_stages" (default 1): each stage update and primitive conversion
+ *   runs in the epilogue of the residual's face kernel, and the time step
+ *   before the first residual, where the residual's path allows it (the
+ *   inviscid second-order FP64 path with the per-entry face kernel);
+ *   0 runs them as separate passes. */
+ibis_status ibis_batch_set_option(ibis_batch* b, const char* key, int64_t value);
 
 const char* ibis_last_error(void);
 

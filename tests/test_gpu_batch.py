@@ -67,3 +67,70 @@ def test_batch_matches_instances_alone(use_graph):
         assert np.array_equal(got[:, :n_i], c_ref[:, :n_i]), i
         assert dt[i] == dts[-1]
         assert t[i] == float(np.cumsum(dts)[-1])
+
+
+def _batch_run(inst, prims, method, nsteps, use_graph, fuse):
+    """Step a batch with the stage updates fused into the residual's face
+    kernel (fuse 1) or as separate passes (fuse 0); returns cons, prim (all
+    rows, ghosts included) and the per-instance times."""
+    import torch
+    from ibis_amd.batch import Batch
+    from ibis_amd.native import check, lib, ptr
+    bt = Batch([(g, c) for g, c in inst])
+    d_prim = to_dev(bt.gather([p(b) for p, b in zip(prims, bt.blocks)]))
+    d_cons = torch.zeros((bt.ncons, bt.Nt), dtype=torch.float64, device="cuda")
+    check(lib().ibis_prim_to_cons(bt.residual_handle(), ptr(d_prim), ptr(d_cons), None),
+          "ibis_prim_to_cons")
+    torch.cuda.synchronize()
+    bt.set_option("fuse_stages", fuse)
+    bt.rk_init(d_cons, d_prim, cfl=0.5, max_time=1e30, method=method)
+    bt.rk_steps(nsteps, use_graph)
+    torch.cuda.synchronize()
+    out = (d_cons.cpu().numpy(), d_prim.cpu().numpy(), bt.times())
+    bt.close()
+    return out
+
+
+@pytest.mark.parametrize("method", ["ssp-rk3", "rk4", "rk3", "euler", "midpoint"])
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_fused_stage_epilogue_equals_separate_passes(method, use_graph):
+    """The stage update and primitive conversion in the face kernel's
+    epilogue (and the time step before the first residual) give the bits of
+    the separate passes: every row of cons and prim, ghosts included, and
+    every instance's t and dt, for tableaux with 1-4 stages (rk4's zero and
+    rk3's negative a_ij are skipped by the stage sums, as the reference's
+    loop skips a_ij < 1e-14)."""
+    specs = _small_specs()[:4]
+    inst = [(g, c) for g, c, _ in _instances(specs)]
+    fss = [f for _, _, f in _instances(specs)]
+    prims = [lambda b, f=f: cases.uniform_prim(f, b.N, b.Nt) for f in fss]
+    print(f"fused stages: {method} graph={use_graph}", flush=True)
+    fused = _batch_run(inst, prims, method, 3, use_graph, 1)
+    sep = _batch_run(inst, prims, method, 3, use_graph, 0)
+    assert fused[0].tobytes() == sep[0].tobytes()
+    assert fused[1].tobytes() == sep[1].tobytes()
+    for a, b in zip(fused[2], sep[2]):
+        assert a.tobytes() == b.tobytes()
+
+
+def test_fused_stage_epilogue_3d():
+    """Two hex boxes (6^3 and 9^3, jittered, permuted), Hanel with the
+    Venkatakrishnan limiter, a smooth non-uniform state: fused and separate
+    stage updates agree bit for bit after three ssp-rk3 steps."""
+    inst = [cases.hex_case(n, flux="hanel", limiter="venkatakrishnan") for n in (6, 9)]
+
+    def prim_of(g_io):
+        def make(b):
+            p = cases.hex_prim(g_io, b.Nt)
+            p[:, b.N:] = p[:, :1]
+            return b.to_grid_order(p)
+        return make
+
+    prims = [prim_of(g) for g, _ in inst]
+    print("fused stages: 3D", flush=True)
+    fused = _batch_run(inst, prims, "ssp-rk3", 3, True, 1)
+    sep = _batch_run(inst, prims, "ssp-rk3", 3, True, 0)
+    assert fused[0].tobytes() == sep[0].tobytes()
+    assert fused[1].tobytes() == sep[1].tobytes()
+    for a, b in zip(fused[2], sep[2]):
+        assert a.tobytes() == b.tobytes()
