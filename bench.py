@@ -352,6 +352,8 @@ def main():
                     help="fix the stage_cells option instead of autotuning")
     ap.add_argument("--cell-prefetch", type=int, default=None, choices=[0, 1],
                     help="fix the cell_prefetch option instead of autotuning")
+    ap.add_argument("--tile", type=int, default=None, choices=[128, 256, 512],
+                    help="fix the cells per tile instead of autotuning")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/primal leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-stepping leg")
     ap.add_argument("--batch-steps", type=int, default=10)
@@ -419,7 +421,8 @@ def main():
                                ("pack_cells", args.pack), ("face_kernel", args.face_kernel),
                                ("stage_offsets", args.stage),
                                ("cell_prefetch", args.cell_prefetch),
-                               ("stage_cells", args.stage_cells)) if v is not None}
+                               ("stage_cells", args.stage_cells), ("tile", args.tile))
+             if v is not None}
     for k, vals in OPTION_SPACE:
         fv.set_option(k, fixed.get(k, vals[0]))
     for _ in range(10):

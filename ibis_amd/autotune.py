@@ -3,9 +3,12 @@
 Every option of ibis_residual_set_option listed here changes only how the
 work is laid out (face-value layout, precomputed offsets, packing, face
 kernel and its block size, LDS staging of offsets or of the cell stencil,
-prefetching), never the arithmetic:
-all choices give the same bits.  The tuner times a caller-supplied workload
-under each candidate and keeps the fastest, by coordinate descent.
+prefetching, cells per tile, which is also the block size of the tile
+kernels), never the arithmetic: all choices give the same bits.  The tuner
+times a caller-supplied workload under each candidate and keeps the
+fastest, by coordinate descent.  A candidate the library refuses for the
+mesh at hand (for example a tile whose face list does not fit in LDS) is
+skipped.
 """
 from __future__ import annotations
 
@@ -14,10 +17,11 @@ from typing import Callable, Dict, Mapping, Tuple
 # each option with its candidate values; the first is the starting point
 OPTION_SPACE = (("face_state_layout", (2, 1, 0)), ("precomputed_offsets", (1, 0)),
                 ("pack_cells", (0, 1)), ("face_kernel", (1, 0, 64, 128, 256)),
-                ("stage_offsets", (1, 0)), ("cell_prefetch", (0, 1)), ("stage_cells", (1, 0)))
+                ("stage_offsets", (1, 0)), ("cell_prefetch", (0, 1)), ("stage_cells", (1, 0)),
+                ("tile", (256, 128, 512)))
 SHORT = {"face_state_layout": "layout", "precomputed_offsets": "offsets", "pack_cells": "pack",
          "face_kernel": "fk", "stage_offsets": "stage", "cell_prefetch": "pf",
-         "stage_cells": "sc"}
+         "stage_cells": "sc", "tile": "tile"}
 
 
 def variant_key(opt: Mapping[str, int]) -> str:
@@ -43,8 +47,12 @@ def coordinate_descent(set_option: Callable[[str, int], None], measure: Callable
     def score(opt):
         key = variant_key(opt)
         if key not in seen:
-            for k, v in opt.items():
-                set_option(k, v)
+            try:
+                for k, v in opt.items():
+                    set_option(k, v)
+            except Exception:  # refused for this mesh: never chosen
+                seen[key] = float("inf")
+                return seen[key]
             seen[key] = measure()
             if log:
                 log(key, seen[key])

@@ -35,6 +35,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -56,6 +57,12 @@ namespace {
             std::string m_ = std::string(#expr) + ": " + hipGetErrorString(e_);         \
             return ibis::fail(IBIS_ERR_HIP, m_.c_str());                                \
         }                                                                               \
+    } while (0)
+
+#define IBIS_TRY(expr)                \
+    do {                              \
+        ibis_status s_ = (expr);      \
+        if (s_ != IBIS_OK) return s_; \
     } while (0)
 
 // primitive rows (ibis::P_*, state.hpp) and gradient/limiter slots
@@ -90,7 +97,7 @@ struct DevMesh {
     const int* b_valid;
     const int* b_tag;
     const int* b_of_ghost;  // G: boundary-face index of each ghost cell
-    // tiles of TILE consecutive cells and the faces touching them
+    // tiles of `tile` consecutive cells and the faces touching them
     int ntiles, maxtf, ntf;           // ntf: entries of all tile lists
     const int* tile_face_off;         // ntiles + 1
     const int* tile_faces;            // face ids, ascending within a tile
@@ -106,11 +113,14 @@ struct DevMesh {
     // face-state records: slot of face f in the face list of its left/right cell
     const unsigned char* slotL;
     const unsigned char* slotR;
+    int tile;      // cells per tile (and threads per block of the tile kernels)
     int n_from_t;  // face normals equal cross(t1, t2) bit for bit (grid-built meshes)
     int planar2d;  // 2D, and every face has t1z = +0 and t2 = (+0, +0, +1) bit for bit
 };
 
-constexpr int TILE = 256;
+constexpr int TILE = 256;     // cells per tile of a new mesh (the option "tile" picks others)
+constexpr int MAXTILE = 512;  // largest tile: the tile kernels' block size is the tile
+constexpr size_t MAX_LDS_BYTES = 160 * 1024;  // LDS of one gfx950 workgroup
 constexpr int NXCD = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin across them
 
 // Logical block for XCD locality: hardware block b runs on XCD b % 8, so
@@ -513,14 +523,14 @@ __global__ void __launch_bounds__(BLOCK)
     for (int q = 0; q < NC; q++) flux[(size_t)q * F + f] = PREMUL ? res[q] * A : res[q];
 }
 
-// Fused face flux + surface integral for one tile of TILE cells.
+// Fused face flux + surface integral for one tile of m.tile cells.
 // Phase 1: every face touching the tile is evaluated once per tile and its
 // flux*area kept in LDS (faces on the tile border are evaluated by both
 // neighbouring tiles).  Phase 2: each cell sums its faces from LDS in its
 // own face order, so the result is the gather of
 // FV/finite_volume.cpp:221-255 with no flux array in HBM and no atomics.
 template <int DIM, int FLUX, bool RECON>
-__global__ void __launch_bounds__(TILE)
+__global__ void __launch_bounds__(MAXTILE)
     k_face_tile(DevMesh m, FaceParams fp, const double* __restrict__ prim,
                 const double* __restrict__ grad, const double* __restrict__ lim,
                 double* __restrict__ dudt) {
@@ -531,7 +541,7 @@ __global__ void __launch_bounds__(TILE)
     const int F = m.F, N = m.N, Nt = m.Nt, S = m.maxtf;
     const int f0 = m.tile_face_off[t];
     const int nfl = m.tile_face_off[t + 1] - f0;
-    for (int k = threadIdx.x; k < nfl; k += TILE) {
+    for (int k = threadIdx.x; k < nfl; k += m.tile) {
         const int f = m.tile_faces[f0 + k];
         double res[NC];
         face_flux_eval<double, DIM, FLUX, RECON>(m, fp, prim, grad, lim, f, res);
@@ -540,7 +550,7 @@ __global__ void __launch_bounds__(TILE)
         for (int q = 0; q < NC; q++) sflux[q * S + k] = res[q] * A;
     }
     __syncthreads();
-    const int c = t * TILE + threadIdx.x;
+    const int c = t * m.tile + threadIdx.x;
     if (c >= N) return;
     double d[NC];
 #pragma unroll
@@ -652,6 +662,28 @@ __global__ void __launch_bounds__(BLOCK)
     ghost_face_values<LAYOUT, W>(m, vs, prim, FL, FR, b);
 }
 
+// Boundary faces of the fused viscous path, one thread per face: the
+// pre-reconstruction actions, the ghost's face values (layout 2), its
+// reconstruction variables as they are now (gconv, 5 x G: the convective
+// gradients read these), then the viscous actions, whose result stays in
+// prim's ghost rows as after the reference's second ghost fill.  The
+// actions of a face read only its valid cell and write only its ghost, so
+// running them back to back per face is running each pass over all faces.
+__global__ void __launch_bounds__(BLOCK)
+    k_bc_visc_fs(DevMesh m, const int* __restrict__ kinds, const double* __restrict__ pars,
+                 int ntab, int has_pre, int has_visc, GasParams gp, VarSet vs,
+                 double* __restrict__ prim, double* __restrict__ FL, double* __restrict__ gconv) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.nb) return;
+    if (has_pre) bc_actions<double>(m, kinds, pars, gp, prim, b);
+    ghost_face_values<2, double>(m, vs, prim, FL, FL, b);
+    const int g = m.b_ghost[b];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        gconv[(size_t)v * m.G + (g - m.N)] = prim[(size_t)vs.row[v] * m.Nt + g];
+    if (has_visc) bc_actions<double>(m, kinds + ntab, pars + (size_t)ntab * IBIS_NPAR, gp, prim, b);
+}
+
 // Offsets x_face - x_cell per (cell, face slot), column-major like the other
 // cell lists: offs[(j*3 + k)*N + i].  Same expression as the kernels use.
 __global__ void __launch_bounds__(BLOCK) k_offsets(DevMesh m, double* __restrict__ offs) {
@@ -699,8 +731,8 @@ __device__ __forceinline__ void load_cell(const DevMesh& m, const VarSet& vs,
     }
 }
 
-// Stage block lb's cell stencil in LDS: its TILE cells' and its halo's
-// values and centroids as 8 planes of SS = TILE + maxhalo doubles (own
+// Stage block lb's cell stencil in LDS: its m.tile cells' and its halo's
+// values and centroids as 8 planes of SS = m.tile + maxhalo doubles (own
 // cells first, then the halo in the tile's halo order).  Each thread
 // gathers its own cell and its first halo cell together, then writes both;
 // further halo cells one per pass.  Ends with the block barrier.
@@ -709,8 +741,8 @@ __device__ __forceinline__ void stage_stencil(const DevMesh& m, const VarSet& vs
                                               const double* __restrict__ pack,
                                               const double* __restrict__ prim, int lb, int i,
                                               double* soff) {
-    static_assert(TILE == BLOCK, "one stencil entry per thread and pass");
-    const int N = m.N, SS = TILE + m.maxhalo;
+    // one stencil entry per thread and pass: the block is the tile
+    const int N = m.N, SS = m.tile + m.maxhalo;
     const int h0 = m.halo_off[lb], nh = m.halo_off[lb + 1] - h0;
     auto put = [&](int k, const double* u, double x, double y, double z) {
 #pragma unroll
@@ -725,36 +757,79 @@ __device__ __forceinline__ void stage_stencil(const DevMesh& m, const VarSet& vs
     if (i < N) load_cell<PACKED>(m, vs, pack, prim, i, u0, x0, y0, z0);
     if (kh < nh) load_cell<PACKED>(m, vs, pack, prim, ch, u1, x1, y1, z1);
     if (i < N) put(threadIdx.x, u0, x0, y0, z0);
-    if (kh < nh) put(TILE + kh, u1, x1, y1, z1);
-    for (int k = kh + BLOCK; k < nh; k += BLOCK) {
+    if (kh < nh) put(m.tile + kh, u1, x1, y1, z1);
+    for (int k = kh + m.tile; k < nh; k += m.tile) {
         double u[5], x, y, z;
         load_cell<PACKED>(m, vs, pack, prim, m.halo_ids[h0 + k], u, x, y, z);
-        put(TILE + k, u, x, y, z);
+        put(m.tile + k, u, x, y, z);
     }
     __syncthreads();
 }
 
+// The stencil of the fused viscous path: 12 planes of SS doubles -- the
+// five reconstruction variables (a ghost's from gconv, its state before the
+// viscous boundary actions, which the convective gradients read), the
+// centroid, then T, vx, vy, vz as the viscous gradients read them (a
+// ghost's after the viscous actions, in prim).  Ends with the block barrier.
+__device__ __forceinline__ void stage_stencil_visc(const DevMesh& m, const VarSet& vs,
+                                                   const double* __restrict__ prim,
+                                                   const double* __restrict__ gconv, int lb, int i,
+                                                   double* soff) {
+    const int N = m.N, Nt = m.Nt, G = m.G, SS = m.tile + m.maxhalo;
+    const int h0 = m.halo_off[lb], nh = m.halo_off[lb + 1] - h0;
+    auto stage = [&](int c, int k) {
+        double u[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            u[v] = c < N ? prim[(size_t)vs.row[v] * Nt + c] : gconv[(size_t)v * G + (c - N)];
+#pragma unroll
+        for (int v = 0; v < 5; v++) soff[v * SS + k] = u[v];
+        soff[5 * SS + k] = m.cen[c];
+        soff[6 * SS + k] = m.cen[Nt + c];
+        soff[7 * SS + k] = m.cen[2 * (size_t)Nt + c];
+        soff[8 * SS + k] = prim[(size_t)P_T * Nt + c];
+        soff[9 * SS + k] = prim[(size_t)P_VX * Nt + c];
+        soff[10 * SS + k] = prim[(size_t)P_VY * Nt + c];
+        soff[11 * SS + k] = prim[(size_t)P_VZ * Nt + c];
+    };
+    if (i < N) stage(i, threadIdx.x);
+    for (int k = threadIdx.x; k < nh; k += m.tile) stage(m.halo_ids[h0 + k], m.tile + k);
+    __syncthreads();
+}
+
+__device__ __forceinline__ bool viscous_slot(int slot) {
+    return slot == G_T || slot == G_VX || slot == G_VY || slot == G_VZ;
+}
+
+// VISC (with SN, layout 2): the fused viscous path's cell kernel -- the
+// stencil of stage_stencil_visc, and before the convective work the viscous
+// gradients of T, vx, vy, vz (the LSQ gradient of k_cell_grad_var, same
+// expressions and order) into their slots, which the convective slots do
+// not overwrite afterwards (the viscous gradients are computed after the
+// convective ones in the reference and overwrite them).
 template <int DIM, int LIM, int LAYOUT, bool OFFS, class W = double, bool PACKED = true,
-          bool PF = false, bool SN = false>
-__global__ void __launch_bounds__(BLOCK)
+          bool PF = false, bool SN = false, bool VISC = false>
+__global__ void __launch_bounds__(MAXTILE)
     k_cell_fs(DevMesh m, VarSet vs, const double* __restrict__ pack, double* __restrict__ grad,
               double* __restrict__ lim, double eps, double venkat_K, int nlogical,
               int write_slots, W* __restrict__ FL, W* __restrict__ FR,
-              const double* __restrict__ offs, const double* __restrict__ prim, int stage) {
+              const double* __restrict__ offs, const double* __restrict__ prim, int stage,
+              const double* __restrict__ gconv) {
     // stage != 0: the limiter pass keeps each face offset in LDS
     // (maxf x 3 doubles per thread) for the reconstruction pass, so the
     // offsets are read from memory once.
-    // SN: the block (one tile of TILE cells) first copies its cells' and its
+    // SN: the block (one tile of m.tile cells) first copies its cells' and its
     // stencil halo's values and centroids into LDS (8 planes of
-    // TILE + maxhalo doubles) with one load per cell, and the neighbour loop
+    // m.tile + maxhalo doubles) with one load per cell, and the neighbour loop
     // reads them from there.  With stage != 0 as well, the offsets take the
     // same LDS once the whole block has finished the neighbour loop.
     extern __shared__ double soff[];
     const int lb = xcd_block(nlogical);
     if (lb >= nlogical) return;
-    const int i = lb * BLOCK + threadIdx.x;
+    const int TS = m.tile;  // cells per block: one tile
+    const int i = lb * TS + threadIdx.x;
     const int N = m.N, F = m.F;
-    const int SS = TILE + m.maxhalo;  // plane length of the LDS stencil
+    const int SS = TS + m.maxhalo;  // plane length of the LDS stencil
     // per-cell inputs that do not depend on the staged stencil, loaded
     // before the staging loop and its barrier so that their latency
     // overlaps them (a thread past the last cell reads cell 0's and
@@ -773,7 +848,10 @@ __global__ void __launch_bounds__(BLOCK)
 #pragma unroll
         for (int j = 0; j < MAXF; j++) kst[j] = j < m.maxf ? m.nbr_lds[(size_t)j * N + il] : 0;
     }
-    if (SN) stage_stencil<PACKED>(m, vs, pack, prim, lb, i, soff);
+    if (SN && VISC)
+        stage_stencil_visc(m, vs, prim, gconv, lb, i, soff);
+    else if (SN)
+        stage_stencil<PACKED>(m, vs, pack, prim, lb, i, soff);
     // with the offsets staged over the stencil, a thread past the last cell
     // stays to the second barrier (it works on LDS slots that are not read
     // back and stores nothing)
@@ -827,6 +905,52 @@ __global__ void __launch_bounds__(BLOCK)
             umax[v] = ibis::kmax(umax[v], un[v]);
         }
     };
+    // VISC: the 2D convective vz gradient is its slot's value from before
+    // this call's viscous gradients
+    double vz_old[3] = {0.0, 0.0, 0.0};
+    if (VISC && DIM == 2 && i < N) {
+        const double* g = grad + (size_t)G_VZ * 3 * N;
+        vz_old[0] = g[i];
+        vz_old[1] = g[N + i];
+        vz_old[2] = g[2 * (size_t)N + i];
+    }
+    if (VISC && SN && i < N) {
+        double tx[4] = {0.0, 0.0, 0.0, 0.0}, ty[4] = {0.0, 0.0, 0.0, 0.0},
+               tz[4] = {0.0, 0.0, 0.0, 0.0}, ti[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) ti[q] = soff[(8 + q) * SS + threadIdx.x];
+#pragma unroll
+        for (int j = 0; j < MAXF; j++) {
+            if (j < nf) {
+                const int k = kst[j];
+                const double dx = soff[5 * SS + k] - xi;
+                const double dy = soff[6 * SS + k] - yi;
+                const double dz = soff[7 * SS + k] - zi;
+                const double alpha_1 = dx / r11s;
+                const double alpha_2 = i22 * (dy - q12 * dx);
+                double alpha_3 = 0.0;
+                if (DIM == 3) alpha_3 = i33 * (dz - q23 * dy + beta * dx);
+                const double w_1 = alpha_1 - q12 * alpha_2 + beta * alpha_3;
+                const double w_2 = alpha_2 - q23 * alpha_3;
+                const double w_3 = alpha_3;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const double diff = soff[(8 + q) * SS + k] - ti[q];
+                    tx[q] += w_1 * diff;
+                    ty[q] += w_2 * diff;
+                    tz[q] += w_3 * diff;
+                }
+            }
+        }
+        const int vslot[4] = {G_T, G_VX, G_VY, G_VZ};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            double* g = grad + (size_t)vslot[q] * 3 * N;
+            g[i] = tx[q];
+            g[N + i] = ty[q];
+            g[2 * (size_t)N + i] = tz[q];
+        }
+    }
     if (SN) {
 #pragma unroll
         for (int j = 0; j < MAXF; j++) {
@@ -875,11 +999,15 @@ __global__ void __launch_bounds__(BLOCK)
     for (int v = 0; v < 5; v++) {
         double* g = grad + (size_t)vs.slot[v] * 3 * N;
         if (v < NG) {
-            if (write_slots) {
+            if (write_slots && !(VISC && viscous_slot(vs.slot[v]))) {
                 g[i] = gx[v];
                 g[N + i] = gy[v];
                 g[2 * (size_t)N + i] = gz[v];
             }
+        } else if (VISC) {  // 2D vz, VISC: the slot's value read above
+            gx[v] = vz_old[0];
+            gy[v] = vz_old[1];
+            gz[v] = vz_old[2];
         } else {  // 2D vz: the gradient stored in its slot
             gx[v] = g[i];
             gy[v] = g[N + i];
@@ -919,9 +1047,9 @@ __global__ void __launch_bounds__(BLOCK)
             const double dx = ndx, dy = ndy, dz = ndz;
             if (j + 1 < MAXF) face_offset(j + 1 < nf ? j + 1 : j, ndx, ndy, ndz);
             if (stage) {
-                soff[(j * 3 + 0) * BLOCK + threadIdx.x] = dx;
-                soff[(j * 3 + 1) * BLOCK + threadIdx.x] = dy;
-                soff[(j * 3 + 2) * BLOCK + threadIdx.x] = dz;
+                soff[(j * 3 + 0) * TS + threadIdx.x] = dx;
+                soff[(j * 3 + 1) * TS + threadIdx.x] = dy;
+                soff[(j * 3 + 2) * TS + threadIdx.x] = dz;
             }
 #pragma unroll
             for (int v = 0; v < 5; v++) {
@@ -972,9 +1100,9 @@ __global__ void __launch_bounds__(BLOCK)
         const int f = m.ell_face[(size_t)j * N + i];
         double dx = ox[j], dy = oy[j], dz = oz[j];
         if (FROM_LDS && stage) {
-            dx = soff[(j * 3 + 0) * BLOCK + threadIdx.x];
-            dy = soff[(j * 3 + 1) * BLOCK + threadIdx.x];
-            dz = soff[(j * 3 + 2) * BLOCK + threadIdx.x];
+            dx = soff[(j * 3 + 0) * TS + threadIdx.x];
+            dy = soff[(j * 3 + 1) * TS + threadIdx.x];
+            dz = soff[(j * 3 + 2) * TS + threadIdx.x];
         }
         W* rec = LAYOUT == 0 ? (m.ell_sign[(size_t)j * N + i] > 0 ? FL : FR) + (size_t)f * 5
                              : FL + ((size_t)i * m.maxf + j) * 5;
@@ -997,7 +1125,7 @@ __global__ void __launch_bounds__(BLOCK)
 // rounding away from the FP64 one.  stage != 0 keeps the face offsets of
 // the limiter pass in LDS as floats for the reconstruction pass.
 template <int DIM, int LIM, bool OFFS, bool SN = false>
-__global__ void __launch_bounds__(BLOCK)
+__global__ void __launch_bounds__(MAXTILE)
     k_cell_fs32(DevMesh m, VarSet vs, const double* __restrict__ prim,
                 const double* __restrict__ grad, double eps, double venkat_K, int nlogical,
                 float* __restrict__ FL, const double* __restrict__ offs, int stage,
@@ -1010,9 +1138,10 @@ __global__ void __launch_bounds__(BLOCK)
     float* soff32 = reinterpret_cast<float*>(sh32);
     const int lb = xcd_block(nlogical);
     if (lb >= nlogical) return;
-    const int i = lb * BLOCK + threadIdx.x;
+    const int TS = m.tile;  // cells per block: one tile
+    const int i = lb * TS + threadIdx.x;
     const int N = m.N, F = m.F;
-    const int SS = TILE + m.maxhalo;
+    const int SS = TS + m.maxhalo;
     // per-cell inputs that do not depend on the stencil, loaded before the
     // staging barrier (a thread past the last cell reads cell 0's)
     const int il = i < N ? i : 0;
@@ -1149,9 +1278,9 @@ __global__ void __launch_bounds__(BLOCK)
             const float dx = ndx, dy = ndy, dz = ndz;
             if (j + 1 < MAXF) offset(j + 1 < nf ? j + 1 : j, ndx, ndy, ndz);
             if (staged) {
-                soff32[(j * 3 + 0) * BLOCK + threadIdx.x] = dx;
-                soff32[(j * 3 + 1) * BLOCK + threadIdx.x] = dy;
-                soff32[(j * 3 + 2) * BLOCK + threadIdx.x] = dz;
+                soff32[(j * 3 + 0) * TS + threadIdx.x] = dx;
+                soff32[(j * 3 + 1) * TS + threadIdx.x] = dy;
+                soff32[(j * 3 + 2) * TS + threadIdx.x] = dz;
             }
 #pragma unroll
             for (int v = 0; v < 5; v++) {
@@ -1192,9 +1321,9 @@ __global__ void __launch_bounds__(BLOCK)
         if (j >= nf) break;
         float dx = ox[j], dy = oy[j], dz = oz[j];
         if (staged) {
-            dx = soff32[(j * 3 + 0) * BLOCK + threadIdx.x];
-            dy = soff32[(j * 3 + 1) * BLOCK + threadIdx.x];
-            dz = soff32[(j * 3 + 2) * BLOCK + threadIdx.x];
+            dx = soff32[(j * 3 + 0) * TS + threadIdx.x];
+            dy = soff32[(j * 3 + 1) * TS + threadIdx.x];
+            dz = soff32[(j * 3 + 2) * TS + threadIdx.x];
         }
 #pragma unroll
         for (int v = 0; v < 5; v++) {
@@ -1230,12 +1359,14 @@ __device__ __forceinline__ void face_flux_values(const DevMesh& m, const FacePar
     face_flux_values_g<DIM, FLUX, R>(m, fp, f, g, lv, rv, res_a);
 }
 
-// gn: the face normal when the caller has loaded it (else read by face id);
-// unused when the normal is cross(t1, t2)
+// The face flux in the global frame before the product with the area
+// (the flux rows of FV/convective_flux.cpp:85-178).  gn: the face normal
+// when the caller has loaded it (else read by face id); unused when the
+// normal is cross(t1, t2)
 template <int DIM, int FLUX, class R>
-__device__ __forceinline__ void face_flux_values_g(const DevMesh& m, const FaceParams& fp, int f,
-                                                   const double* g, const R* lv, const R* rv,
-                                                   double* res_a, const double* gn) {
+__device__ __forceinline__ void face_flux_raw_g(const DevMesh& m, const FaceParams& fp, int f,
+                                                const double* g, const R* lv, const R* rv,
+                                                double* res_d, const double* gn) {
     constexpr int NC = DIM + 2;
     constexpr bool T3 = DIM == 3;
     const int F = m.F;
@@ -1307,15 +1438,27 @@ __device__ __forceinline__ void face_flux_values_g(const DevMesh& m, const FaceP
     } else {
         res[3] = out[4];
     }
+#pragma unroll
+    for (int q = 0; q < NC; q++) res_d[q] = (double)res[q];
+}
+
+// flux * area of face f
+template <int DIM, int FLUX, class R>
+__device__ __forceinline__ void face_flux_values_g(const DevMesh& m, const FaceParams& fp, int f,
+                                                   const double* g, const R* lv, const R* rv,
+                                                   double* res_a, const double* gn) {
+    constexpr int NC = DIM + 2;
+    double res[NC];
+    face_flux_raw_g<DIM, FLUX, R>(m, fp, f, g, lv, rv, res, gn);
     const double A = g[6];
 #pragma unroll
-    for (int q = 0; q < NC; q++) res_a[q] = (double)res[q] * A;
+    for (int q = 0; q < NC; q++) res_a[q] = res[q] * A;
 }
 
 // Fused face flux + surface integral from the face states, one tile of
-// TILE cells per block, driven by the tile's face list.
+// m.tile cells per block, driven by the tile's face list.
 template <int DIM, int FLUX, int LAYOUT, class R = double>
-__global__ void __launch_bounds__(TILE)
+__global__ void __launch_bounds__(MAXTILE)
     k_face_tile_fs(DevMesh m, FaceParams fp, const R* __restrict__ FL,
                    const R* __restrict__ FR, double* __restrict__ dudt) {
     extern __shared__ double sflux[];  // NC x maxtf
@@ -1364,7 +1507,7 @@ __global__ void __launch_bounds__(TILE)
             for (int q = 0; q < 5; q++) v[side][q] = vn[side][q];
 #pragma unroll
         for (int q = 0; q < 7; q++) g[q] = gn[q];
-        const int kn = k + TILE;
+        const int kn = k + m.tile;
         if (kn < nfl) load(kn, fn, vn, gn);
         double res[NC];
         face_flux_values_g<DIM, FLUX, R>(m, fp, f, g, v[0], v[1], res);
@@ -1373,7 +1516,7 @@ __global__ void __launch_bounds__(TILE)
         k = kn;
     }
     __syncthreads();
-    const int c = t * TILE + threadIdx.x;
+    const int c = t * m.tile + threadIdx.x;
     if (c >= N) return;
     double d[NC];
 #pragma unroll
@@ -1455,16 +1598,134 @@ __global__ void __launch_bounds__(BLOCK) k_entries32(DevMesh m, char* __restrict
     c[e] = gf[12 * (size_t)F + f];
 }
 
+struct ViscParams {
+    double R, Cv, Cp, mu0, T0, Ts, Pr;
+};
+
+// GAS/transport_properties.h:27-42; the constants have the scalar type
+template <class S>
+__device__ __forceinline__ S sutherland(S temp, const ViscParams& vp) {
+    const S mu0 = S(vp.mu0), T0 = S(vp.T0), Ts = S(vp.Ts);
+    return mu0 * pow(temp / T0, S(3.0 / 2.0)) * (T0 + Ts) / (temp + Ts);
+}
+
+// GAS/transport_properties.h:64-75
+template <class S>
+__device__ __forceinline__ S conductivity(S temp, const ViscParams& vp) {
+    const S mu = sutherland<S>(temp, vp);
+    return vp.Cp * mu / S(vp.Pr);
+}
+
+template <class S>
+__device__ __forceinline__ void hasselbacher(const S* __restrict__ grad, int N, int slot, S vl,
+                                             S vr, int l, int r, const double* ehat,
+                                             const double* n, double len_e, double edn, S* out) {
+    const S* g = grad + (size_t)slot * 3 * N;
+    const S ax = 0.5 * (g[l] + g[r]);
+    const S ay = 0.5 * (g[N + l] + g[N + r]);
+    const S az = 0.5 * (g[2 * (size_t)N + l] + g[2 * (size_t)N + r]);
+    const S avg_dot_ehat = ax * ehat[0] + ay * ehat[1] + az * ehat[2];
+    const S correction = avg_dot_ehat - (vr - vl) / len_e;
+    out[0] = ax - correction * n[0] / edn;
+    out[1] = ay - correction * n[1] / edn;
+    out[2] = az - correction * n[2] / edn;
+}
+
+// The viscous flux of face f (viscous_flux.cpp:133-184) between cells l
+// and r, whose gradients are in the slots G_T, G_VX, G_VY, G_VZ of grad:
+// Hasselbacher face gradients between two valid cells, the valid cell's own
+// gradients at a boundary face (viscous_flux.cpp:16-80); the face state is
+// the average of the two cells' p, T, v (GAS/flow_state.h:89-91).  vt: the
+// terms the flux rows lose, momentum then energy (dim+1 values); rho_f and
+// T_f: the face density and temperature that estimate_dt reads.
+template <class S, int DIM>
+__device__ __forceinline__ void visc_face(const DevMesh& m, const ViscParams& vp,
+                                          const S* __restrict__ prim, const S* __restrict__ grad,
+                                          int f, int l, int r, S* vt, double* rho_f,
+                                          double* T_f) {
+    const int F = m.F, N = m.N, Nt = m.Nt;
+    const bool lv = l < N, rv = r < N;
+    S gT[3], gvx[3], gvy[3], gvz[3];
+    const double n[3] = {m.geom[f], m.geom[(size_t)F + f], m.geom[2 * (size_t)F + f]};
+    if (!lv || !rv) {  // copy_gradients_to_face, viscous_flux.cpp:16-34
+        const int c = lv ? l : r;
+        for (int k = 0; k < 3; k++) {
+            gT[k] = grad[((size_t)G_T * 3 + k) * N + c];
+            gvx[k] = grad[((size_t)G_VX * 3 + k) * N + c];
+            gvy[k] = grad[((size_t)G_VY * 3 + k) * N + c];
+            gvz[k] = grad[((size_t)G_VZ * 3 + k) * N + c];
+        }
+    } else {  // hasselbacher_average, viscous_flux.cpp:52-80
+        const double ex = m.cen[r] - m.cen[l];
+        const double ey = m.cen[Nt + r] - m.cen[Nt + l];
+        const double ez = m.cen[2 * (size_t)Nt + r] - m.cen[2 * (size_t)Nt + l];
+        const double len_e = sqrt(ex * ex + ey * ey + ez * ez);
+        const double ehat[3] = {ex / len_e, ey / len_e, ez / len_e};
+        const double edn = ehat[0] * n[0] + ehat[1] * n[1] + ehat[2] * n[2];
+        hasselbacher<S>(grad, N, G_VX, prim[(size_t)P_VX * Nt + l], prim[(size_t)P_VX * Nt + r], l,
+                        r, ehat, n, len_e, edn, gvx);
+        hasselbacher<S>(grad, N, G_VY, prim[(size_t)P_VY * Nt + l], prim[(size_t)P_VY * Nt + r], l,
+                        r, ehat, n, len_e, edn, gvy);
+        hasselbacher<S>(grad, N, G_VZ, prim[(size_t)P_VZ * Nt + l], prim[(size_t)P_VZ * Nt + r], l,
+                        r, ehat, n, len_e, edn, gvz);
+        hasselbacher<S>(grad, N, G_T, prim[(size_t)P_T * Nt + l], prim[(size_t)P_T * Nt + r], l, r,
+                        ehat, n, len_e, edn, gT);
+    }
+    // average_flow_states_pT + update_thermo_from_pT (GAS/flow_state.h:89-91)
+    const S p = 0.5 * (prim[(size_t)P_P * Nt + l] + prim[(size_t)P_P * Nt + r]);
+    const S temp = 0.5 * (prim[(size_t)P_T * Nt + l] + prim[(size_t)P_T * Nt + r]);
+    const S vx = 0.5 * (prim[(size_t)P_VX * Nt + l] + prim[(size_t)P_VX * Nt + r]);
+    const S vy = 0.5 * (prim[(size_t)P_VY * Nt + l] + prim[(size_t)P_VY * Nt + r]);
+    const S vz = 0.5 * (prim[(size_t)P_VZ * Nt + l] + prim[(size_t)P_VZ * Nt + r]);
+    *rho_f = ibis::real_part(p / (vp.R * temp));
+    *T_f = ibis::real_part(temp);
+    const S mu = sutherland<S>(temp, vp);
+    const S k = conductivity<S>(temp, vp);
+    const S lambda = -2.0 / 3.0 * mu;
+    const S bulk = lambda * (gvx[0] + gvy[1] + gvz[2]);
+    const S tau_xx = 2.0 * mu * gvx[0] + bulk;
+    const S tau_yy = 2.0 * mu * gvy[1] + bulk;
+    const S tau_zz = 2.0 * mu * gvz[2] + bulk;
+    const S tau_xy = mu * (gvx[1] + gvy[0]);
+    const S tau_xz = mu * (gvx[2] + gvz[0]);
+    const S tau_yz = mu * (gvy[2] + gvz[1]);
+    const S theta_x = vx * tau_xx + vy * tau_xy + vz * tau_xz + k * gT[0];
+    const S theta_y = vx * tau_xy + vy * tau_yy + vz * tau_yz + k * gT[1];
+    const S theta_z = vx * tau_xz + vy * tau_yz + vz * tau_zz + k * gT[2];
+    vt[0] = n[0] * tau_xx + n[1] * tau_xy + n[2] * tau_xz;
+    vt[1] = n[0] * tau_xy + n[1] * tau_yy + n[2] * tau_yz;
+    if (DIM == 3) {
+        vt[2] = n[0] * tau_xz + n[1] * tau_yz + n[2] * tau_zz;
+        vt[3] = n[0] * theta_x + n[1] * theta_y + n[2] * theta_z;
+    } else {
+        vt[2] = n[0] * theta_x + n[1] * theta_y + n[2] * theta_z;
+    }
+}
+
 // k_face_tile_fs over the per-entry face data (layout 2 values); NRM: the
 // normals are stored (they are not cross(t1, t2)).  EPI: the Runge-Kutta
 // stage epilogue (ibis_stage_epilogue) follows each cell's dU/dt; dudt may
 // then be null.
+// The viscous inputs of the fused viscous face kernel: the state after the
+// viscous boundary actions, the cells' viscous gradients (slots G_T, G_VX,
+// G_VY, G_VZ) and the face density and temperature it writes for estimate_dt.
+struct ViscFace {
+    ViscParams vp;
+    const double* prim;
+    const double* grad;
+    double* face_rho;
+    double* face_T;
+};
+
+// VISC: each face's viscous flux (visc_face) is subtracted from its
+// convective flux before the product with the area, as the reference's
+// viscous pass subtracts it from the stored face flux.
 template <int DIM, int FLUX, bool NRM, class R = double, bool P2 = false, bool G32 = false,
-          bool EPI = false>
-__global__ void __launch_bounds__(TILE)
+          bool EPI = false, bool VISC = false>
+__global__ void __launch_bounds__(MAXTILE)
     k_face_entry_fs(DevMesh m, FaceParams fp, const int2* __restrict__ eidx,
                     const double* __restrict__ egeom, const R* __restrict__ FL,
-                    double* __restrict__ dudt, ibis_stage_epilogue epi) {
+                    double* __restrict__ dudt, ibis_stage_epilogue epi, ViscFace vf) {
     extern __shared__ double sflux[];  // NC x maxtf
     const int t = xcd_block(m.ntiles);
     if (t >= m.ntiles) return;
@@ -1546,16 +1807,28 @@ __global__ void __launch_bounds__(TILE)
             for (int q = 0; q < 5; q++) v[side][q] = vn[side][q];
 #pragma unroll
         for (int q = 0; q < NG; q++) g[q] = gn[q];
-        const int kn = k + TILE;
+        const int kn = k + m.tile;
         if (kn < nfl) load(kn, vn, gn);
         double res[NC];
-        face_flux_values_g<DIM, FLUX, R>(m, fp, 0, g, v[0], v[1], res, NRM ? g + 7 : nullptr);
+        if (VISC) {
+            face_flux_raw_g<DIM, FLUX, R>(m, fp, 0, g, v[0], v[1], res, NRM ? g + 7 : nullptr);
+            const int f = m.ent_faces[f0 + k];
+            double vt[DIM + 1], rho_f, T_f;
+            visc_face<double, DIM>(m, vf.vp, vf.prim, vf.grad, f, m.L[f], m.R[f], vt, &rho_f, &T_f);
+            vf.face_rho[f] = rho_f;  // an entry duplicated in two tiles writes the same values
+            vf.face_T[f] = T_f;
+            const double A = g[6];
+#pragma unroll
+            for (int q = 0; q < NC; q++) res[q] = (q == 0 ? res[q] : res[q] - vt[q - 1]) * A;
+        } else {
+            face_flux_values_g<DIM, FLUX, R>(m, fp, 0, g, v[0], v[1], res, NRM ? g + 7 : nullptr);
+        }
 #pragma unroll
         for (int q = 0; q < NC; q++) sflux[q * S + k] = res[q];
         k = kn;
     }
     __syncthreads();
-    const int c = t * TILE + threadIdx.x;
+    const int c = t * m.tile + threadIdx.x;
     if (c >= N) return;
     double d[NC];
 #pragma unroll
@@ -1705,105 +1978,22 @@ __global__ void __launch_bounds__(BLOCK)
 // ---------------------------------------------------------------------------
 // Viscous flux (FV/viscous_flux.cpp:36-184)
 // ---------------------------------------------------------------------------
-struct ViscParams {
-    double R, Cv, Cp, mu0, T0, Ts, Pr;
-};
-
-// GAS/transport_properties.h:27-42; the constants have the scalar type
-template <class S>
-__device__ __forceinline__ S sutherland(S temp, const ViscParams& vp) {
-    const S mu0 = S(vp.mu0), T0 = S(vp.T0), Ts = S(vp.Ts);
-    return mu0 * pow(temp / T0, S(3.0 / 2.0)) * (T0 + Ts) / (temp + Ts);
-}
-
-// GAS/transport_properties.h:64-75
-template <class S>
-__device__ __forceinline__ S conductivity(S temp, const ViscParams& vp) {
-    const S mu = sutherland<S>(temp, vp);
-    return vp.Cp * mu / S(vp.Pr);
-}
-
-template <class S>
-__device__ __forceinline__ void hasselbacher(const S* __restrict__ grad, int N, int slot, S vl,
-                                             S vr, int l, int r, const double* ehat,
-                                             const double* n, double len_e, double edn, S* out) {
-    const S* g = grad + (size_t)slot * 3 * N;
-    const S ax = 0.5 * (g[l] + g[r]);
-    const S ay = 0.5 * (g[N + l] + g[N + r]);
-    const S az = 0.5 * (g[2 * (size_t)N + l] + g[2 * (size_t)N + r]);
-    const S avg_dot_ehat = ax * ehat[0] + ay * ehat[1] + az * ehat[2];
-    const S correction = avg_dot_ehat - (vr - vl) / len_e;
-    out[0] = ax - correction * n[0] / edn;
-    out[1] = ay - correction * n[1] / edn;
-    out[2] = az - correction * n[2] / edn;
-}
-
 template <class S, int DIM>
 __global__ void __launch_bounds__(BLOCK)
     k_visc_flux(DevMesh m, ViscParams vp, const S* __restrict__ prim, const S* __restrict__ grad,
                 S* __restrict__ flux, double* __restrict__ face_rho, double* __restrict__ face_T) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    const int F = m.F, N = m.N, Nt = m.Nt;
+    const int F = m.F;
     if (f >= F) return;
-    const int l = m.L[f], r = m.R[f];
-    const bool lv = l < N, rv = r < N;
-    S gT[3], gvx[3], gvy[3], gvz[3];
-    const double n[3] = {m.geom[f], m.geom[(size_t)F + f], m.geom[2 * (size_t)F + f]};
-    if (!lv || !rv) {  // copy_gradients_to_face, viscous_flux.cpp:16-34
-        const int c = lv ? l : r;
-        for (int k = 0; k < 3; k++) {
-            gT[k] = grad[((size_t)G_T * 3 + k) * N + c];
-            gvx[k] = grad[((size_t)G_VX * 3 + k) * N + c];
-            gvy[k] = grad[((size_t)G_VY * 3 + k) * N + c];
-            gvz[k] = grad[((size_t)G_VZ * 3 + k) * N + c];
-        }
-    } else {  // hasselbacher_average, viscous_flux.cpp:52-80
-        const double ex = m.cen[r] - m.cen[l];
-        const double ey = m.cen[Nt + r] - m.cen[Nt + l];
-        const double ez = m.cen[2 * (size_t)Nt + r] - m.cen[2 * (size_t)Nt + l];
-        const double len_e = sqrt(ex * ex + ey * ey + ez * ez);
-        const double ehat[3] = {ex / len_e, ey / len_e, ez / len_e};
-        const double edn = ehat[0] * n[0] + ehat[1] * n[1] + ehat[2] * n[2];
-        hasselbacher<S>(grad, N, G_VX, prim[(size_t)P_VX * Nt + l], prim[(size_t)P_VX * Nt + r], l,
-                        r, ehat, n, len_e, edn, gvx);
-        hasselbacher<S>(grad, N, G_VY, prim[(size_t)P_VY * Nt + l], prim[(size_t)P_VY * Nt + r], l,
-                        r, ehat, n, len_e, edn, gvy);
-        hasselbacher<S>(grad, N, G_VZ, prim[(size_t)P_VZ * Nt + l], prim[(size_t)P_VZ * Nt + r], l,
-                        r, ehat, n, len_e, edn, gvz);
-        hasselbacher<S>(grad, N, G_T, prim[(size_t)P_T * Nt + l], prim[(size_t)P_T * Nt + r], l, r,
-                        ehat, n, len_e, edn, gT);
-    }
-    // average_flow_states_pT + update_thermo_from_pT (GAS/flow_state.h:89-91)
-    const S p = 0.5 * (prim[(size_t)P_P * Nt + l] + prim[(size_t)P_P * Nt + r]);
-    const S temp = 0.5 * (prim[(size_t)P_T * Nt + l] + prim[(size_t)P_T * Nt + r]);
-    const S vx = 0.5 * (prim[(size_t)P_VX * Nt + l] + prim[(size_t)P_VX * Nt + r]);
-    const S vy = 0.5 * (prim[(size_t)P_VY * Nt + l] + prim[(size_t)P_VY * Nt + r]);
-    const S vz = 0.5 * (prim[(size_t)P_VZ * Nt + l] + prim[(size_t)P_VZ * Nt + r]);
+    S vt[DIM + 1];
+    double rho_f, T_f;
+    visc_face<S, DIM>(m, vp, prim, grad, f, m.L[f], m.R[f], vt, &rho_f, &T_f);
     if (face_rho) {
-        face_rho[f] = ibis::real_part(p / (vp.R * temp));
-        face_T[f] = ibis::real_part(temp);
+        face_rho[f] = rho_f;
+        face_T[f] = T_f;
     }
-    const S mu = sutherland<S>(temp, vp);
-    const S k = conductivity<S>(temp, vp);
-    const S lambda = -2.0 / 3.0 * mu;
-    const S bulk = lambda * (gvx[0] + gvy[1] + gvz[2]);
-    const S tau_xx = 2.0 * mu * gvx[0] + bulk;
-    const S tau_yy = 2.0 * mu * gvy[1] + bulk;
-    const S tau_zz = 2.0 * mu * gvz[2] + bulk;
-    const S tau_xy = mu * (gvx[1] + gvy[0]);
-    const S tau_xz = mu * (gvx[2] + gvz[0]);
-    const S tau_yz = mu * (gvy[2] + gvz[1]);
-    const S theta_x = vx * tau_xx + vy * tau_xy + vz * tau_xz + k * gT[0];
-    const S theta_y = vx * tau_xy + vy * tau_yy + vz * tau_yz + k * gT[1];
-    const S theta_z = vx * tau_xz + vy * tau_yz + vz * tau_zz + k * gT[2];
-    flux[(size_t)F + f] -= n[0] * tau_xx + n[1] * tau_xy + n[2] * tau_xz;
-    flux[2 * (size_t)F + f] -= n[0] * tau_xy + n[1] * tau_yy + n[2] * tau_yz;
-    if (DIM == 3) {
-        flux[3 * (size_t)F + f] -= n[0] * tau_xz + n[1] * tau_yz + n[2] * tau_zz;
-        flux[4 * (size_t)F + f] -= n[0] * theta_x + n[1] * theta_y + n[2] * theta_z;
-    } else {
-        flux[3 * (size_t)F + f] -= n[0] * theta_x + n[1] * theta_y + n[2] * theta_z;
-    }
+#pragma unroll
+    for (int q = 0; q <= DIM; q++) flux[(size_t)(q + 1) * F + f] -= vt[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -2069,6 +2259,9 @@ struct ibis_mesh {
     int ntags = 0;
     std::vector<int> b_in_tag_order;  // device index of the k-th boundary face in tag order
     std::vector<void*> allocs;
+    // tile tables of every tile size built so far (d holds those of TILE)
+    std::map<int, DevMesh> tiles;
+    std::mutex tiles_mu;
 };
 
 int ibis_mesh_ntags(const ibis_mesh* m) { return m ? m->ntags : 0; }
@@ -2081,6 +2274,135 @@ static ibis_status upload(ibis_mesh* m, const std::vector<T>& h, const T** out) 
     m->allocs.push_back(p);
     if (!h.empty()) HIP_TRY(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
     *out = static_cast<const T*>(p);
+    return IBIS_OK;
+}
+
+// Tile tables of one tile size ts: tiles of ts consecutive cells, the faces
+// touching each tile (ascending, and in the per-entry face kernel's order),
+// and each tile's stencil halo.
+struct TileHost {
+    int ts = 0, ntiles = 0, maxtf = 0, maxhalo = 0;
+    std::vector<int> toff, tfaces, hoff, hids, efaces;
+    std::vector<unsigned short> clds, nlds, elds;
+};
+
+// From the cells' face, neighbour and face-count lists (maxf x N, column
+// per slot) and each face's cells and its slot in their lists.
+static ibis_status build_tiles(int ts, int64_t N, int maxf, const std::vector<int>& ef,
+                               const std::vector<int>& en, const std::vector<unsigned char>& nf,
+                               const std::vector<int>& L, const std::vector<int>& R,
+                               const std::vector<unsigned char>& slotL,
+                               const std::vector<unsigned char>& slotR, TileHost* out) {
+    TileHost& T = *out;
+    T.ts = ts;
+    const int ntiles = (int)((N + ts - 1) / ts);
+    T.ntiles = ntiles;
+    // faces touching each tile, and for every (cell, face slot) the face's
+    // position in its tile's list
+    T.toff.assign(ntiles + 1, 0);
+    T.clds.assign((size_t)maxf * N, 0);
+    {
+        std::vector<int> fl;
+        for (int t = 0; t < ntiles; t++) {
+            fl.clear();
+            const int64_t c0 = (int64_t)t * ts, c1 = std::min<int64_t>(N, c0 + ts);
+            for (int64_t c = c0; c < c1; c++)
+                for (int j = 0; j < nf[c]; j++) fl.push_back(ef[(size_t)j * N + c]);
+            std::sort(fl.begin(), fl.end());
+            fl.erase(std::unique(fl.begin(), fl.end()), fl.end());
+            if (fl.size() > 65535)
+                return ibis::fail(IBIS_ERR_INVALID, "tile face list too large for 16-bit positions");
+            T.maxtf = std::max<int>(T.maxtf, (int)fl.size());
+            for (int64_t c = c0; c < c1; c++)
+                for (int j = 0; j < nf[c]; j++) {
+                    auto it = std::lower_bound(fl.begin(), fl.end(), ef[(size_t)j * N + c]);
+                    T.clds[(size_t)j * N + c] = (unsigned short)(it - fl.begin());
+                }
+            T.tfaces.insert(T.tfaces.end(), fl.begin(), fl.end());
+            T.toff[t + 1] = (int)T.tfaces.size();
+        }
+    }
+    // stencil halo of each tile: the neighbours (valid or ghost) of its
+    // cells that lie outside it, ascending; for every (cell, face slot) the
+    // neighbour's position in the tile's LDS stencil (own cells first, then
+    // the halo)
+    T.hoff.assign(ntiles + 1, 0);
+    T.nlds.assign((size_t)maxf * N, 0);
+    {
+        std::vector<int> h;
+        for (int t = 0; t < ntiles; t++) {
+            h.clear();
+            const int64_t c0 = (int64_t)t * ts, c1 = std::min<int64_t>(N, c0 + ts);
+            for (int64_t c = c0; c < c1; c++)
+                for (int j = 0; j < nf[c]; j++) {
+                    const int64_t n = en[(size_t)j * N + c];
+                    if (n < c0 || n >= c1) h.push_back((int)n);
+                }
+            std::sort(h.begin(), h.end());
+            h.erase(std::unique(h.begin(), h.end()), h.end());
+            T.maxhalo = std::max<int>(T.maxhalo, (int)h.size());
+            if (ts + (int64_t)h.size() > 65535)
+                return ibis::fail(IBIS_ERR_INVALID, "ibis_mesh_create: tile stencil too large");
+            for (int64_t c = c0; c < c1; c++)
+                for (int j = 0; j < nf[c]; j++) {
+                    const int64_t n = en[(size_t)j * N + c];
+                    const int64_t k = (n >= c0 && n < c1)
+                                          ? n - c0
+                                          : ts + (std::lower_bound(h.begin(), h.end(), (int)n) - h.begin());
+                    T.nlds[(size_t)j * N + c] = (unsigned short)k;
+                }
+            T.hids.insert(T.hids.end(), h.begin(), h.end());
+            T.hoff[t + 1] = (int)T.hids.size();
+        }
+    }
+    // entry order of the tile lists for the per-entry face kernel: within a
+    // tile, faces by (slot of the lower-numbered cell, slot of the other,
+    // lower cell), so that neighbouring lanes read neighbouring cells of one
+    // face-value plane on both sides whichever side is the face's left;
+    // elds is each (cell, slot)'s position in that order
+    T.efaces.assign(T.tfaces.size(), 0);
+    T.elds.assign((size_t)maxf * N, 0);
+    {
+        std::vector<int> idx, inv;
+        for (int t = 0; t < ntiles; t++) {
+            const int a = T.toff[t], n = T.toff[t + 1] - a;
+            idx.resize(n);
+            inv.resize(n);
+            std::iota(idx.begin(), idx.end(), 0);
+            auto key = [&](int k) {
+                const int f = T.tfaces[a + k];
+                const bool lo_is_left = L[f] < R[f];
+                return std::make_tuple(lo_is_left ? slotL[f] : slotR[f],
+                                       lo_is_left ? slotR[f] : slotL[f], std::min(L[f], R[f]), f);
+            };
+            std::sort(idx.begin(), idx.end(), [&](int x, int y) { return key(x) < key(y); });
+            for (int k = 0; k < n; k++) {
+                T.efaces[a + k] = T.tfaces[a + idx[k]];
+                inv[idx[k]] = k;
+            }
+            const int64_t c0 = (int64_t)t * ts, c1 = std::min<int64_t>(N, c0 + ts);
+            for (int64_t c = c0; c < c1; c++)
+                for (int j = 0; j < nf[c]; j++)
+                    T.elds[(size_t)j * N + c] = (unsigned short)inv[T.clds[(size_t)j * N + c]];
+        }
+    }
+    return IBIS_OK;
+}
+
+static ibis_status upload_tiles(ibis_mesh* m, const TileHost& T, DevMesh* D) {
+    IBIS_TRY(upload(m, T.toff, &D->tile_face_off));
+    IBIS_TRY(upload(m, T.tfaces, &D->tile_faces));
+    IBIS_TRY(upload(m, T.clds, &D->cell_lds));
+    IBIS_TRY(upload(m, T.efaces, &D->ent_faces));
+    IBIS_TRY(upload(m, T.elds, &D->ent_lds));
+    IBIS_TRY(upload(m, T.hoff, &D->halo_off));
+    IBIS_TRY(upload(m, T.hids, &D->halo_ids));
+    IBIS_TRY(upload(m, T.nlds, &D->nbr_lds));
+    D->tile = T.ts;
+    D->ntiles = T.ntiles;
+    D->maxtf = T.maxtf;
+    D->ntf = T.toff[T.ntiles];
+    D->maxhalo = T.maxhalo;
     return IBIS_OK;
 }
 
@@ -2208,70 +2530,6 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
             es[(size_t)j * N + c] = (signed char)d->cell_outsigns[o + j];
         }
     }
-    // tiles of TILE consecutive cells: faces touching each tile, and for
-    // every (cell, face slot) the face's position in its tile's list
-    const int ntiles = (int)((N + TILE - 1) / TILE);
-    std::vector<int> toff(ntiles + 1, 0), tfaces;
-    std::vector<unsigned short> clds((size_t)maxf * N, 0);
-    int maxtf = 0;
-    {
-        std::vector<int> fl;
-        for (int t = 0; t < ntiles; t++) {
-            fl.clear();
-            const int64_t c0 = (int64_t)t * TILE, c1 = std::min<int64_t>(N, c0 + TILE);
-            for (int64_t c = c0; c < c1; c++)
-                for (int64_t j = d->cell_face_off[c]; j < d->cell_face_off[c + 1]; j++)
-                    fl.push_back((int)d->cell_faces[j]);
-            std::sort(fl.begin(), fl.end());
-            fl.erase(std::unique(fl.begin(), fl.end()), fl.end());
-            maxtf = std::max<int>(maxtf, (int)fl.size());
-            for (int64_t c = c0; c < c1; c++) {
-                const int64_t o = d->cell_face_off[c];
-                for (int64_t j = o; j < d->cell_face_off[c + 1]; j++) {
-                    auto it = std::lower_bound(fl.begin(), fl.end(), (int)d->cell_faces[j]);
-                    clds[(size_t)(j - o) * N + c] = (unsigned short)(it - fl.begin());
-                }
-            }
-            tfaces.insert(tfaces.end(), fl.begin(), fl.end());
-            toff[t + 1] = (int)tfaces.size();
-        }
-    }
-    // stencil halo of each tile: the neighbours (valid or ghost) of its
-    // cells that lie outside it, ascending; for every (cell, face slot) the
-    // neighbour's position in the tile's LDS stencil (own cells first, then
-    // the halo)
-    std::vector<int> hoff(ntiles + 1, 0), hids;
-    std::vector<unsigned short> nlds((size_t)maxf * N, 0);
-    int maxhalo = 0;
-    {
-        std::vector<int> h;
-        for (int t = 0; t < ntiles; t++) {
-            h.clear();
-            const int64_t c0 = (int64_t)t * TILE, c1 = std::min<int64_t>(N, c0 + TILE);
-            for (int64_t c = c0; c < c1; c++)
-                for (int64_t j = d->cell_face_off[c]; j < d->cell_face_off[c + 1]; j++) {
-                    const int64_t n = d->cell_neighbours[j];
-                    if (n < c0 || n >= c1) h.push_back((int)n);
-                }
-            std::sort(h.begin(), h.end());
-            h.erase(std::unique(h.begin(), h.end()), h.end());
-            maxhalo = std::max<int>(maxhalo, (int)h.size());
-            if (TILE + (int64_t)h.size() > 65535)
-                return ibis::fail(IBIS_ERR_INVALID, "ibis_mesh_create: tile stencil too large");
-            for (int64_t c = c0; c < c1; c++) {
-                const int64_t o = d->cell_face_off[c];
-                for (int64_t j = o; j < d->cell_face_off[c + 1]; j++) {
-                    const int64_t n = d->cell_neighbours[j];
-                    const int64_t k = (n >= c0 && n < c1)
-                                          ? n - c0
-                                          : TILE + (std::lower_bound(h.begin(), h.end(), (int)n) - h.begin());
-                    nlds[(size_t)(j - o) * N + c] = (unsigned short)k;
-                }
-            }
-            hids.insert(hids.end(), h.begin(), h.end());
-            hoff[t + 1] = (int)hids.size();
-        }
-    }
     std::vector<unsigned char> slotL(F, 0), slotR(F, 0);
     for (int64_t c = 0; c < N; c++)
         for (int64_t j = d->cell_face_off[c]; j < d->cell_face_off[c + 1]; j++) {
@@ -2281,36 +2539,12 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
             else
                 slotR[f] = (unsigned char)(j - d->cell_face_off[c]);
         }
-    // entry order of the tile lists for the per-entry face kernel: within a
-    // tile, faces by (slot of the lower-numbered cell, slot of the other,
-    // lower cell), so that neighbouring lanes read neighbouring cells of one
-    // face-value plane on both sides whichever side is the face's left;
-    // ent_lds is each (cell, slot)'s position in that order
-    std::vector<int> efaces(tfaces.size());
-    std::vector<unsigned short> elds((size_t)maxf * N, 0);
+    TileHost tiles;
     {
-        std::vector<int> idx, inv;
-        for (int t = 0; t < ntiles; t++) {
-            const int a = toff[t], n = toff[t + 1] - a;
-            idx.resize(n);
-            inv.resize(n);
-            std::iota(idx.begin(), idx.end(), 0);
-            auto key = [&](int k) {
-                const int f = tfaces[a + k];
-                const bool lo_is_left = d->face_left[f] < d->face_right[f];
-                return std::make_tuple(lo_is_left ? slotL[f] : slotR[f],
-                                       lo_is_left ? slotR[f] : slotL[f],
-                                       std::min(d->face_left[f], d->face_right[f]), f);
-            };
-            std::sort(idx.begin(), idx.end(), [&](int x, int y) { return key(x) < key(y); });
-            for (int k = 0; k < n; k++) {
-                efaces[a + k] = tfaces[a + idx[k]];
-                inv[idx[k]] = k;
-            }
-            const int64_t c0 = (int64_t)t * TILE, c1 = std::min<int64_t>(N, c0 + TILE);
-            for (int64_t c = c0; c < c1; c++)
-                for (int64_t j = 0; j < d->cell_face_off[c + 1] - d->cell_face_off[c]; j++)
-                    elds[(size_t)j * N + c] = (unsigned short)inv[clds[(size_t)j * N + c]];
+        const ibis_status ts = build_tiles(TILE, N, maxf, ef, en, nf, L, R, slotL, slotR, &tiles);
+        if (ts != IBIS_OK) {
+            delete m;
+            return ts;
         }
     }
     int n_from_t = 1;
@@ -2358,24 +2592,59 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
     UP(b_valid, b_valid);
     UP(b_tag, b_tag);
     UP(b_of_ghost, b_of_ghost);
-    UP(toff, tile_face_off);
-    UP(tfaces, tile_faces);
-    UP(clds, cell_lds);
-    UP(efaces, ent_faces);
-    UP(elds, ent_lds);
-    UP(hoff, halo_off);
-    UP(hids, halo_ids);
-    UP(nlds, nbr_lds);
     UP(slotL, slotL);
     UP(slotR, slotR);
 #undef UP
+    if ((s = upload_tiles(m, tiles, &D)) != IBIS_OK) {
+        ibis_mesh_destroy(m);
+        return s;
+    }
+    m->tiles[TILE] = D;
     D.n_from_t = n_from_t;
     D.planar2d = planar2d;
-    D.ntiles = ntiles;
-    D.maxtf = maxtf;
-    D.ntf = toff[ntiles];
-    D.maxhalo = maxhalo;
     *out = m;
+    return IBIS_OK;
+}
+
+// The tile fields of a tile size: built on first use from the device copies
+// of the cell lists (the mesh keeps every size it has built until it is
+// destroyed), then copied into *D.
+static ibis_status mesh_tiles(const ibis_mesh* mc, int ts, DevMesh* D) {
+    ibis_mesh* m = const_cast<ibis_mesh*>(mc);
+    std::lock_guard<std::mutex> lock(m->tiles_mu);
+    auto it = m->tiles.find(ts);
+    if (it == m->tiles.end()) {
+        const DevMesh& M = m->d;
+        const size_t N = M.N, F = M.F, E = (size_t)M.maxf * N;
+        std::vector<int> ef(E), en(E), L(F), R(F);
+        std::vector<unsigned char> nf(N), slotL(F), slotR(F);
+        HIP_TRY(hipMemcpy(ef.data(), M.ell_face, E * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(en.data(), M.ell_nbr, E * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(nf.data(), M.nfaces, N, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(L.data(), M.L, F * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(R.data(), M.R, F * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(slotL.data(), M.slotL, F, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(slotR.data(), M.slotR, F, hipMemcpyDeviceToHost));
+        TileHost T;
+        IBIS_TRY(build_tiles(ts, (int64_t)N, M.maxf, ef, en, nf, L, R, slotL, slotR, &T));
+        DevMesh t = M;
+        IBIS_TRY(upload_tiles(m, T, &t));
+        it = m->tiles.emplace(ts, t).first;
+    }
+    const DevMesh& t = it->second;
+    D->tile = t.tile;
+    D->ntiles = t.ntiles;
+    D->maxtf = t.maxtf;
+    D->ntf = t.ntf;
+    D->tile_face_off = t.tile_face_off;
+    D->tile_faces = t.tile_faces;
+    D->cell_lds = t.cell_lds;
+    D->ent_faces = t.ent_faces;
+    D->ent_lds = t.ent_lds;
+    D->maxhalo = t.maxhalo;
+    D->halo_off = t.halo_off;
+    D->halo_ids = t.halo_ids;
+    D->nbr_lds = t.nbr_lds;
     return IBIS_OK;
 }
 
@@ -2388,6 +2657,7 @@ ibis_status ibis_mesh_from_grid(const ibis_grid* g, ibis_mesh** out) {
 
 struct ibis_residual {
     const ibis_mesh* m = nullptr;
+    DevMesh d;  // the mesh's device view with the tile tables of option "tile"
     ibis_fv_desc desc;
     bool premul = false;
     double* grad = nullptr;
@@ -2410,6 +2680,8 @@ struct ibis_residual {
     int opt_stage_cells = 0;    // option stage_cells (layout 2, pack_cells 0)
     int opt_keep_grads = 0;     // option keep_gradients: the fused path writes the
                                 // gradient and limiter slots too
+    int opt_fuse_visc = 1;      // option fuse_viscous: the three-kernel viscous path
+    double* gconv = nullptr;    // viscous: the ghosts' reconstruction variables (5 x G)
     int opt_face_kernel = 0; // option face_kernel: 0 tile face lists, 1 the same with
                              // per-entry face data, 64/128/256 slot-major
     int2* ent_idx = nullptr;     // per tile-list entry face data (face_kernel 1)
@@ -2457,7 +2729,7 @@ uint64_t ibis_residual_generation(const ibis_residual* r) { return r ? r->genera
 
 void ibis_residual_destroy(ibis_residual* r) {
     if (!r) return;
-    for (void* p : {(void*)r->grad, (void*)r->lim, (void*)r->flux, (void*)r->pack, (void*)r->FL, (void*)r->FR, (void*)r->offs, (void*)r->ent_idx, (void*)r->ent_geom, r->ent_geom32, (void*)r->offs32, (void*)r->face_rho,
+    for (void* p : {(void*)r->grad, (void*)r->lim, (void*)r->flux, (void*)r->pack, (void*)r->FL, (void*)r->FR, (void*)r->offs, (void*)r->ent_idx, (void*)r->ent_geom, r->ent_geom32, (void*)r->offs32, (void*)r->gconv, (void*)r->face_rho,
                     (void*)r->face_T, (void*)r->kinds, (void*)r->pars, (void*)r->partial,
                     (void*)r->dt_dev, (void*)r->count_dev, (void*)r->k_tmp, (void*)r->flow_tmp,
                     (void*)r->dual_prim, (void*)r->dual_dudt, (void*)r->dual_grad,
@@ -2527,6 +2799,7 @@ ibis_status ibis_residual_create_tables(const ibis_mesh* m, const ibis_fv_desc* 
     const DevMesh& D = m->d;
     ibis_residual* r = new ibis_residual();
     r->m = m;
+    r->d = m->d;
     r->desc = d;
     // the unfused path keeps the flux unmultiplied (the viscous kernel adds to it)
     r->premul = false;
@@ -2554,6 +2827,7 @@ ibis_status ibis_residual_create_tables(const ibis_mesh* m, const ibis_fv_desc* 
     if (d.viscous) {
         AL(r->face_rho, sizeof(double) * F);
         AL(r->face_T, sizeof(double) * F);
+        AL(r->gconv, sizeof(double) * 5 * std::max<size_t>(D.G, 1));
     }
     r->ntab = ntab;
     AL(r->kinds, sizeof(int) * 3 * ntab);
@@ -2590,7 +2864,7 @@ ibis_status ibis_residual_create_tables(const ibis_mesh* m, const ibis_fv_desc* 
     // the fused second-order inviscid path starts from the options that the
     // autotuner picks on the 3D and 2D workloads (every choice gives the
     // same bits; ibis_residual_set_option changes them)
-    if (d.order == 2 && !d.viscous && !r->has_post) {
+    if (d.order == 2 && !r->has_post) {
         // face_kernel 1 indexes the face values with 32-bit entries; a mesh
         // too large for them keeps the tile-list kernel (face_kernel 0)
         const int fk = ibis_entry_indices_fit(D.maxf, D.N, D.G) ? 1 : 0;
@@ -2630,9 +2904,9 @@ void launch_face_tile_f(const DevMesh& D, const FaceParams& fp, bool recon, cons
     const unsigned g = xcd_grid(D.ntiles);
     const size_t shm = (size_t)(DIM + 2) * D.maxtf * sizeof(double);
     if (recon)
-        k_face_tile<DIM, FLUX, true><<<g, TILE, shm, st>>>(D, fp, prim, grad, lim, dudt);
+        k_face_tile<DIM, FLUX, true><<<g, D.tile, shm, st>>>(D, fp, prim, grad, lim, dudt);
     else
-        k_face_tile<DIM, FLUX, false><<<g, TILE, shm, st>>>(D, fp, prim, grad, lim, dudt);
+        k_face_tile<DIM, FLUX, false><<<g, D.tile, shm, st>>>(D, fp, prim, grad, lim, dudt);
 }
 
 template <int DIM>
@@ -2696,7 +2970,7 @@ void launch_face_flux(const DevMesh& D, int flux_kind, const FaceParams& fp, boo
 // thread, only with a limiter (two passes over the faces) and only while
 // the block's share stays within 64 KiB (maxf <= 10); 0 = not staged
 size_t cell_stage_bytes(const DevMesh& D, int lim, int opt) {
-    const size_t b = (size_t)D.maxf * 3 * BLOCK * sizeof(double);
+    const size_t b = (size_t)D.maxf * 3 * D.tile * sizeof(double);
     return (opt && lim != 0 && b <= 65536) ? b : 0;
 }
 
@@ -2707,52 +2981,52 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
     const int stage = shm ? 1 : 0;
     // the LDS cell stencil (option stage_cells), in all precisions; the
     // FP64 and FP32-flux kernels reuse it for the offsets (stage_offsets)
-    const size_t shs0 = (size_t)8 * (TILE + D.maxhalo) * sizeof(double);
+    const size_t shs0 = (size_t)8 * (D.tile + D.maxhalo) * sizeof(double);
     const bool sn = r->opt_stage_cells && shs0 <= 65536;
     const size_t shs = std::max(shs0, shm);
     if (r->prec == IBIS_PREC_FP32_RECON) {
         float* fl = reinterpret_cast<float*>(r->FL);
         if (sn)
-            k_cell_fs32<DIM, LIM, OFFS, true><<<xcd_grid(nlog), BLOCK, std::max(shs0, shm / 2), st>>>(
+            k_cell_fs32<DIM, LIM, OFFS, true><<<xcd_grid(nlog), D.tile, std::max(shs0, shm / 2), st>>>(
                 D, vs, prim, r->grad, d.lim_eps, d.venkat_K, nlog, fl, r->offs, stage, r->offs32);
         else
-            k_cell_fs32<DIM, LIM, OFFS><<<xcd_grid(nlog), BLOCK, shm / 2, st>>>(
+            k_cell_fs32<DIM, LIM, OFFS><<<xcd_grid(nlog), D.tile, shm / 2, st>>>(
                 D, vs, prim, r->grad, d.lim_eps, d.venkat_K, nlog, fl, r->offs, stage, r->offs32);
         return;
     }
     if (r->prec == IBIS_PREC_FP32_FLUX) {
         float* fl = reinterpret_cast<float*>(r->FL);
         if (sn)
-            k_cell_fs<DIM, LIM, 2, OFFS, float, false, false, true><<<xcd_grid(nlog), BLOCK, shs, st>>>(
+            k_cell_fs<DIM, LIM, 2, OFFS, float, false, false, true><<<xcd_grid(nlog), D.tile, shs, st>>>(
                 D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, fl, fl, r->offs,
-                prim, stage);
+                prim, stage, nullptr);
         else
-            k_cell_fs<DIM, LIM, 2, OFFS, float, false><<<xcd_grid(nlog), BLOCK, shm, st>>>(
+            k_cell_fs<DIM, LIM, 2, OFFS, float, false><<<xcd_grid(nlog), D.tile, shm, st>>>(
                 D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, fl, fl, r->offs,
-                prim, stage);
+                prim, stage, nullptr);
         return;
     }
     if (!r->opt_pack && LAYOUT == 2) {
         if (sn) {
             k_cell_fs<DIM, LIM, 2, OFFS, double, false, false, true>
-                <<<xcd_grid(nlog), BLOCK, shs, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps,
+                <<<xcd_grid(nlog), D.tile, shs, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps,
                                                      d.venkat_K, nlog, r->opt_keep_grads, r->FL, r->FR, r->offs,
-                                                     prim, stage);
+                                                     prim, stage, nullptr);
             return;
         }
         if (r->opt_cell_prefetch)
-            k_cell_fs<DIM, LIM, 2, OFFS, double, false, true><<<xcd_grid(nlog), BLOCK, shm, st>>>(
+            k_cell_fs<DIM, LIM, 2, OFFS, double, false, true><<<xcd_grid(nlog), D.tile, shm, st>>>(
                 D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, r->FL, r->FR,
-                r->offs, prim, stage);
+                r->offs, prim, stage, nullptr);
         else
-            k_cell_fs<DIM, LIM, 2, OFFS, double, false><<<xcd_grid(nlog), BLOCK, shm, st>>>(
+            k_cell_fs<DIM, LIM, 2, OFFS, double, false><<<xcd_grid(nlog), D.tile, shm, st>>>(
                 D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, r->FL, r->FR,
-                r->offs, prim, stage);
+                r->offs, prim, stage, nullptr);
         return;
     }
-    k_cell_fs<DIM, LIM, LAYOUT, OFFS><<<xcd_grid(nlog), BLOCK, shm, st>>>(
+    k_cell_fs<DIM, LIM, LAYOUT, OFFS><<<xcd_grid(nlog), D.tile, shm, st>>>(
         D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, r->FL, r->FR, r->offs,
-        prim, stage);
+        prim, stage, nullptr);
 }
 
 template <int DIM, int LIM>
@@ -2811,40 +3085,40 @@ void launch_face_fs_f(const DevMesh& D, const FaceParams& fp, int layout, bool f
     if (f32) {
         const float* fl = reinterpret_cast<const float*>(FL);
         if (face_kernel == 1 && D.n_from_t && DIM == 2 && D.planar2d)
-            k_face_entry_fs<DIM, FLUX, false, float, true><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, fl, dudt,
-                                                                                 ibis_stage_epilogue{});
+            k_face_entry_fs<DIM, FLUX, false, float, true><<<g, D.tile, shm, st>>>(D, fp, eidx, egeom, fl, dudt,
+                                                                                 ibis_stage_epilogue{}, ViscFace{});
         else if (face_kernel == 1 && D.n_from_t && eg32)
-            k_face_entry_fs<DIM, FLUX, false, float, false, true><<<g, TILE, shm, st>>>(
-                D, fp, eidx, reinterpret_cast<const double*>(eg32), fl, dudt, ibis_stage_epilogue{});
+            k_face_entry_fs<DIM, FLUX, false, float, false, true><<<g, D.tile, shm, st>>>(
+                D, fp, eidx, reinterpret_cast<const double*>(eg32), fl, dudt, ibis_stage_epilogue{}, ViscFace{});
         else if (face_kernel == 1 && D.n_from_t)
-            k_face_entry_fs<DIM, FLUX, false, float><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, fl, dudt,
-                                                                          ibis_stage_epilogue{});
+            k_face_entry_fs<DIM, FLUX, false, float><<<g, D.tile, shm, st>>>(D, fp, eidx, egeom, fl, dudt,
+                                                                          ibis_stage_epilogue{}, ViscFace{});
         else if (face_kernel == 1)
-            k_face_entry_fs<DIM, FLUX, true, float><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, fl, dudt,
-                                                                         ibis_stage_epilogue{});
+            k_face_entry_fs<DIM, FLUX, true, float><<<g, D.tile, shm, st>>>(D, fp, eidx, egeom, fl, dudt,
+                                                                         ibis_stage_epilogue{}, ViscFace{});
         else if (face_kernel)
             launch_face_slot_ts<DIM, FLUX, float>(D, fp, face_kernel, fl, dudt, st);
         else
-            k_face_tile_fs<DIM, FLUX, 2, float><<<g, TILE, shm, st>>>(D, fp, fl, fl, dudt);
+            k_face_tile_fs<DIM, FLUX, 2, float><<<g, D.tile, shm, st>>>(D, fp, fl, fl, dudt);
         return;
     }
     if (layout == 2 && face_kernel == 1) {
         const ibis_stage_epilogue none{};
         if (D.n_from_t && DIM == 2 && D.planar2d)
-            k_face_entry_fs<DIM, FLUX, false, double, true><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, FL, dudt,
-                                                                                  none);
+            k_face_entry_fs<DIM, FLUX, false, double, true><<<g, D.tile, shm, st>>>(D, fp, eidx, egeom, FL, dudt,
+                                                                                  none, ViscFace{});
         else if (D.n_from_t)
-            k_face_entry_fs<DIM, FLUX, false><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, FL, dudt, none);
+            k_face_entry_fs<DIM, FLUX, false><<<g, D.tile, shm, st>>>(D, fp, eidx, egeom, FL, dudt, none, ViscFace{});
         else
-            k_face_entry_fs<DIM, FLUX, true><<<g, TILE, shm, st>>>(D, fp, eidx, egeom, FL, dudt, none);
+            k_face_entry_fs<DIM, FLUX, true><<<g, D.tile, shm, st>>>(D, fp, eidx, egeom, FL, dudt, none, ViscFace{});
     } else if (layout == 2 && face_kernel) {
         launch_face_slot_ts<DIM, FLUX, double>(D, fp, face_kernel, FL, dudt, st);
     } else if (layout == 0) {
-        k_face_tile_fs<DIM, FLUX, 0><<<g, TILE, shm, st>>>(D, fp, FL, FR, dudt);
+        k_face_tile_fs<DIM, FLUX, 0><<<g, D.tile, shm, st>>>(D, fp, FL, FR, dudt);
     } else if (layout == 1) {
-        k_face_tile_fs<DIM, FLUX, 1><<<g, TILE, shm, st>>>(D, fp, FL, FR, dudt);
+        k_face_tile_fs<DIM, FLUX, 1><<<g, D.tile, shm, st>>>(D, fp, FL, FR, dudt);
     } else {
-        k_face_tile_fs<DIM, FLUX, 2><<<g, TILE, shm, st>>>(D, fp, FL, FR, dudt);
+        k_face_tile_fs<DIM, FLUX, 2><<<g, D.tile, shm, st>>>(D, fp, FL, FR, dudt);
     }
 }
 
@@ -2857,11 +3131,11 @@ void launch_face_epi_f(const DevMesh& D, const FaceParams& fp, const int2* eidx,
     const unsigned g = xcd_grid(D.ntiles);
     const size_t shm = (size_t)(DIM + 2) * D.maxtf * sizeof(double);
     if (DIM == 2 && D.planar2d)
-        k_face_entry_fs<DIM, FLUX, false, double, true, false, true><<<g, TILE, shm, st>>>(
-            D, fp, eidx, egeom, FL, dudt, epi);
+        k_face_entry_fs<DIM, FLUX, false, double, true, false, true><<<g, D.tile, shm, st>>>(
+            D, fp, eidx, egeom, FL, dudt, epi, ViscFace{});
     else
-        k_face_entry_fs<DIM, FLUX, false, double, false, false, true><<<g, TILE, shm, st>>>(
-            D, fp, eidx, egeom, FL, dudt, epi);
+        k_face_entry_fs<DIM, FLUX, false, double, false, false, true><<<g, D.tile, shm, st>>>(
+            D, fp, eidx, egeom, FL, dudt, epi, ViscFace{});
 }
 
 template <int DIM>
@@ -2898,6 +3172,69 @@ void launch_face_fs(const DevMesh& D, const FaceParams& fp, int flux, int layout
         default: IBIS_FF(IBIS_FLUX_RUSANOV); break;
     }
 #undef IBIS_FF
+}
+
+// the fused viscous path's cell kernel (12-plane stencil in LDS)
+size_t visc_stencil_bytes(const DevMesh& D) {
+    return (size_t)12 * (D.tile + D.maxhalo) * sizeof(double);
+}
+
+template <int DIM, int LIM>
+void launch_cell_visc_l(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
+                        ibis_residual* r, const double* prim, hipStream_t st) {
+    const size_t shm = cell_stage_bytes(D, LIM, r->opt_stage);
+    const int stage = shm ? 1 : 0;
+    const size_t shs = std::max(visc_stencil_bytes(D), shm);
+    k_cell_fs<DIM, LIM, 2, true, double, false, false, true, true>
+        <<<xcd_grid(D.ntiles), D.tile, shs, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps,
+                                                  d.venkat_K, D.ntiles, r->opt_keep_grads, r->FL,
+                                                  r->FR, r->offs, prim, stage, r->gconv);
+}
+
+template <int DIM>
+void launch_cell_visc(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d, ibis_residual* r,
+                      const double* prim, hipStream_t st) {
+    if (d.limiter == 0)
+        launch_cell_visc_l<DIM, 0>(D, vs, d, r, prim, st);
+    else if (d.limiter == 1)
+        launch_cell_visc_l<DIM, 1>(D, vs, d, r, prim, st);
+    else
+        launch_cell_visc_l<DIM, 2>(D, vs, d, r, prim, st);
+}
+
+// the fused viscous path's face kernel (per-entry face data, FP64)
+template <int DIM, int FLUX>
+void launch_face_visc_f(const DevMesh& D, const FaceParams& fp, const int2* eidx,
+                        const double* egeom, const double* FL, double* dudt, const ViscFace& vf,
+                        hipStream_t st) {
+    const unsigned g = xcd_grid(D.ntiles);
+    const size_t shm = (size_t)(DIM + 2) * D.maxtf * sizeof(double);
+    if (DIM == 2 && D.planar2d)
+        k_face_entry_fs<DIM, FLUX, false, double, true, false, false, true><<<g, D.tile, shm, st>>>(
+            D, fp, eidx, egeom, FL, dudt, ibis_stage_epilogue{}, vf);
+    else
+        k_face_entry_fs<DIM, FLUX, false, double, false, false, false, true><<<g, D.tile, shm, st>>>(
+            D, fp, eidx, egeom, FL, dudt, ibis_stage_epilogue{}, vf);
+}
+
+template <int DIM>
+void launch_face_visc(const DevMesh& D, const FaceParams& fp, int flux, const int2* eidx,
+                      const double* egeom, const double* FL, double* dudt, const ViscFace& vf,
+                      hipStream_t st) {
+    switch (flux) {
+        case IBIS_FLUX_HANEL:
+            launch_face_visc_f<DIM, IBIS_FLUX_HANEL>(D, fp, eidx, egeom, FL, dudt, vf, st);
+            break;
+        case IBIS_FLUX_AUSMDV:
+            launch_face_visc_f<DIM, IBIS_FLUX_AUSMDV>(D, fp, eidx, egeom, FL, dudt, vf, st);
+            break;
+        case IBIS_FLUX_LDFSS2:
+            launch_face_visc_f<DIM, IBIS_FLUX_LDFSS2>(D, fp, eidx, egeom, FL, dudt, vf, st);
+            break;
+        default:
+            launch_face_visc_f<DIM, IBIS_FLUX_RUSANOV>(D, fp, eidx, egeom, FL, dudt, vf, st);
+            break;
+    }
 }
 
 // thermo reconstruction pair: (slot, prim row) of the two variables,
@@ -2963,7 +3300,7 @@ void residual_params(const ibis_residual* r, FaceParams* fp, VarSet* vs) {
     fp->gamma = d.gamma;
     fp->delta = d.ldfss_delta;
     vs->n = 5;
-    vs->ngrad = r->m->d.dim == 3 ? 5 : 4;
+    vs->ngrad = r->d.dim == 3 ? 5 : 4;
     vs->row[0] = fp->rowA;
     vs->slot[0] = fp->slotA;
     vs->row[1] = fp->rowB;
@@ -2982,7 +3319,7 @@ void residual_params(const ibis_residual* r, FaceParams* fp, VarSet* vs) {
 template <class S>
 void run_general(ibis_residual* r, const GenBufs<S>& B, S* prim, S* dudt, bool recon,
                  hipStream_t st, Prof& pf) {
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     const ibis_fv_desc& d = r->desc;
     const size_t ntab = r->ntab;
     const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
@@ -3076,7 +3413,7 @@ ibis_status compute_dudt_impl(ibis_residual* r, double* prim, double* dudt,
                               int allow_reconstruction, ibis_stream_t stream,
                               const ibis_stage_epilogue* epi) {
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     const ibis_fv_desc& d = r->desc;
     const bool recon = allow_reconstruction && d.order == 2;
     Prof pf{r, st};
@@ -3088,6 +3425,39 @@ ibis_status compute_dudt_impl(ibis_residual* r, double* prim, double* dudt,
         return ibis::fail(IBIS_ERR_INVALID,
                           "FP32 flux mode needs the inviscid second-order residual without "
                           "flux-override boundaries");
+    // viscous, second order: boundary actions (both passes), cell kernel
+    // (viscous gradients, convective gradients, limiter, face values), face
+    // kernel (convective and viscous flux, surface integral)
+    const bool visc_fused = d.viscous && recon && !r->has_post && !epi && r->opt_fuse_visc &&
+                            r->opt_layout == 2 && r->opt_face_kernel == 1 && r->ent_idx &&
+                            r->offs && r->opt_offsets && !r->opt_pack && r->opt_stage_cells &&
+                            D.n_from_t && visc_stencil_bytes(D) <= 65536 && vs.row[2] == P_VX &&
+                            vs.row[3] == P_VY && vs.row[4] == P_VZ;
+    if (visc_fused) {
+        const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
+        pf.beg();
+        if (D.nb > 0)
+            k_bc_visc_fs<<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, r->ntab,
+                                                          r->has_pre ? 1 : 0, r->has_visc ? 1 : 0,
+                                                          gp, vs, prim, r->FL, r->gconv);
+        pf.end(0);
+        pf.beg();
+        if (D.dim == 3)
+            launch_cell_visc<3>(D, vs, d, r, prim, st);
+        else
+            launch_cell_visc<2>(D, vs, d, r, prim, st);
+        pf.end(1);
+        pf.beg();
+        const ViscFace vf{ViscParams{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr}, prim, r->grad,
+                          r->face_rho, r->face_T};
+        if (D.dim == 3)
+            launch_face_visc<3>(D, fp, d.flux, r->ent_idx, r->ent_geom, r->FL, dudt, vf, st);
+        else
+            launch_face_visc<2>(D, fp, d.flux, r->ent_idx, r->ent_geom, r->FL, dudt, vf, st);
+        pf.end(2);
+        HIP_TRY(hipGetLastError());
+        return IBIS_OK;
+    }
     if (fused) {
         const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
         if (r->has_pre && !recon) {
@@ -3130,7 +3500,7 @@ ibis_status compute_dudt_impl(ibis_residual* r, double* prim, double* dudt,
                 r->prec == IBIS_PREC_FP64 && (r->opt_pack || r->opt_layout != 2);
             if (packed) k_pack<<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, vs, prim, r->pack);
             pf.end(0);
-            const int nlog = (D.N + BLOCK - 1) / BLOCK;
+            const int nlog = D.ntiles;  // one block per tile
             pf.beg();
             if (D.dim == 3)
                 launch_cell_fs<3>(D, vs, d, r, nlog, prim, st);
@@ -3180,7 +3550,7 @@ ibis_status ibis_compute_dudt(ibis_residual* r, double* prim, double* dudt,
 
 int ibis_residual_stage_fusable(const ibis_residual* r) {
     if (!r) return 0;
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     const ibis_fv_desc& d = r->desc;
     return (r->prec == IBIS_PREC_FP64 && !d.viscous && !r->has_post && d.order == 2 &&
             r->opt_layout == 2 && r->opt_face_kernel == 1 && r->ent_idx && D.n_from_t)
@@ -3204,7 +3574,7 @@ ibis_status ibis_compute_dudt_stage(ibis_residual* r, double* prim, double* dudt
 // tangent state and scratch, allocated (and zeroed) on first use
 static hipError_t ensure_dual(ibis_residual* r) {
     if (r->dual_prim) return hipSuccess;
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     const size_t N = D.N, F = D.F, Nt = D.Nt;
     const int nc = D.dim + 2;
     const size_t sz = sizeof(ibis::Dual);
@@ -3235,7 +3605,7 @@ ibis_status ibis_compute_dudt_jvp(ibis_residual* r, double* prim, double* prim_d
     if (!r || !prim || !prim_dot || !dudt || !dudt_dot)
         return ibis::fail(IBIS_ERR_INVALID, "ibis_compute_dudt_jvp: null argument");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     const ibis_fv_desc& d = r->desc;
     HIP_TRY(ensure_dual(r));
     const long np = (long)NPRIM * D.Nt;
@@ -3260,7 +3630,7 @@ ibis_status ibis_steady_matvec(ibis_residual* r, const double* cons, const doubl
         return ibis::fail(IBIS_ERR_INVALID, "ibis_steady_matvec: null argument");
     if (!(dt_star > 0.0)) return ibis::fail(IBIS_ERR_INVALID, "ibis_steady_matvec: dt_star <= 0");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     const ibis_fv_desc& d = r->desc;
     HIP_TRY(ensure_dual(r));
     const int nc = D.dim + 2;
@@ -3287,7 +3657,7 @@ ibis_status ibis_steady_rhs(ibis_residual* r, double* prim, double* dudt, double
     if (!r || !prim || !dudt || !rhs) return ibis::fail(IBIS_ERR_INVALID, "ibis_steady_rhs: null argument");
     ibis_status s = ibis_compute_dudt(r, prim, dudt, allow_reconstruction, stream);
     if (s != IBIS_OK) return s;
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     k_rhs_vector<<<nblocks(D.N), BLOCK, 0, st>>>(D.N, D.Nt, D.dim + 2, dudt, rhs);
     HIP_TRY(hipGetLastError());
@@ -3299,7 +3669,7 @@ ibis_status ibis_cons_to_prim_jvp(ibis_residual* r, const double* cons, const do
     if (!r || !cons || !cons_dot || !prim || !prim_dot)
         return ibis::fail(IBIS_ERR_INVALID, "ibis_cons_to_prim_jvp: null argument");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     HIP_TRY(ensure_dual(r));
     const int nc = D.dim + 2;
     const long ncv = (long)nc * D.Nt;
@@ -3320,7 +3690,7 @@ ibis_status ibis_residual_set_precision(ibis_residual* r, int mode) {
     if (mode != IBIS_PREC_FP64 && mode != IBIS_PREC_FP32_FLUX && mode != IBIS_PREC_FP32_RECON)
         return ibis::fail(IBIS_ERR_INVALID, "Unknown precision mode");
     r->generation++;
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     if (mode != IBIS_PREC_FP64 && r->ent_idx && !r->ent_geom32 && D.ntf > 0 && D.n_from_t) {
         HIP_TRY(hipMalloc(&r->ent_geom32, (size_t)32 * D.ntf));
         k_entries32<<<nblocks(D.ntf), BLOCK>>>(D, reinterpret_cast<char*>(r->ent_geom32));
@@ -3340,12 +3710,38 @@ ibis_status ibis_residual_set_precision(ibis_residual* r, int mode) {
 
 ibis_status ibis_residual_sizes(const ibis_residual* r, int64_t* sizes) {
     if (!r || !sizes) return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_sizes: null argument");
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     sizes[0] = D.dim;
     sizes[1] = D.N;
     sizes[2] = D.Nt;
     sizes[3] = D.F;
     sizes[4] = D.G;
+    return IBIS_OK;
+}
+
+// The per-entry face data (face_kernel 1) of the tile lists in r->d, and its
+// float copies when the residual has them (reduced-precision modes).
+static ibis_status build_entries(ibis_residual* r) {
+    const DevMesh& D = r->d;
+    const bool had32 = r->ent_geom32 != nullptr;
+    for (void** p : {(void**)&r->ent_idx, (void**)&r->ent_geom, (void**)&r->ent_geom32})
+        if (*p) {
+            HIP_TRY(hipFree(*p));
+            *p = nullptr;
+        }
+    const size_t ne = std::max(D.ntf, 1);
+    // entry geometry: 1 = with normals, 2 = planar 2D, 0 = t1, t2, area
+    const int with_n = !D.n_from_t ? 1 : (D.dim == 2 && D.planar2d ? 2 : 0);
+    HIP_TRY(hipMalloc(&r->ent_idx, sizeof(int2) * ne));
+    HIP_TRY(hipMalloc(&r->ent_geom, sizeof(double) * (with_n == 1 ? 10 : 7) * ne));
+    if (D.ntf > 0) k_entries<<<nblocks(D.ntf), BLOCK>>>(D, with_n, r->ent_idx, r->ent_geom);
+    HIP_TRY(hipGetLastError());
+    if (had32 && D.ntf > 0 && D.n_from_t) {
+        HIP_TRY(hipMalloc(&r->ent_geom32, (size_t)32 * D.ntf));
+        k_entries32<<<nblocks(D.ntf), BLOCK>>>(D, reinterpret_cast<char*>(r->ent_geom32));
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipDeviceSynchronize());
     return IBIS_OK;
 }
 
@@ -3361,7 +3757,7 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
     }
     if (k == "precomputed_offsets") {
         if (value && !r->offs) {
-            const DevMesh& D = r->m->d;
+            const DevMesh& D = r->d;
             HIP_TRY(hipMalloc(&r->offs, sizeof(double) * 3 * (size_t)D.maxf * D.N));
             k_offsets<<<nblocks(D.N), BLOCK>>>(D, r->offs);
             HIP_TRY(hipGetLastError());
@@ -3373,22 +3769,28 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
     if (k == "face_kernel") {
         if (value != 0 && value != 1 && value != 64 && value != 128 && value != 256)
             return ibis::fail(IBIS_ERR_INVALID, "face_kernel must be 0, 1, 64, 128 or 256");
-        const DevMesh& D = r->m->d;
+        const DevMesh& D = r->d;
         if (value > 1 && (size_t)(D.dim + 2) * D.maxf * value * sizeof(double) > 65536)
             return ibis::fail(IBIS_ERR_INVALID, "face_kernel: block does not fit in LDS");
         if (value == 1 && !r->ent_idx) {
             if (!ibis_entry_indices_fit(D.maxf, D.N, D.G))
                 return ibis::fail(IBIS_ERR_INVALID, "face_kernel 1: mesh too large for 32-bit value indices");
-            const size_t ne = std::max(D.ntf, 1);
-            // entry geometry: 1 = with normals, 2 = planar 2D, 0 = t1, t2, area
-            const int with_n = !D.n_from_t ? 1 : (D.dim == 2 && D.planar2d ? 2 : 0);
-            HIP_TRY(hipMalloc(&r->ent_idx, sizeof(int2) * ne));
-            HIP_TRY(hipMalloc(&r->ent_geom, sizeof(double) * (with_n == 1 ? 10 : 7) * ne));
-            if (D.ntf > 0) k_entries<<<nblocks(D.ntf), BLOCK>>>(D, with_n, r->ent_idx, r->ent_geom);
-            HIP_TRY(hipGetLastError());
-            HIP_TRY(hipDeviceSynchronize());
+            IBIS_TRY(build_entries(r));
         }
         r->opt_face_kernel = (int)value;
+        return IBIS_OK;
+    }
+    if (k == "tile") {
+        if (value != 128 && value != 256 && value != 512)
+            return ibis::fail(IBIS_ERR_INVALID, "tile must be 128, 256 or 512");
+        if (value == r->d.tile) return IBIS_OK;
+        DevMesh D = r->d;
+        IBIS_TRY(mesh_tiles(r->m, (int)value, &D));
+        // the tile kernels keep (dim+2) doubles per tile-list entry in LDS
+        if ((size_t)(D.dim + 2) * D.maxtf * sizeof(double) > MAX_LDS_BYTES)
+            return ibis::fail(IBIS_ERR_INVALID, "tile: a tile's face list does not fit in LDS");
+        r->d = D;
+        if (r->ent_idx) IBIS_TRY(build_entries(r));
         return IBIS_OK;
     }
     if (k == "pack_cells") {
@@ -3415,6 +3817,10 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
         r->opt_keep_grads = value ? 1 : 0;
         return IBIS_OK;
     }
+    if (k == "fuse_viscous") {
+        r->opt_fuse_visc = value ? 1 : 0;
+        return IBIS_OK;
+    }
     return ibis::fail(IBIS_ERR_INVALID, ("unknown option " + k).c_str());
 }
 
@@ -3425,7 +3831,7 @@ namespace {
 
 ibis_status ensure_adjoint(ibis_residual* r) {
     if (r->adj_prim_c) return IBIS_OK;
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     const size_t N = D.N, Nt = D.Nt, F = D.F, nb = std::max(D.nb, 1);
     const int nc = D.dim + 2;
     auto al = [&](void** p, size_t bytes) -> hipError_t {
@@ -3478,7 +3884,7 @@ ibis_status adjoint_supported(const ibis_residual* r) {
 template <int DIM>
 void launch_vjp(ibis_residual* r, const VarSet& vs, const FaceParams& fp, const AdjBufs& B,
                 bool active, hipStream_t st) {
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     const ibis_fv_desc& d = r->desc;
     const bool visc = d.viscous != 0;
     const bool ord1 = d.order == 1;
@@ -3556,7 +3962,7 @@ ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* 
     s = ensure_adjoint(r);
     if (s != IBIS_OK) return s;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     const ibis_fv_desc& d = r->desc;
     const size_t N = D.N, F = D.F, Nt = D.Nt;
     const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
@@ -3635,7 +4041,7 @@ ibis_status ibis_cons_to_prim_vjp(ibis_residual* r, const double* cons, const do
     if (!r || !cons || !prim_bar || !cons_bar)
         return ibis::fail(IBIS_ERR_INVALID, "ibis_cons_to_prim_vjp: null argument");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     if (D.dim == 3)
         k_cons_to_prim_vjp<3><<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, r->desc.R, r->desc.Cv, cons,
                                                                 prim_bar, cons_bar);
@@ -3652,7 +4058,7 @@ ibis_status ibis_wall_force(ibis_residual* r, double* prim, uint64_t force_tags,
     ibis_status s = ensure_adjoint(r);
     if (s != IBIS_OK) return s;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     if (r->prec != IBIS_PREC_FP64)
         return ibis::fail(IBIS_ERR_INVALID, "ibis_wall_force is FP64 only");
     Prof pf{r, st};
@@ -3705,7 +4111,7 @@ ibis_status ibis_estimate_dt(ibis_residual* r, const double* prim, double* dt_ho
                              ibis_stream_t stream) {
     if (!r || !prim || !dt_host) return ibis::fail(IBIS_ERR_INVALID, "ibis_estimate_dt: null argument");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     const ibis_fv_desc& d = r->desc;
     ViscParams vp{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr};
     const unsigned nb = nblocks(D.N);
@@ -3723,7 +4129,7 @@ ibis_status ibis_residual_cell_dt(ibis_residual* r, const double* prim, double* 
                                   ibis_stream_t stream) {
     if (!r || !prim || !dt_cells) return ibis::fail(IBIS_ERR_INVALID, "ibis_residual_cell_dt: null argument");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     const ibis_fv_desc& d = r->desc;
     ViscParams vp{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr};
     k_dt_cells<<<nblocks(D.N), BLOCK, 0, st>>>(D, prim, r->face_rho, r->face_T, d.viscous ? 1 : 0,
@@ -3736,7 +4142,7 @@ ibis_status ibis_count_bad_cells(ibis_residual* r, const double* prim, uint64_t*
                                  ibis_stream_t stream) {
     if (!r || !prim || !n_host) return ibis::fail(IBIS_ERR_INVALID, "ibis_count_bad_cells: null argument");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     k_zero_u64<<<1, 1, 0, st>>>(r->count_dev);
     k_count_bad<<<nblocks(D.N), BLOCK, 0, st>>>(D.N, D.Nt, prim, r->count_dev);
     HIP_TRY(hipGetLastError());
@@ -3753,7 +4159,7 @@ ibis_status ibis_cons_to_prim(ibis_residual* r, const double* cons, double* prim
                               ibis_stream_t stream) {
     if (!r || !cons || !prim) return ibis::fail(IBIS_ERR_INVALID, "ibis_cons_to_prim: null argument");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     if (D.dim == 3)
         k_cons_to_prim<double, 3><<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, r->desc.R, r->desc.Cv, cons, prim);
     else
@@ -3766,7 +4172,7 @@ ibis_status ibis_prim_to_cons(ibis_residual* r, const double* prim, double* cons
                               ibis_stream_t stream) {
     if (!r || !cons || !prim) return ibis::fail(IBIS_ERR_INVALID, "ibis_prim_to_cons: null argument");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     if (D.dim == 3)
         k_prim_to_cons<3><<<nblocks(D.Nt), BLOCK, 0, st>>>(D.Nt, prim, cons);
     else
@@ -3779,7 +4185,7 @@ ibis_status ibis_apply_time_derivative(ibis_residual* r, const double* u0, doubl
                                        const double* k, double dt, ibis_stream_t stream) {
     if (!r || !u0 || !u1 || !k) return ibis::fail(IBIS_ERR_INVALID, "ibis_apply_time_derivative: null argument");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     const long n = (long)(D.dim + 2) * D.Nt;
     k_axpy<<<nblocks(n), BLOCK, 0, st>>>(n, u0, u1, k, dt);
     HIP_TRY(hipGetLastError());
@@ -3830,7 +4236,7 @@ ibis_status ibis_rk_step(ibis_residual* r, int s, const double* a, const double*
                          double plot_remaining, double* dt_taken, ibis_stream_t stream) {
     if (!r || !a || !b || !cons || !prim || !dt_prev || s < 1 || s > 8)
         return ibis::fail(IBIS_ERR_INVALID, "ibis_rk_step: bad argument");
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     const size_t ncv = (size_t)(D.dim + 2) * D.Nt;
     while ((int)r->k.size() < s) {
         double* p = nullptr;
@@ -3892,7 +4298,7 @@ ibis_status ibis_rk_step_noalloc(ibis_residual* r, int s, const double* a, const
                                  double* dt_taken, ibis_stream_t stream) {
     if (!r || !a || !b || !cons || !prim || !dt_prev || s < 1 || s > 8)
         return ibis::fail(IBIS_ERR_INVALID, "ibis_rk_step_noalloc: bad argument");
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     const size_t nc = (size_t)(D.dim + 2);
     const size_t ncv = nc * D.Nt;
     const size_t need = ((size_t)s + 1) * ncv + (size_t)NPRIM * D.Nt;  // doubles
@@ -3937,7 +4343,7 @@ ibis_status ibis_compute_convective_gradient(ibis_residual* r, double* prim,
     if (!r || !prim)
         return ibis::fail(IBIS_ERR_INVALID, "ibis_compute_convective_gradient: null argument");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     const ibis_fv_desc& d = r->desc;
     const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
     if (r->has_pre) k_bc<double><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, prim);
@@ -3957,7 +4363,7 @@ ibis_status ibis_compute_viscous_gradient(ibis_residual* r, double* prim, ibis_s
     if (!r || !prim)
         return ibis::fail(IBIS_ERR_INVALID, "ibis_compute_viscous_gradient: null argument");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const DevMesh& D = r->m->d;
+    const DevMesh& D = r->d;
     const ibis_fv_desc& d = r->desc;
     const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
     if (r->has_pre) k_bc<double><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, prim);

@@ -313,11 +313,11 @@ ibis_status ibis_cell_gradients(const ibis_residual* r, const double** grads);
 ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** limiters,
                                   double** flux, int* flux_has_area);
 
-/* Implementation choices of the 2nd-order inviscid path; every choice gives
- * the same bits.  A residual of that path is created with face_state_layout
- * 2, pack_cells 0, precomputed_offsets 1, face_kernel 1 and stage_cells 1
- * (the autotuner's choice on the 3D and 2D workloads); others start at 0 /
- * layout 0.  Keys:
+/* Implementation choices of the 2nd-order path; every choice gives the same
+ * bits.  A second-order residual without flux-override boundaries is created
+ * with face_state_layout 2, pack_cells 0, precomputed_offsets 1, face_kernel
+ * 1, stage_cells 1 and tile 256 (the autotuner's choice on the 3D and 2D
+ * workloads); others start at 0 / layout 0 / tile 256.  Keys:
  *   "face_state_layout"   0: face states indexed by face (left/right arrays)
  *                         1: face states stored per (cell, face slot)
  *                         2: per (face slot, variable) planes over cells
@@ -335,7 +335,7 @@ ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** lim
  *                            limiter pass in LDS for the reconstruction
  *                            pass (limited schemes, at most 10 faces/cell)
  *                         0: both passes read them from memory
- *   "stage_cells"         1: each block of the cell kernel copies its 256
+ *   "stage_cells"         1: each block of the cell kernel copies its tile's
  *                            cells' and their outside neighbours' values
  *                            and centroids into LDS once, and the stencil
  *                            loop reads them there (layout 2, pack_cells 0;
@@ -350,15 +350,30 @@ ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** lim
  *                            gradient and limiter storage (the reference's
  *                            cell_grad_ and limiters after compute_dudt);
  *                            0 (default) skips those stores
- * Setting precomputed_offsets allocates and synchronises once. */
+ *   "tile"                128, 256 (default) or 512 cells per tile, which
+ *                            is also the block size of the tile kernels;
+ *                            the mesh builds a size's tile tables on first
+ *                            use (kept until ibis_mesh_destroy)
+ *   "fuse_viscous"        1 (default): a second-order viscous residual runs
+ *                            as three kernels -- boundary actions of both
+ *                            passes, cell kernel (viscous gradients,
+ *                            convective gradients, limiter, face values),
+ *                            face kernel (convective and viscous flux,
+ *                            surface integral) -- with layout 2, offsets,
+ *                            face_kernel 1, stage_cells, pack_cells 0;
+ *                            0: the reference-shaped general path
+ * Setting precomputed_offsets, face_kernel 1 or a new tile size allocates
+ * and synchronises once. */
 ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t value);
 
 /* Per-kernel timing with HIP events recorded around every launch of
  * ibis_compute_dudt on its stream.  Kernel slots:
  *   0 boundary actions before reconstruction (k_bc; on the fused path
  *     k_bc_ghost_fs, or k_bc, k_pack and k_ghost_fs)
+ *   0 also k_bc_visc_fs (both boundary passes) on the fused viscous path
  *   1 cell stencil: k_cell_grad_var, or k_cell_fs on the fused path
- *     (gradients, limiter and face values)
+ *     (gradients, limiter and face values; viscous gradients too on the
+ *     fused viscous path)
  *   2 face flux: k_face_flux, or k_face_*_fs on the fused path (flux and
  *     surface integral)
  *   3 k_post_flux   4 k_bc (pre-viscous)   5 k_cell_grad_var (viscous)

@@ -5,7 +5,8 @@ from ibis_amd.autotune import OPTION_SPACE, SHORT, coordinate_descent, variant_k
 def test_finds_the_separable_minimum_and_leaves_it_set():
     # a cost that is a sum over options: coordinate descent finds its minimum
     best_val = {"face_state_layout": 1, "precomputed_offsets": 0, "pack_cells": 1,
-                "face_kernel": 128, "stage_offsets": 0, "cell_prefetch": 1, "stage_cells": 0}
+                "face_kernel": 128, "stage_offsets": 0, "cell_prefetch": 1, "stage_cells": 0,
+                "tile": 512}
     state = {}
 
     def measure():
@@ -53,3 +54,17 @@ def test_near_ties_keep_the_earlier_value():
     assert opt["stage_offsets"] == 1
     assert opt["face_kernel"] == 0
     assert t == 980.0
+
+
+def test_refused_candidates_are_skipped():
+    # the library refuses tile 512 for this mesh: the tuner goes on without it
+    state = {}
+
+    def setter(k, v):
+        if k == "tile" and v == 512:
+            raise RuntimeError("tile: a tile's face list does not fit in LDS")
+        state[k] = v
+
+    opt, t, seen = coordinate_descent(setter, lambda: 10.0 - state["tile"] / 128.0)
+    assert opt["tile"] == 256 and state["tile"] == 256
+    assert any(v == float("inf") for v in seen.values())

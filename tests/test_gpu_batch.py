@@ -69,7 +69,7 @@ def test_batch_matches_instances_alone(use_graph):
         assert t[i] == float(np.cumsum(dts)[-1])
 
 
-def _batch_run(inst, prims, method, nsteps, use_graph, fuse):
+def _batch_run(inst, prims, method, nsteps, use_graph, fuse, options=None):
     """Step a batch with the stage updates fused into the residual's face
     kernel (fuse 1) or as separate passes (fuse 0); returns cons, prim (all
     rows, ghosts included) and the per-instance times."""
@@ -83,6 +83,8 @@ def _batch_run(inst, prims, method, nsteps, use_graph, fuse):
           "ibis_prim_to_cons")
     torch.cuda.synchronize()
     bt.set_option("fuse_stages", fuse)
+    for k, v in (options or {}).items():
+        bt.set_option(k, v)
     bt.rk_init(d_cons, d_prim, cfl=0.5, max_time=1e30, method=method)
     bt.rk_steps(nsteps, use_graph)
     torch.cuda.synchronize()
@@ -113,10 +115,12 @@ def test_fused_stage_epilogue_equals_separate_passes(method, use_graph):
         assert a.tobytes() == b.tobytes()
 
 
-def test_fused_stage_epilogue_3d():
+@pytest.mark.parametrize("tile", [256, 128, 512])
+def test_fused_stage_epilogue_3d(tile):
     """Two hex boxes (6^3 and 9^3, jittered, permuted), Hanel with the
-    Venkatakrishnan limiter, a smooth non-uniform state: fused and separate
-    stage updates agree bit for bit after three ssp-rk3 steps."""
+    Venkatakrishnan limiter, a smooth non-uniform state: fused stage
+    updates (with tiles of 128, 256 or 512 cells) and separate ones (256)
+    agree bit for bit after three ssp-rk3 steps."""
     inst = [cases.hex_case(n, flux="hanel", limiter="venkatakrishnan") for n in (6, 9)]
 
     def prim_of(g_io):
@@ -127,8 +131,8 @@ def test_fused_stage_epilogue_3d():
         return make
 
     prims = [prim_of(g) for g, _ in inst]
-    print("fused stages: 3D", flush=True)
-    fused = _batch_run(inst, prims, "ssp-rk3", 3, True, 1)
+    print(f"fused stages: 3D, tile {tile}", flush=True)
+    fused = _batch_run(inst, prims, "ssp-rk3", 3, True, 1, {"tile": tile})
     sep = _batch_run(inst, prims, "ssp-rk3", 3, True, 0)
     assert fused[0].tobytes() == sep[0].tobytes()
     assert fused[1].tobytes() == sep[1].tobytes()

@@ -124,6 +124,48 @@ def test_implementation_options_give_identical_bits(layout, pack, fk, stage, pf,
     _check(g_io, cfg, prim, exact=True, options=opts)
 
 
+TILE_OPTIONS = [{"tile": 128}, {"tile": 512},
+                {"tile": 512, "face_kernel": 0}, {"tile": 128, "stage_cells": 0, "cell_prefetch": 1},
+                {"tile": 512, "face_state_layout": 0, "pack_cells": 1, "face_kernel": 0}]
+
+
+@pytest.mark.parametrize("opts", TILE_OPTIONS, ids=lambda o: "-".join(f"{k}{v}" for k, v in o.items()))
+def test_tile_sizes_give_identical_bits(opts):
+    """Tiles of 128 and 512 cells (the tile kernels' block size follows)
+    on the 12^3 hex box, the wedge, the first-order wedge and the
+    hex/pyramid mesh, bit for bit against the oracle."""
+    from ibis_amd import gridgen
+    g_io, cfg = cases.hex_case(12, flux="hanel", limiter="venkatakrishnan")
+    _check(g_io, cfg, cases.hex_prim(g_io, 12 ** 3 + 6 * 12 ** 2), exact=True, options=opts)
+    g_io, cfg, fs = cases.wedge_case(40, 20)
+    prim = cases.perturbed_prim(g_io, fs, 40 * 20 + 2 * (40 + 20), seed=21)
+    _check(g_io, cfg, prim, exact=True, options=opts)
+    g_io, cfg, fs = cases.wedge_case(40, 20, order=1)
+    _check(g_io, cfg, prim, exact=True, options=opts)
+    g_io, cfg = cases.hex_case(8, flux="hanel", limiter="venkatakrishnan")
+    gh = gridgen.split_hexes(g_io, 3)
+    _check(gh, cfg, cases.hex_prim(gh, gh.num_cells + 6 * 64), exact=True, options=opts)
+
+
+def test_tile_option_switches_on_one_residual():
+    """One residual switched between tile sizes (512, 128, 256, 512) and to
+    the FP32-flux mode and back: every FP64 residual equals the oracle's bit
+    for bit, so the per-entry face data follows each switch."""
+    g_io, cfg = cases.hex_case(10, flux="ausmdv", limiter="barth_jespersen")
+    prim = cases.hex_prim(g_io, 10 ** 3 + 6 * 10 ** 2)
+    og, ofv = oracle_pair(g_io, cfg)
+    d_cpu = ofv.compute_dudt(prim.copy())
+    gb, fv = gpu_objects(g_io, cfg)
+    for tile in (512, 128, 256, 512):
+        fv.set_option("tile", tile)
+        fv.set_precision("fp32_flux")
+        gpu_dudt(gb, fv, prim)
+        fv.set_precision("fp64")
+        d_gpu, _ = gpu_dudt(gb, fv, prim)
+        print(f"tile {tile}", flush=True)
+        assert np.array_equal(d_gpu[:, :og.N], d_cpu[:, :og.N]), tile
+
+
 MIXED_OPTIONS = [{}, {"face_state_layout": 0, "face_kernel": 0, "stage_cells": 0,
                       "precomputed_offsets": 0, "pack_cells": 1},
                  {"stage_cells": 0, "cell_prefetch": 1, "face_kernel": 0},
@@ -329,3 +371,65 @@ def test_renumbering_invariance_hex160():
     n = 160 ** 3
     assert np.array_equal(outs[0][:, :n], outs[1][:, :n])
     assert np.isfinite(outs[0][:, :n]).all()
+
+
+def _visc_cases():
+    """Viscous cases for the fused viscous path: the plate (AUSMDV,
+    Barth-Jespersen, fixed-temperature no-slip wall), the same with Hanel and
+    Venkatakrishnan and rho_p, a split-quad plate (triangles and quads),
+    a jittered permuted 3D box with slip and no-slip walls, and a subsonic
+    wedge with viscous walls."""
+    from ibis_amd import config as C
+    from ibis_amd import gridgen
+    out = {}
+    g_io, cfg, fs = cases.plate_case(64, 32)
+    out["plate"] = (g_io, cfg, cases.perturbed_prim(g_io, fs, 64 * 32 + 2 * (64 + 32), seed=2))
+    g_io, cfg, fs = cases.plate_case(48, 24, flux="hanel")
+    cfg.limiter, cfg.thermo_interpolator = "venkatakrishnan", "rho_p"
+    out["plate_hanel_vk"] = (g_io, cfg, cases.perturbed_prim(g_io, fs, 48 * 24 + 2 * 72, seed=3))
+    gm = gridgen.split_quads(g_io, 2)
+    out["plate_triquad"] = (gm, cfg, cases.perturbed_prim(gm, fs, gm.num_cells + 2 * 72, seed=4))
+    g_io, cfg = cases.hex_case(10, flux="ausmdv", limiter="barth_jespersen")
+    cfg.viscous = True
+    tags = sorted(cfg.boundaries)
+    cfg.boundaries[tags[0]] = C.fixed_temperature_no_slip_wall(320.0)
+    cfg.boundaries[tags[1]] = C.slip_wall()
+    prim = cases.hex_prim(g_io, 10 ** 3 + 6 * 10 ** 2)
+    prim[:, 10 ** 3:] = prim[:, :1]
+    out["hex_walls"] = (g_io, cfg, prim)
+    g_io, cfg, fs = cases.wedge_case(48, 24, mach=0.5)
+    cfg.viscous = True
+    cfg.boundaries = {"inflow": C.subsonic_inflow(fs), "outflow": C.subsonic_outflow(fs.p * 0.98),
+                      "wall": C.fixed_temperature_no_slip_wall(300.0)}
+    out["wedge_subsonic"] = (g_io, cfg, cases.perturbed_prim(g_io, fs, 48 * 24 + 2 * 72, seed=5))
+    return out
+
+
+@pytest.mark.parametrize("name", ["plate", "plate_hanel_vk", "plate_triquad", "hex_walls",
+                                  "wedge_subsonic"])
+def test_fused_viscous_path_equals_general_path(name):
+    """The three-kernel viscous path (boundary actions, cell kernel with the
+    viscous gradients, face kernel with the viscous flux) against the
+    reference-shaped general path on the same device: dU/dt, the ghost rows,
+    estimate_dt (face density and temperature) and, with keep_gradients,
+    every gradient and limiter slot, bit for bit; and against the oracle at
+    the viscous bar."""
+    import torch
+    g_io, cfg, prim_ref = _visc_cases()[name]
+    _check(g_io, cfg, prim_ref, exact=False)
+    gb, fv = gpu_objects(g_io, cfg)
+    outs = []
+    for fuse in (1, 0):
+        fv.set_option("fuse_viscous", fuse)
+        fv.set_option("keep_gradients", 1)
+        d, p = gpu_dudt(gb, fv, prim_ref)
+        dt = fv.estimate_dt(to_dev(gb.to_grid_order(p)))
+        grads = fv.cell_gradients().cpu().numpy()
+        torch.cuda.synchronize()
+        outs.append((d, p, dt, grads))
+    n = gb.N
+    print(f"{name}: N={n}", flush=True)
+    assert outs[0][0][:, :n].tobytes() == outs[1][0][:, :n].tobytes()
+    assert outs[0][1].tobytes() == outs[1][1].tobytes()
+    assert outs[0][2] == outs[1][2]
+    assert outs[0][3].tobytes() == outs[1][3].tobytes()
