@@ -1710,16 +1710,30 @@ __device__ __forceinline__ void visc_face(const DevMesh& m, const ViscParams& vp
 // viscous boundary actions, the cells' viscous gradients (slots G_T, G_VX,
 // G_VY, G_VZ) and the face density and temperature it writes for estimate_dt.
 struct ViscFace {
-    ViscParams vp;
-    const double* prim;
-    const double* grad;
-    double* face_rho;
-    double* face_T;
+    const double* vterm;  // (dim+1) x F: k_visc_terms of each face
 };
 
-// VISC: each face's viscous flux (visc_face) is subtracted from its
-// convective flux before the product with the area, as the reference's
-// viscous pass subtracts it from the stored face flux.
+// The viscous terms of every face (visc_face), in face order, and the face
+// density and temperature for estimate_dt.
+template <int DIM>
+__global__ void __launch_bounds__(BLOCK)
+    k_visc_terms(DevMesh m, ViscParams vp, const double* __restrict__ prim,
+                 const double* __restrict__ grad, double* __restrict__ vterm,
+                 double* __restrict__ face_rho, double* __restrict__ face_T) {
+    const int f = blockIdx.x * BLOCK + threadIdx.x;
+    const int F = m.F;
+    if (f >= F) return;
+    double vt[DIM + 1], rho_f, T_f;
+    visc_face<double, DIM>(m, vp, prim, grad, f, m.L[f], m.R[f], vt, &rho_f, &T_f);
+    face_rho[f] = rho_f;
+    face_T[f] = T_f;
+#pragma unroll
+    for (int q = 0; q <= DIM; q++) vterm[(size_t)q * F + f] = vt[q];
+}
+
+// VISC: each entry's viscous terms (k_visc_terms of its face) are
+// subtracted from its convective flux before the product with the area, as
+// the reference's viscous pass subtracts them from the stored face flux.
 template <int DIM, int FLUX, bool NRM, class R = double, bool P2 = false, bool G32 = false,
           bool EPI = false, bool VISC = false>
 __global__ void __launch_bounds__(MAXTILE)
@@ -1812,14 +1826,11 @@ __global__ void __launch_bounds__(MAXTILE)
         double res[NC];
         if (VISC) {
             face_flux_raw_g<DIM, FLUX, R>(m, fp, 0, g, v[0], v[1], res, NRM ? g + 7 : nullptr);
-            const int f = m.ent_faces[f0 + k];
-            double vt[DIM + 1], rho_f, T_f;
-            visc_face<double, DIM>(m, vf.vp, vf.prim, vf.grad, f, m.L[f], m.R[f], vt, &rho_f, &T_f);
-            vf.face_rho[f] = rho_f;  // an entry duplicated in two tiles writes the same values
-            vf.face_T[f] = T_f;
+            const size_t f = m.ent_faces[f0 + k];
             const double A = g[6];
 #pragma unroll
-            for (int q = 0; q < NC; q++) res[q] = (q == 0 ? res[q] : res[q] - vt[q - 1]) * A;
+            for (int q = 0; q < NC; q++)
+                res[q] = (q == 0 ? res[q] : res[q] - vf.vterm[(size_t)(q - 1) * m.F + f]) * A;
         } else {
             face_flux_values_g<DIM, FLUX, R>(m, fp, 0, g, v[0], v[1], res, NRM ? g + 7 : nullptr);
         }
@@ -2682,6 +2693,7 @@ struct ibis_residual {
                                 // gradient and limiter slots too
     int opt_fuse_visc = 1;      // option fuse_viscous: the three-kernel viscous path
     double* gconv = nullptr;    // viscous: the ghosts' reconstruction variables (5 x G)
+    double* vterm = nullptr;    // viscous: (dim+1) x F viscous terms of the fused path
     int opt_face_kernel = 0; // option face_kernel: 0 tile face lists, 1 the same with
                              // per-entry face data, 64/128/256 slot-major
     int2* ent_idx = nullptr;     // per tile-list entry face data (face_kernel 1)
@@ -2729,7 +2741,7 @@ uint64_t ibis_residual_generation(const ibis_residual* r) { return r ? r->genera
 
 void ibis_residual_destroy(ibis_residual* r) {
     if (!r) return;
-    for (void* p : {(void*)r->grad, (void*)r->lim, (void*)r->flux, (void*)r->pack, (void*)r->FL, (void*)r->FR, (void*)r->offs, (void*)r->ent_idx, (void*)r->ent_geom, r->ent_geom32, (void*)r->offs32, (void*)r->gconv, (void*)r->face_rho,
+    for (void* p : {(void*)r->grad, (void*)r->lim, (void*)r->flux, (void*)r->pack, (void*)r->FL, (void*)r->FR, (void*)r->offs, (void*)r->ent_idx, (void*)r->ent_geom, r->ent_geom32, (void*)r->offs32, (void*)r->gconv, (void*)r->vterm, (void*)r->face_rho,
                     (void*)r->face_T, (void*)r->kinds, (void*)r->pars, (void*)r->partial,
                     (void*)r->dt_dev, (void*)r->count_dev, (void*)r->k_tmp, (void*)r->flow_tmp,
                     (void*)r->dual_prim, (void*)r->dual_dudt, (void*)r->dual_grad,
@@ -2828,6 +2840,7 @@ ibis_status ibis_residual_create_tables(const ibis_mesh* m, const ibis_fv_desc* 
         AL(r->face_rho, sizeof(double) * F);
         AL(r->face_T, sizeof(double) * F);
         AL(r->gconv, sizeof(double) * 5 * std::max<size_t>(D.G, 1));
+        AL(r->vterm, sizeof(double) * (D.dim + 1) * F);
     }
     r->ntab = ntab;
     AL(r->kinds, sizeof(int) * 3 * ntab);
@@ -3430,6 +3443,7 @@ ibis_status compute_dudt_impl(ibis_residual* r, double* prim, double* dudt,
     // kernel (convective and viscous flux, surface integral)
     const bool visc_fused = d.viscous && recon && !r->has_post && !epi && r->opt_fuse_visc &&
                             r->opt_layout == 2 && r->opt_face_kernel == 1 && r->ent_idx &&
+                            r->vterm &&
                             r->offs && r->opt_offsets && !r->opt_pack && r->opt_stage_cells &&
                             D.n_from_t && visc_stencil_bytes(D) <= 65536 && vs.row[2] == P_VX &&
                             vs.row[3] == P_VY && vs.row[4] == P_VZ;
@@ -3448,8 +3462,16 @@ ibis_status compute_dudt_impl(ibis_residual* r, double* prim, double* dudt,
             launch_cell_visc<2>(D, vs, d, r, prim, st);
         pf.end(1);
         pf.beg();
-        const ViscFace vf{ViscParams{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr}, prim, r->grad,
-                          r->face_rho, r->face_T};
+        const ViscParams vp{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr};
+        if (D.dim == 3)
+            k_visc_terms<3><<<nblocks(D.F), BLOCK, 0, st>>>(D, vp, prim, r->grad, r->vterm,
+                                                           r->face_rho, r->face_T);
+        else
+            k_visc_terms<2><<<nblocks(D.F), BLOCK, 0, st>>>(D, vp, prim, r->grad, r->vterm,
+                                                           r->face_rho, r->face_T);
+        pf.end(6);
+        pf.beg();
+        const ViscFace vf{r->vterm};
         if (D.dim == 3)
             launch_face_visc<3>(D, fp, d.flux, r->ent_idx, r->ent_geom, r->FL, dudt, vf, st);
         else

@@ -355,13 +355,15 @@ ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** lim
  *                            the mesh builds a size's tile tables on first
  *                            use (kept until ibis_mesh_destroy)
  *   "fuse_viscous"        1 (default): a second-order viscous residual runs
- *                            as three kernels -- boundary actions of both
+ *                            as four kernels -- boundary actions of both
  *                            passes, cell kernel (viscous gradients,
  *                            convective gradients, limiter, face values),
- *                            face kernel (convective and viscous flux,
- *                            surface integral) -- with layout 2, offsets,
- *                            face_kernel 1, stage_cells, pack_cells 0;
- *                            0: the reference-shaped general path
+ *                            viscous terms per tile-list entry, face
+ *                            kernel (convective flux minus the viscous
+ *                            terms, surface integral) -- with layout 2,
+ *                            offsets, face_kernel 1, stage_cells,
+ *                            pack_cells 0; 0: the reference-shaped
+ *                            general path
  * Setting precomputed_offsets, face_kernel 1 or a new tile size allocates
  * and synchronises once. */
 ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t value);
@@ -377,7 +379,7 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
  *   2 face flux: k_face_flux, or k_face_*_fs on the fused path (flux and
  *     surface integral)
  *   3 k_post_flux   4 k_bc (pre-viscous)   5 k_cell_grad_var (viscous)
- *   6 k_visc_flux   7 k_integral
+ *   6 k_visc_flux (k_visc_terms on the fused viscous path)   7 k_integral
  * ibis_residual_kernel_times synchronises, adds the elapsed times of all
  * completed launches into total_ms[slot] / counts[slot] (IBIS_NKERNELS
  * entries each) and, with reset != 0, clears the accumulators. */
