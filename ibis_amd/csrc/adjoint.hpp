@@ -48,8 +48,9 @@ struct AdjBufs {
     double* QRb;           // 5 x F
     double* VLb;           // 4 x F: T, vx, vy, vz of the left cell (viscous)
     double* VRb;           // 4 x F
-    double* GLb;           // 12 x F: viscous gradients of T, vx, vy, vz of the left cell
-    double* GRb;           // 12 x F
+    double* Gb;            // 12 x F: viscous gradients of T, vx, vy, vz of either cell
+                           // (equal for the two cells of an interior face; a
+                           // boundary face's goes to its valid cell)
     // cell cotangents
     double* cself;         // 5 x N
     double* cg;            // 5 x 3 x N
@@ -399,20 +400,19 @@ __global__ void __launch_bounds__(BLOCK)
     // face averages: half to each side
     double Lv[4] = {0.5 * tempb, 0.5 * vxb, 0.5 * vyb, 0.5 * vzb};
     double Rv[4] = {0.5 * tempb, 0.5 * vxb, 0.5 * vyb, 0.5 * vzb};
-    double GL[4][3], GR[4][3];
-    if (!interior) {  // the face gradient is the valid cell's gradient
+    // the gradient bars of the cells: the valid cell's gradient is the face
+    // gradient at a boundary face (the ghost side gets none, and no kernel
+    // reads it); both cells get half the Hasselbacher average's bar
+    double G[4][3];
+    if (!interior) {
         for (int k = 0; k < 4; k++)
-            for (int d = 0; d < 3; d++) {
-                GL[k][d] = lv ? gb[k][d] : 0.0;
-                GR[k][d] = lv ? 0.0 : gb[k][d];
-            }
+            for (int d = 0; d < 3; d++) G[k][d] = gb[k][d];
     } else {  // Hasselbacher average
         for (int k = 0; k < 4; k++) {
             const double corrb = -(gb[k][0] * n[0] + gb[k][1] * n[1] + gb[k][2] * n[2]) / edn;
             for (int d = 0; d < 3; d++) {
                 const double ab = gb[k][d] + corrb * ehat[d];
-                GL[k][d] = 0.5 * ab;
-                GR[k][d] = 0.5 * ab;
+                G[k][d] = 0.5 * ab;
             }
             Rv[k] += -corrb / len_e;
             Lv[k] += corrb / len_e;
@@ -421,10 +421,7 @@ __global__ void __launch_bounds__(BLOCK)
     for (int k = 0; k < 4; k++) {
         B.VLb[(size_t)k * F + f] = Lv[k];
         B.VRb[(size_t)k * F + f] = Rv[k];
-        for (int d = 0; d < 3; d++) {
-            B.GLb[(size_t)(k * 3 + d) * F + f] = GL[k][d];
-            B.GRb[(size_t)(k * 3 + d) * F + f] = GR[k][d];
-        }
+        for (int d = 0; d < 3; d++) B.Gb[(size_t)(k * 3 + d) * F + f] = G[k][d];
     }
 }
 
@@ -589,8 +586,7 @@ __global__ void __launch_bounds__(BLOCK)
                 const bool left = fe >= 0;
                 const int f = left ? fe : ~fe;
                 self += (left ? B.VLb : B.VRb)[(size_t)k * F + f];
-                for (int d = 0; d < 3; d++)
-                    gb[d] += (left ? B.GLb : B.GRb)[(size_t)(k * 3 + d) * F + f];
+                for (int d = 0; d < 3; d++) gb[d] += B.Gb[(size_t)(k * 3 + d) * F + f];
             }
             for (int j = 0; j < nf; j++) {
                 const double* w = B.lsqw + (size_t)j * 3 * N + i;

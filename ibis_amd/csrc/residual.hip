@@ -3866,7 +3866,7 @@ ibis_status ensure_adjoint(ibis_residual* r) {
     HIP_TRY(al((void**)&r->adj_vgrad, sizeof(double) * NGRAD * 3 * N));
     // face and cell cotangents: 7 rows each for the reconstructed values
     // (5 used by the second-order path, 7 prim rows by the first-order one)
-    HIP_TRY(al((void**)&r->adj_face, sizeof(double) * (7 + 7 + 4 + 4 + 12 + 12) * F));
+    HIP_TRY(al((void**)&r->adj_face, sizeof(double) * (7 + 7 + 4 + 4 + 12) * F));
     HIP_TRY(al((void**)&r->adj_cell, sizeof(double) * (7 + 15 + 5 + 4 + 12) * N));
     HIP_TRY(al((void**)&r->adj_extj, sizeof(int) * 5 * N));
     HIP_TRY(al((void**)&r->adj_bcb, sizeof(double) * NPRIM * nb));
@@ -4036,8 +4036,7 @@ ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* 
     B.QRb = fa + 7 * F;
     B.VLb = fa + 14 * F;
     B.VRb = fa + 18 * F;
-    B.GLb = fa + 22 * F;
-    B.GRb = fa + 34 * F;
+    B.Gb = fa + 22 * F;
     double* ca = r->adj_cell;
     B.cself = ca;
     B.cg = ca + 7 * N;
