@@ -669,10 +669,13 @@ __global__ void __launch_bounds__(BLOCK)
 // prim's ghost rows as after the reference's second ghost fill.  The
 // actions of a face read only its valid cell and write only its ghost, so
 // running them back to back per face is running each pass over all faces.
+// prim_c (the adjoint's copy of the state between the two passes): when not
+// null, the ghost's whole row is also stored there.
 __global__ void __launch_bounds__(BLOCK)
     k_bc_visc_fs(DevMesh m, const int* __restrict__ kinds, const double* __restrict__ pars,
                  int ntab, int has_pre, int has_visc, GasParams gp, VarSet vs,
-                 double* __restrict__ prim, double* __restrict__ FL, double* __restrict__ gconv) {
+                 double* __restrict__ prim, double* __restrict__ FL, double* __restrict__ gconv,
+                 double* __restrict__ prim_c) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= m.nb) return;
     if (has_pre) bc_actions<double>(m, kinds, pars, gp, prim, b);
@@ -681,6 +684,10 @@ __global__ void __launch_bounds__(BLOCK)
 #pragma unroll
     for (int v = 0; v < 5; v++)
         gconv[(size_t)v * m.G + (g - m.N)] = prim[(size_t)vs.row[v] * m.Nt + g];
+    if (prim_c) {
+#pragma unroll
+        for (int k = 0; k < NPRIM; k++) prim_c[(size_t)k * m.Nt + g] = prim[(size_t)k * m.Nt + g];
+    }
     if (has_visc) bc_actions<double>(m, kinds + ntab, pars + (size_t)ntab * IBIS_NPAR, gp, prim, b);
 }
 
@@ -804,9 +811,10 @@ __device__ __forceinline__ bool viscous_slot(int slot) {
 // VISC (with SN, layout 2): the fused viscous path's cell kernel -- the
 // stencil of stage_stencil_visc, and before the convective work the viscous
 // gradients of T, vx, vy, vz (the LSQ gradient of k_cell_grad_var, same
-// expressions and order) into their slots, which the convective slots do
-// not overwrite afterwards (the viscous gradients are computed after the
-// convective ones in the reference and overwrite them).
+// expressions and order) into their slots of vgrad.  When vgrad is grad,
+// the convective slots do not overwrite them afterwards (the viscous
+// gradients are computed after the convective ones in the reference and
+// overwrite them); the adjoint passes its own vgrad and keeps both.
 template <int DIM, int LIM, int LAYOUT, bool OFFS, class W = double, bool PACKED = true,
           bool PF = false, bool SN = false, bool VISC = false>
 __global__ void __launch_bounds__(MAXTILE)
@@ -814,7 +822,7 @@ __global__ void __launch_bounds__(MAXTILE)
               double* __restrict__ lim, double eps, double venkat_K, int nlogical,
               int write_slots, W* __restrict__ FL, W* __restrict__ FR,
               const double* __restrict__ offs, const double* __restrict__ prim, int stage,
-              const double* __restrict__ gconv) {
+              const double* __restrict__ gconv, double* __restrict__ vgrad) {
     // stage != 0: the limiter pass keeps each face offset in LDS
     // (maxf x 3 doubles per thread) for the reconstruction pass, so the
     // offsets are read from memory once.
@@ -945,7 +953,7 @@ __global__ void __launch_bounds__(MAXTILE)
         const int vslot[4] = {G_T, G_VX, G_VY, G_VZ};
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            double* g = grad + (size_t)vslot[q] * 3 * N;
+            double* g = vgrad + (size_t)vslot[q] * 3 * N;
             g[i] = tx[q];
             g[N + i] = ty[q];
             g[2 * (size_t)N + i] = tz[q];
@@ -999,7 +1007,7 @@ __global__ void __launch_bounds__(MAXTILE)
     for (int v = 0; v < 5; v++) {
         double* g = grad + (size_t)vs.slot[v] * 3 * N;
         if (v < NG) {
-            if (write_slots && !(VISC && viscous_slot(vs.slot[v]))) {
+            if (write_slots && !(VISC && vgrad == grad && viscous_slot(vs.slot[v]))) {
                 g[i] = gx[v];
                 g[N + i] = gy[v];
                 g[2 * (size_t)N + i] = gz[v];
@@ -3012,11 +3020,11 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
         if (sn)
             k_cell_fs<DIM, LIM, 2, OFFS, float, false, false, true><<<xcd_grid(nlog), D.tile, shs, st>>>(
                 D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, fl, fl, r->offs,
-                prim, stage, nullptr);
+                prim, stage, nullptr, nullptr);
         else
             k_cell_fs<DIM, LIM, 2, OFFS, float, false><<<xcd_grid(nlog), D.tile, shm, st>>>(
                 D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, fl, fl, r->offs,
-                prim, stage, nullptr);
+                prim, stage, nullptr, nullptr);
         return;
     }
     if (!r->opt_pack && LAYOUT == 2) {
@@ -3024,22 +3032,22 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
             k_cell_fs<DIM, LIM, 2, OFFS, double, false, false, true>
                 <<<xcd_grid(nlog), D.tile, shs, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps,
                                                      d.venkat_K, nlog, r->opt_keep_grads, r->FL, r->FR, r->offs,
-                                                     prim, stage, nullptr);
+                                                     prim, stage, nullptr, nullptr);
             return;
         }
         if (r->opt_cell_prefetch)
             k_cell_fs<DIM, LIM, 2, OFFS, double, false, true><<<xcd_grid(nlog), D.tile, shm, st>>>(
                 D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, r->FL, r->FR,
-                r->offs, prim, stage, nullptr);
+                r->offs, prim, stage, nullptr, nullptr);
         else
             k_cell_fs<DIM, LIM, 2, OFFS, double, false><<<xcd_grid(nlog), D.tile, shm, st>>>(
                 D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, r->FL, r->FR,
-                r->offs, prim, stage, nullptr);
+                r->offs, prim, stage, nullptr, nullptr);
         return;
     }
     k_cell_fs<DIM, LIM, LAYOUT, OFFS><<<xcd_grid(nlog), D.tile, shm, st>>>(
         D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, r->FL, r->FR, r->offs,
-        prim, stage, nullptr);
+        prim, stage, nullptr, nullptr);
 }
 
 template <int DIM, int LIM>
@@ -3192,27 +3200,43 @@ size_t visc_stencil_bytes(const DevMesh& D) {
     return (size_t)12 * (D.tile + D.maxhalo) * sizeof(double);
 }
 
+// vgrad: where the viscous gradients go (r->grad for the residual; the
+// adjoint keeps them apart); write_slots: also the convective gradients
+// and limiters
 template <int DIM, int LIM>
 void launch_cell_visc_l(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
-                        ibis_residual* r, const double* prim, hipStream_t st) {
+                        ibis_residual* r, const double* prim, double* vgrad, int write_slots,
+                        hipStream_t st) {
     const size_t shm = cell_stage_bytes(D, LIM, r->opt_stage);
     const int stage = shm ? 1 : 0;
     const size_t shs = std::max(visc_stencil_bytes(D), shm);
     k_cell_fs<DIM, LIM, 2, true, double, false, false, true, true>
         <<<xcd_grid(D.ntiles), D.tile, shs, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps,
-                                                  d.venkat_K, D.ntiles, r->opt_keep_grads, r->FL,
-                                                  r->FR, r->offs, prim, stage, r->gconv);
+                                                  d.venkat_K, D.ntiles, write_slots, r->FL,
+                                                  r->FR, r->offs, prim, stage, r->gconv, vgrad);
 }
 
 template <int DIM>
 void launch_cell_visc(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d, ibis_residual* r,
-                      const double* prim, hipStream_t st) {
+                      const double* prim, double* vgrad, int write_slots, hipStream_t st) {
     if (d.limiter == 0)
-        launch_cell_visc_l<DIM, 0>(D, vs, d, r, prim, st);
+        launch_cell_visc_l<DIM, 0>(D, vs, d, r, prim, vgrad, write_slots, st);
     else if (d.limiter == 1)
-        launch_cell_visc_l<DIM, 1>(D, vs, d, r, prim, st);
+        launch_cell_visc_l<DIM, 1>(D, vs, d, r, prim, vgrad, write_slots, st);
     else
-        launch_cell_visc_l<DIM, 2>(D, vs, d, r, prim, st);
+        launch_cell_visc_l<DIM, 2>(D, vs, d, r, prim, vgrad, write_slots, st);
+}
+
+// the fused viscous path can run this residual (second order, the options
+// its kernels need, a stencil that fits the LDS)
+bool visc_fusable(const ibis_residual* r, const VarSet& vs) {
+    const DevMesh& D = r->d;
+    const ibis_fv_desc& d = r->desc;
+    return d.viscous && d.order == 2 && !r->has_post && r->prec == IBIS_PREC_FP64 &&
+           r->opt_fuse_visc && r->opt_layout == 2 && r->opt_face_kernel == 1 && r->ent_idx &&
+           r->vterm && r->offs && r->opt_offsets && !r->opt_pack && r->opt_stage_cells &&
+           D.n_from_t && visc_stencil_bytes(D) <= 65536 && vs.row[2] == P_VX &&
+           vs.row[3] == P_VY && vs.row[4] == P_VZ;
 }
 
 // the fused viscous path's face kernel (per-entry face data, FP64)
@@ -3441,25 +3465,20 @@ ibis_status compute_dudt_impl(ibis_residual* r, double* prim, double* dudt,
     // viscous, second order: boundary actions (both passes), cell kernel
     // (viscous gradients, convective gradients, limiter, face values), face
     // kernel (convective and viscous flux, surface integral)
-    const bool visc_fused = d.viscous && recon && !r->has_post && !epi && r->opt_fuse_visc &&
-                            r->opt_layout == 2 && r->opt_face_kernel == 1 && r->ent_idx &&
-                            r->vterm &&
-                            r->offs && r->opt_offsets && !r->opt_pack && r->opt_stage_cells &&
-                            D.n_from_t && visc_stencil_bytes(D) <= 65536 && vs.row[2] == P_VX &&
-                            vs.row[3] == P_VY && vs.row[4] == P_VZ;
+    const bool visc_fused = recon && !epi && visc_fusable(r, vs);
     if (visc_fused) {
         const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
         pf.beg();
         if (D.nb > 0)
             k_bc_visc_fs<<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, r->ntab,
                                                           r->has_pre ? 1 : 0, r->has_visc ? 1 : 0,
-                                                          gp, vs, prim, r->FL, r->gconv);
+                                                          gp, vs, prim, r->FL, r->gconv, nullptr);
         pf.end(0);
         pf.beg();
         if (D.dim == 3)
-            launch_cell_visc<3>(D, vs, d, r, prim, st);
+            launch_cell_visc<3>(D, vs, d, r, prim, r->grad, r->opt_keep_grads, st);
         else
-            launch_cell_visc<2>(D, vs, d, r, prim, st);
+            launch_cell_visc<2>(D, vs, d, r, prim, r->grad, r->opt_keep_grads, st);
         pf.end(1);
         pf.beg();
         const ViscParams vp{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr};
@@ -3991,7 +4010,24 @@ ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* 
     FaceParams fp;
     VarSet vs;
     residual_params(r, &fp, &vs);
-    // forward: the residual's state, gradients and limiters (recomputed)
+    // forward: the residual's state, gradients and limiters (recomputed);
+    // on the fused viscous path with its own kernels: the boundary kernel
+    // stores the ghosts' rows between the two passes in prim_c (whose valid
+    // rows are prim's), the cell kernel writes the convective gradients and
+    // limiters and puts the viscous gradients in adj_vgrad
+    if (visc_fusable(r, vs)) {
+        HIP_TRY(hipMemcpyAsync(r->adj_prim_c, prim, sizeof(double) * NPRIM * Nt,
+                               hipMemcpyDeviceToDevice, st));
+        if (D.nb > 0)
+            k_bc_visc_fs<<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, r->ntab,
+                                                          r->has_pre ? 1 : 0, r->has_visc ? 1 : 0,
+                                                          gp, vs, prim, r->FL, r->gconv,
+                                                          r->adj_prim_c);
+        if (D.dim == 3)
+            launch_cell_visc<3>(D, vs, d, r, prim, r->adj_vgrad, 1, st);
+        else
+            launch_cell_visc<2>(D, vs, d, r, prim, r->adj_vgrad, 1, st);
+    } else {
     if (r->has_pre) k_bc<double><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, prim);
     HIP_TRY(hipMemcpyAsync(r->adj_prim_c, prim, sizeof(double) * NPRIM * Nt,
                            hipMemcpyDeviceToDevice, st));
@@ -4020,6 +4056,7 @@ ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* 
             launch_cell_grad<double, 3>(D, vv, 0, prim, r->adj_vgrad, r->lim, 0.0, 0.0, st);
         else
             launch_cell_grad<double, 2>(D, vv, 0, prim, r->adj_vgrad, r->lim, 0.0, 0.0, st);
+    }
     }
     AdjBufs B;
     B.prim_c = r->adj_prim_c;
