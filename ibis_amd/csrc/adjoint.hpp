@@ -65,6 +65,9 @@ struct AdjBufs {
     // x_face - x_cell, offs[(j*3 + k)*N + c]
     const double* lsqw;
     const double* offs;
+    // the forward pass's face values (layout 2, as the fused cell kernel
+    // wrote them), or null: then the reverse recomputes them
+    const double* FLv;
 };
 
 // LSQ weights of cell c for a neighbour at centroid offset (dx, dy, dz)
@@ -171,20 +174,25 @@ __global__ void __launch_bounds__(BLOCK)
     for (int side = 0; side < 2; side++) {
         const int c = side ? m.R[f] : m.L[f];
         const bool valid = c < N;
-        const double dx = m.geom[9 * (size_t)F + f] - m.cen[c];
-        const double dy = m.geom[10 * (size_t)F + f] - m.cen[Nt + c];
-        const double dz = m.geom[11 * (size_t)F + f] - m.cen[2 * (size_t)Nt + c];
-        for (int v = 0; v < 5; v++) {
-            const double u = B.prim_c[(size_t)vs.row[v] * Nt + c];
-            double gx = 0.0, gy = 0.0, gz = 0.0, L = 1.0;
-            if (valid) {
-                const double* g = B.cgrad + (size_t)vs.slot[v] * 3 * N;
-                gx = g[c];
-                gy = g[N + c];
-                gz = g[2 * (size_t)N + c];
-                if (fp.limited) L = B.clim[(size_t)vs.slot[v] * N + c];
+        if (B.FLv) {  // the same values, as the forward cell kernel stored them
+            const int slot = side ? m.slotR[f] : m.slotL[f];
+            for (int v = 0; v < 5; v++) q[side][v] = B.FLv[soa_index(m, c, slot, v)];
+        } else {
+            const double dx = m.geom[9 * (size_t)F + f] - m.cen[c];
+            const double dy = m.geom[10 * (size_t)F + f] - m.cen[Nt + c];
+            const double dz = m.geom[11 * (size_t)F + f] - m.cen[2 * (size_t)Nt + c];
+            for (int v = 0; v < 5; v++) {
+                const double u = B.prim_c[(size_t)vs.row[v] * Nt + c];
+                double gx = 0.0, gy = 0.0, gz = 0.0, L = 1.0;
+                if (valid) {
+                    const double* g = B.cgrad + (size_t)vs.slot[v] * 3 * N;
+                    gx = g[c];
+                    gy = g[N + c];
+                    gz = g[2 * (size_t)N + c];
+                    if (fp.limited) L = B.clim[(size_t)vs.slot[v] * N + c];
+                }
+                q[side][v] = u + L * (gx * dx + gy * dy + gz * dz);
             }
-            q[side][v] = u + L * (gx * dx + gy * dy + gz * dz);
         }
         double rho = 0.0, p = 0.0, T = 0.0, e = 0.0;
         const double a = q[side][0], b = q[side][1];

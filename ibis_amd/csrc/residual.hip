@@ -2699,7 +2699,9 @@ struct ibis_residual {
     int opt_stage_cells = 0;    // option stage_cells (layout 2, pack_cells 0)
     int opt_keep_grads = 0;     // option keep_gradients: the fused path writes the
                                 // gradient and limiter slots too
-    int opt_fuse_visc = 1;      // option fuse_viscous: the three-kernel viscous path
+    int opt_fuse_visc = 1;      // option fuse_viscous: the four-kernel viscous path
+    int force_slots = 0;        // the adjoint's forward pass: the fused cell kernel
+                                // writes the gradient and limiter slots
     double* gconv = nullptr;    // viscous: the ghosts' reconstruction variables (5 x G)
     double* vterm = nullptr;    // viscous: (dim+1) x F viscous terms of the fused path
     int opt_face_kernel = 0; // option face_kernel: 0 tile face lists, 1 the same with
@@ -3000,6 +3002,7 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
                       int nlog, const double* prim, hipStream_t st) {
     const size_t shm = cell_stage_bytes(D, LIM, r->opt_stage);
     const int stage = shm ? 1 : 0;
+    const int ws = (r->opt_keep_grads || r->force_slots) ? 1 : 0;
     // the LDS cell stencil (option stage_cells), in all precisions; the
     // FP64 and FP32-flux kernels reuse it for the offsets (stage_offsets)
     const size_t shs0 = (size_t)8 * (D.tile + D.maxhalo) * sizeof(double);
@@ -3019,11 +3022,11 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
         float* fl = reinterpret_cast<float*>(r->FL);
         if (sn)
             k_cell_fs<DIM, LIM, 2, OFFS, float, false, false, true><<<xcd_grid(nlog), D.tile, shs, st>>>(
-                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, fl, fl, r->offs,
+                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, ws, fl, fl, r->offs,
                 prim, stage, nullptr, nullptr);
         else
             k_cell_fs<DIM, LIM, 2, OFFS, float, false><<<xcd_grid(nlog), D.tile, shm, st>>>(
-                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, fl, fl, r->offs,
+                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, ws, fl, fl, r->offs,
                 prim, stage, nullptr, nullptr);
         return;
     }
@@ -3031,22 +3034,22 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
         if (sn) {
             k_cell_fs<DIM, LIM, 2, OFFS, double, false, false, true>
                 <<<xcd_grid(nlog), D.tile, shs, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps,
-                                                     d.venkat_K, nlog, r->opt_keep_grads, r->FL, r->FR, r->offs,
+                                                     d.venkat_K, nlog, ws, r->FL, r->FR, r->offs,
                                                      prim, stage, nullptr, nullptr);
             return;
         }
         if (r->opt_cell_prefetch)
             k_cell_fs<DIM, LIM, 2, OFFS, double, false, true><<<xcd_grid(nlog), D.tile, shm, st>>>(
-                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, r->FL, r->FR,
+                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, ws, r->FL, r->FR,
                 r->offs, prim, stage, nullptr, nullptr);
         else
             k_cell_fs<DIM, LIM, 2, OFFS, double, false><<<xcd_grid(nlog), D.tile, shm, st>>>(
-                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, r->FL, r->FR,
+                D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, ws, r->FL, r->FR,
                 r->offs, prim, stage, nullptr, nullptr);
         return;
     }
     k_cell_fs<DIM, LIM, LAYOUT, OFFS><<<xcd_grid(nlog), D.tile, shm, st>>>(
-        D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, r->opt_keep_grads, r->FL, r->FR, r->offs,
+        D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, ws, r->FL, r->FR, r->offs,
         prim, stage, nullptr, nullptr);
 }
 
@@ -4015,7 +4018,11 @@ ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* 
     // stores the ghosts' rows between the two passes in prim_c (whose valid
     // rows are prim's), the cell kernel writes the convective gradients and
     // limiters and puts the viscous gradients in adj_vgrad
-    if (visc_fusable(r, vs)) {
+    const bool vis_fused = visc_fusable(r, vs);
+    const bool inv_fused = !d.viscous && d.order == 2 && !r->has_post &&
+                           r->prec == IBIS_PREC_FP64 && r->opt_layout == 2 && r->offs &&
+                           r->opt_offsets && !r->opt_pack && r->opt_stage_cells;
+    if (vis_fused) {
         HIP_TRY(hipMemcpyAsync(r->adj_prim_c, prim, sizeof(double) * NPRIM * Nt,
                                hipMemcpyDeviceToDevice, st));
         if (D.nb > 0)
@@ -4027,6 +4034,24 @@ ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* 
             launch_cell_visc<3>(D, vs, d, r, prim, r->adj_vgrad, 1, st);
         else
             launch_cell_visc<2>(D, vs, d, r, prim, r->adj_vgrad, 1, st);
+    } else if (inv_fused) {
+        // the inviscid second-order path's own kernels: boundary actions and
+        // ghost face values, then the cell kernel writing the slots too
+        if (D.nb > 0) {
+            if (r->has_pre)
+                k_bc_ghost_fs<2><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, vs,
+                                                                  prim, r->FL, r->FR);
+            else
+                k_ghost_fs<2><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
+        }
+        HIP_TRY(hipMemcpyAsync(r->adj_prim_c, prim, sizeof(double) * NPRIM * Nt,
+                               hipMemcpyDeviceToDevice, st));
+        r->force_slots = 1;
+        if (D.dim == 3)
+            launch_cell_fs<3>(D, vs, d, r, D.ntiles, prim, st);
+        else
+            launch_cell_fs<2>(D, vs, d, r, D.ntiles, prim, st);
+        r->force_slots = 0;
     } else {
     if (r->has_pre) k_bc<double><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, prim);
     HIP_TRY(hipMemcpyAsync(r->adj_prim_c, prim, sizeof(double) * NPRIM * Nt,
@@ -4084,6 +4109,7 @@ ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* 
     B.bcb = r->adj_bcb;
     B.pb = prim_bar;
     B.lsqw = r->adj_lsqw;
+    B.FLv = (vis_fused || inv_fused) ? r->FL : nullptr;
     B.offs = r->offs;
     const bool active = limiter_mode == IBIS_LIMITER_ACTIVE;
     if (D.dim == 3)
