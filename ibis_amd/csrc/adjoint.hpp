@@ -53,11 +53,13 @@ struct AdjBufs {
                            // boundary face's goes to its valid cell)
     // cell cotangents
     double* cself;         // 5 x N
-    double* cg;            // 5 x 3 x N
-    double* cext;          // 5 x N: bar of the limiter's extreme neighbour value
-    int* cextj;            // 5 x N: its slot in the cell's list (-1 none)
     double* vself;         // 4 x N
-    double* vg;            // 4 x 3 x N
+    // what cell c's stencil sends to the neighbour in its slot j, as the
+    // value that neighbour adds: send[(j*9 + q)*N + c], q < 5 the
+    // reconstruction variables (LSQ weight . gradient bar, plus the
+    // limiter's extreme-value bar when that neighbour is the extreme),
+    // q = 5 + k the viscous T, vx, vy, vz (LSQ weight . gradient bar)
+    double* send;
     double* bcb;           // 7 x nb: what each ghost sends to its valid cell
     double* pb;            // 7 x Nt output
     // geometry tables, built once: the LSQ weights of each cell's stencil
@@ -582,9 +584,13 @@ __global__ void __launch_bounds__(BLOCK)
             gb[0] = gb[1] = gb[2] = 0.0;  // 2D vz: no gradient is computed
         }
         B.cself[(size_t)v * N + i] = self;
-        for (int k = 0; k < 3; k++) B.cg[((size_t)v * 3 + k) * N + i] = gb[k];
-        B.cext[(size_t)v * N + i] = ext;
-        B.cextj[(size_t)v * N + i] = extj;
+        for (int j = 0; j < nf; j++) {
+            const double* w = B.lsqw + (size_t)j * 3 * N + i;
+            double x = 0.0;
+            if (v < ngrad) x = w[0] * gb[0] + w[N] * gb[1] + w[2 * (size_t)N] * gb[2];
+            if (extj == j) x += ext;
+            B.send[((size_t)j * 9 + v) * N + i] = x;
+        }
     }
     if (VISC) {
         for (int k = 0; k < 4; k++) {
@@ -601,7 +607,11 @@ __global__ void __launch_bounds__(BLOCK)
                 self -= w[0] * gb[0] + w[N] * gb[1] + w[2 * (size_t)N] * gb[2];
             }
             B.vself[(size_t)k * N + i] = self;
-            for (int d = 0; d < 3; d++) B.vg[((size_t)k * 3 + d) * N + i] = gb[d];
+            for (int j = 0; j < nf; j++) {
+                const double* w = B.lsqw + (size_t)j * 3 * N + i;
+                B.send[((size_t)j * 9 + 5 + k) * N + i] =
+                    w[0] * gb[0] + w[N] * gb[1] + w[2 * (size_t)N] * gb[2];
+            }
         }
     }
 }
@@ -743,10 +753,7 @@ __global__ void __launch_bounds__(BLOCK)
     const int f = m.b_face[b], g = m.b_ghost[b], va = m.b_valid[b], t = m.b_tag[b];
     double vst[NPRIM];  // the valid cell's state, which the actions read
     for (int k = 0; k < NPRIM; k++) vst[k] = B.prim_c[(size_t)k * Nt + va];
-    const int ngrad = DIM == 3 ? 5 : 4;
     const int j = slot_of(m, f, va);
-    const double w[3] = {B.lsqw[(size_t)(j * 3 + 0) * N + va], B.lsqw[(size_t)(j * 3 + 1) * N + va],
-                         B.lsqw[(size_t)(j * 3 + 2) * N + va]};
     double gb[NPRIM], vb[NPRIM];
     for (int k = 0; k < NPRIM; k++) gb[k] = vb[k] = 0.0;
     if (VISC) {  // viscous path reads the ghost after the viscous actions
@@ -754,8 +761,7 @@ __global__ void __launch_bounds__(BLOCK)
         const int rows[4] = {P_T, P_VX, P_VY, P_VZ};
         for (int k = 0; k < 4; k++) {
             double x = (gl ? B.VLb : B.VRb)[(size_t)k * F + f];
-            x += w[0] * B.vg[((size_t)k * 3 + 0) * N + va] + w[1] * B.vg[((size_t)k * 3 + 1) * N + va] +
-                 w[2] * B.vg[((size_t)k * 3 + 2) * N + va];
+            x += B.send[((size_t)j * 9 + 5 + k) * N + va];
             gb[rows[k]] += x;
         }
         for (int a = IBIS_MAX_ACT - 1; a >= 0; a--)
@@ -769,11 +775,7 @@ __global__ void __launch_bounds__(BLOCK)
         const bool gl = m.L[f] == g;
         for (int v = 0; v < 5; v++) {
             double x = (gl ? B.QLb : B.QRb)[(size_t)v * F + f];  // ghost side: plain copy
-            if (v < ngrad)
-                x += w[0] * B.cg[((size_t)v * 3 + 0) * N + va] +
-                     w[1] * B.cg[((size_t)v * 3 + 1) * N + va] +
-                     w[2] * B.cg[((size_t)v * 3 + 2) * N + va];
-            if (B.cextj[(size_t)v * N + va] == j) x += B.cext[(size_t)v * N + va];
+            x += B.send[((size_t)j * 9 + v) * N + va];
             gb[vs.row[v]] += x;
         }
     }
@@ -789,7 +791,6 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_gather(DevMesh m, VarSet vs, AdjB
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int N = m.N, Nt = m.Nt;
     if (i >= N) return;
-    const int ngrad = DIM == 3 ? 5 : 4;
     double out[NPRIM];
     for (int k = 0; k < NPRIM; k++) out[k] = 0.0;
     if (ORD1)
@@ -810,21 +811,9 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_gather(DevMesh m, VarSet vs, AdjB
         }
         // cell nb's stencil holds this cell at slot s
         const int s = slot_of(m, f, nb);
-        const double w[3] = {B.lsqw[(size_t)(s * 3 + 0) * N + nb], B.lsqw[(size_t)(s * 3 + 1) * N + nb],
-                             B.lsqw[(size_t)(s * 3 + 2) * N + nb]};
-        for (int v = 0; v < (ORD1 ? 0 : 5); v++) {
-            double x = 0.0;
-            if (v < ngrad)
-                x = w[0] * B.cg[((size_t)v * 3 + 0) * N + nb] + w[1] * B.cg[((size_t)v * 3 + 1) * N + nb] +
-                    w[2] * B.cg[((size_t)v * 3 + 2) * N + nb];
-            if (B.cextj[(size_t)v * N + nb] == s) x += B.cext[(size_t)v * N + nb];
-            out[vs.row[v]] += x;
-        }
+        for (int v = 0; v < (ORD1 ? 0 : 5); v++) out[vs.row[v]] += B.send[((size_t)s * 9 + v) * N + nb];
         if (VISC)
-            for (int k = 0; k < 4; k++)
-                out[vrows[k]] += w[0] * B.vg[((size_t)k * 3 + 0) * N + nb] +
-                                 w[1] * B.vg[((size_t)k * 3 + 1) * N + nb] +
-                                 w[2] * B.vg[((size_t)k * 3 + 2) * N + nb];
+            for (int k = 0; k < 4; k++) out[vrows[k]] += B.send[((size_t)s * 9 + 5 + k) * N + nb];
     }
     for (int k = 0; k < NPRIM; k++) B.pb[(size_t)k * Nt + i] = out[k];
 }

@@ -2735,7 +2735,6 @@ struct ibis_residual {
     // adjoint (VJP) scratch, allocated on first use
     std::vector<void*> adj;
     double *adj_prim_c = nullptr, *adj_vgrad = nullptr, *adj_face = nullptr, *adj_cell = nullptr;
-    int* adj_extj = nullptr;
     double *adj_bcb = nullptr, *adj_zero = nullptr, *adj_dudt = nullptr, *adj_force = nullptr;
     double* adj_lsqw = nullptr;  // LSQ weight table of the reverse kernels
     int freeze_lim = 0;  // option freeze_limiter (tangent residual only)
@@ -3889,8 +3888,7 @@ ibis_status ensure_adjoint(ibis_residual* r) {
     // face and cell cotangents: 7 rows each for the reconstructed values
     // (5 used by the second-order path, 7 prim rows by the first-order one)
     HIP_TRY(al((void**)&r->adj_face, sizeof(double) * (7 + 7 + 4 + 4 + 12) * F));
-    HIP_TRY(al((void**)&r->adj_cell, sizeof(double) * (7 + 15 + 5 + 4 + 12) * N));
-    HIP_TRY(al((void**)&r->adj_extj, sizeof(int) * 5 * N));
+    HIP_TRY(al((void**)&r->adj_cell, sizeof(double) * (7 + 4 + 9 * (size_t)D.maxf) * N));
     HIP_TRY(al((void**)&r->adj_bcb, sizeof(double) * NPRIM * nb));
     HIP_TRY(al((void**)&r->adj_zero, sizeof(double) * nc * Nt));
     HIP_TRY(al((void**)&r->adj_dudt, sizeof(double) * nc * Nt));
@@ -4101,11 +4099,8 @@ ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* 
     B.Gb = fa + 22 * F;
     double* ca = r->adj_cell;
     B.cself = ca;
-    B.cg = ca + 7 * N;
-    B.cext = ca + 22 * N;
-    B.vself = ca + 27 * N;
-    B.vg = ca + 31 * N;
-    B.cextj = r->adj_extj;
+    B.vself = ca + 7 * N;
+    B.send = ca + 11 * N;
     B.bcb = r->adj_bcb;
     B.pb = prim_bar;
     B.lsqw = r->adj_lsqw;
