@@ -469,13 +469,15 @@ def main():
     # float): time and drift per conserved variable against the FP64
     # residual (itself equal to the oracle)
     fp32 = None
-    if not args.no_fp32 and args.workload != "plate":
+    if not args.no_fp32:
         fv.compute_dudt(prim, gb, dudt)
         d64 = dudt.clone()
         names = (["mass", "momentum_x", "momentum_y", "momentum_z", "energy"] if gb.dim == 3
                  else ["mass", "momentum_x", "momentum_y", "energy"])
         fp32 = {}
-        for mode in ("fp32_flux", "fp32_recon"):
+        # the FP32-reconstruction mode is inviscid-only
+        modes = ("fp32_flux",) if cfg.viscous else ("fp32_flux", "fp32_recon")
+        for mode in modes:
             fv.set_precision(mode)
             d32 = torch.zeros_like(dudt)
             for _ in range(2):

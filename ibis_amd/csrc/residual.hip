@@ -671,15 +671,16 @@ __global__ void __launch_bounds__(BLOCK)
 // running them back to back per face is running each pass over all faces.
 // prim_c (the adjoint's copy of the state between the two passes): when not
 // null, the ghost's whole row is also stored there.
+template <class W = double>
 __global__ void __launch_bounds__(BLOCK)
     k_bc_visc_fs(DevMesh m, const int* __restrict__ kinds, const double* __restrict__ pars,
                  int ntab, int has_pre, int has_visc, GasParams gp, VarSet vs,
-                 double* __restrict__ prim, double* __restrict__ FL, double* __restrict__ gconv,
+                 double* __restrict__ prim, W* __restrict__ FL, double* __restrict__ gconv,
                  double* __restrict__ prim_c) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= m.nb) return;
     if (has_pre) bc_actions<double>(m, kinds, pars, gp, prim, b);
-    ghost_face_values<2, double>(m, vs, prim, FL, FL, b);
+    ghost_face_values<2, W>(m, vs, prim, FL, FL, b);
     const int g = m.b_ghost[b];
 #pragma unroll
     for (int v = 0; v < 5; v++)
@@ -3212,6 +3213,14 @@ void launch_cell_visc_l(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& 
     const size_t shm = cell_stage_bytes(D, LIM, r->opt_stage);
     const int stage = shm ? 1 : 0;
     const size_t shs = std::max(visc_stencil_bytes(D), shm);
+    if (r->prec == IBIS_PREC_FP32_FLUX) {  // float face values
+        float* fl = reinterpret_cast<float*>(r->FL);
+        k_cell_fs<DIM, LIM, 2, true, float, false, false, true, true>
+            <<<xcd_grid(D.ntiles), D.tile, shs, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps,
+                                                      d.venkat_K, D.ntiles, write_slots, fl, fl,
+                                                      r->offs, prim, stage, r->gconv, vgrad);
+        return;
+    }
     k_cell_fs<DIM, LIM, 2, true, double, false, false, true, true>
         <<<xcd_grid(D.ntiles), D.tile, shs, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps,
                                                   d.venkat_K, D.ntiles, write_slots, r->FL,
@@ -3234,7 +3243,7 @@ void launch_cell_visc(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
 bool visc_fusable(const ibis_residual* r, const VarSet& vs) {
     const DevMesh& D = r->d;
     const ibis_fv_desc& d = r->desc;
-    return d.viscous && d.order == 2 && !r->has_post && r->prec == IBIS_PREC_FP64 &&
+    return d.viscous && d.order == 2 && !r->has_post && r->prec != IBIS_PREC_FP32_RECON &&
            r->opt_fuse_visc && r->opt_layout == 2 && r->opt_face_kernel == 1 && r->ent_idx &&
            r->vterm && r->offs && r->opt_offsets && !r->opt_pack && r->opt_stage_cells &&
            D.n_from_t && visc_stencil_bytes(D) <= 65536 && vs.row[2] == P_VX &&
@@ -3244,10 +3253,24 @@ bool visc_fusable(const ibis_residual* r, const VarSet& vs) {
 // the fused viscous path's face kernel (per-entry face data, FP64)
 template <int DIM, int FLUX>
 void launch_face_visc_f(const DevMesh& D, const FaceParams& fp, const int2* eidx,
-                        const double* egeom, const double* FL, double* dudt, const ViscFace& vf,
-                        hipStream_t st) {
+                        const double* egeom, const void* eg32, const double* FL, bool f32,
+                        double* dudt, const ViscFace& vf, hipStream_t st) {
     const unsigned g = xcd_grid(D.ntiles);
     const size_t shm = (size_t)(DIM + 2) * D.maxtf * sizeof(double);
+    const ibis_stage_epilogue none{};
+    if (f32) {  // float face values and flux arithmetic, FP64 viscous terms and sums
+        const float* fl = reinterpret_cast<const float*>(FL);
+        if (DIM == 2 && D.planar2d)
+            k_face_entry_fs<DIM, FLUX, false, float, true, false, false, true><<<g, D.tile, shm, st>>>(
+                D, fp, eidx, egeom, fl, dudt, none, vf);
+        else if (eg32)
+            k_face_entry_fs<DIM, FLUX, false, float, false, true, false, true><<<g, D.tile, shm, st>>>(
+                D, fp, eidx, reinterpret_cast<const double*>(eg32), fl, dudt, none, vf);
+        else
+            k_face_entry_fs<DIM, FLUX, false, float, false, false, false, true><<<g, D.tile, shm, st>>>(
+                D, fp, eidx, egeom, fl, dudt, none, vf);
+        return;
+    }
     if (DIM == 2 && D.planar2d)
         k_face_entry_fs<DIM, FLUX, false, double, true, false, false, true><<<g, D.tile, shm, st>>>(
             D, fp, eidx, egeom, FL, dudt, ibis_stage_epilogue{}, vf);
@@ -3258,22 +3281,16 @@ void launch_face_visc_f(const DevMesh& D, const FaceParams& fp, const int2* eidx
 
 template <int DIM>
 void launch_face_visc(const DevMesh& D, const FaceParams& fp, int flux, const int2* eidx,
-                      const double* egeom, const double* FL, double* dudt, const ViscFace& vf,
-                      hipStream_t st) {
+                      const double* egeom, const void* eg32, const double* FL, bool f32,
+                      double* dudt, const ViscFace& vf, hipStream_t st) {
+#define IBIS_FV(K) launch_face_visc_f<DIM, K>(D, fp, eidx, egeom, eg32, FL, f32, dudt, vf, st)
     switch (flux) {
-        case IBIS_FLUX_HANEL:
-            launch_face_visc_f<DIM, IBIS_FLUX_HANEL>(D, fp, eidx, egeom, FL, dudt, vf, st);
-            break;
-        case IBIS_FLUX_AUSMDV:
-            launch_face_visc_f<DIM, IBIS_FLUX_AUSMDV>(D, fp, eidx, egeom, FL, dudt, vf, st);
-            break;
-        case IBIS_FLUX_LDFSS2:
-            launch_face_visc_f<DIM, IBIS_FLUX_LDFSS2>(D, fp, eidx, egeom, FL, dudt, vf, st);
-            break;
-        default:
-            launch_face_visc_f<DIM, IBIS_FLUX_RUSANOV>(D, fp, eidx, egeom, FL, dudt, vf, st);
-            break;
+        case IBIS_FLUX_HANEL: IBIS_FV(IBIS_FLUX_HANEL); break;
+        case IBIS_FLUX_AUSMDV: IBIS_FV(IBIS_FLUX_AUSMDV); break;
+        case IBIS_FLUX_LDFSS2: IBIS_FV(IBIS_FLUX_LDFSS2); break;
+        default: IBIS_FV(IBIS_FLUX_RUSANOV); break;
     }
+#undef IBIS_FV
 }
 
 // thermo reconstruction pair: (slot, prim row) of the two variables,
@@ -3460,21 +3477,28 @@ ibis_status compute_dudt_impl(ibis_residual* r, double* prim, double* dudt,
     VarSet vs;
     residual_params(r, &fp, &vs);
     const bool fused = !d.viscous && !r->has_post;
-    if (r->prec != IBIS_PREC_FP64 && !(fused && recon))
+    const bool visc_fused = recon && !epi && visc_fusable(r, vs);
+    if (r->prec != IBIS_PREC_FP64 && !(fused && recon) &&
+        !(visc_fused && r->prec == IBIS_PREC_FP32_FLUX))
         return ibis::fail(IBIS_ERR_INVALID,
-                          "FP32 flux mode needs the inviscid second-order residual without "
-                          "flux-override boundaries");
+                          "the reduced-precision modes need the second-order residual without "
+                          "flux-override boundaries (FP32 reconstruction: inviscid only; "
+                          "viscous: with the options of the fused viscous path)");
     // viscous, second order: boundary actions (both passes), cell kernel
     // (viscous gradients, convective gradients, limiter, face values), face
     // kernel (convective and viscous flux, surface integral)
-    const bool visc_fused = recon && !epi && visc_fusable(r, vs);
     if (visc_fused) {
+        const bool f32 = r->prec == IBIS_PREC_FP32_FLUX;
         const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
         pf.beg();
-        if (D.nb > 0)
-            k_bc_visc_fs<<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, r->ntab,
-                                                          r->has_pre ? 1 : 0, r->has_visc ? 1 : 0,
-                                                          gp, vs, prim, r->FL, r->gconv, nullptr);
+        if (D.nb > 0 && f32)
+            k_bc_visc_fs<float><<<nblocks(D.nb), BLOCK, 0, st>>>(
+                D, r->kinds, r->pars, r->ntab, r->has_pre ? 1 : 0, r->has_visc ? 1 : 0, gp, vs,
+                prim, reinterpret_cast<float*>(r->FL), r->gconv, nullptr);
+        else if (D.nb > 0)
+            k_bc_visc_fs<double><<<nblocks(D.nb), BLOCK, 0, st>>>(
+                D, r->kinds, r->pars, r->ntab, r->has_pre ? 1 : 0, r->has_visc ? 1 : 0, gp, vs,
+                prim, r->FL, r->gconv, nullptr);
         pf.end(0);
         pf.beg();
         if (D.dim == 3)
@@ -3494,9 +3518,11 @@ ibis_status compute_dudt_impl(ibis_residual* r, double* prim, double* dudt,
         pf.beg();
         const ViscFace vf{r->vterm};
         if (D.dim == 3)
-            launch_face_visc<3>(D, fp, d.flux, r->ent_idx, r->ent_geom, r->FL, dudt, vf, st);
+            launch_face_visc<3>(D, fp, d.flux, r->ent_idx, r->ent_geom, r->ent_geom32, r->FL, f32,
+                                dudt, vf, st);
         else
-            launch_face_visc<2>(D, fp, d.flux, r->ent_idx, r->ent_geom, r->FL, dudt, vf, st);
+            launch_face_visc<2>(D, fp, d.flux, r->ent_idx, r->ent_geom, r->ent_geom32, r->FL, f32,
+                                dudt, vf, st);
         pf.end(2);
         HIP_TRY(hipGetLastError());
         return IBIS_OK;
@@ -4024,10 +4050,9 @@ ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* 
         HIP_TRY(hipMemcpyAsync(r->adj_prim_c, prim, sizeof(double) * NPRIM * Nt,
                                hipMemcpyDeviceToDevice, st));
         if (D.nb > 0)
-            k_bc_visc_fs<<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, r->ntab,
-                                                          r->has_pre ? 1 : 0, r->has_visc ? 1 : 0,
-                                                          gp, vs, prim, r->FL, r->gconv,
-                                                          r->adj_prim_c);
+            k_bc_visc_fs<double><<<nblocks(D.nb), BLOCK, 0, st>>>(
+                D, r->kinds, r->pars, r->ntab, r->has_pre ? 1 : 0, r->has_visc ? 1 : 0, gp, vs,
+                prim, r->FL, r->gconv, r->adj_prim_c);
         if (D.dim == 3)
             launch_cell_visc<3>(D, vs, d, r, prim, r->adj_vgrad, 1, st);
         else

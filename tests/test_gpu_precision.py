@@ -93,15 +93,63 @@ def test_mixed_elements_fp32_drift(mode):
         assert mx < 1e-3, (name, mx)
 
 
-def test_fp32_flux_rejected_where_unsupported():
+def test_fp32_modes_rejected_where_unsupported():
+    """The reduced-precision modes need a fused second-order path: the
+    viscous plate on the general path (fuse_viscous 0) refuses both, a
+    first-order residual refuses both; unknown mode names raise."""
     import torch
     from ibis_amd.native import IbisError
     g_io, cfg, fs = cases.plate_case(16, 8)
     gb, fv = gpu_objects(g_io, cfg)
     prim = to_dev(gb.to_grid_order(cases.perturbed_prim(g_io, fs, gb.Nt, seed=2)))
     d = torch.zeros((gb.dim + 2, gb.Nt), dtype=torch.float64, device="cuda")
-    fv.set_precision("fp32_flux")
-    with pytest.raises(IbisError):
-        fv.compute_dudt(prim, gb, d)
+    fv.set_option("fuse_viscous", 0)
+    for mode in ("fp32_flux", "fp32_recon"):
+        fv.set_precision(mode)
+        with pytest.raises(IbisError):
+            fv.compute_dudt(prim.clone(), gb, d)
+    fv.set_precision("fp64")
+    g_io, cfg, fs = cases.wedge_case(24, 12, order=1)
+    gb, fv = gpu_objects(g_io, cfg)
+    prim = to_dev(gb.to_grid_order(cases.perturbed_prim(g_io, fs, gb.Nt, seed=3)))
+    d = torch.zeros((gb.dim + 2, gb.Nt), dtype=torch.float64, device="cuda")
+    for mode in ("fp32_flux", "fp32_recon"):
+        fv.set_precision(mode)
+        with pytest.raises(IbisError):
+            fv.compute_dudt(prim.clone(), gb, d)
     with pytest.raises(IbisError):
         fv.set_precision("bf16")
+
+
+@pytest.mark.parametrize("name", ["plate", "hex_walls"])
+def test_viscous_fp32_flux_drift(name):
+    """The FP32-flux mode on the fused viscous path (float face values and
+    convective flux; FP64 viscous terms, area products and sums) drifts
+    from the FP64 residual by less than 1e-3 per conserved variable, and
+    switching back restores the FP64 bits.  The FP32-reconstruction mode
+    stays inviscid-only and is refused."""
+    import torch
+    from test_gpu_residual import _visc_cases
+    from ibis_amd.native import IbisError
+    g_io, cfg, prim_ref = _visc_cases()[name]
+    gb, fv = gpu_objects(g_io, cfg)
+    prim = to_dev(gb.to_grid_order(prim_ref))
+    d64 = torch.zeros((gb.dim + 2, gb.Nt), dtype=torch.float64, device="cuda")
+    d32 = torch.zeros_like(d64)
+    fv.compute_dudt(prim.clone(), gb, d64)
+    fv.set_precision("fp32_flux")
+    fv.compute_dudt(prim.clone(), gb, d32)
+    fv.set_precision("fp64")
+    d64b = torch.zeros_like(d64)
+    fv.compute_dudt(prim.clone(), gb, d64b)
+    torch.cuda.synchronize()
+    assert torch.equal(d64, d64b)
+    dr = drift(d32.cpu().numpy(), d64.cpu().numpy(), gb.N)
+    print(f"{name} viscous fp32_flux drift {dr}", flush=True)
+    for q, (mx, rms) in dr.items():
+        assert mx < 1e-3 and rms < 1e-3, (q, mx, rms)
+    assert not torch.equal(d32, d64)  # the float path did run
+    fv.set_precision("fp32_recon")
+    with pytest.raises(IbisError):
+        fv.compute_dudt(prim.clone(), gb, d32)
+    fv.set_precision("fp64")
