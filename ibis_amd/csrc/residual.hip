@@ -114,6 +114,8 @@ struct DevMesh {
     const unsigned char* slotL;
     const unsigned char* slotR;
     int tile;      // cells per tile (and threads per block of the tile kernels)
+    int lsq_from_cen;  // lsq equals its expression of the centroids bit for bit, so a
+                       // kernel with the stencil's centroids in LDS forms it there
     int n_from_t;  // face normals equal cross(t1, t2) bit for bit (grid-built meshes)
     int planar2d;  // 2D, and every face has t1z = +0 and t2 = (+0, +0, +1) bit for bit
 };
@@ -845,12 +847,19 @@ __global__ void __launch_bounds__(MAXTILE)
     // discards them)
     const int il = i < N ? i : 0;
     const int nf = m.nfaces[il];
-    const double r11 = m.lsq[il], r12 = m.lsq[N + il], r22 = m.lsq[2 * (size_t)N + il];
-    double r13 = 0.0, r23 = 0.0, r33 = 0.0;
-    if (DIM == 3) {
-        r13 = m.lsq[3 * (size_t)N + il];
-        r23 = m.lsq[4 * (size_t)N + il];
-        r33 = m.lsq[5 * (size_t)N + il];
+    // the LSQ factors: formed from the staged stencil's centroids below
+    // when the mesh's equal that expression (SN), else read
+    const bool lsq_lds = SN && m.lsq_from_cen;
+    double r11 = 1.0, r12 = 0.0, r22 = 1.0, r13 = 0.0, r23 = 0.0, r33 = 0.0;
+    if (!lsq_lds) {
+        r11 = m.lsq[il];
+        r12 = m.lsq[N + il];
+        r22 = m.lsq[2 * (size_t)N + il];
+        if (DIM == 3) {
+            r13 = m.lsq[3 * (size_t)N + il];
+            r23 = m.lsq[4 * (size_t)N + il];
+            r33 = m.lsq[5 * (size_t)N + il];
+        }
     }
     int kst[MAXF];  // the neighbours' positions in the LDS stencil
     if (SN) {
@@ -877,6 +886,32 @@ __global__ void __launch_bounds__(MAXTILE)
         zi = soff[7 * SS + threadIdx.x];
     } else {
         load_cell<PACKED>(m, vs, pack, prim, i, ui, xi, yi, zi);
+    }
+    if (SN && lsq_lds) {  // GR/gradient.h:119-162, the grid builder's expressions and order
+        double sxx = 0.0, sxy = 0.0, sxz = 0.0, syy = 0.0, syz = 0.0, szz = 0.0;
+#pragma unroll
+        for (int j = 0; j < MAXF; j++) {
+            if (j < nf) {
+                const int k = kst[j];
+                const double dx = soff[5 * SS + k] - xi;
+                const double dy = soff[6 * SS + k] - yi;
+                const double dz = soff[7 * SS + k] - zi;
+                sxx += dx * dx;
+                sxy += dx * dy;
+                sxz += dx * dz;
+                syy += dy * dy;
+                syz += dy * dz;
+                szz += dz * dz;
+            }
+        }
+        r11 = sqrt(sxx);
+        r12 = 1.0 / r11 * sxy;
+        r22 = sqrt(syy - r12 * r12);
+        if (DIM == 3) {
+            r13 = 1.0 / r11 * sxz;
+            r23 = 1.0 / r22 * (syz - r12 / r11 * sxz);
+            r33 = sqrt(szz - (r13 * r13 + r23 * r23));
+        }
     }
     const double beta = (r12 * r23 - r13 * r23) / (r11 * r22);  // GR/gradient.h:89 (sic)
     const double r11s = r11 * r11;
@@ -2567,6 +2602,35 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
             return ts;
         }
     }
+    // the LSQ factors as the grid builder forms them from the centroids
+    // (WLSGradient::compute_weights, GR/gradient.h:119-162): when a caller's
+    // mesh holds exactly these, the staged cell kernel recomputes them from
+    // its LDS stencil instead of reading 8 * nl bytes per cell
+    int lsq_from_cen = 1;
+    for (int64_t i = 0; i < N && lsq_from_cen; i++) {
+        const double* c = d->centroid;
+        double sxx = 0.0, sxy = 0.0, sxz = 0.0, syy = 0.0, syz = 0.0, szz = 0.0;
+        for (int64_t j = d->cell_face_off[i]; j < d->cell_face_off[i + 1]; j++) {
+            const int64_t n = d->cell_neighbours[j];
+            const double dx = c[n] - c[i], dy = c[Nt + n] - c[Nt + i],
+                         dz = c[2 * Nt + n] - c[2 * Nt + i];
+            sxx += dx * dx;
+            sxy += dx * dy;
+            sxz += dx * dz;
+            syy += dy * dy;
+            syz += dy * dz;
+            szz += dz * dz;
+        }
+        const double r11 = std::sqrt(sxx);
+        const double r12 = 1.0 / r11 * sxy;
+        const double r22 = std::sqrt(syy - r12 * r12);
+        const double r13 = 1.0 / r11 * sxz;
+        const double r23 = 1.0 / r22 * (syz - r12 / r11 * sxz);
+        const double r33 = std::sqrt(szz - (r13 * r13 + r23 * r23));
+        const double want[6] = {r11, r12, r22, r13, r23, r33};
+        for (int a = 0; a < D.nl; a++)
+            if (std::memcmp(&want[a], d->lsq + (size_t)a * N + i, 8)) lsq_from_cen = 0;
+    }
     int n_from_t = 1;
     for (int64_t f = 0; f < F && n_from_t; f++) {
         const double* gg = d->face_geom;
@@ -2622,6 +2686,7 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
     m->tiles[TILE] = D;
     D.n_from_t = n_from_t;
     D.planar2d = planar2d;
+    D.lsq_from_cen = lsq_from_cen;
     *out = m;
     return IBIS_OK;
 }
