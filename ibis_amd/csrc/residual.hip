@@ -1190,12 +1190,19 @@ __global__ void __launch_bounds__(MAXTILE)
     // staging barrier (a thread past the last cell reads cell 0's)
     const int il = i < N ? i : 0;
     const int nf = m.nfaces[il];
-    const double l11 = m.lsq[il], l12 = m.lsq[N + il], l22 = m.lsq[2 * (size_t)N + il];
-    double l13 = 0.0, l23 = 0.0, l33 = 0.0;
-    if (DIM == 3) {
-        l13 = m.lsq[3 * (size_t)N + il];
-        l23 = m.lsq[4 * (size_t)N + il];
-        l33 = m.lsq[5 * (size_t)N + il];
+    // the LSQ factors in FP64: formed from the LDS stencil as k_cell_fs
+    // does when the mesh's equal that expression, else read
+    const bool lsq_lds = SN && m.lsq_from_cen;
+    double l11 = 1.0, l12 = 0.0, l22 = 1.0, l13 = 0.0, l23 = 0.0, l33 = 0.0;
+    if (!lsq_lds) {
+        l11 = m.lsq[il];
+        l12 = m.lsq[N + il];
+        l22 = m.lsq[2 * (size_t)N + il];
+        if (DIM == 3) {
+            l13 = m.lsq[3 * (size_t)N + il];
+            l23 = m.lsq[4 * (size_t)N + il];
+            l33 = m.lsq[5 * (size_t)N + il];
+        }
     }
     int kst[MAXF];  // the neighbours' positions in the LDS stencil
     if (SN) {
@@ -1219,6 +1226,32 @@ __global__ void __launch_bounds__(MAXTILE)
         zi = sh32[7 * SS + threadIdx.x];
     } else {
         load_cell<false>(m, vs, nullptr, prim, i, ui, xi, yi, zi);
+    }
+    if (SN && lsq_lds) {  // GR/gradient.h:119-162, the grid builder's expressions and order
+        double sxx = 0.0, sxy = 0.0, sxz = 0.0, syy = 0.0, syz = 0.0, szz = 0.0;
+#pragma unroll
+        for (int j = 0; j < MAXF; j++) {
+            if (j < nf) {
+                const int k = kst[j];
+                const double dx = sh32[5 * SS + k] - xi;
+                const double dy = sh32[6 * SS + k] - yi;
+                const double dz = sh32[7 * SS + k] - zi;
+                sxx += dx * dx;
+                sxy += dx * dy;
+                sxz += dx * dz;
+                syy += dy * dy;
+                syz += dy * dz;
+                szz += dz * dz;
+            }
+        }
+        l11 = sqrt(sxx);
+        l12 = 1.0 / l11 * sxy;
+        l22 = sqrt(syy - l12 * l12);
+        if (DIM == 3) {
+            l13 = 1.0 / l11 * sxz;
+            l23 = 1.0 / l22 * (syz - l12 / l11 * sxz);
+            l33 = sqrt(szz - (l13 * l13 + l23 * l23));
+        }
     }
     const float r11 = (float)l11, r12 = (float)l12, r22 = (float)l22;
     const float r13 = (float)l13, r23 = (float)l23, r33 = (float)l33;
