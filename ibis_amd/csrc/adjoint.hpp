@@ -574,23 +574,19 @@ __global__ void __launch_bounds__(BLOCK)
                 for (int k = 0; k < 3; k++) gb[k] += d2b * dstar[k];
             }
         }
-        // own LSQ stencil: g = sum_j w_j (u_j - u_i)
-        if (v < ngrad) {
-            for (int j = 0; j < nf; j++) {
-                const double* w = B.lsqw + (size_t)j * 3 * N + i;
-                self -= w[0] * gb[0] + w[N] * gb[1] + w[2 * (size_t)N] * gb[2];
-            }
-        } else {
-            gb[0] = gb[1] = gb[2] = 0.0;  // 2D vz: no gradient is computed
-        }
-        B.cself[(size_t)v * N + i] = self;
+        // own LSQ stencil g = sum_j w_j (u_j - u_i): the cell's own term and
+        // what slot j's neighbour receives (2D vz: no gradient is computed)
         for (int j = 0; j < nf; j++) {
             const double* w = B.lsqw + (size_t)j * 3 * N + i;
             double x = 0.0;
-            if (v < ngrad) x = w[0] * gb[0] + w[N] * gb[1] + w[2 * (size_t)N] * gb[2];
+            if (v < ngrad) {
+                x = w[0] * gb[0] + w[N] * gb[1] + w[2 * (size_t)N] * gb[2];
+                self -= x;
+            }
             if (extj == j) x += ext;
             B.send[((size_t)j * 9 + v) * N + i] = x;
         }
+        B.cself[(size_t)v * N + i] = self;
     }
     if (VISC) {
         for (int k = 0; k < 4; k++) {
@@ -604,14 +600,11 @@ __global__ void __launch_bounds__(BLOCK)
             }
             for (int j = 0; j < nf; j++) {
                 const double* w = B.lsqw + (size_t)j * 3 * N + i;
-                self -= w[0] * gb[0] + w[N] * gb[1] + w[2 * (size_t)N] * gb[2];
+                const double x = w[0] * gb[0] + w[N] * gb[1] + w[2 * (size_t)N] * gb[2];
+                self -= x;
+                B.send[((size_t)j * 9 + 5 + k) * N + i] = x;
             }
             B.vself[(size_t)k * N + i] = self;
-            for (int j = 0; j < nf; j++) {
-                const double* w = B.lsqw + (size_t)j * 3 * N + i;
-                B.send[((size_t)j * 9 + 5 + k) * N + i] =
-                    w[0] * gb[0] + w[N] * gb[1] + w[2 * (size_t)N] * gb[2];
-            }
         }
     }
 }
