@@ -14,6 +14,7 @@
 //   k_vjp_visc_face  viscous flux (viscous_flux.cpp:133-184), Hasselbacher
 //                    face gradients and face averages -> bars of T, v of
 //                    both sides and of the two cells' viscous gradients
+//   k_vjp_face_cv    the two above in one pass (second-order viscous)
 //   k_vjp_cell       MUSCL reconstruction, limiter (active or frozen) and
 //                    the cell's own LSQ stencil
 //   k_vjp_ghost      ghost cells: their stencil contributions, then the
@@ -22,9 +23,9 @@
 //   k_vjp_gather     per valid cell: its own terms plus what its
 //                    neighbours' stencils and its ghosts send back
 //
-// Nothing is stored by the forward pass beyond what the residual keeps
-// anyway (gradients, limiters, the state before and after the viscous
-// ghost fill): face states and fluxes are recomputed.  All sums are
+// The forward pass keeps the gradients, the limiters, the ghosts' state
+// before and after the viscous ghost fill and, on the fused paths, the face
+// values its cell kernel wrote; fluxes are recomputed.  All sums are
 // gathers in a fixed order, so the result is deterministic.
 #ifndef IBIS_ADJOINT_HPP
 #define IBIS_ADJOINT_HPP
@@ -34,6 +35,9 @@
 struct AdjBufs {
     // forward state
     const double* prim_c;  // 7 x Nt, ghosts after the pre-reconstruction actions
+    // or, when set, the ghosts' rows after the pre-reconstruction actions
+    // (7 x G), prim_c then holding only the valid rows (prim's own)
+    const double* prim_cg;
     const double* prim_v;  // 7 x Nt, ghosts after the viscous actions
     const double* cgrad;   // NGRAD x 3 x N convective gradients
     const double* clim;    // NGRAD x N limiters
@@ -94,6 +98,14 @@ __device__ __forceinline__ void lsq_weights(const DevMesh& m, int c, double dx, 
     w[0] = a1 - q12 * a2 + beta * a3;
     w[1] = a2 - q23 * a3;
     w[2] = a3;
+}
+
+// w . gb for a weight triple stored at stride N (the table below); in 2D the
+// z-weight is 0 (lsq_weights), so the z terms are left out
+template <int DIM>
+__device__ __forceinline__ double lsq_dot(const double* w, int N, const double* gb) {
+    if (DIM == 3) return w[0] * gb[0] + w[N] * gb[1] + w[2 * (size_t)N] * gb[2];
+    return w[0] * gb[0] + w[N] * gb[1];
 }
 
 // the weight table: lsq_weights of cell i for the neighbour in each of its
@@ -158,19 +170,15 @@ __device__ __forceinline__ bool constant_flux_face(const DevMesh& m, const AdjBu
     return false;
 }
 
-// Convective face flux backwards: bars of the reconstructed values
+// Convective face flux of face f backwards, from its cotangent fb: bars of
+// the reconstructed values
 template <int DIM, int FLUX>
-__global__ void __launch_bounds__(BLOCK)
-    k_vjp_conv_face(DevMesh m, FaceParams fp, VarSet vs, AdjBufs B) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void conv_face_rev(const DevMesh& m, const FaceParams& fp,
+                                              const VarSet& vs, const AdjBufs& B, int f,
+                                              const double (&fb)[DIM + 2]) {
     const int F = m.F, N = m.N, Nt = m.Nt;
-    if (f >= F) return;
     constexpr int NC = DIM + 2;
     constexpr bool T3 = DIM == 3;
-    double fb[NC];
-    face_flux_bar<DIM>(m, B, f, fb);
-    if (constant_flux_face(m, B, f))
-        for (int q = 0; q < NC; q++) fb[q] = 0.0;
     // forward: reconstruction of both sides (FV/convective_flux.cpp:243-420)
     double q[2][5], st[2][NPRIM];
     for (int side = 0; side < 2; side++) {
@@ -184,7 +192,9 @@ __global__ void __launch_bounds__(BLOCK)
             const double dy = m.geom[10 * (size_t)F + f] - m.cen[Nt + c];
             const double dz = m.geom[11 * (size_t)F + f] - m.cen[2 * (size_t)Nt + c];
             for (int v = 0; v < 5; v++) {
-                const double u = B.prim_c[(size_t)vs.row[v] * Nt + c];
+                const double u = (B.prim_cg && c >= N)
+                                     ? B.prim_cg[(size_t)vs.row[v] * m.G + (c - N)]
+                                     : B.prim_c[(size_t)vs.row[v] * Nt + c];
                 double gx = 0.0, gy = 0.0, gz = 0.0, L = 1.0;
                 if (valid) {
                     const double* g = B.cgrad + (size_t)vs.slot[v] * 3 * N;
@@ -251,6 +261,19 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
+// Convective face flux backwards, one thread per face
+template <int DIM, int FLUX>
+__global__ void __launch_bounds__(BLOCK)
+    k_vjp_conv_face(DevMesh m, FaceParams fp, VarSet vs, AdjBufs B) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F) return;
+    double fb[DIM + 2];
+    face_flux_bar<DIM>(m, B, f, fb);
+    if (constant_flux_face(m, B, f))
+        for (int q = 0; q < DIM + 2; q++) fb[q] = 0.0;
+    conv_face_rev<DIM, FLUX>(m, fp, vs, B, f, fb);
+}
+
 // First-order convective face flux backwards (copy_reconstruct,
 // FV/convective_flux.cpp:212-240): the face states are the two cells'
 // whole prim rows, rotated to the face frame, so the flux's cotangents go
@@ -314,16 +337,14 @@ __device__ __forceinline__ double sutherland_dT(double T, double mu, const ViscP
     return mu * (1.5 / T - 1.0 / (T + vp.Ts));
 }
 
-// Viscous face flux backwards (viscous_flux.cpp:16-184)
+// Viscous face flux of face f backwards (viscous_flux.cpp:16-184), from its
+// cotangent fb
 template <int DIM>
-__global__ void __launch_bounds__(BLOCK)
-    k_vjp_visc_face(DevMesh m, ViscParams vp, AdjBufs B) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void visc_face_rev(const DevMesh& m, const ViscParams& vp,
+                                              const AdjBufs& B, int f,
+                                              const double (&fb)[DIM + 2]) {
     const int F = m.F, N = m.N, Nt = m.Nt;
-    if (f >= F) return;
     constexpr int NC = DIM + 2;
-    double fb[NC];
-    face_flux_bar<DIM>(m, B, f, fb);
     const int l = m.L[f], r = m.R[f];
     const bool lv = l < N, rv = r < N;
     const double n[3] = {m.geom[f], m.geom[(size_t)F + f], m.geom[2 * (size_t)F + f]};
@@ -431,8 +452,36 @@ __global__ void __launch_bounds__(BLOCK)
     for (int k = 0; k < 4; k++) {
         B.VLb[(size_t)k * F + f] = Lv[k];
         B.VRb[(size_t)k * F + f] = Rv[k];
-        for (int d = 0; d < 3; d++) B.Gb[(size_t)(k * 3 + d) * F + f] = G[k][d];
+        // in 2D the z component is not stored: its one reader (k_vjp_cell)
+        // multiplies it by an LSQ z-weight, which is 0 in 2D
+        for (int d = 0; d < DIM; d++) B.Gb[(size_t)(k * 3 + d) * F + f] = G[k][d];
     }
+}
+
+// Viscous face flux backwards, one thread per face
+template <int DIM>
+__global__ void __launch_bounds__(BLOCK) k_vjp_visc_face(DevMesh m, ViscParams vp, AdjBufs B) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F) return;
+    double fb[DIM + 2];
+    face_flux_bar<DIM>(m, B, f, fb);
+    visc_face_rev<DIM>(m, vp, B, f, fb);
+}
+
+// The convective and viscous face reverses of a second-order viscous
+// residual in one pass over the faces: each face's flux cotangent is formed
+// once (the convective one then zeroed on a constant-flux boundary face)
+template <int DIM, int FLUX>
+__global__ void __launch_bounds__(BLOCK)
+    k_vjp_face_cv(DevMesh m, FaceParams fp, VarSet vs, ViscParams vp, AdjBufs B) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F) return;
+    double fb[DIM + 2];
+    face_flux_bar<DIM>(m, B, f, fb);
+    visc_face_rev<DIM>(m, vp, B, f, fb);
+    if (constant_flux_face(m, B, f))
+        for (int q = 0; q < DIM + 2; q++) fb[q] = 0.0;
+    conv_face_rev<DIM, FLUX>(m, fp, vs, B, f, fb);
 }
 
 // Per valid cell: reconstruction, limiter and own-stencil terms
@@ -474,6 +523,8 @@ __global__ void __launch_bounds__(BLOCK)
     for (int v = 0; v < (ORD1 ? 0 : 5); v++) {
         const int row = vs.row[v], slot = vs.slot[v];
         const double* u = B.prim_c + (size_t)row * Nt;
+        // ghost c's value at ug[c - N]
+        const double* ug = B.prim_cg ? B.prim_cg + (size_t)row * m.G : u + N;
         const double ui = u[i];
         const double* gg = B.cgrad + (size_t)slot * 3 * N;
         const double g[3] = {gg[i], gg[N + i], gg[2 * (size_t)N + i]};
@@ -487,7 +538,7 @@ __global__ void __launch_bounds__(BLOCK)
                                  sd[(j * 3 + 2) * BLOCK + tid]};
             self += qb;
             phib += qb * (g[0] * d[0] + g[1] * d[1] + g[2] * d[2]);
-            for (int k = 0; k < 3; k++) gb[k] += qb * phi * d[k];
+            for (int k = 0; k < DIM; k++) gb[k] += qb * phi * d[k];
         }
         double ext = 0.0;
         int extj = -1;
@@ -497,7 +548,8 @@ __global__ void __launch_bounds__(BLOCK)
             double umax = ui, umin = ui;
             int jmax = -1, jmin = -1;
             for (int j = 0; j < nf; j++) {
-                const double un = u[sn[j * BLOCK + tid]];
+                const int c = sn[j * BLOCK + tid];
+                const double un = c < N ? u[c] : ug[c - N];
                 if (!(umax > un)) {
                     umax = un;
                     jmax = j;
@@ -571,7 +623,7 @@ __global__ void __launch_bounds__(BLOCK)
                     self += d1b;  // the extreme is u_i itself: d1 = 0 identically
                 }
                 self -= d1b;
-                for (int k = 0; k < 3; k++) gb[k] += d2b * dstar[k];
+                for (int k = 0; k < DIM; k++) gb[k] += d2b * dstar[k];
             }
         }
         // own LSQ stencil g = sum_j w_j (u_j - u_i): the cell's own term and
@@ -580,7 +632,7 @@ __global__ void __launch_bounds__(BLOCK)
             const double* w = B.lsqw + (size_t)j * 3 * N + i;
             double x = 0.0;
             if (v < ngrad) {
-                x = w[0] * gb[0] + w[N] * gb[1] + w[2 * (size_t)N] * gb[2];
+                x = lsq_dot<DIM>(w, N, gb);
                 self -= x;
             }
             if (extj == j) x += ext;
@@ -596,11 +648,11 @@ __global__ void __launch_bounds__(BLOCK)
                 const bool left = fe >= 0;
                 const int f = left ? fe : ~fe;
                 self += (left ? B.VLb : B.VRb)[(size_t)k * F + f];
-                for (int d = 0; d < 3; d++) gb[d] += B.Gb[(size_t)(k * 3 + d) * F + f];
+                for (int d = 0; d < DIM; d++) gb[d] += B.Gb[(size_t)(k * 3 + d) * F + f];
             }
             for (int j = 0; j < nf; j++) {
                 const double* w = B.lsqw + (size_t)j * 3 * N + i;
-                const double x = w[0] * gb[0] + w[N] * gb[1] + w[2 * (size_t)N] * gb[2];
+                const double x = lsq_dot<DIM>(w, N, gb);
                 self -= x;
                 B.send[((size_t)j * 9 + 5 + k) * N + i] = x;
             }

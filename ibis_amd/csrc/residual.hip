@@ -671,8 +671,8 @@ __global__ void __launch_bounds__(BLOCK)
 // prim's ghost rows as after the reference's second ghost fill.  The
 // actions of a face read only its valid cell and write only its ghost, so
 // running them back to back per face is running each pass over all faces.
-// prim_c (the adjoint's copy of the state between the two passes): when not
-// null, the ghost's whole row is also stored there.
+// prim_c (the adjoint's record of the ghosts between the two passes): when
+// not null, the ghost's whole row is also stored there, 7 x G.
 template <class W = double>
 __global__ void __launch_bounds__(BLOCK)
     k_bc_visc_fs(DevMesh m, const int* __restrict__ kinds, const double* __restrict__ pars,
@@ -689,7 +689,8 @@ __global__ void __launch_bounds__(BLOCK)
         gconv[(size_t)v * m.G + (g - m.N)] = prim[(size_t)vs.row[v] * m.Nt + g];
     if (prim_c) {
 #pragma unroll
-        for (int k = 0; k < NPRIM; k++) prim_c[(size_t)k * m.Nt + g] = prim[(size_t)k * m.Nt + g];
+        for (int k = 0; k < NPRIM; k++)
+            prim_c[(size_t)k * m.G + (g - m.N)] = prim[(size_t)k * m.Nt + g];
     }
     if (has_visc) bc_actions<double>(m, kinds + ntab, pars + (size_t)ntab * IBIS_NPAR, gp, prim, b);
 }
@@ -4062,15 +4063,23 @@ void launch_vjp(ibis_residual* r, const VarSet& vs, const FaceParams& fp, const 
             case IBIS_FLUX_HANEL: k_vjp_conv_face1<DIM, IBIS_FLUX_HANEL><<<nf, BLOCK, 0, st>>>(D, fp, B); break;
             default: k_vjp_conv_face1<DIM, IBIS_FLUX_RUSANOV><<<nf, BLOCK, 0, st>>>(D, fp, B); break;
         }
-    } else {
+    }
+    ViscParams vp{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr};
+    if (!ord1 && visc) {  // both face reverses in one kernel
+        switch (d.flux) {
+            case IBIS_FLUX_AUSMDV: k_vjp_face_cv<DIM, IBIS_FLUX_AUSMDV><<<nf, BLOCK, 0, st>>>(D, fp, vs, vp, B); break;
+            case IBIS_FLUX_HANEL: k_vjp_face_cv<DIM, IBIS_FLUX_HANEL><<<nf, BLOCK, 0, st>>>(D, fp, vs, vp, B); break;
+            default: k_vjp_face_cv<DIM, IBIS_FLUX_RUSANOV><<<nf, BLOCK, 0, st>>>(D, fp, vs, vp, B); break;
+        }
+    } else if (!ord1) {
         switch (d.flux) {
             case IBIS_FLUX_AUSMDV: k_vjp_conv_face<DIM, IBIS_FLUX_AUSMDV><<<nf, BLOCK, 0, st>>>(D, fp, vs, B); break;
             case IBIS_FLUX_HANEL: k_vjp_conv_face<DIM, IBIS_FLUX_HANEL><<<nf, BLOCK, 0, st>>>(D, fp, vs, B); break;
             default: k_vjp_conv_face<DIM, IBIS_FLUX_RUSANOV><<<nf, BLOCK, 0, st>>>(D, fp, vs, B); break;
         }
+    } else if (visc) {
+        k_vjp_visc_face<DIM><<<nf, BLOCK, 0, st>>>(D, vp, B);
     }
-    ViscParams vp{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr};
-    if (visc) k_vjp_visc_face<DIM><<<nf, BLOCK, 0, st>>>(D, vp, B);
     // LDS of k_vjp_cell: maxf x 3 doubles and 2 x maxf ints per thread
     const size_t vshm = (size_t)D.maxf * BLOCK * (3 * sizeof(double) + 2 * sizeof(int));
 #define CELL(LIM, ACT, VIS) \
@@ -4137,16 +4146,15 @@ ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* 
     residual_params(r, &fp, &vs);
     // forward: the residual's state, gradients and limiters (recomputed);
     // on the fused viscous path with its own kernels: the boundary kernel
-    // stores the ghosts' rows between the two passes in prim_c (whose valid
-    // rows are prim's), the cell kernel writes the convective gradients and
-    // limiters and puts the viscous gradients in adj_vgrad
+    // stores the ghosts' rows between the two passes in adj_prim_c (7 x G;
+    // the valid rows are prim's), the cell kernel writes the convective
+    // gradients and limiters and puts the viscous gradients in adj_vgrad;
+    // on the fused inviscid path the state the reverse reads is prim itself
     const bool vis_fused = visc_fusable(r, vs);
     const bool inv_fused = !d.viscous && d.order == 2 && !r->has_post &&
                            r->prec == IBIS_PREC_FP64 && r->opt_layout == 2 && r->offs &&
                            r->opt_offsets && !r->opt_pack && r->opt_stage_cells;
     if (vis_fused) {
-        HIP_TRY(hipMemcpyAsync(r->adj_prim_c, prim, sizeof(double) * NPRIM * Nt,
-                               hipMemcpyDeviceToDevice, st));
         if (D.nb > 0)
             k_bc_visc_fs<double><<<nblocks(D.nb), BLOCK, 0, st>>>(
                 D, r->kinds, r->pars, r->ntab, r->has_pre ? 1 : 0, r->has_visc ? 1 : 0, gp, vs,
@@ -4165,8 +4173,6 @@ ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* 
             else
                 k_ghost_fs<2><<<nblocks(D.nb), BLOCK, 0, st>>>(D, vs, prim, r->FL, r->FR);
         }
-        HIP_TRY(hipMemcpyAsync(r->adj_prim_c, prim, sizeof(double) * NPRIM * Nt,
-                               hipMemcpyDeviceToDevice, st));
         r->force_slots = 1;
         if (D.dim == 3)
             launch_cell_fs<3>(D, vs, d, r, D.ntiles, prim, st);
@@ -4205,7 +4211,8 @@ ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* 
     }
     }
     AdjBufs B;
-    B.prim_c = r->adj_prim_c;
+    B.prim_c = (vis_fused || inv_fused) ? prim : r->adj_prim_c;
+    B.prim_cg = vis_fused ? r->adj_prim_c : nullptr;
     B.prim_v = prim;
     B.cgrad = r->grad;
     B.clim = r->lim;
