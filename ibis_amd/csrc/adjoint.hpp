@@ -358,7 +358,9 @@ __device__ __forceinline__ void visc_face_rev(const DevMesh& m, const ViscParams
     if (!interior) {
         const int c = lv ? l : r;
         for (int k = 0; k < 4; k++)
-            for (int d = 0; d < 3; d++) g[k][d] = B.vgrad[((size_t)slots[k] * 3 + d) * N + c];
+            for (int d = 0; d < 3; d++)
+                g[k][d] = d < 2 ? B.vgrad[((size_t)slots[k] * 3 + d) * N + c]
+                                : grad_z<double, DIM>(B.vgrad + (size_t)slots[k] * 3 * N, N, c);
     } else {
         const double ex = m.cen[r] - m.cen[l];
         const double ey = m.cen[Nt + r] - m.cen[Nt + l];
@@ -372,7 +374,7 @@ __device__ __forceinline__ void visc_face_rev(const DevMesh& m, const ViscParams
             const double* gg = B.vgrad + (size_t)slots[k] * 3 * N;
             const double ax = 0.5 * (gg[l] + gg[r]);
             const double ay = 0.5 * (gg[N + l] + gg[N + r]);
-            const double az = 0.5 * (gg[2 * (size_t)N + l] + gg[2 * (size_t)N + r]);
+            const double az = 0.5 * (grad_z<double, DIM>(gg, N, l) + grad_z<double, DIM>(gg, N, r));
             const double vl = P[(size_t)rows[k] * Nt + l], vr = P[(size_t)rows[k] * Nt + r];
             const double corr = ax * ehat[0] + ay * ehat[1] + az * ehat[2] - (vr - vl) / len_e;
             g[k][0] = ax - corr * n[0] / edn;

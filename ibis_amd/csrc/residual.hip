@@ -808,6 +808,14 @@ __device__ __forceinline__ void stage_stencil_visc(const DevMesh& m, const VarSe
     __syncthreads();
 }
 
+// In 2D every LSQ gradient's z component is +0 exactly (its weight is 0 and
+// the sum starts at +0; the gradient buffers start zeroed), so it is taken as
+// 0 there instead of being loaded (the fused cell kernel does not store it).
+template <class S, int DIM>
+__device__ __forceinline__ S grad_z(const S* __restrict__ g, int N, int c) {
+    return DIM == 3 ? g[2 * (size_t)N + c] : S(0.0);
+}
+
 __device__ __forceinline__ bool viscous_slot(int slot) {
     return slot == G_T || slot == G_VX || slot == G_VY || slot == G_VZ;
 }
@@ -957,7 +965,7 @@ __global__ void __launch_bounds__(MAXTILE)
         const double* g = grad + (size_t)G_VZ * 3 * N;
         vz_old[0] = g[i];
         vz_old[1] = g[N + i];
-        vz_old[2] = g[2 * (size_t)N + i];
+        vz_old[2] = grad_z<double, DIM>(g, N, i);
     }
     if (VISC && SN && i < N) {
         double tx[4] = {0.0, 0.0, 0.0, 0.0}, ty[4] = {0.0, 0.0, 0.0, 0.0},
@@ -983,17 +991,18 @@ __global__ void __launch_bounds__(MAXTILE)
                     const double diff = soff[(8 + q) * SS + k] - ti[q];
                     tx[q] += w_1 * diff;
                     ty[q] += w_2 * diff;
-                    tz[q] += w_3 * diff;
+                    if (DIM == 3) tz[q] += w_3 * diff;
                 }
             }
         }
+        // in 2D the z components are not stored: they are +0 (grad_z)
         const int vslot[4] = {G_T, G_VX, G_VY, G_VZ};
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             double* g = vgrad + (size_t)vslot[q] * 3 * N;
             g[i] = tx[q];
             g[N + i] = ty[q];
-            g[2 * (size_t)N + i] = tz[q];
+            if (DIM == 3) g[2 * (size_t)N + i] = tz[q];
         }
     }
     if (SN) {
@@ -1694,14 +1703,14 @@ __device__ __forceinline__ S conductivity(S temp, const ViscParams& vp) {
     return vp.Cp * mu / S(vp.Pr);
 }
 
-template <class S>
+template <class S, int DIM>
 __device__ __forceinline__ void hasselbacher(const S* __restrict__ grad, int N, int slot, S vl,
                                              S vr, int l, int r, const double* ehat,
                                              const double* n, double len_e, double edn, S* out) {
     const S* g = grad + (size_t)slot * 3 * N;
     const S ax = 0.5 * (g[l] + g[r]);
     const S ay = 0.5 * (g[N + l] + g[N + r]);
-    const S az = 0.5 * (g[2 * (size_t)N + l] + g[2 * (size_t)N + r]);
+    const S az = 0.5 * (grad_z<S, DIM>(g, N, l) + grad_z<S, DIM>(g, N, r));
     const S avg_dot_ehat = ax * ehat[0] + ay * ehat[1] + az * ehat[2];
     const S correction = avg_dot_ehat - (vr - vl) / len_e;
     out[0] = ax - correction * n[0] / edn;
@@ -1727,12 +1736,16 @@ __device__ __forceinline__ void visc_face(const DevMesh& m, const ViscParams& vp
     const double n[3] = {m.geom[f], m.geom[(size_t)F + f], m.geom[2 * (size_t)F + f]};
     if (!lv || !rv) {  // copy_gradients_to_face, viscous_flux.cpp:16-34
         const int c = lv ? l : r;
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < 2; k++) {
             gT[k] = grad[((size_t)G_T * 3 + k) * N + c];
             gvx[k] = grad[((size_t)G_VX * 3 + k) * N + c];
             gvy[k] = grad[((size_t)G_VY * 3 + k) * N + c];
             gvz[k] = grad[((size_t)G_VZ * 3 + k) * N + c];
         }
+        gT[2] = grad_z<S, DIM>(grad + (size_t)G_T * 3 * N, N, c);
+        gvx[2] = grad_z<S, DIM>(grad + (size_t)G_VX * 3 * N, N, c);
+        gvy[2] = grad_z<S, DIM>(grad + (size_t)G_VY * 3 * N, N, c);
+        gvz[2] = grad_z<S, DIM>(grad + (size_t)G_VZ * 3 * N, N, c);
     } else {  // hasselbacher_average, viscous_flux.cpp:52-80
         const double ex = m.cen[r] - m.cen[l];
         const double ey = m.cen[Nt + r] - m.cen[Nt + l];
@@ -1740,13 +1753,13 @@ __device__ __forceinline__ void visc_face(const DevMesh& m, const ViscParams& vp
         const double len_e = sqrt(ex * ex + ey * ey + ez * ez);
         const double ehat[3] = {ex / len_e, ey / len_e, ez / len_e};
         const double edn = ehat[0] * n[0] + ehat[1] * n[1] + ehat[2] * n[2];
-        hasselbacher<S>(grad, N, G_VX, prim[(size_t)P_VX * Nt + l], prim[(size_t)P_VX * Nt + r], l,
+        hasselbacher<S, DIM>(grad, N, G_VX, prim[(size_t)P_VX * Nt + l], prim[(size_t)P_VX * Nt + r], l,
                         r, ehat, n, len_e, edn, gvx);
-        hasselbacher<S>(grad, N, G_VY, prim[(size_t)P_VY * Nt + l], prim[(size_t)P_VY * Nt + r], l,
+        hasselbacher<S, DIM>(grad, N, G_VY, prim[(size_t)P_VY * Nt + l], prim[(size_t)P_VY * Nt + r], l,
                         r, ehat, n, len_e, edn, gvy);
-        hasselbacher<S>(grad, N, G_VZ, prim[(size_t)P_VZ * Nt + l], prim[(size_t)P_VZ * Nt + r], l,
+        hasselbacher<S, DIM>(grad, N, G_VZ, prim[(size_t)P_VZ * Nt + l], prim[(size_t)P_VZ * Nt + r], l,
                         r, ehat, n, len_e, edn, gvz);
-        hasselbacher<S>(grad, N, G_T, prim[(size_t)P_T * Nt + l], prim[(size_t)P_T * Nt + r], l, r,
+        hasselbacher<S, DIM>(grad, N, G_T, prim[(size_t)P_T * Nt + l], prim[(size_t)P_T * Nt + r], l, r,
                         ehat, n, len_e, edn, gT);
     }
     // average_flow_states_pT + update_thermo_from_pT (GAS/flow_state.h:89-91)
