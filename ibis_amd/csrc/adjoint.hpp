@@ -364,7 +364,8 @@ __device__ __forceinline__ void visc_face_rev(const DevMesh& m, const ViscParams
     } else {
         const double ex = m.cen[r] - m.cen[l];
         const double ey = m.cen[Nt + r] - m.cen[Nt + l];
-        const double ez = m.cen[2 * (size_t)Nt + r] - m.cen[2 * (size_t)Nt + l];
+        const double ez =
+            DIM == 2 && m.flat_z ? 0.0 : m.cen[2 * (size_t)Nt + r] - m.cen[2 * (size_t)Nt + l];
         len_e = sqrt(ex * ex + ey * ey + ez * ez);
         ehat[0] = ex / len_e;
         ehat[1] = ey / len_e;
@@ -510,7 +511,9 @@ __global__ void __launch_bounds__(BLOCK)
         const int f = m.ell_face[(size_t)j * N + i];
         sf[j * BLOCK + tid] = m.ell_sign[(size_t)j * N + i] > 0 ? f : ~f;
         if (ACTIVE) sn[j * BLOCK + tid] = m.ell_nbr[(size_t)j * N + i];
-        for (int k = 0; k < 3; k++) sd[(j * 3 + k) * BLOCK + tid] = B.offs[(size_t)(j * 3 + k) * N + i];
+        for (int k = 0; k < 3; k++)  // z: +0 on a flat 2D mesh
+            sd[(j * 3 + k) * BLOCK + tid] =
+                (k == 2 && DIM == 2 && m.flat_z) ? 0.0 : B.offs[(size_t)(j * 3 + k) * N + i];
     }
     if (ORD1) {
         for (int k = 0; k < NPRIM; k++) {

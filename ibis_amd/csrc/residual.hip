@@ -118,6 +118,9 @@ struct DevMesh {
                        // kernel with the stencil's centroids in LDS forms it there
     int n_from_t;  // face normals equal cross(t1, t2) bit for bit (grid-built meshes)
     int planar2d;  // 2D, and every face has t1z = +0 and t2 = (+0, +0, +1) bit for bit
+    int flat_z;    // 2D, and every cell and face centroid has the same finite z bits: every
+                   // z difference of centroids (the face offsets' z, e_z) is +0, and the
+                   // 2D kernels take it as 0 without loading it
 };
 
 constexpr int TILE = 256;     // cells per tile of a new mesh (the option "tile" picks others)
@@ -719,7 +722,7 @@ template <bool PACKED>
 __device__ __forceinline__ void load_cell(const DevMesh& m, const VarSet& vs,
                                           const double* __restrict__ pack,
                                           const double* __restrict__ prim, int c, double* u,
-                                          double& x, double& y, double& z) {
+                                          double& x, double& y, double& z, bool loadz = true) {
     if (PACKED) {
         const double2* P = reinterpret_cast<const double2*>(pack);
         const double2 a0 = P[(size_t)c * 4 + 0], a1 = P[(size_t)c * 4 + 1];
@@ -738,7 +741,7 @@ __device__ __forceinline__ void load_cell(const DevMesh& m, const VarSet& vs,
         for (int v = 0; v < 5; v++) u[v] = prim[(size_t)vs.row[v] * Nt + c];
         x = m.cen[c];
         y = m.cen[Nt + c];
-        z = m.cen[2 * (size_t)Nt + c];
+        z = loadz ? m.cen[2 * (size_t)Nt + c] : 0.0;
     }
 }
 
@@ -752,26 +755,29 @@ __device__ __forceinline__ void stage_stencil(const DevMesh& m, const VarSet& vs
                                               const double* __restrict__ pack,
                                               const double* __restrict__ prim, int lb, int i,
                                               double* soff) {
-    // one stencil entry per thread and pass: the block is the tile
+    // one stencil entry per thread and pass: the block is the tile.  On a
+    // flat 2D mesh the centroids' z plane is not staged (every z difference
+    // is +0 there and the kernels take it so).
     const int N = m.N, SS = m.tile + m.maxhalo;
     const int h0 = m.halo_off[lb], nh = m.halo_off[lb + 1] - h0;
+    const bool loadz = !(m.dim == 2 && m.flat_z);
     auto put = [&](int k, const double* u, double x, double y, double z) {
 #pragma unroll
         for (int v = 0; v < 5; v++) soff[v * SS + k] = u[v];
         soff[5 * SS + k] = x;
         soff[6 * SS + k] = y;
-        soff[7 * SS + k] = z;
+        if (loadz) soff[7 * SS + k] = z;
     };
     const int kh = threadIdx.x;
     const int ch = kh < nh ? m.halo_ids[h0 + kh] : 0;
     double u0[5], x0, y0, z0, u1[5], x1, y1, z1;
-    if (i < N) load_cell<PACKED>(m, vs, pack, prim, i, u0, x0, y0, z0);
-    if (kh < nh) load_cell<PACKED>(m, vs, pack, prim, ch, u1, x1, y1, z1);
+    if (i < N) load_cell<PACKED>(m, vs, pack, prim, i, u0, x0, y0, z0, loadz);
+    if (kh < nh) load_cell<PACKED>(m, vs, pack, prim, ch, u1, x1, y1, z1, loadz);
     if (i < N) put(threadIdx.x, u0, x0, y0, z0);
     if (kh < nh) put(m.tile + kh, u1, x1, y1, z1);
     for (int k = kh + m.tile; k < nh; k += m.tile) {
         double u[5], x, y, z;
-        load_cell<PACKED>(m, vs, pack, prim, m.halo_ids[h0 + k], u, x, y, z);
+        load_cell<PACKED>(m, vs, pack, prim, m.halo_ids[h0 + k], u, x, y, z, loadz);
         put(m.tile + k, u, x, y, z);
     }
     __syncthreads();
@@ -788,6 +794,7 @@ __device__ __forceinline__ void stage_stencil_visc(const DevMesh& m, const VarSe
                                                    double* soff) {
     const int N = m.N, Nt = m.Nt, G = m.G, SS = m.tile + m.maxhalo;
     const int h0 = m.halo_off[lb], nh = m.halo_off[lb + 1] - h0;
+    const bool loadz = !(m.dim == 2 && m.flat_z);  // as stage_stencil
     auto stage = [&](int c, int k) {
         double u[5];
 #pragma unroll
@@ -797,7 +804,7 @@ __device__ __forceinline__ void stage_stencil_visc(const DevMesh& m, const VarSe
         for (int v = 0; v < 5; v++) soff[v * SS + k] = u[v];
         soff[5 * SS + k] = m.cen[c];
         soff[6 * SS + k] = m.cen[Nt + c];
-        soff[7 * SS + k] = m.cen[2 * (size_t)Nt + c];
+        if (loadz) soff[7 * SS + k] = m.cen[2 * (size_t)Nt + c];
         soff[8 * SS + k] = prim[(size_t)P_T * Nt + c];
         soff[9 * SS + k] = prim[(size_t)P_VX * Nt + c];
         soff[10 * SS + k] = prim[(size_t)P_VY * Nt + c];
@@ -1068,17 +1075,18 @@ __global__ void __launch_bounds__(MAXTILE)
             gz[v] = g[2 * (size_t)N + i];
         }
     }
-    // x_face - x_cell of the cell's face in slot j
+    // x_face - x_cell of the cell's face in slot j (z: +0 on a flat 2D mesh)
+    const bool zflat = DIM == 2 && m.flat_z;
     auto face_offset = [&](int j, double& dx, double& dy, double& dz) {
         if (OFFS) {
             dx = offs[(size_t)(j * 3 + 0) * N + i];
             dy = offs[(size_t)(j * 3 + 1) * N + i];
-            dz = offs[(size_t)(j * 3 + 2) * N + i];
+            dz = zflat ? 0.0 : offs[(size_t)(j * 3 + 2) * N + i];
         } else {
             const int f = m.ell_face[(size_t)j * N + i];
             dx = m.geom[9 * (size_t)F + f] - xi;
             dy = m.geom[10 * (size_t)F + f] - yi;
-            dz = m.geom[11 * (size_t)F + f] - zi;
+            dz = zflat ? 0.0 : m.geom[11 * (size_t)F + f] - zi;
         }
     };
     double phi[5] = {1.0, 1.0, 1.0, 1.0, 1.0};
@@ -1327,21 +1335,23 @@ __global__ void __launch_bounds__(MAXTILE)
         gz[v] = (float)g[2 * (size_t)N + i];
     }
     // offs32: the offsets already rounded to float (the same values), when
-    // the mode made them
+    // the mode made them.  z: +0 on a flat 2D mesh (whose stencil staging
+    // leaves the centroids' z out)
+    const bool zflat = DIM == 2 && m.flat_z;
     auto offset = [&](int j, float& dx, float& dy, float& dz) {
         if (OFFS && offs32) {
             dx = offs32[(size_t)(j * 3 + 0) * N + i];
             dy = offs32[(size_t)(j * 3 + 1) * N + i];
-            dz = offs32[(size_t)(j * 3 + 2) * N + i];
+            dz = zflat ? 0.0f : offs32[(size_t)(j * 3 + 2) * N + i];
         } else if (OFFS) {
             dx = (float)offs[(size_t)(j * 3 + 0) * N + i];
             dy = (float)offs[(size_t)(j * 3 + 1) * N + i];
-            dz = (float)offs[(size_t)(j * 3 + 2) * N + i];
+            dz = zflat ? 0.0f : (float)offs[(size_t)(j * 3 + 2) * N + i];
         } else {
             const int f = m.ell_face[(size_t)j * N + i];
             dx = (float)(m.geom[9 * (size_t)F + f] - xi);
             dy = (float)(m.geom[10 * (size_t)F + f] - yi);
-            dz = (float)(m.geom[11 * (size_t)F + f] - zi);
+            dz = zflat ? 0.0f : (float)(m.geom[11 * (size_t)F + f] - zi);
         }
     };
     const bool staged = LIM != 0 && stage;
@@ -1749,7 +1759,8 @@ __device__ __forceinline__ void visc_face(const DevMesh& m, const ViscParams& vp
     } else {  // hasselbacher_average, viscous_flux.cpp:52-80
         const double ex = m.cen[r] - m.cen[l];
         const double ey = m.cen[Nt + r] - m.cen[Nt + l];
-        const double ez = m.cen[2 * (size_t)Nt + r] - m.cen[2 * (size_t)Nt + l];
+        const double ez =
+            DIM == 2 && m.flat_z ? 0.0 : m.cen[2 * (size_t)Nt + r] - m.cen[2 * (size_t)Nt + l];
         const double len_e = sqrt(ex * ex + ey * ey + ez * ez);
         const double ehat[3] = {ex / len_e, ey / len_e, ez / len_e};
         const double edn = ehat[0] * n[0] + ehat[1] * n[1] + ehat[2] * n[2];
@@ -2698,6 +2709,11 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
             std::memcmp(gg + 7 * F + f, &zero, 8) || std::memcmp(gg + 8 * F + f, &one, 8))
             planar2d = 0;
     }
+    int flat_z = d->dim == 2 && std::isfinite(d->centroid[2 * Nt]) ? 1 : 0;
+    for (int64_t c = 0; c < Nt && flat_z; c++)
+        if (std::memcmp(d->centroid + 2 * Nt + c, d->centroid + 2 * Nt, 8)) flat_z = 0;
+    for (int64_t f = 0; f < F && flat_z; f++)
+        if (std::memcmp(d->face_geom + 11 * F + f, d->centroid + 2 * Nt, 8)) flat_z = 0;
     std::vector<double> geom(d->face_geom, d->face_geom + 13 * F);
     std::vector<double> cen(d->centroid, d->centroid + 3 * Nt);
     std::vector<double> vol(d->volume, d->volume + Nt);
@@ -2733,6 +2749,7 @@ ibis_status ibis_mesh_create(const ibis_mesh_desc* d, ibis_mesh** out) {
     m->tiles[TILE] = D;
     D.n_from_t = n_from_t;
     D.planar2d = planar2d;
+    D.flat_z = flat_z;
     D.lsq_from_cen = lsq_from_cen;
     *out = m;
     return IBIS_OK;

@@ -77,6 +77,37 @@ def test_wedge_subsonic_boundaries(layout):
     _check(g_io, cfg, prim, exact=True, options={"face_state_layout": layout})
 
 
+@pytest.mark.parametrize("zmode", ["flat", "uneven"])
+def test_2d_mesh_off_the_z0_plane(zmode):
+    """A 2D wedge lifted to z = 1.5 (every centroid at the same z: the 2D
+    kernels take each z difference as +0 without loading it) and one whose
+    vertices' z differ (they load it): the fused inviscid residual with and
+    without precomputed offsets bit for bit, and the fused viscous one at the
+    viscous bar, against the oracle."""
+    from ibis_amd import config as C
+    from ibis_amd import gridgen
+
+    def lift(g_io):
+        if zmode == "flat":
+            g_io.xyz[:, 2] = 1.5
+        else:
+            g_io.xyz[:, 2] = 1.5 + 1e-3 * gridgen.uniform01(17, g_io.xyz.shape[0])
+        return g_io
+
+    g_io, cfg, fs = cases.wedge_case(40, 20)
+    g_io = lift(g_io)
+    prim = cases.perturbed_prim(g_io, fs, 40 * 20 + 2 * (40 + 20), seed=23)
+    print(f"2D off z=0: {zmode}", flush=True)
+    _check(g_io, cfg, prim, exact=True)
+    _check(g_io, cfg, prim, exact=True, options={"precomputed_offsets": 0})
+    g_io, cfg, fs = cases.wedge_case(40, 20, mach=0.5)
+    g_io = lift(g_io)
+    cfg.viscous = True
+    cfg.boundaries = {"inflow": C.subsonic_inflow(fs), "outflow": C.subsonic_outflow(fs.p * 0.98),
+                      "wall": C.fixed_temperature_no_slip_wall(300.0)}
+    _check(g_io, cfg, cases.perturbed_prim(g_io, fs, 40 * 20 + 2 * 60, seed=24), exact=False)
+
+
 def test_wedge_ldfss2_tolerance():
     g_io, cfg, fs = cases.wedge_case(48, 24, flux="ldfss2")
     prim = cases.perturbed_prim(g_io, fs, g_io.num_cells + 2 * (48 + 24), seed=12)
