@@ -532,7 +532,7 @@ __global__ void __launch_bounds__(BLOCK)
         const double* ug = B.prim_cg ? B.prim_cg + (size_t)row * m.G : u + N;
         const double ui = u[i];
         const double* gg = B.cgrad + (size_t)slot * 3 * N;
-        const double g[3] = {gg[i], gg[N + i], gg[2 * (size_t)N + i]};
+        const double g[3] = {gg[i], gg[N + i], grad_z<double, DIM>(gg, N, i)};
         const double phi = LIM != 0 ? B.clim[(size_t)slot * N + i] : 1.0;
         double self = 0.0, phib = 0.0, gb[3] = {0.0, 0.0, 0.0};
         // MUSCL: q = u + phi * (g . d) on each face of the cell
@@ -833,6 +833,8 @@ __global__ void __launch_bounds__(BLOCK)
         bc_action_vjp(m, kinds[t * IBIS_MAX_ACT + a], f, pars + ((size_t)t * IBIS_MAX_ACT + a) * IBIS_NPAR,
                       vst, gp, gb, vb);
     for (int k = 0; k < NPRIM; k++) B.bcb[(size_t)k * m.nb + b] = vb[k];
+    // the output has no ghost rows: this ghost's are zero
+    for (int k = 0; k < NPRIM; k++) B.pb[(size_t)k * Nt + g] = 0.0;
 }
 
 // Per valid cell: own terms + neighbours' stencils + ghosts' actions

@@ -1063,7 +1063,7 @@ __global__ void __launch_bounds__(MAXTILE)
             if (write_slots && !(VISC && vgrad == grad && viscous_slot(vs.slot[v]))) {
                 g[i] = gx[v];
                 g[N + i] = gy[v];
-                g[2 * (size_t)N + i] = gz[v];
+                if (DIM == 3) g[2 * (size_t)N + i] = gz[v];  // 2D: +0, not stored (grad_z)
             }
         } else if (VISC) {  // 2D vz, VISC: the slot's value read above
             gx[v] = vz_old[0];
@@ -4150,7 +4150,9 @@ void launch_vjp(ibis_residual* r, const VarSet& vs, const FaceParams& fp, const 
         k_vjp_gather<DIM, false, true><<<nc, BLOCK, 0, st>>>(D, vs, B);
     else
         k_vjp_gather<DIM, false><<<nc, BLOCK, 0, st>>>(D, vs, B);
-    if (D.G > 0) k_zero_rows<<<nblocks(D.G), BLOCK, 0, st>>>(NPRIM, D.N, D.Nt, B.pb);
+    // the ghost kernel zeroes the output's ghost rows of the ghosts on a
+    // boundary face; any others are zeroed here
+    if (D.G > D.nb) k_zero_rows<<<nblocks(D.G), BLOCK, 0, st>>>(NPRIM, D.N, D.Nt, B.pb);
 }
 
 }  // namespace
