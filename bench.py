@@ -159,7 +159,7 @@ def adjoint_leg(steps):
 
 
 def rk_leg(nsteps=500):
-    """Config 1 (192 x 96 wedge, uniform Mach-3 free stream, ssp-rk3 at CFL
+    """The 192 x 96 wedge (uniform Mach-3 free stream, ssp-rk3 at CFL
     0.5): time per step of the plain driver (RungeKutta.take_step: the dt
     goes to the host every step) and of the device-resident driver (a batch
     of one: dt on the device, each step one HIP graph replay), same states."""
@@ -193,7 +193,7 @@ def rk_leg(nsteps=500):
     bt.rk_steps(nsteps, use_graph=True)
     t_graph = (time.perf_counter() - w) / nsteps
     same = bool(np.array_equal(bt.scatter(bc.cpu().numpy(), 0)[:, :gb.N], dc.cpu().numpy()[:, :gb.N]))
-    out = {"workload": "wedge 192x96 (config 1), ssp-rk3, CFL 0.5", "steps": nsteps,
+    out = {"workload": "wedge 192x96, ssp-rk3, CFL 0.5", "steps": nsteps,
            "plain_us_per_step": round(t_plain * 1e6, 1), "graph_us_per_step": round(t_graph * 1e6, 1),
            "speedup": round(t_plain / t_graph, 3), "identical_fp64": same}
     print(f"[bench] rk driver: {out}", file=sys.stderr, flush=True)

@@ -140,7 +140,7 @@ def test_plate_wall_force_gradient_full_size():
 
 
 @pytest.mark.timeout(900)
-def test_batch64_config5_matches_one_at_a_time_and_oracle():
+def test_batch64_matches_one_at_a_time_and_oracle():
     """64 seeded instances (10k-400k cells, 13.0M in all): three ssp-rk3
     steps of the batch, each step one HIP graph replay, equal three steps
     of every instance on its own FiniteVolume + RungeKutta, bit for bit

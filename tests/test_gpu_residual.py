@@ -120,7 +120,7 @@ def test_wedge_first_order_when_reconstruction_disallowed():
     _check(g_io, cfg, prim, exact=True, allow_reconstruction=False)
 
 
-def test_wedge_config1_full_size():
+def test_wedge_192x96_full_size():
     """192 x 96 wedge, AUSMDV / Barth-Jespersen / rho_u, uniform Mach 3 and
     perturbed states."""
     g_io, cfg, fs = cases.wedge_case(192, 96)
@@ -285,7 +285,7 @@ def test_plate_viscous():
 
 
 @pytest.mark.timeout(300)
-def test_plate_viscous_config2_full_size():
+def test_plate_viscous_1024x512_full_size():
     g_io, cfg, fs = cases.plate_case(1024, 512)
     prim = cases.perturbed_prim(g_io, fs, 1024 * 512 + 2 * (1024 + 512), seed=2)
     _check(g_io, cfg, prim, exact=False)
@@ -335,8 +335,8 @@ def test_rk_steps_match_oracle():
 
 
 @pytest.mark.timeout(900)
-def test_wedge_config1_2000_steps():
-    """The first workload as specified: 192 x 96 wedge, uniform Mach-3 free
+def test_wedge_192x96_2000_steps():
+    """The 192 x 96 wedge workload: uniform Mach-3 free
     stream, ssp-rk3 at CFL 0.5, 2000 steps.  The plain path (a host dt per
     step) equals the oracle's state after steps 1, 10, 100 and 2000 and its
     dt at every step; the device-resident path (a batch of one, each step
@@ -366,7 +366,7 @@ def test_wedge_config1_2000_steps():
             torch.cuda.synchronize()
             c_gpu = gb.to_reference_order(d_cons.cpu().numpy())
             assert np.array_equal(c_gpu[:, :n], c_cpu[:, :n]), step
-            print(f"config 1: step {step}, t = {t:.6e}, state identical to the oracle", flush=True)
+            print(f"wedge 192x96: step {step}, t = {t:.6e}, state identical to the oracle", flush=True)
     bt = Batch([(g_io, cfg)])
     b0 = bt.blocks[0]
     bp = to_dev(bt.gather([b0.to_grid_order(prim)]))
