@@ -439,8 +439,9 @@ def _visc_cases():
 @pytest.mark.parametrize("name", ["plate", "plate_hanel_vk", "plate_triquad", "hex_walls",
                                   "wedge_subsonic"])
 def test_fused_viscous_path_equals_general_path(name):
-    """The three-kernel viscous path (boundary actions, cell kernel with the
-    viscous gradients, face kernel with the viscous flux) against the
+    """The four-kernel viscous path (boundary actions, cell kernel with the
+    viscous gradients, viscous terms per face, face kernel with the viscous
+    flux) against the
     reference-shaped general path on the same device: dU/dt, the ghost rows,
     estimate_dt (face density and temperature) and, with keep_gradients,
     every gradient and limiter slot, bit for bit; and against the oracle at
