@@ -151,113 +151,131 @@ struct GasParams {
 
 // S is double for the residual and ibis::Dual for its tangent (JVP); the
 // geometry and the action parameters are plain doubles in both.
-// the actions of boundary face b, in the order of its tag's list
-template <class S>
-__device__ __forceinline__ void bc_actions(const DevMesh& m, const int* __restrict__ kinds,
-                                           const double* __restrict__ pars, const GasParams& gp,
-                                           S* __restrict__ prim, int b) {
-    const int f = m.b_face[b], gh = m.b_ghost[b], va = m.b_valid[b], t = m.b_tag[b];
-    const int Nt = m.Nt, F = m.F;
-    S* rho = prim + (size_t)P_RHO * Nt;
-    S* p = prim + (size_t)P_P * Nt;
-    S* temp = prim + (size_t)P_T * Nt;
-    S* e = prim + (size_t)P_E * Nt;
-    S* vx = prim + (size_t)P_VX * Nt;
-    S* vy = prim + (size_t)P_VY * Nt;
-    S* vz = prim + (size_t)P_VZ * Nt;
+// The actions of tag t on boundary face f, in the order of the tag's list,
+// through an accessor X: X.v(row) reads the valid cell's prim row (the
+// actions only read it), X.g(row) and X.set(row, x) read and write the
+// ghost's (entries no action writes keep what the ghost held).
+template <class S, class Acc>
+__device__ __forceinline__ void bc_actions_on(const DevMesh& m, const int* __restrict__ kinds,
+                                              const double* __restrict__ pars,
+                                              const GasParams& gp, int f, int t, Acc& X) {
+    const int F = m.F;
+    // one copy of the action switch (unrolled over the list it costs about
+    // a hundred registers)
+#pragma unroll 1
     for (int a = 0; a < IBIS_MAX_ACT; a++) {
         const int kind = kinds[t * IBIS_MAX_ACT + a];
         if (kind == IBIS_BC_NONE) continue;
         const double* par = pars + (t * IBIS_MAX_ACT + a) * IBIS_NPAR;
         switch (kind) {
             case IBIS_BC_FLOW_STATE_COPY:  // boundary.cpp:15-45
-                temp[gh] = S(par[0]);
-                p[gh] = S(par[1]);
-                rho[gh] = S(par[2]);
-                e[gh] = S(par[3]);
-                vx[gh] = S(par[4]);
-                vy[gh] = S(par[5]);
-                vz[gh] = S(par[6]);
+                X.set(P_T, S(par[0]));
+                X.set(P_P, S(par[1]));
+                X.set(P_RHO, S(par[2]));
+                X.set(P_E, S(par[3]));
+                X.set(P_VX, S(par[4]));
+                X.set(P_VY, S(par[5]));
+                X.set(P_VZ, S(par[6]));
                 break;
             case IBIS_BC_INTERNAL_COPY:  // boundary.cpp:100-136
-                temp[gh] = temp[va];
-                p[gh] = p[va];
-                rho[gh] = rho[va];
-                e[gh] = e[va];
-                vx[gh] = vx[va];
-                vy[gh] = vy[va];
-                vz[gh] = vz[va];
+                X.set(P_T, X.v(P_T));
+                X.set(P_P, X.v(P_P));
+                X.set(P_RHO, X.v(P_RHO));
+                X.set(P_E, X.v(P_E));
+                X.set(P_VX, X.v(P_VX));
+                X.set(P_VY, X.v(P_VY));
+                X.set(P_VZ, X.v(P_VZ));
                 break;
             case IBIS_BC_REFLECT_NORMAL: {  // boundary.cpp:142-206
-                temp[gh] = temp[va];
-                p[gh] = p[va];
-                rho[gh] = rho[va];
-                e[gh] = e[va];
-                const S x = vx[va], y = vy[va], z = vz[va];
+                X.set(P_T, X.v(P_T));
+                X.set(P_P, X.v(P_P));
+                X.set(P_RHO, X.v(P_RHO));
+                X.set(P_E, X.v(P_E));
+                const S x = X.v(P_VX), y = X.v(P_VY), z = X.v(P_VZ);
                 double nx = m.geom[0 * F + f], ny = m.geom[1 * F + f], nz = m.geom[2 * F + f];
                 double t1x = m.geom[3 * F + f], t1y = m.geom[4 * F + f], t1z = m.geom[5 * F + f];
                 double t2x = m.geom[6 * F + f], t2y = m.geom[7 * F + f], t2z = m.geom[8 * F + f];
                 const S xs = -(x * nx + y * ny + z * nz);
                 const S ys = x * t1x + y * t1y + z * t1z;
                 const S zs = x * t2x + y * t2y + z * t2z;
-                vx[gh] = xs * nx + ys * t1x + zs * t2x;
-                vy[gh] = xs * ny + ys * t1y + zs * t2y;
-                vz[gh] = xs * nz + ys * t1z + zs * t2z;
+                X.set(P_VX, xs * nx + ys * t1x + zs * t2x);
+                X.set(P_VY, xs * ny + ys * t1y + zs * t2y);
+                X.set(P_VZ, xs * nz + ys * t1z + zs * t2z);
                 break;
             }
             case IBIS_BC_VEL_REFLECT:  // boundary.cpp:209-241
-                vx[gh] = -vx[va];
-                vy[gh] = -vy[va];
-                vz[gh] = -vz[va];
+                X.set(P_VX, -X.v(P_VX));
+                X.set(P_VY, -X.v(P_VY));
+                X.set(P_VZ, -X.v(P_VZ));
                 break;
             case IBIS_BC_FIX_TEMPERATURE:  // boundary.cpp:244-277
-                temp[gh] = 2.0 * S(par[0]) - temp[va];
+                X.set(P_T, 2.0 * S(par[0]) - X.v(P_T));
                 break;
             case IBIS_BC_SUBSONIC_INFLOW: {  // boundary.cpp:286-342
                 double nx = m.geom[0 * F + f], ny = m.geom[1 * F + f], nz = m.geom[2 * F + f];
                 const S in_vx = S(par[4]), in_vy = S(par[5]), in_vz = S(par[6]);
                 const S vel_correct =
-                    nx * (in_vx - vx[va]) + ny * (in_vy - vy[va]) + nz * (in_vz - vz[va]);
-                const S a = sqrt(S(gp.gamma * gp.R) * temp[va]);
-                const S rho_ref = rho[va];
+                    nx * (in_vx - X.v(P_VX)) + ny * (in_vy - X.v(P_VY)) + nz * (in_vz - X.v(P_VZ));
+                const S a = sqrt(S(gp.gamma * gp.R) * X.v(P_T));
+                const S rho_ref = X.v(P_RHO);
                 const S p_in = S(par[1]);
-                const S p_face = 0.5 * (p_in + p[va] - rho_ref * a * vel_correct);
+                const S p_face = 0.5 * (p_in + X.v(P_P) - rho_ref * a * vel_correct);
                 const S rho_face = S(par[2]) + (p_face - p_in) / a / a;
                 const S vx_face = in_vx - nx * (p_in - p_face) / (rho_ref * a);
                 const S vy_face = in_vy - ny * (p_in - p_face) / (rho_ref * a);
                 const S vz_face = in_vz - nz * (p_in - p_face) / (rho_ref * a);
-                p[gh] = 2.0 * p_face - p[va];
-                rho[gh] = 2.0 * rho_face - rho[va];
-                temp[gh] = p[gh] / (rho[gh] * gp.R);
-                e[gh] = gp.Cv * temp[gh];
-                vx[gh] = 2.0 * vx_face - vx[va];
-                vy[gh] = 2.0 * vy_face - vy[va];
-                vz[gh] = 2.0 * vz_face - vz[va];
+                X.set(P_P, 2.0 * p_face - X.v(P_P));
+                X.set(P_RHO, 2.0 * rho_face - X.v(P_RHO));
+                X.set(P_T, X.g(P_P) / (X.g(P_RHO) * gp.R));
+                X.set(P_E, gp.Cv * X.g(P_T));
+                X.set(P_VX, 2.0 * vx_face - X.v(P_VX));
+                X.set(P_VY, 2.0 * vy_face - X.v(P_VY));
+                X.set(P_VZ, 2.0 * vz_face - X.v(P_VZ));
                 break;
             }
             case IBIS_BC_SUBSONIC_OUTFLOW: {  // boundary.cpp:348-399
                 double nx = m.geom[0 * F + f], ny = m.geom[1 * F + f], nz = m.geom[2 * F + f];
                 const S p_face = S(par[0]);
-                const S rho_ref = rho[va];
-                const S p_ref = p[va];
-                const S a = sqrt(S(gp.gamma * gp.R) * temp[va]);
-                const S rho_face = rho[va] + (p_face - p_ref) / a / a;
-                const S vx_face = vx[va] + nx * (p_ref - p_face) / (rho_ref * a);
-                const S vy_face = vy[va] + ny * (p_ref - p_face) / (rho_ref * a);
-                const S vz_face = vz[va] + nz * (p_ref - p_face) / (rho_ref * a);
-                p[gh] = 2.0 * p_face - p[va];
-                rho[gh] = 2.0 * rho_face - rho[va];
-                temp[gh] = p[gh] / (rho[gh] * gp.R);
-                e[gh] = gp.Cv * temp[gh];
-                vx[gh] = 2.0 * vx_face - vx[va];
-                vy[gh] = 2.0 * vy_face - vy[va];
-                vz[gh] = 2.0 * vz_face - vz[va];
+                const S rho_ref = X.v(P_RHO);
+                const S p_ref = X.v(P_P);
+                const S a = sqrt(S(gp.gamma * gp.R) * X.v(P_T));
+                const S rho_face = X.v(P_RHO) + (p_face - p_ref) / a / a;
+                const S vx_face = X.v(P_VX) + nx * (p_ref - p_face) / (rho_ref * a);
+                const S vy_face = X.v(P_VY) + ny * (p_ref - p_face) / (rho_ref * a);
+                const S vz_face = X.v(P_VZ) + nz * (p_ref - p_face) / (rho_ref * a);
+                X.set(P_P, 2.0 * p_face - X.v(P_P));
+                X.set(P_RHO, 2.0 * rho_face - X.v(P_RHO));
+                X.set(P_T, X.g(P_P) / (X.g(P_RHO) * gp.R));
+                X.set(P_E, gp.Cv * X.g(P_T));
+                X.set(P_VX, 2.0 * vx_face - X.v(P_VX));
+                X.set(P_VY, 2.0 * vy_face - X.v(P_VY));
+                X.set(P_VZ, 2.0 * vz_face - X.v(P_VZ));
                 break;
             }
             default:
                 break;
         }
     }
+}
+
+// the accessor of prim's rows: each action reads and writes only what it uses
+template <class S>
+struct PrimRows {
+    S* prim;
+    int Nt, va, gh;
+    __device__ __forceinline__ S v(int row) const { return prim[(size_t)row * Nt + va]; }
+    __device__ __forceinline__ S g(int row) const { return prim[(size_t)row * Nt + gh]; }
+    __device__ __forceinline__ void set(int row, S x) { prim[(size_t)row * Nt + gh] = x; }
+};
+
+// the actions of boundary face b, in the order of its tag's list, on prim
+template <class S>
+__device__ __forceinline__ void bc_actions(const DevMesh& m, const int* __restrict__ kinds,
+                                           const double* __restrict__ pars, const GasParams& gp,
+                                           S* __restrict__ prim, int b) {
+    const int f = m.b_face[b], t = m.b_tag[b];
+    PrimRows<S> X{prim, m.Nt, m.b_valid[b], m.b_ghost[b]};
+    bc_actions_on<S>(m, kinds, pars, gp, f, t, X);
 }
 
 template <class S>
@@ -750,7 +768,109 @@ __device__ __forceinline__ void load_cell(const DevMesh& m, const VarSet& vs,
 // cells first, then the halo in the tile's halo order).  Each thread
 // gathers its own cell and its first halo cell together, then writes both;
 // further halo cells one per pass.  Ends with the block barrier.
-template <bool PACKED>
+// The boundary pass folded into the cell kernel's stencil staging (the
+// option fold_boundary): a ghost's only neighbour is the valid cell of its
+// boundary face, so exactly one tile has it in its stencil halo, and the
+// block of that tile runs the ghost's boundary actions while it stages it.
+struct BcFold {
+    const int* kinds;     // the pre-reconstruction then the viscous action tables
+    const double* pars;
+    int ntab, has_pre, has_visc;
+    GasParams gp;
+    double* prim;         // the state: the ghost rows are written through this
+    double* FL;           // the face values (layout 2, FP64): the ghost entries
+    double* prim_cg;      // when set (the adjoint's forward pass): each ghost's row after
+                          // the pre-reconstruction actions, 7 x G
+};
+
+// the accessor of the folded pass: the ghost's row in registers, loaded
+// before the actions; the valid cell's read from the state
+struct GhostRegs {
+    const double* prim;
+    int Nt, va;
+    double gr[NPRIM];
+    __device__ __forceinline__ double v(int row) const { return prim[(size_t)row * Nt + va]; }
+    __device__ __forceinline__ double g(int row) const { return gr[row]; }
+    __device__ __forceinline__ void set(int row, double x) { gr[row] = x; }
+};
+
+// prim row `row` of a register copy (row is a kernel argument, the same in
+// every lane: a chain of selects keeps the copy in registers)
+__device__ __forceinline__ double row_entry(const double (&r)[NPRIM], int row) {
+    double x = r[0];
+#pragma unroll
+    for (int k = 1; k < NPRIM; k++) x = row == k ? r[k] : x;
+    return x;
+}
+
+// Ghost c of the stencil halo: its boundary face's pre-reconstruction
+// actions, its face values (as ghost_face_values), its viscous actions and
+// its prim row written back, as k_bc_ghost_fs / k_bc_visc_fs do.  u: the
+// reconstruction variables after the pre-reconstruction actions; tv: T,
+// vx, vy, vz after the viscous ones.
+__device__ __forceinline__ void fold_ghost(const DevMesh& m, const VarSet& vs, const BcFold& bf,
+                                           int c, double (&u)[5], double (&tv)[4]) {
+    const int N = m.N, Nt = m.Nt, F = m.F;
+    const int b = m.b_of_ghost[c - N];
+    const int f = m.b_face[b], t = m.b_tag[b];
+    GhostRegs X{bf.prim, Nt, m.b_valid[b], {}};
+#pragma unroll
+    for (int k = 0; k < NPRIM; k++) X.gr[k] = bf.prim[(size_t)k * Nt + c];
+    if (bf.has_pre) bc_actions_on<double>(m, bf.kinds, bf.pars, bf.gp, f, t, X);
+    const double dx = m.geom[9 * (size_t)F + f] - m.cen[c];
+    const double dy = m.geom[10 * (size_t)F + f] - m.cen[Nt + c];
+    const double dz = m.geom[11 * (size_t)F + f] - m.cen[2 * (size_t)Nt + c];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        u[v] = row_entry(X.gr, vs.row[v]);
+        const double gx = 0.0, gy = 0.0, gz = 0.0;
+        bf.FL[soa_index(m, c, 0, v)] = u[v] + 1.0 * (gx * dx + gy * dy + gz * dz);
+    }
+    if (bf.prim_cg) {
+#pragma unroll
+        for (int k = 0; k < NPRIM; k++) bf.prim_cg[(size_t)k * m.G + (c - N)] = X.gr[k];
+    }
+    if (bf.has_visc)
+        bc_actions_on<double>(m, bf.kinds + bf.ntab, bf.pars + (size_t)bf.ntab * IBIS_NPAR, bf.gp,
+                              f, t, X);
+    tv[0] = X.gr[P_T];
+    tv[1] = X.gr[P_VX];
+    tv[2] = X.gr[P_VY];
+    tv[3] = X.gr[P_VZ];
+#pragma unroll
+    for (int k = 0; k < NPRIM; k++) bf.prim[(size_t)k * Nt + c] = X.gr[k];
+}
+
+// FOLD: the ghosts of the tile's stencil halo (a suffix of the ascending
+// halo list) through fold_ghost into their stencil entries, before the
+// cell kernel loads anything else, so that the actions' registers are not
+// added to those the kernel keeps live; VISC: also T, vx, vy, vz.  The
+// staging that follows leaves these entries alone; its barrier publishes
+// them.
+template <bool VISC>
+__device__ __forceinline__ void fold_halo_ghosts(const DevMesh& m, const VarSet& vs,
+                                                 const BcFold& bf, int lb, double* soff) {
+    const int N = m.N, Nt = m.Nt, SS = m.tile + m.maxhalo;
+    const int h0 = m.halo_off[lb], nh = m.halo_off[lb + 1] - h0;
+    for (int k = threadIdx.x; k < nh; k += m.tile) {
+        const int c = m.halo_ids[h0 + k];
+        if (c < N) continue;
+        double u[5], tv[4];
+        fold_ghost(m, vs, bf, c, u, tv);
+        const int e = m.tile + k;
+#pragma unroll
+        for (int v = 0; v < 5; v++) soff[v * SS + e] = u[v];
+        soff[5 * SS + e] = m.cen[c];
+        soff[6 * SS + e] = m.cen[Nt + c];
+        soff[7 * SS + e] = m.cen[2 * (size_t)Nt + c];
+        if (VISC) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) soff[(8 + q) * SS + e] = tv[q];
+        }
+    }
+}
+
+template <bool PACKED, bool FOLD>
 __device__ __forceinline__ void stage_stencil(const DevMesh& m, const VarSet& vs,
                                               const double* __restrict__ pack,
                                               const double* __restrict__ prim, int lb, int i,
@@ -768,16 +888,20 @@ __device__ __forceinline__ void stage_stencil(const DevMesh& m, const VarSet& vs
         soff[6 * SS + k] = y;
         if (loadz) soff[7 * SS + k] = z;
     };
+    // FOLD: the halo's ghosts are staged already (fold_halo_ghosts)
     const int kh = threadIdx.x;
     const int ch = kh < nh ? m.halo_ids[h0 + kh] : 0;
+    const bool hv = kh < nh && !(FOLD && ch >= N);
     double u0[5], x0, y0, z0, u1[5], x1, y1, z1;
     if (i < N) load_cell<PACKED>(m, vs, pack, prim, i, u0, x0, y0, z0, loadz);
-    if (kh < nh) load_cell<PACKED>(m, vs, pack, prim, ch, u1, x1, y1, z1, loadz);
+    if (hv) load_cell<PACKED>(m, vs, pack, prim, ch, u1, x1, y1, z1, loadz);
     if (i < N) put(threadIdx.x, u0, x0, y0, z0);
-    if (kh < nh) put(m.tile + kh, u1, x1, y1, z1);
+    if (hv) put(m.tile + kh, u1, x1, y1, z1);
     for (int k = kh + m.tile; k < nh; k += m.tile) {
+        const int c = m.halo_ids[h0 + k];
+        if (FOLD && c >= N) continue;
         double u[5], x, y, z;
-        load_cell<PACKED>(m, vs, pack, prim, m.halo_ids[h0 + k], u, x, y, z, loadz);
+        load_cell<PACKED>(m, vs, pack, prim, c, u, x, y, z, loadz);
         put(m.tile + k, u, x, y, z);
     }
     __syncthreads();
@@ -788,6 +912,7 @@ __device__ __forceinline__ void stage_stencil(const DevMesh& m, const VarSet& vs
 // viscous boundary actions, which the convective gradients read), the
 // centroid, then T, vx, vy, vz as the viscous gradients read them (a
 // ghost's after the viscous actions, in prim).  Ends with the block barrier.
+template <bool FOLD>
 __device__ __forceinline__ void stage_stencil_visc(const DevMesh& m, const VarSet& vs,
                                                    const double* __restrict__ prim,
                                                    const double* __restrict__ gconv, int lb, int i,
@@ -796,19 +921,22 @@ __device__ __forceinline__ void stage_stencil_visc(const DevMesh& m, const VarSe
     const int h0 = m.halo_off[lb], nh = m.halo_off[lb + 1] - h0;
     const bool loadz = !(m.dim == 2 && m.flat_z);  // as stage_stencil
     auto stage = [&](int c, int k) {
-        double u[5];
+        if (FOLD && c >= N) return;  // staged by fold_halo_ghosts
+        double u[5], tv[4];
 #pragma unroll
         for (int v = 0; v < 5; v++)
             u[v] = c < N ? prim[(size_t)vs.row[v] * Nt + c] : gconv[(size_t)v * G + (c - N)];
+        tv[0] = prim[(size_t)P_T * Nt + c];
+        tv[1] = prim[(size_t)P_VX * Nt + c];
+        tv[2] = prim[(size_t)P_VY * Nt + c];
+        tv[3] = prim[(size_t)P_VZ * Nt + c];
 #pragma unroll
         for (int v = 0; v < 5; v++) soff[v * SS + k] = u[v];
         soff[5 * SS + k] = m.cen[c];
         soff[6 * SS + k] = m.cen[Nt + c];
         if (loadz) soff[7 * SS + k] = m.cen[2 * (size_t)Nt + c];
-        soff[8 * SS + k] = prim[(size_t)P_T * Nt + c];
-        soff[9 * SS + k] = prim[(size_t)P_VX * Nt + c];
-        soff[10 * SS + k] = prim[(size_t)P_VY * Nt + c];
-        soff[11 * SS + k] = prim[(size_t)P_VZ * Nt + c];
+#pragma unroll
+        for (int q = 0; q < 4; q++) soff[(8 + q) * SS + k] = tv[q];
     };
     if (i < N) stage(i, threadIdx.x);
     for (int k = threadIdx.x; k < nh; k += m.tile) stage(m.halo_ids[h0 + k], m.tile + k);
@@ -835,13 +963,16 @@ __device__ __forceinline__ bool viscous_slot(int slot) {
 // gradients are computed after the convective ones in the reference and
 // overwrite them); the adjoint passes its own vgrad and keeps both.
 template <int DIM, int LIM, int LAYOUT, bool OFFS, class W = double, bool PACKED = true,
-          bool PF = false, bool SN = false, bool VISC = false>
+          bool PF = false, bool SN = false, bool VISC = false, bool FOLD = false>
 __global__ void __launch_bounds__(MAXTILE)
     k_cell_fs(DevMesh m, VarSet vs, const double* __restrict__ pack, double* __restrict__ grad,
               double* __restrict__ lim, double eps, double venkat_K, int nlogical,
               int write_slots, W* __restrict__ FL, W* __restrict__ FR,
               const double* __restrict__ offs, const double* __restrict__ prim, int stage,
-              const double* __restrict__ gconv, double* __restrict__ vgrad) {
+              const double* __restrict__ gconv, double* __restrict__ vgrad,
+              BcFold bf = BcFold{}) {
+    // FOLD (FP64, SN): the boundary pass runs first, on the tile's halo ghosts
+    // (fold_halo_ghosts, bf), then the staging of the rest of the stencil.
     // stage != 0: the limiter pass keeps each face offset in LDS
     // (maxf x 3 doubles per thread) for the reconstruction pass, so the
     // offsets are read from memory once.
@@ -853,6 +984,9 @@ __global__ void __launch_bounds__(MAXTILE)
     extern __shared__ double soff[];
     const int lb = xcd_block(nlogical);
     if (lb >= nlogical) return;
+    static_assert(!FOLD || (SN && std::is_same<W, double>::value && !PACKED),
+                  "the folded boundary pass: LDS stencil, FP64 face values from the state");
+    if (FOLD) fold_halo_ghosts<VISC>(m, vs, bf, lb, soff);
     const int TS = m.tile;  // cells per block: one tile
     const int i = lb * TS + threadIdx.x;
     const int N = m.N, F = m.F;
@@ -883,9 +1017,9 @@ __global__ void __launch_bounds__(MAXTILE)
         for (int j = 0; j < MAXF; j++) kst[j] = j < m.maxf ? m.nbr_lds[(size_t)j * N + il] : 0;
     }
     if (SN && VISC)
-        stage_stencil_visc(m, vs, prim, gconv, lb, i, soff);
+        stage_stencil_visc<FOLD>(m, vs, prim, gconv, lb, i, soff);
     else if (SN)
-        stage_stencil<PACKED>(m, vs, pack, prim, lb, i, soff);
+        stage_stencil<PACKED, FOLD>(m, vs, pack, prim, lb, i, soff);
     // with the offsets staged over the stencil, a thread past the last cell
     // stays to the second barrier (it works on LDS slots that are not read
     // back and stores nothing)
@@ -1226,7 +1360,7 @@ __global__ void __launch_bounds__(MAXTILE)
     if (SN) {
 #pragma unroll
         for (int j = 0; j < MAXF; j++) kst[j] = j < m.maxf ? m.nbr_lds[(size_t)j * N + il] : 0;
-        stage_stencil<false>(m, vs, nullptr, prim, lb, i, sh32);
+        stage_stencil<false, false>(m, vs, nullptr, prim, lb, i, sh32);
     }
     // with the offsets staged over the stencil, a thread past the last cell
     // stays to the second barrier (it works on LDS slots that are not read
@@ -2830,6 +2964,8 @@ struct ibis_residual {
     int opt_keep_grads = 0;     // option keep_gradients: the fused path writes the
                                 // gradient and limiter slots too
     int opt_fuse_visc = 1;      // option fuse_viscous: the four-kernel viscous path
+    int opt_fold = 0;           // option fold_boundary: the fused paths' boundary pass
+                                // runs in the cell kernel's stencil staging
     int force_slots = 0;        // the adjoint's forward pass: the fused cell kernel
                                 // writes the gradient and limiter slots
     double* gconv = nullptr;    // viscous: the ghosts' reconstruction variables (5 x G)
@@ -3000,6 +3136,9 @@ ibis_status ibis_residual_create_tables(const ibis_mesh* m, const ibis_fv_desc* 
     r->has_pre = r->has_pre && D.nb > 0;
     r->has_visc = r->has_visc && D.nb > 0;
     r->has_post = r->has_post && D.nb > 0;
+    // the boundary pass in the cell kernel's staging pays on 2D meshes; in
+    // 3D the action code's registers cost the cell kernel occupancy
+    r->opt_fold = D.dim == 2 ? 1 : 0;
     if (hipMemcpy(r->kinds, kinds3, 3 * (size_t)ntab * sizeof(int), hipMemcpyHostToDevice) !=
             hipSuccess ||
         hipMemcpy(r->pars, pars3, 3 * (size_t)ntab * IBIS_NPAR * sizeof(double),
@@ -3126,16 +3265,31 @@ size_t cell_stage_bytes(const DevMesh& D, int lim, int opt) {
     return (opt && lim != 0 && b <= 65536) ? b : 0;
 }
 
+// the cell stencil goes to LDS: the option stage_cells, when its 8 planes
+// fit one block's 64 KiB
+bool stencil_in_lds(const ibis_residual* r) {
+    const DevMesh& D = r->d;
+    return r->opt_stage_cells && (size_t)8 * (D.tile + D.maxhalo) * sizeof(double) <= 65536;
+}
+
+// the inviscid fused path's boundary pass runs in the cell kernel's staging
+// (option fold_boundary): FP64, layout 2 gathered from the state, stencil in LDS
+bool fold_on(const ibis_residual* r) {
+    return r->opt_fold && r->prec == IBIS_PREC_FP64 && r->opt_layout == 2 && !r->opt_pack &&
+           stencil_in_lds(r);
+}
+
+// bf: the folded boundary pass (only when fold_on), or null
 template <int DIM, int LIM, int LAYOUT, bool OFFS>
 void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d, ibis_residual* r,
-                      int nlog, const double* prim, hipStream_t st) {
+                      int nlog, const double* prim, hipStream_t st, const BcFold* bf) {
     const size_t shm = cell_stage_bytes(D, LIM, r->opt_stage);
     const int stage = shm ? 1 : 0;
     const int ws = (r->opt_keep_grads || r->force_slots) ? 1 : 0;
     // the LDS cell stencil (option stage_cells), in all precisions; the
     // FP64 and FP32-flux kernels reuse it for the offsets (stage_offsets)
     const size_t shs0 = (size_t)8 * (D.tile + D.maxhalo) * sizeof(double);
-    const bool sn = r->opt_stage_cells && shs0 <= 65536;
+    const bool sn = stencil_in_lds(r);
     const size_t shs = std::max(shs0, shm);
     if (r->prec == IBIS_PREC_FP32_RECON) {
         float* fl = reinterpret_cast<float*>(r->FL);
@@ -3161,10 +3315,16 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
     }
     if (!r->opt_pack && LAYOUT == 2) {
         if (sn) {
-            k_cell_fs<DIM, LIM, 2, OFFS, double, false, false, true>
-                <<<xcd_grid(nlog), D.tile, shs, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps,
-                                                     d.venkat_K, nlog, ws, r->FL, r->FR, r->offs,
-                                                     prim, stage, nullptr, nullptr);
+            if (bf)
+                k_cell_fs<DIM, LIM, 2, OFFS, double, false, false, true, false, true>
+                    <<<xcd_grid(nlog), D.tile, shs, st>>>(
+                        D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, ws, r->FL,
+                        r->FR, r->offs, prim, stage, nullptr, nullptr, *bf);
+            else
+                k_cell_fs<DIM, LIM, 2, OFFS, double, false, false, true>
+                    <<<xcd_grid(nlog), D.tile, shs, st>>>(
+                        D, vs, r->pack, r->grad, r->lim, d.lim_eps, d.venkat_K, nlog, ws, r->FL,
+                        r->FR, r->offs, prim, stage, nullptr, nullptr);
             return;
         }
         if (r->opt_cell_prefetch)
@@ -3184,27 +3344,27 @@ void launch_cell_fs_t(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
 
 template <int DIM, int LIM>
 void launch_cell_fs_l(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d, ibis_residual* r,
-                      int nlog, const double* prim, hipStream_t st) {
+                      int nlog, const double* prim, hipStream_t st, const BcFold* bf) {
     const int sel = r->opt_layout * 2 + (r->opt_offsets ? 1 : 0);
     switch (sel) {
-        case 0: launch_cell_fs_t<DIM, LIM, 0, false>(D, vs, d, r, nlog, prim, st); break;
-        case 1: launch_cell_fs_t<DIM, LIM, 0, true>(D, vs, d, r, nlog, prim, st); break;
-        case 2: launch_cell_fs_t<DIM, LIM, 1, false>(D, vs, d, r, nlog, prim, st); break;
-        case 3: launch_cell_fs_t<DIM, LIM, 1, true>(D, vs, d, r, nlog, prim, st); break;
-        case 4: launch_cell_fs_t<DIM, LIM, 2, false>(D, vs, d, r, nlog, prim, st); break;
-        default: launch_cell_fs_t<DIM, LIM, 2, true>(D, vs, d, r, nlog, prim, st); break;
+        case 0: launch_cell_fs_t<DIM, LIM, 0, false>(D, vs, d, r, nlog, prim, st, bf); break;
+        case 1: launch_cell_fs_t<DIM, LIM, 0, true>(D, vs, d, r, nlog, prim, st, bf); break;
+        case 2: launch_cell_fs_t<DIM, LIM, 1, false>(D, vs, d, r, nlog, prim, st, bf); break;
+        case 3: launch_cell_fs_t<DIM, LIM, 1, true>(D, vs, d, r, nlog, prim, st, bf); break;
+        case 4: launch_cell_fs_t<DIM, LIM, 2, false>(D, vs, d, r, nlog, prim, st, bf); break;
+        default: launch_cell_fs_t<DIM, LIM, 2, true>(D, vs, d, r, nlog, prim, st, bf); break;
     }
 }
 
 template <int DIM>
 void launch_cell_fs(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d, ibis_residual* r,
-                    int nlog, const double* prim, hipStream_t st) {
+                    int nlog, const double* prim, hipStream_t st, const BcFold* bf = nullptr) {
     if (d.limiter == 0)
-        launch_cell_fs_l<DIM, 0>(D, vs, d, r, nlog, prim, st);
+        launch_cell_fs_l<DIM, 0>(D, vs, d, r, nlog, prim, st, bf);
     else if (d.limiter == 1)
-        launch_cell_fs_l<DIM, 1>(D, vs, d, r, nlog, prim, st);
+        launch_cell_fs_l<DIM, 1>(D, vs, d, r, nlog, prim, st, bf);
     else
-        launch_cell_fs_l<DIM, 2>(D, vs, d, r, nlog, prim, st);
+        launch_cell_fs_l<DIM, 2>(D, vs, d, r, nlog, prim, st, bf);
 }
 
 template <int DIM, int FLUX, int TS, class R>
@@ -3338,7 +3498,7 @@ size_t visc_stencil_bytes(const DevMesh& D) {
 template <int DIM, int LIM>
 void launch_cell_visc_l(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d,
                         ibis_residual* r, const double* prim, double* vgrad, int write_slots,
-                        hipStream_t st) {
+                        hipStream_t st, const BcFold* bf) {
     const size_t shm = cell_stage_bytes(D, LIM, r->opt_stage);
     const int stage = shm ? 1 : 0;
     const size_t shs = std::max(visc_stencil_bytes(D), shm);
@@ -3350,21 +3510,31 @@ void launch_cell_visc_l(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& 
                                                       r->offs, prim, stage, r->gconv, vgrad);
         return;
     }
-    k_cell_fs<DIM, LIM, 2, true, double, false, false, true, true>
-        <<<xcd_grid(D.ntiles), D.tile, shs, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps,
-                                                  d.venkat_K, D.ntiles, write_slots, r->FL,
-                                                  r->FR, r->offs, prim, stage, r->gconv, vgrad);
+    if (bf)
+        k_cell_fs<DIM, LIM, 2, true, double, false, false, true, true, true>
+            <<<xcd_grid(D.ntiles), D.tile, shs, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps,
+                                                      d.venkat_K, D.ntiles, write_slots, r->FL,
+                                                      r->FR, r->offs, prim, stage, r->gconv,
+                                                      vgrad, *bf);
+    else
+        k_cell_fs<DIM, LIM, 2, true, double, false, false, true, true>
+            <<<xcd_grid(D.ntiles), D.tile, shs, st>>>(D, vs, r->pack, r->grad, r->lim, d.lim_eps,
+                                                      d.venkat_K, D.ntiles, write_slots, r->FL,
+                                                      r->FR, r->offs, prim, stage, r->gconv,
+                                                      vgrad);
 }
 
+// bf: the boundary pass folded into the staging (FP64 only), or null
 template <int DIM>
 void launch_cell_visc(const DevMesh& D, const VarSet& vs, const ibis_fv_desc& d, ibis_residual* r,
-                      const double* prim, double* vgrad, int write_slots, hipStream_t st) {
+                      const double* prim, double* vgrad, int write_slots, hipStream_t st,
+                      const BcFold* bf = nullptr) {
     if (d.limiter == 0)
-        launch_cell_visc_l<DIM, 0>(D, vs, d, r, prim, vgrad, write_slots, st);
+        launch_cell_visc_l<DIM, 0>(D, vs, d, r, prim, vgrad, write_slots, st, bf);
     else if (d.limiter == 1)
-        launch_cell_visc_l<DIM, 1>(D, vs, d, r, prim, vgrad, write_slots, st);
+        launch_cell_visc_l<DIM, 1>(D, vs, d, r, prim, vgrad, write_slots, st, bf);
     else
-        launch_cell_visc_l<DIM, 2>(D, vs, d, r, prim, vgrad, write_slots, st);
+        launch_cell_visc_l<DIM, 2>(D, vs, d, r, prim, vgrad, write_slots, st, bf);
 }
 
 // the fused viscous path can run this residual (second order, the options
@@ -3619,21 +3789,28 @@ ibis_status compute_dudt_impl(ibis_residual* r, double* prim, double* dudt,
     if (visc_fused) {
         const bool f32 = r->prec == IBIS_PREC_FP32_FLUX;
         const GasParams gp{d.R, d.Cv, d.Cp, d.gamma};
+        // option fold_boundary (FP64): both boundary passes run in the cell
+        // kernel's staging of each ghost
+        const bool fold = !f32 && r->opt_fold;
+        const BcFold bf{r->kinds, r->pars, r->ntab, r->has_pre ? 1 : 0, r->has_visc ? 1 : 0,
+                        gp, prim, r->FL, nullptr};
         pf.beg();
         if (D.nb > 0 && f32)
             k_bc_visc_fs<float><<<nblocks(D.nb), BLOCK, 0, st>>>(
                 D, r->kinds, r->pars, r->ntab, r->has_pre ? 1 : 0, r->has_visc ? 1 : 0, gp, vs,
                 prim, reinterpret_cast<float*>(r->FL), r->gconv, nullptr);
-        else if (D.nb > 0)
+        else if (D.nb > 0 && !fold)
             k_bc_visc_fs<double><<<nblocks(D.nb), BLOCK, 0, st>>>(
                 D, r->kinds, r->pars, r->ntab, r->has_pre ? 1 : 0, r->has_visc ? 1 : 0, gp, vs,
                 prim, r->FL, r->gconv, nullptr);
         pf.end(0);
         pf.beg();
         if (D.dim == 3)
-            launch_cell_visc<3>(D, vs, d, r, prim, r->grad, r->opt_keep_grads, st);
+            launch_cell_visc<3>(D, vs, d, r, prim, r->grad, r->opt_keep_grads, st,
+                                fold ? &bf : nullptr);
         else
-            launch_cell_visc<2>(D, vs, d, r, prim, r->grad, r->opt_keep_grads, st);
+            launch_cell_visc<2>(D, vs, d, r, prim, r->grad, r->opt_keep_grads, st,
+                                fold ? &bf : nullptr);
         pf.end(1);
         pf.beg();
         const ViscParams vp{d.R, d.Cv, d.Cp, d.mu0, d.T0, d.Ts, d.Pr};
@@ -3665,9 +3842,13 @@ ibis_status compute_dudt_impl(ibis_residual* r, double* prim, double* dudt,
         }
         if (recon) {
             // face-state path: boundary actions + ghost face states, pack,
-            // cell reconstruction, fused face flux + integral
+            // cell reconstruction, fused face flux + integral; with the option
+            // fold_boundary the boundary pass runs in the cell kernel's staging
+            const bool fold = fold_on(r);
+            const BcFold bf{r->kinds, r->pars, r->ntab, r->has_pre ? 1 : 0, 0, gp, prim, r->FL,
+                            nullptr};
             pf.beg();
-            if (D.nb > 0) {
+            if (D.nb > 0 && !fold) {
                 const unsigned g = nblocks(D.nb);
                 const int* kd = r->kinds;
                 const double* pr = r->pars;
@@ -3701,9 +3882,9 @@ ibis_status compute_dudt_impl(ibis_residual* r, double* prim, double* dudt,
             const int nlog = D.ntiles;  // one block per tile
             pf.beg();
             if (D.dim == 3)
-                launch_cell_fs<3>(D, vs, d, r, nlog, prim, st);
+                launch_cell_fs<3>(D, vs, d, r, nlog, prim, st, fold ? &bf : nullptr);
             else
-                launch_cell_fs<2>(D, vs, d, r, nlog, prim, st);
+                launch_cell_fs<2>(D, vs, d, r, nlog, prim, st, fold ? &bf : nullptr);
             pf.end(1);
             pf.beg();
             const bool f32 = r->prec != IBIS_PREC_FP64;
@@ -4019,6 +4200,10 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
         r->opt_fuse_visc = value ? 1 : 0;
         return IBIS_OK;
     }
+    if (k == "fold_boundary") {
+        r->opt_fold = value ? 1 : 0;
+        return IBIS_OK;
+    }
     return ibis::fail(IBIS_ERR_INVALID, ("unknown option " + k).c_str());
 }
 
@@ -4186,19 +4371,27 @@ ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* 
     const bool inv_fused = !d.viscous && d.order == 2 && !r->has_post &&
                            r->prec == IBIS_PREC_FP64 && r->opt_layout == 2 && r->offs &&
                            r->opt_offsets && !r->opt_pack && r->opt_stage_cells;
+    // the option fold_boundary: the boundary passes run in the cell kernel's
+    // staging, which also records the ghosts between the viscous passes
     if (vis_fused) {
-        if (D.nb > 0)
+        const bool fold = r->opt_fold;
+        const BcFold bf{r->kinds, r->pars, r->ntab, r->has_pre ? 1 : 0, r->has_visc ? 1 : 0,
+                        gp, prim, r->FL, r->adj_prim_c};
+        if (D.nb > 0 && !fold)
             k_bc_visc_fs<double><<<nblocks(D.nb), BLOCK, 0, st>>>(
                 D, r->kinds, r->pars, r->ntab, r->has_pre ? 1 : 0, r->has_visc ? 1 : 0, gp, vs,
                 prim, r->FL, r->gconv, r->adj_prim_c);
         if (D.dim == 3)
-            launch_cell_visc<3>(D, vs, d, r, prim, r->adj_vgrad, 1, st);
+            launch_cell_visc<3>(D, vs, d, r, prim, r->adj_vgrad, 1, st, fold ? &bf : nullptr);
         else
-            launch_cell_visc<2>(D, vs, d, r, prim, r->adj_vgrad, 1, st);
+            launch_cell_visc<2>(D, vs, d, r, prim, r->adj_vgrad, 1, st, fold ? &bf : nullptr);
     } else if (inv_fused) {
         // the inviscid second-order path's own kernels: boundary actions and
         // ghost face values, then the cell kernel writing the slots too
-        if (D.nb > 0) {
+        const bool fold = fold_on(r);
+        const BcFold bf{r->kinds, r->pars, r->ntab, r->has_pre ? 1 : 0, 0, gp, prim, r->FL,
+                        nullptr};
+        if (D.nb > 0 && !fold) {
             if (r->has_pre)
                 k_bc_ghost_fs<2><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, vs,
                                                                   prim, r->FL, r->FR);
@@ -4207,9 +4400,9 @@ ibis_status ibis_compute_dudt_vjp(ibis_residual* r, double* prim, const double* 
         }
         r->force_slots = 1;
         if (D.dim == 3)
-            launch_cell_fs<3>(D, vs, d, r, D.ntiles, prim, st);
+            launch_cell_fs<3>(D, vs, d, r, D.ntiles, prim, st, fold ? &bf : nullptr);
         else
-            launch_cell_fs<2>(D, vs, d, r, D.ntiles, prim, st);
+            launch_cell_fs<2>(D, vs, d, r, D.ntiles, prim, st, fold ? &bf : nullptr);
         r->force_slots = 0;
     } else {
     if (r->has_pre) k_bc<double><<<nblocks(D.nb), BLOCK, 0, st>>>(D, r->kinds, r->pars, gp, prim);

@@ -358,12 +358,22 @@ ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** lim
  *                            as four kernels -- boundary actions of both
  *                            passes, cell kernel (viscous gradients,
  *                            convective gradients, limiter, face values),
- *                            viscous terms per tile-list entry, face
- *                            kernel (convective flux minus the viscous
- *                            terms, surface integral) -- with layout 2,
- *                            offsets, face_kernel 1, stage_cells,
- *                            pack_cells 0; 0: the reference-shaped
- *                            general path
+ *                            viscous terms per face, face kernel
+ *                            (convective flux minus the viscous terms,
+ *                            surface integral) -- with layout 2, offsets,
+ *                            face_kernel 1, stage_cells, pack_cells 0; 0:
+ *                            the reference-shaped general path
+ *   "fold_boundary"       1 (the default on 2D meshes): on the fused paths
+ *                            in FP64 with the cell stencil in LDS, the
+ *                            boundary actions (both passes when viscous),
+ *                            the ghosts' face values and the ghost rows of
+ *                            prim are formed by the cell kernel's block
+ *                            whose tile has the ghost in its stencil halo,
+ *                            before it stages the rest of the stencil,
+ *                            instead of by a kernel of their own; 0 (the
+ *                            default on 3D meshes, where the action code's
+ *                            registers cost the cell kernel occupancy):
+ *                            the boundary kernel
  * Setting precomputed_offsets, face_kernel 1 or a new tile size allocates
  * and synchronises once. */
 ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t value);
@@ -371,7 +381,8 @@ ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t 
 /* Per-kernel timing with HIP events recorded around every launch of
  * ibis_compute_dudt on its stream.  Kernel slots:
  *   0 boundary actions before reconstruction (k_bc; on the fused path
- *     k_bc_ghost_fs, or k_bc, k_pack and k_ghost_fs)
+ *     k_bc_ghost_fs, or k_bc, k_pack and k_ghost_fs; nothing when the
+ *     option fold_boundary puts them in the cell kernel)
  *   0 also k_bc_visc_fs (both boundary passes) on the fused viscous path
  *   1 cell stencil: k_cell_grad_var, or k_cell_fs on the fused path
  *     (gradients, limiter and face values; viscous gradients too on the

@@ -68,13 +68,17 @@ def test_wedge_variants(kw):
 @pytest.mark.parametrize("layout", [0, 2])
 def test_wedge_subsonic_boundaries(layout):
     """Subsonic inflow / outflow and slip-wall actions on the second-order
-    inviscid path (boundary actions fused with the ghost face values)."""
+    inviscid path: boundary actions fused with the ghost face values (layout
+    0), and run in the cell kernel's staging of each ghost (layout 2, the
+    option fold_boundary) or as a kernel of their own (fold_boundary 0)."""
     from ibis_amd import config as C
     g_io, cfg, fs = cases.wedge_case(48, 24, mach=0.5)
     cfg.boundaries = {"inflow": C.subsonic_inflow(fs), "outflow": C.subsonic_outflow(fs.p * 0.98),
                       "wall": C.slip_wall()}
     prim = cases.perturbed_prim(g_io, fs, g_io.num_cells + 2 * (48 + 24), seed=15)
     _check(g_io, cfg, prim, exact=True, options={"face_state_layout": layout})
+    if layout == 2:
+        _check(g_io, cfg, prim, exact=True, options={"fold_boundary": 0})
 
 
 @pytest.mark.parametrize("zmode", ["flat", "uneven"])
@@ -155,7 +159,7 @@ def test_implementation_options_give_identical_bits(layout, pack, fk, stage, pf,
     _check(g_io, cfg, prim, exact=True, options=opts)
 
 
-TILE_OPTIONS = [{"tile": 128}, {"tile": 512},
+TILE_OPTIONS = [{"tile": 128}, {"tile": 512}, {"fold_boundary": 0}, {"fold_boundary": 1},
                 {"tile": 512, "face_kernel": 0}, {"tile": 128, "stage_cells": 0, "cell_prefetch": 1},
                 {"tile": 512, "face_state_layout": 0, "pack_cells": 1, "face_kernel": 0}]
 
@@ -451,8 +455,11 @@ def test_fused_viscous_path_equals_general_path(name):
     _check(g_io, cfg, prim_ref, exact=False)
     gb, fv = gpu_objects(g_io, cfg)
     outs = []
-    for fuse in (1, 0):
+    # the fused path with its boundary passes folded into the cell kernel's
+    # staging (the default) and as a kernel of their own, then the general path
+    for fuse, fold in ((1, 1), (1, 0), (0, 1)):
         fv.set_option("fuse_viscous", fuse)
+        fv.set_option("fold_boundary", fold)
         fv.set_option("keep_gradients", 1)
         d, p = gpu_dudt(gb, fv, prim_ref)
         dt = fv.estimate_dt(to_dev(gb.to_grid_order(p)))
@@ -461,7 +468,8 @@ def test_fused_viscous_path_equals_general_path(name):
         outs.append((d, p, dt, grads))
     n = gb.N
     print(f"{name}: N={n}", flush=True)
-    assert outs[0][0][:, :n].tobytes() == outs[1][0][:, :n].tobytes()
-    assert outs[0][1].tobytes() == outs[1][1].tobytes()
-    assert outs[0][2] == outs[1][2]
-    assert outs[0][3].tobytes() == outs[1][3].tobytes()
+    for o in outs[:2]:
+        assert o[0][:, :n].tobytes() == outs[2][0][:, :n].tobytes()
+        assert o[1].tobytes() == outs[2][1].tobytes()
+        assert o[2] == outs[2][2]
+        assert o[3].tobytes() == outs[2][3].tobytes()
