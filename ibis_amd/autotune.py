@@ -4,7 +4,8 @@ Every option of ibis_residual_set_option listed here changes only how the
 work is laid out (face-value layout, precomputed offsets, packing, face
 kernel and its block size, LDS staging of offsets or of the cell stencil,
 prefetching, cells per tile, which is also the block size of the tile
-kernels), never the arithmetic: all choices give the same bits.  The tuner
+kernels, whether the cell kernel runs the boundary actions of its stencil's
+ghosts), never the arithmetic: all choices give the same bits.  The tuner
 times a caller-supplied workload under each candidate and keeps the
 fastest, by coordinate descent.  A candidate the library refuses for the
 mesh at hand (for example a tile whose face list does not fit in LDS) is
@@ -18,10 +19,10 @@ from typing import Callable, Dict, Mapping, Tuple
 OPTION_SPACE = (("face_state_layout", (2, 1, 0)), ("precomputed_offsets", (1, 0)),
                 ("pack_cells", (0, 1)), ("face_kernel", (1, 0, 64, 128, 256)),
                 ("stage_offsets", (1, 0)), ("cell_prefetch", (0, 1)), ("stage_cells", (1, 0)),
-                ("tile", (256, 128, 512)))
+                ("tile", (256, 128, 512)), ("fold_boundary", (1, 0)))
 SHORT = {"face_state_layout": "layout", "precomputed_offsets": "offsets", "pack_cells": "pack",
          "face_kernel": "fk", "stage_offsets": "stage", "cell_prefetch": "pf",
-         "stage_cells": "sc", "tile": "tile"}
+         "stage_cells": "sc", "tile": "tile", "fold_boundary": "fold"}
 
 
 def variant_key(opt: Mapping[str, int]) -> str:

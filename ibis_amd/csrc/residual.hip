@@ -124,6 +124,7 @@ struct DevMesh {
 };
 
 constexpr int TILE = 256;     // cells per tile of a new mesh (the option "tile" picks others)
+constexpr int FOLD_MAX_CELLS = 1 << 20;  // 2D meshes up to this size start with fold_boundary on
 constexpr int MAXTILE = 512;  // largest tile: the tile kernels' block size is the tile
 constexpr size_t MAX_LDS_BYTES = 160 * 1024;  // LDS of one gfx950 workgroup
 constexpr int NXCD = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin across them
@@ -3136,9 +3137,11 @@ ibis_status ibis_residual_create_tables(const ibis_mesh* m, const ibis_fv_desc* 
     r->has_pre = r->has_pre && D.nb > 0;
     r->has_visc = r->has_visc && D.nb > 0;
     r->has_post = r->has_post && D.nb > 0;
-    // the boundary pass in the cell kernel's staging pays on 2D meshes; in
-    // 3D the action code's registers cost the cell kernel occupancy
-    r->opt_fold = D.dim == 2 ? 1 : 0;
+    // the boundary pass in the cell kernel's staging saves the boundary
+    // kernel (a fixed latency) and costs the cell kernel occupancy (a cost per
+    // cell): it pays on 2D meshes up to about a million cells; in 3D the
+    // action code's registers cost more than it saves
+    r->opt_fold = (D.dim == 2 && D.N <= FOLD_MAX_CELLS) ? 1 : 0;
     if (hipMemcpy(r->kinds, kinds3, 3 * (size_t)ntab * sizeof(int), hipMemcpyHostToDevice) !=
             hipSuccess ||
         hipMemcpy(r->pars, pars3, 3 * (size_t)ntab * IBIS_NPAR * sizeof(double),

@@ -363,7 +363,8 @@ ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** lim
  *                            surface integral) -- with layout 2, offsets,
  *                            face_kernel 1, stage_cells, pack_cells 0; 0:
  *                            the reference-shaped general path
- *   "fold_boundary"       1 (the default on 2D meshes): on the fused paths
+ *   "fold_boundary"       1 (the default on 2D meshes of at most 2^20
+ *                            cells): on the fused paths
  *                            in FP64 with the cell stencil in LDS, the
  *                            boundary actions (both passes when viscous),
  *                            the ghosts' face values and the ghost rows of
@@ -371,9 +372,11 @@ ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** lim
  *                            whose tile has the ghost in its stencil halo,
  *                            before it stages the rest of the stencil,
  *                            instead of by a kernel of their own; 0 (the
- *                            default on 3D meshes, where the action code's
- *                            registers cost the cell kernel occupancy):
- *                            the boundary kernel
+ *                            default otherwise: the action code's registers
+ *                            cost the cell kernel occupancy, a cost per
+ *                            cell that outweighs the boundary kernel saved
+ *                            on larger and on 3D meshes): the boundary
+ *                            kernel
  * Setting precomputed_offsets, face_kernel 1 or a new tile size allocates
  * and synchronises once. */
 ibis_status ibis_residual_set_option(ibis_residual* r, const char* key, int64_t value);

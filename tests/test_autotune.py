@@ -6,7 +6,7 @@ def test_finds_the_separable_minimum_and_leaves_it_set():
     # a cost that is a sum over options: coordinate descent finds its minimum
     best_val = {"face_state_layout": 1, "precomputed_offsets": 0, "pack_cells": 1,
                 "face_kernel": 128, "stage_offsets": 0, "cell_prefetch": 1, "stage_cells": 0,
-                "tile": 512}
+                "tile": 512, "fold_boundary": 0}
     state = {}
 
     def measure():
