@@ -371,6 +371,13 @@ static StageArgs stage_terms(const ibis_batch* b, int i) {
     return sa;
 }
 
+// 1 when a step runs with the stage updates in the face kernel and its
+// last stage also writes the local time steps of the new state
+static bool step_carries_dt(ibis_batch* b) {
+    return b->fuse_stages && ibis_residual_stage_fusable(b->r) && b->s <= IBIS_EPI_MAXK &&
+           ibis_residual_stage_dt_fusable(b->r);
+}
+
 // One step with the stage updates in the residual's face kernel: the
 // time step first (it reads only the valid cells' state, which the
 // residual does not change), then each residual's epilogue evaluates the
@@ -378,11 +385,17 @@ static StageArgs stage_terms(const ibis_batch* b, int i) {
 // its own dU/dt in registers, with the same expressions in the same order
 // as the separate stage pass; a pass over the ghost rows keeps them as the
 // separate pass leaves them.  The last stage's dU/dt is not stored.
+// When step_carries_dt, the last stage's epilogue also writes the local
+// time steps of the step's result (the same bits as ibis_residual_cell_dt
+// of that state), so the next step starts from them and no step reads the
+// state and face geometry again for its time step; ibis_batch_rk_steps
+// computes them for the first step of each call.
 static ibis_status batch_step_fused(ibis_batch* b) {
     hipStream_t st = b->stream;
     ibis_stream_t sh = reinterpret_cast<ibis_stream_t>(st);
     const int s = b->s;
-    IBIS_TRY(ibis_residual_cell_dt(b->r, b->prim, b->dt_cells, sh));
+    const bool carry_dt = step_carries_dt(b);
+    if (!carry_dt) IBIS_TRY(ibis_residual_cell_dt(b->r, b->prim, b->dt_cells, sh));
     batch_dt(b, st);
     for (int i = 0; i < s; i++) {
         ibis_stage_epilogue e;
@@ -417,6 +430,7 @@ static ibis_status batch_step_fused(ibis_batch* b) {
             e.c_reg = b->bw[s - 1];
             e.cons_out = b->cons;
             e.prim_out = b->prim;
+            if (carry_dt) e.dt_cells_out = b->dt_cells;
             dudt = nullptr;
             for (int j = 0; j < s; j++) {
                 sa.k[j] = b->k[j];
@@ -469,6 +483,11 @@ ibis_status ibis_batch_rk_steps(ibis_batch* b, int nsteps, int use_graph) {
         return ibis::fail(IBIS_ERR_INVALID, "ibis_batch_rk_steps: call ibis_batch_rk_init first");
     // the state may have been written on any stream of the device
     HIP_TRY(hipDeviceSynchronize());
+    // the local time steps of the state as the caller left it, for a first
+    // step that takes them from the previous one's last stage
+    if (nsteps > 0 && step_carries_dt(b))
+        IBIS_TRY(ibis_residual_cell_dt(b->r, b->prim, b->dt_cells,
+                                       reinterpret_cast<ibis_stream_t>(b->stream)));
     if (!use_graph) {
         for (int n = 0; n < nsteps; n++) IBIS_TRY(batch_step(b));
     } else {

@@ -42,7 +42,10 @@ ibis_status ibis_residual_cell_dt(ibis_residual* r, const double* prim, double* 
 //   u + k_0 (c_0 dt) + ... + k_{nk-1} (c_{nk-1} dt) [+ dU/dt (c_reg dt)]
 // of each conserved row left to right (dt = dt[inst_of_cell[cell]]),
 // writes it to cons_out (unless null) and its primitive rows to prim_out.
-// Ghost rows are not touched.
+// With dt_cells_out set (ibis_residual_stage_dt_fusable), it also writes
+// each valid cell's local time step of the new primitive state, the same
+// bits ibis_residual_cell_dt gives for that state.  Ghost rows are not
+// touched.
 constexpr int IBIS_EPI_MAXK = 8;
 struct ibis_stage_epilogue {
     const double* u = nullptr;
@@ -56,11 +59,18 @@ struct ibis_stage_epilogue {
     double* cons_out = nullptr;
     double* prim_out = nullptr;
     double R = 0.0, Cv = 0.0;
+    double* dt_cells_out = nullptr;
+    // filled in by ibis_compute_dudt_stage from the residual's description
+    double dt_gamma = 0.0, dt_R = 0.0, dt_sf = 0.0;
 };
 // 1 when ibis_compute_dudt_stage fuses the epilogue into this residual's
 // kernels (the inviscid second-order path with the per-entry face kernel
 // in FP64); else the caller runs the stage update separately.
 int ibis_residual_stage_fusable(const ibis_residual* r);
+// 1 when, in addition, the epilogue can write the local time steps
+// (dt_cells_out): an inviscid residual whose tile face lists fit in LDS with
+// each entry's normal and area beside its flux.
+int ibis_residual_stage_dt_fusable(const ibis_residual* r);
 ibis_status ibis_compute_dudt_stage(ibis_residual* r, double* prim, double* dudt,
                                     const ibis_stage_epilogue* epi, ibis_stream_t stream);
 

@@ -1939,6 +1939,38 @@ __device__ __forceinline__ void visc_face(const DevMesh& m, const ViscParams& vp
     }
 }
 
+// estimate_dt's per-cell term (FV/finite_volume.cpp:143-176): the local
+// time step of one cell from its velocity and temperature, with
+// face(j, nx, ny, nz, area) giving the normal and area of its face slot j
+// (slot order) and face_id(j) the face index for the viscous term.
+template <class Face, class FaceId>
+__device__ __forceinline__ double cell_dt_of(double vx, double vy, double vz, double temp,
+                                             double volume, int nf, const double* __restrict__ face_rho,
+                                             const double* __restrict__ face_T, int viscous,
+                                             double sf, const ViscParams& vp, double gamma,
+                                             Face face, FaceId face_id) {
+    const double a = sqrt(gamma * vp.R * temp);
+    double src = 0.0, srv = 0.0;
+    for (int j = 0; j < nf; j++) {
+        double nx, ny, nz, area;
+        face(j, nx, ny, nz, area);
+        const double dot = vx * nx + vy * ny + vz * nz;
+        const double sig_vel = fabs(dot) + a;
+        src += sig_vel * area;
+        if (viscous) {
+            const int f = face_id(j);
+            const double tf = face_T[f];
+            const double mu = sutherland(tf, vp);
+            const double k = vp.Cp * sutherland(tf, vp) / vp.Pr;
+            const double rho = face_rho[f];
+            const double Pr = mu * vp.Cp / k;
+            const double tmp = (gamma / rho) * (mu / Pr) * area * area;
+            srv += tmp / volume;
+        }
+    }
+    return volume / (src + sf * srv);
+}
+
 // k_face_tile_fs over the per-entry face data (layout 2 values); NRM: the
 // normals are stored (they are not cross(t1, t2)).  EPI: the Runge-Kutta
 // stage epilogue (ibis_stage_epilogue) follows each cell's dU/dt; dudt may
@@ -1977,7 +2009,7 @@ __global__ void __launch_bounds__(MAXTILE)
     k_face_entry_fs(DevMesh m, FaceParams fp, const int2* __restrict__ eidx,
                     const double* __restrict__ egeom, const R* __restrict__ FL,
                     double* __restrict__ dudt, ibis_stage_epilogue epi, ViscFace vf) {
-    extern __shared__ double sflux[];  // NC x maxtf
+    extern __shared__ double sflux[];  // NC x maxtf [+ 4 x maxtf: normals, areas]
     const int t = xcd_block(m.ntiles);
     if (t >= m.ntiles) return;
     constexpr int NC = DIM + 2;
@@ -2073,6 +2105,16 @@ __global__ void __launch_bounds__(MAXTILE)
         }
 #pragma unroll
         for (int q = 0; q < NC; q++) sflux[q * S + k] = res[q];
+        if (EPI && epi.dt_cells_out) {
+            // the normal as the flux takes it (cross(t1, t2), GR/interface.h:249,
+            // equal to the stored one for these meshes) and the area, for the
+            // cells' local time steps below
+            double* sgeo = sflux + NC * S;
+            sgeo[k] = NRM ? g[NG - 3] : g[1] * g[5] - g[2] * g[4];
+            sgeo[S + k] = NRM ? g[NG - 2] : g[2] * g[3] - g[0] * g[5];
+            sgeo[2 * S + k] = NRM ? g[NG - 1] : g[0] * g[4] - g[1] * g[3];
+            sgeo[3 * S + k] = g[6];
+        }
         k = kn;
     }
     __syncthreads();
@@ -2119,6 +2161,23 @@ __global__ void __launch_bounds__(MAXTILE)
     ibis::cons_to_prim_cell<double, DIM>(qv, epi.R, epi.Cv, out);
 #pragma unroll
     for (int q = 0; q < NPRIM; q++) epi.prim_out[(size_t)q * Nt + c] = out[q];
+    if (epi.dt_cells_out) {
+        // the local time step of the new state (inviscid): cell_dt's terms
+        // with the faces' normals and areas from LDS, in the cell's slot order
+        const double* sgeo = sflux + NC * S;
+        auto face = [&](int j, double& nx, double& ny, double& nz, double& area) {
+            const int k = m.ent_lds[(size_t)j * N + c];
+            nx = sgeo[k];
+            ny = sgeo[S + k];
+            nz = sgeo[2 * S + k];
+            area = sgeo[3 * S + k];
+        };
+        auto face_id = [](int) { return 0; };
+        ViscParams vp{};
+        vp.R = epi.dt_R;
+        epi.dt_cells_out[c] = cell_dt_of(out[P_VX], out[P_VY], out[P_VZ], out[P_T], V, nf, nullptr,
+                                         nullptr, 0, epi.dt_sf, vp, epi.dt_gamma, face, face_id);
+    }
 }
 
 // Fused face flux + surface integral, slot-major variant for face values in
@@ -2301,31 +2360,19 @@ __device__ __forceinline__ double cell_dt(const DevMesh& m, const double* __rest
                                           const double* __restrict__ face_rho,
                                           const double* __restrict__ face_T, int viscous,
                                           double sf, const ViscParams& vp, double gamma, int i) {
-    const int N = m.N, Nt = m.Nt, F = m.F;
-    const double vx = prim[(size_t)P_VX * Nt + i], vy = prim[(size_t)P_VY * Nt + i],
-                 vz = prim[(size_t)P_VZ * Nt + i];
-    const double a = sqrt(gamma * vp.R * prim[(size_t)P_T * Nt + i]);
-    double src = 0.0, srv = 0.0;
-    const double volume = m.vol[i];
-    const int nf = m.nfaces[i];
-    for (int j = 0; j < nf; j++) {
-        const int f = m.ell_face[(size_t)j * N + i];
-        const double area = m.geom[12 * (size_t)F + f];
-        const double dot =
-            vx * m.geom[f] + vy * m.geom[(size_t)F + f] + vz * m.geom[2 * (size_t)F + f];
-        const double sig_vel = fabs(dot) + a;
-        src += sig_vel * area;
-        if (viscous) {
-            const double tf = face_T[f];
-            const double mu = sutherland(tf, vp);
-            const double k = vp.Cp * sutherland(tf, vp) / vp.Pr;
-            const double rho = face_rho[f];
-            const double Pr = mu * vp.Cp / k;
-            const double tmp = (gamma / rho) * (mu / Pr) * area * area;
-            srv += tmp / volume;
-        }
-    }
-    return volume / (src + sf * srv);
+    const int N = m.N, Nt = m.Nt;
+    const size_t F = m.F;
+    auto face_id = [&](int j) { return m.ell_face[(size_t)j * N + i]; };
+    auto face = [&](int j, double& nx, double& ny, double& nz, double& area) {
+        const int f = face_id(j);
+        area = m.geom[12 * F + f];
+        nx = m.geom[f];
+        ny = m.geom[F + f];
+        nz = m.geom[2 * F + f];
+    };
+    return cell_dt_of(prim[(size_t)P_VX * Nt + i], prim[(size_t)P_VY * Nt + i],
+                      prim[(size_t)P_VZ * Nt + i], prim[(size_t)P_T * Nt + i], m.vol[i],
+                      m.nfaces[i], face_rho, face_T, viscous, sf, vp, gamma, face, face_id);
 }
 
 __global__ void __launch_bounds__(BLOCK)
@@ -3445,7 +3492,8 @@ void launch_face_epi_f(const DevMesh& D, const FaceParams& fp, const int2* eidx,
                        const double* egeom, const double* FL, double* dudt,
                        const ibis_stage_epilogue& epi, hipStream_t st) {
     const unsigned g = xcd_grid(D.ntiles);
-    const size_t shm = (size_t)(DIM + 2) * D.maxtf * sizeof(double);
+    // fluxes, then (for the local time steps) normals and areas
+    const size_t shm = (size_t)(DIM + 2 + (epi.dt_cells_out ? 4 : 0)) * D.maxtf * sizeof(double);
     if (DIM == 2 && D.planar2d)
         k_face_entry_fs<DIM, FLUX, false, double, true, false, true><<<g, D.tile, shm, st>>>(
             D, fp, eidx, egeom, FL, dudt, epi, ViscFace{});
@@ -3940,6 +3988,12 @@ int ibis_residual_stage_fusable(const ibis_residual* r) {
                : 0;
 }
 
+int ibis_residual_stage_dt_fusable(const ibis_residual* r) {
+    if (!ibis_residual_stage_fusable(r)) return 0;
+    const DevMesh& D = r->d;
+    return (size_t)(D.dim + 6) * D.maxtf * sizeof(double) <= MAX_LDS_BYTES ? 1 : 0;
+}
+
 ibis_status ibis_compute_dudt_stage(ibis_residual* r, double* prim, double* dudt,
                                     const ibis_stage_epilogue* epi, ibis_stream_t stream) {
     if (!r || !prim || !epi || !epi->u || !epi->inst_of_cell || !epi->dt || !epi->prim_out ||
@@ -3950,7 +4004,16 @@ ibis_status ibis_compute_dudt_stage(ibis_residual* r, double* prim, double* dudt
     if (!ibis_residual_stage_fusable(r))
         return ibis::fail(IBIS_ERR_INVALID,
                           "ibis_compute_dudt_stage: the residual's path has no stage epilogue");
-    return compute_dudt_impl(r, prim, dudt, 1, stream, epi);
+    if (epi->dt_cells_out && !ibis_residual_stage_dt_fusable(r))
+        return ibis::fail(IBIS_ERR_INVALID,
+                          "ibis_compute_dudt_stage: the residual's epilogue cannot write time steps");
+    // the time-step constants are the residual's own (as ibis_residual_cell_dt)
+    ibis_stage_epilogue e = *epi;
+    const ibis_fv_desc& d = r->desc;
+    e.dt_gamma = d.gamma;
+    e.dt_R = d.R;
+    e.dt_sf = d.signal_factor;
+    return compute_dudt_impl(r, prim, dudt, 1, stream, &e);
 }
 
 // tangent state and scratch, allocated (and zeroed) on first use

@@ -551,7 +551,10 @@ ibis_status ibis_batch_rk_init(ibis_batch* b, int nstages, const double* a, cons
 /* nsteps explicit steps of every instance on the batch's own stream, each
  * instance with its own dt; with use_graph the first call captures one step
  * into a HIP graph that later steps replay.  Synchronises the device before
- * and the batch's stream after. */
+ * and the batch's stream after.  The caller may write the bound state
+ * between calls: each call takes its first time step from the state as it
+ * finds it (with the stage updates fused, later steps take theirs from the
+ * local time steps the previous step's last stage wrote, the same bits). */
 ibis_status ibis_batch_rk_steps(ibis_batch* b, int nsteps, int use_graph);
 /* Per-instance time and last dt (n_inst each; either may be NULL). */
 ibis_status ibis_batch_times(ibis_batch* b, double* t, double* dt_prev);
