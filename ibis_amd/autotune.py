@@ -29,6 +29,16 @@ def variant_key(opt: Mapping[str, int]) -> str:
     return "_".join(f"{SHORT[k]}{opt[k]}" for k, _ in OPTION_SPACE)
 
 
+def canonical(opt: Mapping[str, int]) -> Dict[str, int]:
+    """The same kernels under one name: cell_prefetch applies only to the
+    cell kernel that reads its stencil from memory, so with stage_cells 1
+    both of its values launch the same code."""
+    out = dict(opt)
+    if out.get("stage_cells", 0) == 1 and "cell_prefetch" in out:
+        out["cell_prefetch"] = 0
+    return out
+
+
 def coordinate_descent(set_option: Callable[[str, int], None], measure: Callable[[], float],
                        fixed: Mapping[str, int] = None,
                        log: Callable[[str, float], None] = None,
@@ -46,7 +56,9 @@ def coordinate_descent(set_option: Callable[[str, int], None], measure: Callable
     seen: Dict[str, float] = {}
 
     def score(opt):
-        key = variant_key(opt)
+        # equivalent options share one measurement, so noise between two
+        # runs of the same kernels cannot pick either
+        key = variant_key(canonical(opt))
         if key not in seen:
             try:
                 for k, v in opt.items():

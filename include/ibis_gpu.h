@@ -343,7 +343,9 @@ ibis_status ibis_residual_scratch(ibis_residual* r, double** grads, double** lim
  *                            that LDS after the stencil loop)
  *   "cell_prefetch"       1: the cell kernel loads all neighbour indices
  *                            first and each neighbour's values while the
- *                            previous one is summed (layout 2, pack_cells 0)
+ *                            previous one is summed (layout 2, pack_cells 0,
+ *                            inviscid; no effect when stage_cells stages
+ *                            the stencil in LDS)
  *   "freeze_limiter"      1: the tangent (JVP) treats the limiter values
  *                            as constants; 0 (default) differentiates them
  *   "keep_gradients"      1: the fused second-order path also writes the

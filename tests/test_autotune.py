@@ -68,3 +68,23 @@ def test_refused_candidates_are_skipped():
     opt, t, seen = coordinate_descent(setter, lambda: 10.0 - state["tile"] / 128.0)
     assert opt["tile"] == 256 and state["tile"] == 256
     assert any(v == float("inf") for v in seen.values())
+
+
+def test_equivalent_variants_share_one_measurement():
+    """With stage_cells 1 both cell_prefetch values launch the same kernels:
+    the tuner measures them once, so a later (warmer) run of the same code
+    cannot move the choice to cell_prefetch 1."""
+    state = {}
+    runs = []
+
+    def measure():
+        runs.append(dict(state))
+        # every run is faster than the last (clocks rising), whatever the options
+        return 100.0 - len(runs)
+
+    opt, _, seen = coordinate_descent(lambda k, v: state.__setitem__(k, v), measure,
+                                      fixed={k: vals[0] for k, vals in OPTION_SPACE
+                                             if k not in ("cell_prefetch",)})
+    assert opt["stage_cells"] == 1 and opt["cell_prefetch"] == 0
+    assert len(runs) == 1
+    assert all(r["cell_prefetch"] == 0 for r in runs)
