@@ -58,47 +58,42 @@ constexpr double DMAX = 1.79769313486231570815e+308;
 // then each instance reduces its chunks chunk_off[i] .. chunk_off[i+1].
 constexpr int CHUNK = 8192;
 
-// The time step of every instance in one launch: block k reduces chunk k
-// of dt_cells to partial[k]; the last block to finish (a ticket counter)
-// reduces each instance's chunks to stable[i] and applies
-// RungeKutta::estimate_dt's rules (runge_kutta.cpp:108-119), then resets
-// the counter for the next launch.  The minimum of finite values does not
-// depend on the order of the reduction.
 __global__ void __launch_bounds__(BB)
-    k_dt_step(int nchunks, const int64_t* __restrict__ chunk_lo, const double* __restrict__ dt_cells,
-              double* partial, unsigned* ticket, int n_inst, const int32_t* __restrict__ chunk_off,
-              double* __restrict__ stable, double cfl, const double* __restrict__ dt_prev,
-              const double* __restrict__ t, double max_time, double* __restrict__ dt) {
+    k_chunk_min(const int64_t* __restrict__ chunk_lo, const double* __restrict__ dt_cells,
+                double* __restrict__ partial) {
     __shared__ double sm[BB / 64];
-    __shared__ int last;
     const int k = blockIdx.x;
-    if (k < nchunks) {
-        const int64_t lo = chunk_lo[k], hi = chunk_lo[k + 1];
-        double v = DMAX;
-        for (int64_t c = lo + threadIdx.x; c < hi; c += BB) v = kmin(dt_cells[c], v);
-        for (int o = 32; o > 0; o >>= 1) v = kmin(__shfl_down(v, o, 64), v);
-        if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int w = 1; w < BB / 64; w++) v = kmin(sm[w], v);
-            partial[k] = v;
-        }
-    }
-    if (threadIdx.x == 0) {
-        __threadfence();  // partial[k] is visible before the ticket counts it
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1 ? 1 : 0;
-    }
+    const int64_t lo = chunk_lo[k], hi = chunk_lo[k + 1];
+    double v = DMAX;
+    for (int64_t c = lo + threadIdx.x; c < hi; c += BB) v = kmin(dt_cells[c], v);
+    for (int o = 32; o > 0; o >>= 1) v = kmin(__shfl_down(v, o, 64), v);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (!last) return;
-    __threadfence();  // every block's partial is visible here
-    for (int i = threadIdx.x; i < n_inst; i += BB) {
-        double v = DMAX;
-        for (int j = chunk_off[i]; j < chunk_off[i + 1]; j++) v = kmin(partial[j], v);
-        stable[i] = v;
-        const double d = kmin(cfl * v, 1.5 * dt_prev[i]);
-        dt[i] = kmin(d, max_time - t[i]);
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < BB / 64; w++) v = kmin(sm[w], v);
+        partial[k] = v;
     }
-    if (threadIdx.x == 0) atomicExch(ticket, 0u);
+}
+
+__global__ void __launch_bounds__(64)
+    k_seg_min(const int32_t* __restrict__ chunk_off, const double* __restrict__ partial,
+              double* __restrict__ stable) {
+    const int inst = blockIdx.x;
+    double v = DMAX;
+    for (int k = chunk_off[inst] + threadIdx.x; k < chunk_off[inst + 1]; k += 64)
+        v = kmin(partial[k], v);
+    for (int o = 32; o > 0; o >>= 1) v = kmin(__shfl_down(v, o, 64), v);
+    if (threadIdx.x == 0) stable[inst] = v;
+}
+
+// RungeKutta::estimate_dt (runge_kutta.cpp:108-119) per instance
+__global__ void k_dt_rules(int n, const double* __restrict__ stable, double cfl,
+                           const double* __restrict__ dt_prev, const double* __restrict__ t,
+                           double max_time, double* __restrict__ dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = kmin(cfl * stable[i], 1.5 * dt_prev[i]);
+    dt[i] = kmin(d, max_time - t[i]);
 }
 
 __global__ void k_advance(int n, const double* __restrict__ dt, double* __restrict__ dt_prev,
@@ -157,7 +152,6 @@ struct ibis_batch {
     int64_t* chunk_lo = nullptr;    // device, nchunks + 1: cell range of each chunk
     int32_t* chunk_off = nullptr;   // device, n_inst + 1: chunk range of each instance
     double* partial = nullptr;      // device, nchunks
-    unsigned* ticket = nullptr;     // device, 1: blocks of k_dt_step done (0 between launches)
     double R = 0.0, Cv = 0.0;
     double* dt_cells = nullptr;
     double* stable = nullptr;
@@ -188,7 +182,7 @@ void ibis_batch_destroy(ibis_batch* b) {
     for (void* p : {(void*)b->cell_off, (void*)b->inst_of_cell, (void*)b->chunk_lo,
                     (void*)b->chunk_off, (void*)b->partial, (void*)b->dt_cells,
                     (void*)b->stable, (void*)b->dt, (void*)b->dt_prev, (void*)b->t,
-                    (void*)b->flow_tmp, (void*)b->ticket})
+                    (void*)b->flow_tmp})
         if (p) (void)hipFree(p);
     for (double* p : b->k) (void)hipFree(p);
     ibis_residual_destroy(b->r);
@@ -287,8 +281,6 @@ ibis_status ibis_batch_create(const ibis_mesh* m, int n_inst, const ibis_fv_desc
     if (e == hipSuccess) e = hipMalloc((void**)&b->chunk_off, sizeof(int32_t) * (n_inst + 1));
     if (e == hipSuccess) e = hipMemcpy(b->chunk_off, coff.data(), sizeof(int32_t) * (n_inst + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc((void**)&b->partial, sizeof(double) * std::max(b->nchunks, 1));
-    if (e == hipSuccess) e = hipMalloc((void**)&b->ticket, sizeof(unsigned));
-    if (e == hipSuccess) e = hipMemset(b->ticket, 0, sizeof(unsigned));
     if (e == hipSuccess) e = hipMalloc((void**)&b->dt_cells, sizeof(double) * b->N);
     for (double** p : {&b->stable, &b->dt, &b->dt_prev, &b->t})
         if (e == hipSuccess) e = hipMalloc((void**)p, sizeof(double) * n_inst);
@@ -354,12 +346,12 @@ static ibis_status stage_prim(ibis_batch* b, const StageArgs& sa, double* cons_o
     return stage_prim(b, sa, cons_out, prim, 0, b->Nt);
 }
 
-// the time step of every instance from the local time steps in b->dt_cells
+// the time step of every instance from the state in b->prim
 static void batch_dt(ibis_batch* b, hipStream_t st) {
-    k_dt_step<<<std::max(b->nchunks, 1), BB, 0, st>>>(b->nchunks, b->chunk_lo, b->dt_cells,
-                                                     b->partial, b->ticket, b->n_inst, b->chunk_off,
-                                                     b->stable, b->cfl, b->dt_prev, b->t,
-                                                     b->max_time, b->dt);
+    if (b->nchunks) k_chunk_min<<<b->nchunks, BB, 0, st>>>(b->chunk_lo, b->dt_cells, b->partial);
+    k_seg_min<<<b->n_inst, 64, 0, st>>>(b->chunk_off, b->partial, b->stable);
+    k_dt_rules<<<nblk(b->n_inst), BB, 0, st>>>(b->n_inst, b->stable, b->cfl, b->dt_prev, b->t,
+                                              b->max_time, b->dt);
 }
 
 // the stored stages of stage i's value (runge_kutta.cpp:151-191): k_0
