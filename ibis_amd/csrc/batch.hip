@@ -56,7 +56,9 @@ constexpr double DMAX = 1.79769313486231570815e+308;
 // stable_dt[i] = min over the valid cells of instance i, in two passes:
 // every chunk (at most CHUNK cells of one instance) reduces to partial[k],
 // then each instance reduces its chunks chunk_off[i] .. chunk_off[i+1].
-constexpr int CHUNK = 8192;
+// 1024 cells: four loads per thread of a 256-thread block, so that even one
+// small instance spreads its minimum over many blocks
+constexpr int CHUNK = 1024;
 
 __global__ void __launch_bounds__(BB)
     k_chunk_min(const int64_t* __restrict__ chunk_lo, const double* __restrict__ dt_cells,
@@ -383,8 +385,8 @@ static bool step_carries_dt(ibis_batch* b) {
 // residual does not change), then each residual's epilogue evaluates the
 // next stage's value (or the step's final one) from the stored stages and
 // its own dU/dt in registers, with the same expressions in the same order
-// as the separate stage pass; a pass over the ghost rows keeps them as the
-// separate pass leaves them.  The last stage's dU/dt is not stored.
+// as the separate stage pass, and the ghost rows (spread over the tiles) as
+// the separate pass leaves them.  The last stage's dU/dt is not stored.
 // When step_carries_dt, the last stage's epilogue also writes the local
 // time steps of the step's result (the same bits as ibis_residual_cell_dt
 // of that state), so the next step starts from them and no step reads the
@@ -404,7 +406,6 @@ static ibis_status batch_step_fused(ibis_batch* b) {
         e.dt = b->dt;
         e.R = b->R;
         e.Cv = b->Cv;
-        StageArgs sa{};
         double* dudt = b->k[i];
         if (i + 1 < s) {
             // stage i+1: the stored k_0 .. k_{i-1}, then this residual's k_i
@@ -418,7 +419,6 @@ static ibis_status batch_step_fused(ibis_batch* b) {
             e.use_reg = reg ? 1 : 0;
             e.c_reg = reg ? all.c[all.nk - 1] : 0.0;
             e.prim_out = b->flow_tmp;
-            sa = all;
         } else {
             // the step's value: k_0 .. k_{s-2} stored, k_{s-1} in registers
             for (int j = 0; j + 1 < s; j++) {
@@ -432,19 +432,12 @@ static ibis_status batch_step_fused(ibis_batch* b) {
             e.prim_out = b->prim;
             if (carry_dt) e.dt_cells_out = b->dt_cells;
             dudt = nullptr;
-            for (int j = 0; j < s; j++) {
-                sa.k[j] = b->k[j];
-                sa.c[j] = b->bw[j];
-            }
-            sa.nk = s;
         }
+        // the ghost rows (their dU/dt rows stay zero) in the same epilogue,
+        // as the separate pass evaluates them
+        e.ghost_rows = 1;
         double* src = i == 0 ? b->prim : b->flow_tmp;
         IBIS_TRY(ibis_compute_dudt_stage(b->r, src, dudt, &e, sh));
-        // ghost rows (their dU/dt rows stay zero): as the separate pass
-        if (i + 1 < s)
-            IBIS_TRY(stage_prim(b, sa, nullptr, b->flow_tmp, b->N, b->Nt));
-        else
-            IBIS_TRY(stage_prim(b, sa, b->cons, b->prim, b->N, b->Nt));
     }
     k_advance<<<nblk(b->n_inst), BB, 0, st>>>(b->n_inst, b->dt, b->dt_prev, b->t);
     HIP_TRY(hipGetLastError());

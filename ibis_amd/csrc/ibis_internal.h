@@ -60,6 +60,11 @@ struct ibis_stage_epilogue {
     double* prim_out = nullptr;
     double R = 0.0, Cv = 0.0;
     double* dt_cells_out = nullptr;
+    // 1: the ghost rows too, with the residual's own dU/dt term as +0 (its
+    // ghost rows are zero), as a separate stage pass over them gives
+    int ghost_rows = 0;
+    // set by ibis_compute_dudt_stage: tile t takes rows N + t*ghost_per_tile ...
+    int ghost_per_tile = 0;
     // filled in by ibis_compute_dudt_stage from the residual's description
     double dt_gamma = 0.0, dt_R = 0.0, dt_sf = 0.0;
 };

@@ -2118,6 +2118,27 @@ __global__ void __launch_bounds__(MAXTILE)
         k = kn;
     }
     __syncthreads();
+    if (EPI && epi.ghost_per_tile > 0) {
+        // this tile's share of the ghost rows: the stage value and its
+        // primitives, as the separate stage pass evaluates them
+        const int g1 = min(Nt, N + (t + 1) * epi.ghost_per_tile);
+        for (int g = N + t * epi.ghost_per_tile + threadIdx.x; g < g1; g += blockDim.x) {
+            const double dtg = epi.dt[epi.inst_of_cell[g]];
+            double qv[NC], out[NPRIM];
+#pragma unroll
+            for (int q = 0; q < NC; q++) {
+                const size_t e = (size_t)q * Nt + g;
+                double v = epi.u[e];
+                for (int j = 0; j < epi.nk; j++) v = v + epi.k[j][e] * (epi.c[j] * dtg);
+                if (epi.use_reg) v = v + 0.0 * (epi.c_reg * dtg);
+                qv[q] = v;
+                if (epi.cons_out) epi.cons_out[e] = v;
+            }
+            ibis::cons_to_prim_cell<double, DIM>(qv, epi.R, epi.Cv, out);
+#pragma unroll
+            for (int q = 0; q < NPRIM; q++) epi.prim_out[(size_t)q * Nt + g] = out[q];
+        }
+    }
     const int c = t * m.tile + threadIdx.x;
     if (c >= N) return;
     double d[NC];
@@ -4013,6 +4034,11 @@ ibis_status ibis_compute_dudt_stage(ibis_residual* r, double* prim, double* dudt
     e.dt_gamma = d.gamma;
     e.dt_R = d.R;
     e.dt_sf = d.signal_factor;
+    const DevMesh& D = r->d;
+    // at least a tile's worth of rows per tile (one per thread): the first
+    // few tiles take them all and the others pay nothing on their way
+    e.ghost_per_tile =
+        e.ghost_rows && D.Nt > D.N ? std::max(D.tile, (D.Nt - D.N + D.ntiles - 1) / D.ntiles) : 0;
     return compute_dudt_impl(r, prim, dudt, 1, stream, &e);
 }
 
