@@ -367,6 +367,9 @@ def main():
     ap.add_argument("--batch-profile", action="store_true",
                     help="only time the batched steps (fused and separate stage updates), "
                          "for a kernel profile of the batch")
+    ap.add_argument("--rk-profile", type=int, default=0, metavar="STEPS",
+                    help="only run the RK driver leg (192 x 96 wedge) with STEPS timed steps, "
+                         "for a kernel profile of the device-resident driver")
     ap.add_argument("--no-fp32", action="store_true",
                     help="skip the FP32-flux / FP64-accumulate comparison leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -390,6 +393,9 @@ def main():
         sys.exit(2)
     if args.batch_profile:
         batch_profile_leg(args.batch_steps)
+        return
+    if args.rk_profile:
+        print(json.dumps({"rk_driver": rk_leg(args.rk_profile)}))
         return
 
     import torch
